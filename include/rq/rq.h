@@ -1,0 +1,145 @@
+/*
+ * rq — MI355X-native ray-query engine: C ABI.
+ *
+ * Drop-in GPU_LBVH backend for the acceleration-structure path of
+ * Alhajras/Raytracer-Data-structures. The reference has no plugin interface;
+ * its boundary is two switch statements over free functions (build side
+ * main.cpp:791-843, query side main.cpp:330-388). Each entry point below names
+ * the reference function it replaces.
+ *
+ * Conventions
+ *  - Every function returns RQ_OK (0) or a negative RQ_E_* code; nothing
+ *    throws across the ABI. rq_last_error() describes the last failure.
+ *  - Host arrays belong to the caller. Device memory behind opaque handles
+ *    belongs to the library and is charged against the context's HBM budget;
+ *    exceeding it returns RQ_E_NOMEM (admission failure) without allocating.
+ *  - Ray and hit buffers passed to queries are DEVICE pointers (SoA).
+ *  - `stream` is a hipStream_t passed as void* (NULL = default stream); calls
+ *    are asynchronous on it and rq_sync() waits.
+ *  - One rq_ctx is used by one host thread at a time.
+ */
+#ifndef RQ_RQ_H
+#define RQ_RQ_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RQ_OK 0
+#define RQ_E_INVALID (-1)     /* bad argument or size                        */
+#define RQ_E_NOMEM (-2)       /* HBM budget exceeded or allocation failed    */
+#define RQ_E_HIP (-3)         /* HIP runtime error                           */
+#define RQ_E_IO (-4)          /* file could not be read or parsed            */
+#define RQ_E_NODEVICE (-5)    /* no usable gfx950 device                     */
+#define RQ_E_UNSUPPORTED (-6) /* feature not available for this input       */
+
+/* Morton normalisation (accelerators.h:577 vs Karras 2012). */
+#define RQ_MORTON_REF_AFFINE 0 /* (centroid + 30) / 1000, bit-exact with the reference */
+#define RQ_MORTON_SCENE_AABB 1 /* (centroid - lo) / (hi - lo) over centroid bounds     */
+
+#define RQ_PRIM_TRIANGLE 0
+
+typedef struct rq_ctx rq_ctx;
+typedef struct rq_scene rq_scene;
+typedef struct rq_bvh rq_bvh;
+
+typedef struct {
+  const float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+} rq_rays; /* device SoA; a hit needs t in (tmin, tmax); tmin >= tmax = inactive */
+
+typedef struct {
+  int32_t* prim; /* primitive index, -1 on miss          */
+  float* t;      /* hit distance (+inf on miss), may be NULL */
+  float* u;      /* barycentrics, may be NULL            */
+  float* v;
+} rq_hits; /* device SoA */
+
+typedef struct {
+  uint32_t n_prims;    /* leaves                               */
+  uint32_t n_internal; /* n_prims - 1 (0 when n_prims <= 1)   */
+  uint32_t height;     /* longest root-to-leaf edge count      */
+  int32_t morton_mode;
+  float root_lo[3], root_hi[3];
+  size_t device_bytes; /* HBM held by the BVH                  */
+} rq_bvh_info;
+
+/* ---- context ---------------------------------------------------------- */
+int rq_create(rq_ctx** ctx, int device, size_t hbm_budget_bytes);
+int rq_destroy(rq_ctx* ctx);
+const char* rq_last_error(const rq_ctx* ctx);
+const char* rq_version(void);
+size_t rq_bytes_in_use(const rq_ctx* ctx);
+
+/* ---- scene -------------------------------------------------------------
+ * Replaces createScene_new (main.cpp:599-721) + the per-object SceneObject
+ * vector: host SoA vertices and a triangle index list (3 per triangle, 0-based)
+ * are copied into device-resident SoA buffers. */
+int rq_scene_upload(rq_ctx* ctx, const float* vx, const float* vy, const float* vz, uint32_t nv,
+                    const uint32_t* idx, uint32_t ntri, int prim_kind, rq_scene** out,
+                    void* stream);
+/* Replace the vertex positions of an uploaded scene (same counts); the next
+ * rq_rebuild_lbvh sees them. */
+int rq_scene_update_vertices(rq_ctx* ctx, rq_scene* s, const float* vx, const float* vy,
+                             const float* vz, void* stream);
+
+/* ---- build -------------------------------------------------------------
+ * Replaces constructLBVHTree (accelerators.h:570-586) and constructBVHNew
+ * (accelerators.h:246-337): Morton codes -> 8-bit-digit radix sort ->
+ * Karras internal-node emission -> bottom-up atomic AABB refit, all on the GPU. */
+int rq_build_lbvh(rq_ctx* ctx, const rq_scene* s, int morton_mode, rq_bvh** out, void* stream);
+/* Rebuild in place from the scene's current vertices (no allocation). */
+int rq_rebuild_lbvh(rq_ctx* ctx, rq_bvh* b, void* stream);
+int rq_bvh_info_get(rq_ctx* ctx, const rq_bvh* b, rq_bvh_info* info);
+/* Download the build for parity checks (host arrays, any may be NULL):
+ * codes[n] in primitive order, sorted_idx[n], left/right[n-1] (child >= 0 is an
+ * internal node, child < 0 is leaf ~child), boxes[(2n-1)*6] internal nodes then
+ * leaves, (lo xyz, hi xyz). */
+int rq_bvh_export(rq_ctx* ctx, const rq_bvh* b, uint32_t* codes, uint32_t* sorted_idx,
+                  int32_t* left, int32_t* right, float* boxes);
+
+/* ---- queries -----------------------------------------------------------
+ * Closest hit replaces boxIntersect (accelerators.h:668-690) plus the
+ * candidate loop of castRay (main.cpp:343-358); the result equals the NONE
+ * brute-force loop (main.cpp:373-387): smallest t, lowest index on ties.
+ * Any-hit replaces kdtreeIntersect (accelerators.h:997-1086): occluded[i] = 1
+ * iff some primitive has t in (tmin, tmax). */
+int rq_closest_hit(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
+                   void* stream);
+int rq_any_hit(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, uint8_t* occluded,
+               void* stream);
+/* Instrumented variants: per-ray node-record visits and triangle tests of the
+ * canonical traversal (device uint32 arrays). Slower; for measurement. */
+int rq_closest_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                           rq_hits* hits, uint32_t* visits, uint32_t* tests, void* stream);
+int rq_any_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                       uint8_t* occluded, uint32_t* visits, uint32_t* tests, void* stream);
+
+/* ---- device buffers (for callers without their own allocator) ---------- */
+int rq_device_alloc(rq_ctx* ctx, size_t bytes, void** dptr);
+int rq_device_free(rq_ctx* ctx, void* dptr);
+int rq_memcpy_h2d(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+int rq_memcpy_d2h(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+int rq_memset_d(rq_ctx* ctx, void* dst, int value, size_t bytes, void* stream);
+int rq_sync(rq_ctx* ctx, void* stream);
+/* Reads and clears the device error bits of past queries (synchronises the
+ * device): bit 0 = a ray needed a deeper traversal stack than provided. */
+int rq_device_errors(rq_ctx* ctx, uint32_t* bits);
+int rq_stream_create(rq_ctx* ctx, void** stream);
+int rq_stream_destroy(rq_ctx* ctx, void* stream);
+
+/* ---- per-kernel timing (HIP events on the launch stream) ---------------
+ * When enabled, every kernel the library launches is bracketed by events on
+ * its own stream; rq_profile_read sums the durations per kernel name. */
+int rq_profile_enable(rq_ctx* ctx, int on);
+int rq_profile_reset(rq_ctx* ctx);
+/* Returns the number of kernel names; fills up to cap entries. */
+int rq_profile_read(rq_ctx* ctx, char names[][48], double* total_ms, uint64_t* launches, int cap);
+
+/* ---- handles ---------------------------------------------------------- */
+int rq_release(rq_ctx* ctx, void* handle); /* rq_scene* or rq_bvh* */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,86 @@
+/*
+ * rq — wavefront renderer on top of the ray-query ABI (C ABI).
+ *
+ * Replaces render() (main.cpp:541-566) + castRay (main.cpp:291-500) for the
+ * GPU_LBVH data structure. The reference walks pixels one by one and recurses
+ * per ray; here each stage is one batched launch over all samples:
+ * raygen -> closest hit -> shade (spawn shadow / bounce rays) -> [closest hit
+ * of bounce rays -> shade] -> any-hit of shadow rays -> resolve.
+ *
+ * Shading is castRay's DIFFUSE_AND_GLOSSY branch (main.cpp:447-492) with a
+ * real shadow test in place of the trace_more stub (main.cpp:235-245): the
+ * shadow ray starts at the hit point offset by bias along the normal and is
+ * occluded if any triangle lies strictly between it and the light. With
+ * bounce = 1 a mirror ray reflect(d, N) is traced from every primary hit
+ * (origin offset as main.cpp:438-441) and its direct light is added weighted
+ * by fresnel(d, N, 2) (main.cpp:85-103, :444); a missed bounce sees the sky.
+ */
+#ifndef RQ_RENDER_H
+#define RQ_RENDER_H
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rq.h"
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Camera jitter source. The camera is the reference's (main.cpp:544-557):
+ * origin 0, fov 30 degrees, direction normalize(xx, yy, -1), a jitter pair
+ * (r1, r2) per sample always applied. */
+#define RQ_CAMERA_REF_JITTER 0 /* std::mt19937 (default seed) + uniform_real_distribution<double>, main.cpp:503-508 */
+#define RQ_CAMERA_PHILOX 1     /* Philox4x32-10 keyed by seed, counter = sample index */
+
+typedef struct {
+  uint32_t width, height, spp;
+  int32_t camera;
+  uint64_t seed;
+  float light[3];    /* point light position (main.cpp:775: (0, 3, 30)) */
+  float emission[3]; /* light emission (1, 1, 1) */
+  float surface[3];  /* surfaceColor added per light (main.cpp:490) */
+  float sky[3];      /* miss colour (0.6, 0.8, 1) (main.cpp:312, :341) */
+  float bias;        /* 1e-4 (main.cpp:404) */
+  int32_t bounce;    /* 0: primary + shadow; 1: + mirror bounce and its shadow ray */
+} rq_render_params;
+
+typedef struct {
+  uint64_t primary, bounce, shadow; /* rays traced in the last frame (active only) */
+} rq_frame_counts;
+
+typedef struct rq_renderer rq_renderer;
+
+/* Fills p with the reference defaults (640x480, 1 spp, REF_JITTER, light
+ * (0,3,30), emission 1, surface 0, sky (0.6,0.8,1), bias 1e-4, no bounce). */
+void rq_render_defaults(rq_render_params* p);
+int rq_renderer_create(rq_ctx* ctx, const rq_render_params* p, rq_renderer** out);
+int rq_renderer_destroy(rq_ctx* ctx, rq_renderer* r);
+/* Enqueue one frame on `stream` against bvh (its scene supplies normals). */
+int rq_render_frame(rq_ctx* ctx, rq_renderer* r, const rq_bvh* bvh, void* stream);
+int rq_render_counts(rq_ctx* ctx, rq_renderer* r, rq_frame_counts* out);
+/* Image after the last frame: rgb = per-pixel sum over samples (w*h*3
+ * floats, may be NULL); ppm = (uint8)(min(1, rgb/spp) * 255) (main.cpp:521-523). */
+int rq_render_read_image(rq_ctx* ctx, rq_renderer* r, float* rgb, uint8_t* ppm);
+/* Per-sample results of the last frame, any pointer may be NULL:
+ * primary/bounce hit ids (-1 = miss or no ray), shadow flags, sample colour (3 floats). */
+int rq_render_read_samples(rq_ctx* ctx, rq_renderer* r, int32_t* hit0, int32_t* hit1,
+                           uint8_t* shadow0, uint8_t* shadow1, float* sample_rgb);
+/* Device SoA of the primary rays of the last frame (valid until destroy). */
+int rq_render_primary_rays(rq_ctx* ctx, rq_renderer* r, rq_rays* out, uint32_t* n);
+
+/* Batch ray generators into caller-owned device SoA buffers. */
+int rq_raygen_camera(rq_ctx* ctx, const rq_render_params* p, const double* jitter_dev,
+                     float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
+                     float* tmin, float* tmax, void* stream);
+/* n rays: origin uniform in [lo, hi], direction uniform on the sphere
+ * (rejection from the cube, at most 64 Philox draws, then +z), t in (0, inf). */
+int rq_raygen_uniform(rq_ctx* ctx, uint32_t n, uint64_t seed, const float lo[3], const float hi[3],
+                      float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
+                      float* tmin, float* tmax, void* stream);
+
+/* P6 PPM writer (main.cpp:518-524 layout). */
+int rq_write_ppm(const char* path, uint32_t w, uint32_t h, const uint8_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,84 @@
+"""Build librq.so (HIP kernels for gfx950 + host C ABI) in-tree.
+
+    python raytracer-data-structures_amd/build.py
+
+Every source is compiled on each call (objects under _build/, then linked into
+raytracer-data-structures_amd/librq.so). Device code is built with
+-ffp-contract=off (parity with the CPU oracle needs one rounding per written
+operation) and with packed FP32 instructions disabled.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+INCLUDE = PKG.parent / "include"
+BUILD = PKG / "_build"
+LIB = PKG / "librq.so"
+
+HIP_SOURCES = ["build_kernels.hip", "trace_kernels.hip", "render_kernels.hip"]
+HIP_HOST_SOURCES = ["rq_api.cpp", "render.cpp"]
+CXX_SOURCES = ["host/scene_gen.cpp"]
+
+ARCH = os.environ.get("RQ_OFFLOAD_ARCH", "gfx950")
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+          f"-I{CSRC}", f"-I{INCLUDE}", "-Wall", "-Wno-unused-value", "-Wno-unused-result",
+          "-Wno-unused-function"]
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops"]
+
+
+def _hipcc() -> str:
+    for cand in ("/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found: the gfx950 build needs ROCm")
+
+
+def _run(cmd: list[str]) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    noise = ("packed-fp32-ops", "is not a recognized feature")
+    err = "\n".join(l for l in r.stderr.splitlines() if not any(n in l for n in noise))
+    if r.returncode != 0:
+        raise RuntimeError(f"build failed: {' '.join(cmd)}\n{err}")
+    if err.strip():
+        sys.stderr.write(err + "\n")
+
+
+def build(verbose: bool = False) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    hipcc = _hipcc()
+    jobs = []
+    objs = []
+    for src in HIP_SOURCES:
+        obj = BUILD / (Path(src).stem + ".o")
+        jobs.append([hipcc, *HIP_FLAGS, *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
+    for src in HIP_HOST_SOURCES:
+        obj = BUILD / (Path(src).stem + ".o")
+        jobs.append([hipcc, *HIP_FLAGS, *COMMON, "-x", "hip", "-c", str(CSRC / src), "-o",
+                     str(obj)])
+        objs.append(obj)
+    for src in CXX_SOURCES:
+        obj = BUILD / (Path(src).stem + ".o")
+        jobs.append(["g++", *COMMON, "-c", str(CSRC / src), "-o", str(obj)])
+        objs.append(obj)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for f in [ex.submit(_run, j) for j in jobs]:
+            f.result()
+    tmp = LIB.with_suffix(".so.tmp")
+    _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

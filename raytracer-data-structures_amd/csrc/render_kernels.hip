@@ -1,0 +1,333 @@
+// Ray generation and shading kernels of the wavefront renderer (gfx950).
+//
+// Replace the per-pixel loop of render() (main.cpp:549-564) and the shading
+// part of castRay (main.cpp:394-497). Arithmetic follows the reference's
+// operation order (Vec3 ops of geometry.h:84-134, no contraction); only
+// powf (specular, main.cpp:475) is not correctly rounded on either side.
+#include <algorithm>
+
+#include "rq/rq_render.h"
+#include "rq_render_internal.h"
+
+namespace rq {
+namespace {
+
+struct V3 {
+  float x, y, z;
+};
+__device__ inline V3 sub(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline V3 add(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V3 mul(V3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ inline float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ inline V3 cross(V3 a, V3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ inline V3 normalize(V3 a) {
+  float n = (a.x * a.x + a.y * a.y) + a.z * a.z;
+  if (n > 0.0f) {
+    float f = 1.0f / sqrtf(n);
+    a.x *= f, a.y *= f, a.z *= f;
+  }
+  return a;
+}
+// std::max(0.f, x) / std::min semantics (comparison-based, not fmaxf).
+__device__ inline float max0(float x) { return (0.0f < x) ? x : 0.0f; }
+
+// ---- Philox4x32-10 (Salmon et al. 2011) -------------------------------------
+__device__ inline void philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint64_t seed,
+                              uint32_t out[4]) {
+  uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0), lo0 = 0xD2511F53u * c0;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2), lo1 = 0xCD9E8D57u * c2;
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0, c1 = lo1, c2 = n2, c3 = lo0;
+    k0 += 0x9E3779B9u, k1 += 0xBB67AE85u;
+  }
+  out[0] = c0, out[1] = c1, out[2] = c2, out[3] = c3;
+}
+__device__ inline double u53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+__device__ inline float u24(uint32_t a) { return (float)(a >> 8) * (1.0f / 16777216.0f); }
+
+// ---- ray generation ---------------------------------------------------------
+// main.cpp:554-557 with the C++ promotions of the reference expression:
+// (x + r) is double; invWidth, angle, aspectratio are floats widened to
+// double; the product is narrowed to float; then Vec3::normalize in float.
+__global__ __launch_bounds__(256) void k_raygen_camera(CamParams cp, const double* __restrict__ jit,
+                                                        float* ox, float* oy, float* oz, float* dx,
+                                                        float* dy, float* dz, float* tmin,
+                                                        float* tmax, uint32_t* counters) {
+  const uint64_t n = (uint64_t)cp.w * cp.h * cp.spp;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (counters && i < 4) counters[i] = 0u;
+  if (i >= n) return;
+  const uint64_t pix = i / cp.spp;
+  const uint32_t x = (uint32_t)(pix % cp.w), y = (uint32_t)(pix / cp.w);
+  double r1, r2;
+  if (jit) {
+    r1 = jit[2 * i], r2 = jit[2 * i + 1];
+  } else {
+    uint32_t o[4];
+    philox((uint32_t)i, (uint32_t)(i >> 32), 0u, 0x43414D31u, cp.seed, o);
+    r1 = u53(o[0], o[1]), r2 = u53(o[2], o[3]);
+  }
+  const float xx =
+      (float)((2.0 * (((double)x + r1) * (double)cp.invW) - 1.0) * (double)cp.angle * (double)cp.aspect);
+  const float yy = (float)((1.0 - 2.0 * (((double)y + r2) * (double)cp.invH)) * (double)cp.angle);
+  const V3 d = normalize({xx, yy, -1.0f});
+  ox[i] = 0.0f, oy[i] = 0.0f, oz[i] = 0.0f;
+  dx[i] = d.x, dy[i] = d.y, dz[i] = d.z;
+  tmin[i] = 0.0f, tmax[i] = INFINITY;
+}
+
+__global__ __launch_bounds__(256) void k_raygen_uniform(uint32_t n, uint64_t seed, float lox,
+                                                         float loy, float loz, float ex, float ey,
+                                                         float ez, float* ox, float* oy, float* oz,
+                                                         float* dx, float* dy, float* dz,
+                                                         float* tmin, float* tmax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t o[4];
+  philox(i, 0u, 0u, 0x4F524947u, seed, o);
+  ox[i] = lox + ex * u24(o[0]);
+  oy[i] = loy + ey * u24(o[1]);
+  oz[i] = loz + ez * u24(o[2]);
+  V3 d{0.0f, 0.0f, 1.0f};
+  for (uint32_t a = 0; a < 64; ++a) {
+    philox(i, a, 1u, 0x44495221u, seed, o);
+    const float x = u24(o[0]) * 2.0f - 1.0f, y = u24(o[1]) * 2.0f - 1.0f,
+                z = u24(o[2]) * 2.0f - 1.0f;
+    const float r2 = (x * x + y * y) + z * z;
+    if (r2 > 1e-6f && r2 <= 1.0f) {
+      const float f = 1.0f / sqrtf(r2);
+      d = {x * f, y * f, z * f};
+      break;
+    }
+  }
+  dx[i] = d.x, dy[i] = d.y, dz[i] = d.z;
+  tmin[i] = 0.0f, tmax[i] = INFINITY;
+}
+
+// ---- shading ---------------------------------------------------------------
+struct Geo {
+  V3 p, n;
+};
+__device__ inline V3 vtx(const SceneView& s, uint32_t i) { return {s.vx[i], s.vy[i], s.vz[i]}; }
+// Hit point o + d*t and the geometric normal normalize(e1 x e2) turned to face
+// against the ray (main.cpp:396-406 with the triangle normal for the sphere's).
+__device__ inline Geo hit_geo(const SceneView& s, int32_t prim, V3 o, V3 d, float t) {
+  const uint32_t a = s.idx[3 * prim], b = s.idx[3 * prim + 1], c = s.idx[3 * prim + 2];
+  const V3 v0 = vtx(s, a), v1 = vtx(s, b), v2 = vtx(s, c);
+  Geo g;
+  g.p = add(o, mul(d, t));
+  g.n = normalize(cross(sub(v1, v0), sub(v2, v0)));
+  if (dot(d, g.n) > 0.0f) g.n = {-g.n.x, -g.n.y, -g.n.z};
+  return g;
+}
+__device__ inline V3 reflect(V3 I, V3 N) { return sub(I, mul(N, 2.0f * dot(I, N))); }
+// main.cpp:85-103
+__device__ inline float fresnel(V3 I, V3 N, float ior) {
+  float c = dot(I, N);
+  c = (c < 1.0f) ? c : 1.0f;
+  c = (-1.0f < c) ? c : -1.0f;
+  float etai = 1.0f, etat = ior;
+  if (c > 0.0f) {
+    float tmp = etai;
+    etai = etat, etat = tmp;
+  }
+  const float sint = etai / etat * sqrtf(max0(1.0f - c * c));
+  if (sint >= 1.0f) return 1.0f;
+  const float cost = sqrtf(max0(1.0f - sint * sint));
+  c = fabsf(c);
+  const float Rs = ((etat * c) - (etai * cost)) / ((etat * c) + (etai * cost));
+  const float Rp = ((etai * c) - (etat * cost)) / ((etai * c) + (etat * cost));
+  return (Rs * Rs + Rp * Rp) / 2.0f;
+}
+// main.cpp:463-490 for one light; diffuse = getDiffuseColor(0.2) * 0.8.
+__device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, bool shadow) {
+  const V3 L{sp.light[0], sp.light[1], sp.light[2]};
+  const V3 ld = normalize(sub(L, g.p));
+  const float LdotN = max0(dot(ld, g.n));
+  const float vis = shadow ? 0.0f : 1.0f;
+  const V3 amt{(sp.emission[0] * vis) * LdotN, (sp.emission[1] * vis) * LdotN,
+               (sp.emission[2] * vis) * LdotN};
+  const V3 R = reflect({-ld.x, -ld.y, -ld.z}, g.n);
+  const float s25 = powf(max0(-dot(R, d)), 25.0f);
+  const float k0 = 0.815f * 0.8f, k1 = 0.235f * 0.8f, k2 = 0.031f * 0.8f;
+  V3 c;
+  c.x = ((amt.x * k0) / 2.0f + (s25 * sp.emission[0]) * 0.5f) + sp.surface[0];
+  c.y = ((amt.y * k1) / 2.0f + (s25 * sp.emission[1]) * 0.5f) + sp.surface[1];
+  c.z = ((amt.z * k2) / 2.0f + (s25 * sp.emission[2]) * 0.5f) + sp.surface[2];
+  return c;
+}
+__device__ inline void put_ray(const RaySoA& R, uint64_t i, V3 o, V3 d, float tmin, float tmax) {
+  R.ox[i] = o.x, R.oy[i] = o.y, R.oz[i] = o.z;
+  R.dx[i] = d.x, R.dy[i] = d.y, R.dz[i] = d.z;
+  R.tmin[i] = tmin, R.tmax[i] = tmax;
+}
+__device__ inline void put_inactive(const RaySoA& R, uint64_t i) {
+  R.tmin[i] = 0.0f, R.tmax[i] = 0.0f;
+}
+__device__ inline void shadow_ray(const RaySoA& R, uint64_t i, V3 d, const Geo& g,
+                                  const ShadeParams& sp) {
+  const V3 L{sp.light[0], sp.light[1], sp.light[2]};
+  const V3 so = dot(d, g.n) < 0.0f ? add(g.p, mul(g.n, sp.bias)) : sub(g.p, mul(g.n, sp.bias));
+  V3 ld = sub(L, g.p);
+  const float dist2 = dot(ld, ld);
+  ld = normalize(ld);
+  put_ray(R, i, so, ld, 0.0f, sqrtf(dist2));
+}
+__device__ inline V3 get_o(const RaySoA& R, uint64_t i) { return {R.ox[i], R.oy[i], R.oz[i]}; }
+__device__ inline V3 get_d(const RaySoA& R, uint64_t i) { return {R.dx[i], R.dy[i], R.dz[i]}; }
+
+__device__ inline void count_wave(uint32_t* ctr, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (uint32_t)__popcll(m));
+}
+
+// Stage 1: from primary hits spawn the shadow ray (slot i) and the bounce ray.
+__global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams sp, uint32_t n,
+                                                        RaySoA prim, const int32_t* __restrict__ h0,
+                                                        const float* __restrict__ t0, RaySoA shad,
+                                                        RaySoA bnc, uint32_t* counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < n;
+  const int32_t p = in ? h0[i] : -1;
+  count_wave(&counters[0], in);
+  count_wave(&counters[2], p >= 0);
+  if (sp.bounce) count_wave(&counters[1], p >= 0);
+  if (!in) return;
+  if (p < 0) {
+    put_inactive(shad, i);
+    if (sp.bounce) put_inactive(bnc, i);
+    return;
+  }
+  const V3 o = get_o(prim, i), d = get_d(prim, i);
+  const Geo g = hit_geo(s, p, o, d, t0[i]);
+  shadow_ray(shad, i, d, g, sp);
+  if (sp.bounce) {
+    const V3 rd = normalize(reflect(d, g.n));
+    const V3 ro =
+        dot(rd, g.n) < 0.0f ? sub(g.p, mul(g.n, sp.bias)) : add(g.p, mul(g.n, sp.bias));
+    put_ray(bnc, i, ro, rd, 0.0f, INFINITY);
+  }
+}
+
+// Stage 2: from bounce hits spawn their shadow ray (slot n + i).
+__global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams sp, uint32_t n,
+                                                       RaySoA bnc, const int32_t* __restrict__ h1,
+                                                       const float* __restrict__ t1, RaySoA shad,
+                                                       uint32_t* counters) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < n;
+  const int32_t p = in ? h1[i] : -1;
+  count_wave(&counters[2], p >= 0);
+  if (!in) return;
+  if (p < 0) {
+    put_inactive(shad, (uint64_t)n + i);
+    return;
+  }
+  const V3 o = get_o(bnc, i), d = get_d(bnc, i);
+  const Geo g = hit_geo(s, p, o, d, t1[i]);
+  shadow_ray(shad, (uint64_t)n + i, d, g, sp);
+}
+
+// Final: per pixel, colour every sample and sum them in sample order
+// (main.cpp:551-560).
+__global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp, uint32_t npix,
+                                                      uint32_t spp, RaySoA prim, RaySoA bnc,
+                                                      const int32_t* __restrict__ h0,
+                                                      const float* __restrict__ t0,
+                                                      const int32_t* __restrict__ h1,
+                                                      const float* __restrict__ t1,
+                                                      const uint8_t* __restrict__ occ,
+                                                      float* __restrict__ srgb,
+                                                      float* __restrict__ img) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= npix) return;
+  const uint32_t n = npix * spp;
+  const V3 sky{sp.sky[0], sp.sky[1], sp.sky[2]};
+  V3 acc{0.0f, 0.0f, 0.0f};
+  for (uint32_t k = 0; k < spp; ++k) {
+    const uint32_t i = px * spp + k;
+    const int32_t p = h0[i];
+    V3 col = sky;
+    if (p >= 0) {
+      const V3 o = get_o(prim, i), d = get_d(prim, i);
+      const Geo g = hit_geo(s, p, o, d, t0[i]);
+      col = direct(d, g, sp, occ[i] != 0);
+      if (sp.bounce) {
+        const float kr = fresnel(d, g.n, 2.0f);
+        V3 bc = sky;
+        const int32_t q = h1[i];
+        if (q >= 0) {
+          const V3 o1 = get_o(bnc, i), d1 = get_d(bnc, i);
+          const Geo g1 = hit_geo(s, q, o1, d1, t1[i]);
+          bc = direct(d1, g1, sp, occ[n + i] != 0);
+        }
+        col = {col.x + kr * bc.x, col.y + kr * bc.y, col.z + kr * bc.z};
+      }
+    }
+    if (srgb) srgb[3 * (size_t)i] = col.x, srgb[3 * (size_t)i + 1] = col.y,
+                  srgb[3 * (size_t)i + 2] = col.z;
+    acc = {acc.x + col.x, acc.y + col.y, acc.z + col.z};
+  }
+  img[3 * (size_t)px] = acc.x, img[3 * (size_t)px + 1] = acc.y, img[3 * (size_t)px + 2] = acc.z;
+}
+
+inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_raygen_camera(const LaunchCtx& lc, const CamParams& cp, const double* jit,
+                                const RaySoA& R, uint32_t* counters) {
+  const uint64_t n = (uint64_t)cp.w * cp.h * cp.spp;
+  if (n == 0) return hipSuccess;
+  ProfScope ps(lc, "raygen_camera");
+  hipLaunchKernelGGL(k_raygen_camera, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, cp, jit, R.ox,
+                     R.oy, R.oz, R.dx, R.dy, R.dz, R.tmin, R.tmax, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed, const float lo[3],
+                                 const float hi[3], const RaySoA& R) {
+  if (n == 0) return hipSuccess;
+  ProfScope ps(lc, "raygen_uniform");
+  const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  hipLaunchKernelGGL(k_raygen_uniform, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, n, seed, lo[0],
+                     lo[1], lo[2], ex, ey, ez, R.ox, R.oy, R.oz, R.dx, R.dy, R.dz, R.tmin, R.tmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
+                                uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
+                                const RaySoA& shad, const RaySoA& bnc, uint32_t* counters) {
+  ProfScope ps(lc, "shade_primary");
+  hipLaunchKernelGGL(k_shade_primary, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, prim,
+                     h0, t0, shad, bnc, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
+                               uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
+                               const RaySoA& shad, uint32_t* counters) {
+  ProfScope ps(lc, "shade_bounce");
+  hipLaunchKernelGGL(k_shade_bounce, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, bnc,
+                     h1, t1, shad, counters);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_final(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
+                              uint32_t npix, uint32_t spp, const RaySoA& prim, const RaySoA& bnc,
+                              const int32_t* h0, const float* t0, const int32_t* h1,
+                              const float* t1, const uint8_t* occ, float* srgb, float* img) {
+  ProfScope ps(lc, "shade_final");
+  hipLaunchKernelGGL(k_shade_final, dim3(cdiv(npix, 256)), dim3(256), 0, lc.stream, s, sp, npix,
+                     spp, prim, bnc, h0, t0, h1, t1, occ, srgb, img);
+  return hipGetLastError();
+}
+
+}  // namespace rq

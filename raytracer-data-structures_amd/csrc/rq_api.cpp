@@ -1,0 +1,528 @@
+// C ABI of librq (include/rq/rq.h): contexts, HBM budget accounting, scene
+// upload, LBVH build/rebuild/export, closest/any-hit queries, per-kernel
+// event timing. Host code only; kernels live in the .hip files.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "rq_ctx.h"
+
+namespace rq {
+
+struct Profiler {
+  struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::map<std::string, std::pair<double, uint64_t>> totals;
+  void resolve() {
+    for (auto& p : pending) {
+      float ms = 0.0f;
+      hipEventSynchronize(p.b);
+      hipEventElapsedTime(&ms, p.a, p.b);
+      auto& t = totals[p.name];
+      t.first += ms;
+      t.second += 1;
+      hipEventDestroy(p.a);
+      hipEventDestroy(p.b);
+    }
+    pending.clear();
+  }
+};
+
+void prof_begin(Profiler* p, hipStream_t s, const char* name, hipEvent_t* ev) {
+  (void)p, (void)name;
+  hipEventCreate(ev);
+  hipEventRecord(*ev, s);
+}
+void prof_end(Profiler* p, hipStream_t s, const char* name, hipEvent_t ev) {
+  hipEvent_t b;
+  hipEventCreate(&b);
+  hipEventRecord(b, s);
+  p->pending.push_back({name, ev, b});
+  if (p->pending.size() > 4096) p->resolve();
+}
+
+int set_err(rq_ctx* c, int code, const char* fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+int hip_err(rq_ctx* c, hipError_t e, const char* what) {
+  return set_err(c, RQ_E_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int ctx_alloc(rq_ctx* c, size_t bytes, void** p) {
+  *p = nullptr;
+  if (bytes == 0) return RQ_OK;
+  if (c->used + bytes > c->budget)
+    return set_err(c, RQ_E_NOMEM, "HBM budget exceeded: %zu in use + %zu requested > %zu budget",
+                   c->used, bytes, c->budget);
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(c, RQ_E_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  }
+  c->used += bytes;
+  c->sizes[*p] = bytes;
+  return RQ_OK;
+}
+
+void ctx_free(rq_ctx* c, void* p) {
+  if (!p) return;
+  auto it = c->sizes.find(p);
+  if (it != c->sizes.end()) {
+    c->used -= it->second;
+    c->sizes.erase(it);
+  }
+  hipFree(p);
+}
+
+LaunchCtx launch_ctx(rq_ctx* c, void* stream) {
+  return LaunchCtx{(hipStream_t)stream, c->prof_on ? c->prof : nullptr};
+}
+
+uint32_t* next_counter_slot(rq_ctx* c) {
+  uint32_t s = c->counter_seq++ % COUNTER_SLOTS;
+  return c->counters + 2 * s;
+}
+
+}  // namespace rq
+
+using namespace rq;
+
+namespace {
+constexpr uint32_t SCENE_MAGIC = 0x53435131u;  // "SCQ1"
+constexpr uint32_t BVH_MAGIC = 0x42565131u;    // "BVQ1"
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
+  BuildBuffers& B = b->B;
+  B.ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  size_t nn = std::max<uint32_t>(n, 1);
+  size_t ni = n > 1 ? n - 1 : 1;
+  size_t sz[] = {4 * nn,
+                 4 * nn,
+                 4 * nn,
+                 4 * nn,
+                 4 * nn,
+                 (size_t)SORT_PASSES * std::max<uint32_t>(B.ntiles, 1) * 256 * 4,
+                 (size_t)MISC_WORDS * 4,
+                 4 * (2 * nn),
+                 4 * ni,
+                 8 * ni,
+                 64 * ni,
+                 48 * nn};
+  size_t total = 0;
+  for (size_t s : sz) total += align256(s);
+  void* p;
+  int rc = ctx_alloc(c, total, &p);
+  if (rc) return rc;
+  char* q = (char*)p;
+  void** dst[] = {(void**)&B.codes, (void**)&B.keys_a, (void**)&B.keys_b, (void**)&B.vals_a,
+                  (void**)&B.vals_b, (void**)&B.status, (void**)&B.misc, (void**)&B.parent,
+                  (void**)&B.flags, (void**)&B.slot_h, (void**)&B.nodes, (void**)&B.tris};
+  for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); ++i) {
+    *dst[i] = q;
+    q += align256(sz[i]);
+  }
+  b->base = p;
+  b->bytes = total;
+  return RQ_OK;
+}
+
+int check_ctx(rq_ctx* c) { return c ? RQ_OK : RQ_E_INVALID; }
+
+}  // namespace
+
+extern "C" {
+
+const char* rq_version(void) { return "rq 0.1 (gfx950)"; }
+
+int rq_create(rq_ctx** out, int device, size_t budget) {
+  if (!out) return RQ_E_INVALID;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RQ_E_NODEVICE;
+  if (device < 0 || device >= ndev) return RQ_E_INVALID;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return RQ_E_NODEVICE;
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return RQ_E_NODEVICE;
+  if (hipSetDevice(device) != hipSuccess) return RQ_E_NODEVICE;
+  auto* c = new rq_ctx();
+  c->device = device;
+  c->budget = budget ? budget : (size_t)prop.totalGlobalMem;
+  c->prof = new Profiler();
+  c->trace_grid = trace_grid_blocks(device);
+  // counter slots + one trailing word of traversal error bits
+  const size_t cbytes = (COUNTER_SLOTS * 2 + 1) * sizeof(uint32_t);
+  int rc = ctx_alloc(c, cbytes, (void**)&c->counters);
+  if (rc == RQ_OK) {
+    c->errword = c->counters + COUNTER_SLOTS * 2;
+    hipMemset(c->counters, 0, cbytes);
+    c->gdepth = 64 - TRACE_LDS_DEPTH;  // LBVH height <= 63 (prefix lengths 2..63)
+    size_t gbytes = (size_t)c->trace_grid * TRACE_BLOCK * c->gdepth * sizeof(int32_t);
+    rc = ctx_alloc(c, gbytes, (void**)&c->gstack);
+  }
+  if (rc == RQ_OK && hipDeviceSynchronize() != hipSuccess) rc = RQ_E_HIP;
+  if (rc) {
+    rq_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return RQ_OK;
+}
+
+int rq_destroy(rq_ctx* c) {
+  if (!c) return RQ_E_INVALID;
+  hipSetDevice(c->device);
+  hipDeviceSynchronize();
+  if (c->prof) c->prof->resolve();
+  for (auto& kv : c->sizes) hipFree(kv.first);
+  c->sizes.clear();
+  delete c->prof;
+  delete c;
+  return RQ_OK;
+}
+
+const char* rq_last_error(const rq_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+size_t rq_bytes_in_use(const rq_ctx* c) { return c ? c->used : 0; }
+
+int rq_scene_upload(rq_ctx* c, const float* vx, const float* vy, const float* vz, uint32_t nv,
+                    const uint32_t* idx, uint32_t ntri, int prim_kind, rq_scene** out,
+                    void* stream) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (!out) return set_err(c, RQ_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (prim_kind != RQ_PRIM_TRIANGLE)
+    return set_err(c, RQ_E_UNSUPPORTED, "primitive kind %d not supported", prim_kind);
+  if (ntri > MAX_PRIMS) return set_err(c, RQ_E_INVALID, "too many triangles (%u)", ntri);
+  if ((nv && (!vx || !vy || !vz)) || (ntri && !idx))
+    return set_err(c, RQ_E_INVALID, "null vertex or index array");
+  for (uint64_t k = 0; k < 3ull * ntri; ++k)
+    if (idx[k] >= nv) return set_err(c, RQ_E_INVALID, "index %u out of range (nv=%u)", idx[k], nv);
+  hipSetDevice(c->device);
+  auto* s = new rq_scene();
+  s->magic = SCENE_MAGIC;
+  s->nv = nv;
+  s->ntri = ntri;
+  size_t vb = align256((size_t)nv * 4), ib = align256((size_t)ntri * 12);
+  void* p;
+  int rc = ctx_alloc(c, 3 * vb + ib, &p);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->base = p;
+  s->bytes = 3 * vb + ib;
+  s->vx = (float*)p;
+  s->vy = (float*)((char*)p + vb);
+  s->vz = (float*)((char*)p + 2 * vb);
+  s->idx = (uint32_t*)((char*)p + 3 * vb);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (nv) {
+    e = hipMemcpyAsync(s->vx, vx, (size_t)nv * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->vy, vy, (size_t)nv * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->vz, vz, (size_t)nv * 4, hipMemcpyHostToDevice, st);
+  }
+  if (e == hipSuccess && ntri)
+    e = hipMemcpyAsync(s->idx, idx, (size_t)ntri * 12, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // host arrays may go away after return
+  if (e != hipSuccess) {
+    ctx_free(c, p);
+    delete s;
+    return hip_err(c, e, "scene upload");
+  }
+  *out = s;
+  return RQ_OK;
+}
+
+int rq_scene_update_vertices(rq_ctx* c, rq_scene* s, const float* vx, const float* vy,
+                             const float* vz, void* stream) {
+  if (check_ctx(c) || !s || s->magic != SCENE_MAGIC) return set_err(c, RQ_E_INVALID, "bad scene");
+  hipStream_t st = (hipStream_t)stream;
+  size_t b = (size_t)s->nv * 4;
+  hipError_t e = hipMemcpyAsync(s->vx, vx, b, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->vy, vy, b, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(s->vz, vz, b, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "vertex update");
+}
+
+int rq_build_lbvh(rq_ctx* c, const rq_scene* s, int mode, rq_bvh** out, void* stream) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (!out) return set_err(c, RQ_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (!s || s->magic != SCENE_MAGIC) return set_err(c, RQ_E_INVALID, "bad scene handle");
+  if (mode != RQ_MORTON_REF_AFFINE && mode != RQ_MORTON_SCENE_AABB)
+    return set_err(c, RQ_E_INVALID, "unknown morton mode %d", mode);
+  hipSetDevice(c->device);
+  auto* b = new rq_bvh();
+  b->magic = BVH_MAGIC;
+  b->scene = s;
+  b->mode = mode;
+  b->n = s->ntri;
+  int rc = bvh_alloc(c, b, b->n);
+  if (rc) {
+    delete b;
+    return rc;
+  }
+  rc = rq_rebuild_lbvh(c, b, stream);
+  if (rc) {
+    ctx_free(c, b->base);
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return RQ_OK;
+}
+
+int rq_rebuild_lbvh(rq_ctx* c, rq_bvh* b, void* stream) {
+  if (check_ctx(c) || !b || b->magic != BVH_MAGIC) return set_err(c, RQ_E_INVALID, "bad bvh");
+  const rq_scene* s = b->scene;
+  hipError_t e = launch_build(launch_ctx(c, stream), s->vx, s->vy, s->vz, s->idx, b->n, b->mode,
+                              b->B);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "lbvh build launch");
+}
+
+int rq_bvh_info_get(rq_ctx* c, const rq_bvh* b, rq_bvh_info* info) {
+  if (check_ctx(c) || !b || b->magic != BVH_MAGIC || !info)
+    return set_err(c, RQ_E_INVALID, "bad argument");
+  std::memset(info, 0, sizeof *info);
+  info->n_prims = b->n;
+  info->n_internal = b->n > 1 ? b->n - 1 : 0;
+  info->morton_mode = b->mode;
+  info->device_bytes = b->bytes;
+  if (b->n == 0) return RQ_OK;
+  uint32_t misc[MISC_WORDS];
+  hipError_t e = hipMemcpy(misc, b->B.misc, sizeof misc, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(c, e, "bvh info");
+  std::memcpy(info->root_lo, misc + MISC_ROOT, 12);
+  std::memcpy(info->root_hi, misc + MISC_ROOT + 3, 12);
+  info->height = misc[MISC_HEIGHT];
+  if (misc[MISC_ERR]) return set_err(c, RQ_E_HIP, "build reported error bits 0x%x", misc[MISC_ERR]);
+  return RQ_OK;
+}
+
+int rq_bvh_export(rq_ctx* c, const rq_bvh* b, uint32_t* codes, uint32_t* sorted_idx,
+                  int32_t* left, int32_t* right, float* boxes) {
+  if (check_ctx(c) || !b || b->magic != BVH_MAGIC) return set_err(c, RQ_E_INVALID, "bad bvh");
+  const uint32_t n = b->n;
+  if (n == 0) return RQ_OK;
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_err(c, e, "export sync");
+  if (codes) e = hipMemcpy(codes, b->B.codes, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && sorted_idx)
+    e = hipMemcpy(sorted_idx, b->B.vals_a, (size_t)n * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(c, e, "export copy");
+  if (!left && !right && !boxes) return RQ_OK;
+  const uint32_t ni = n - 1;
+  std::vector<NodeRec> nodes(ni);
+  std::vector<int32_t> parent(2 * (size_t)n - 1);
+  std::vector<float> misc(MISC_WORDS);
+  if (ni) e = hipMemcpy(nodes.data(), b->B.nodes, (size_t)ni * 64, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && ni)
+    e = hipMemcpy(parent.data(), b->B.parent, parent.size() * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(misc.data(), b->B.misc, MISC_WORDS * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(c, e, "export copy");
+  uint32_t ebits;
+  std::memcpy(&ebits, misc.data() + MISC_ERR, 4);
+  if (ebits) return set_err(c, RQ_E_HIP, "build reported error bits 0x%x", ebits);
+  for (uint32_t i = 0; i < ni; ++i) {
+    int32_t l, r;
+    std::memcpy(&l, &nodes[i].lo0.w, 4);
+    std::memcpy(&r, &nodes[i].hi0.w, 4);
+    if (left) left[i] = l;
+    if (right) right[i] = r;
+  }
+  if (boxes) {
+    auto slot = [&](uint32_t node, uint32_t side, float* out6) {
+      const NodeRec& q = nodes[node];
+      const float4& lo = side ? q.lo1 : q.lo0;
+      const float4& hi = side ? q.hi1 : q.hi0;
+      out6[0] = lo.x, out6[1] = lo.y, out6[2] = lo.z, out6[3] = hi.x, out6[4] = hi.y,
+      out6[5] = hi.z;
+    };
+    // every node except the root has its box in its parent's child slot
+    for (uint32_t u = 0; u < 2 * n - 1; ++u) {
+      float* out6 = boxes + 6 * (size_t)u;
+      if (u == 0 && n > 1) {
+        std::memcpy(out6, misc.data() + MISC_ROOT, 24);
+      } else if (n == 1) {
+        std::memcpy(out6, misc.data() + MISC_ROOT, 24);
+      } else {
+        int32_t pe = parent[u];
+        slot((uint32_t)pe >> 1, (uint32_t)pe & 1u, out6);
+      }
+    }
+  }
+  return RQ_OK;
+}
+
+static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
+                 uint8_t* occ, uint32_t* visits, uint32_t* tests, bool any, bool counted,
+                 void* stream) {
+  if (check_ctx(c) || !b || b->magic != BVH_MAGIC || !rays)
+    return set_err(c, RQ_E_INVALID, "bad argument");
+  if (n == 0) return RQ_OK;
+  if (!rays->ox || !rays->oy || !rays->oz || !rays->dx || !rays->dy || !rays->dz ||
+      !rays->tmin || !rays->tmax)
+    return set_err(c, RQ_E_INVALID, "null ray array");
+  if (any ? !occ : (!hits || !hits->prim)) return set_err(c, RQ_E_INVALID, "null output array");
+  if (counted && (!visits || !tests)) return set_err(c, RQ_E_INVALID, "null count array");
+  TraceArgs a{};
+  a.nodes = b->B.nodes;
+  a.tris = b->B.tris;
+  a.nprims = b->n;
+  a.rays = *rays;
+  a.nrays = n;
+  if (!any) a.prim = hits->prim, a.t = hits->t, a.u = hits->u, a.v = hits->v;
+  a.occ = occ;
+  a.visits = visits;
+  a.tests = tests;
+  a.gstack = c->gstack;
+  a.gdepth = c->gdepth;
+  a.counters = next_counter_slot(c);
+  a.err = c->errword;
+  a.grid = c->trace_grid;
+  hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "trace launch");
+}
+
+int rq_closest_hit(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
+                   void* stream) {
+  return query(c, b, rays, n, hits, nullptr, nullptr, nullptr, false, false, stream);
+}
+int rq_any_hit(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, uint8_t* occ,
+               void* stream) {
+  return query(c, b, rays, n, nullptr, occ, nullptr, nullptr, true, false, stream);
+}
+int rq_closest_hit_counted(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                           rq_hits* hits, uint32_t* visits, uint32_t* tests, void* stream) {
+  return query(c, b, rays, n, hits, nullptr, visits, tests, false, true, stream);
+}
+int rq_any_hit_counted(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, uint8_t* occ,
+                       uint32_t* visits, uint32_t* tests, void* stream) {
+  return query(c, b, rays, n, nullptr, occ, visits, tests, true, true, stream);
+}
+
+int rq_device_alloc(rq_ctx* c, size_t bytes, void** p) {
+  if (check_ctx(c) || !p) return RQ_E_INVALID;
+  hipSetDevice(c->device);
+  return ctx_alloc(c, bytes, p);
+}
+int rq_device_free(rq_ctx* c, void* p) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (p && !c->sizes.count(p)) return set_err(c, RQ_E_INVALID, "pointer not owned by context");
+  ctx_free(c, p);
+  return RQ_OK;
+}
+int rq_memcpy_h2d(rq_ctx* c, void* dst, const void* src, size_t bytes, void* stream) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "memcpy h2d");
+}
+int rq_memcpy_d2h(rq_ctx* c, void* dst, const void* src, size_t bytes, void* stream) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
+  if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "memcpy d2h");
+}
+int rq_memset_d(rq_ctx* c, void* dst, int value, size_t bytes, void* stream) {
+  hipError_t e = hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "memset");
+}
+int rq_sync(rq_ctx* c, void* stream) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream sync");
+}
+int rq_stream_create(rq_ctx* c, void** stream) {
+  if (check_ctx(c) || !stream) return RQ_E_INVALID;
+  hipSetDevice(c->device);
+  hipStream_t s;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_err(c, e, "stream create");
+  *stream = (void*)s;
+  return RQ_OK;
+}
+int rq_stream_destroy(rq_ctx* c, void* stream) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  hipError_t e = hipStreamDestroy((hipStream_t)stream);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream destroy");
+}
+
+int rq_device_errors(rq_ctx* c, uint32_t* bits) {
+  if (check_ctx(c) || !bits) return RQ_E_INVALID;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(bits, c->errword, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && *bits) e = hipMemset(c->errword, 0, 4);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "device error word");
+}
+
+int rq_profile_enable(rq_ctx* c, int on) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  c->prof_on = on != 0;
+  return RQ_OK;
+}
+int rq_profile_reset(rq_ctx* c) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  c->prof->resolve();
+  c->prof->totals.clear();
+  return RQ_OK;
+}
+int rq_profile_read(rq_ctx* c, char names[][48], double* total_ms, uint64_t* launches, int cap) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  c->prof->resolve();
+  int k = 0;
+  for (auto& kv : c->prof->totals) {
+    if (k < cap) {
+      std::snprintf(names[k], 48, "%s", kv.first.c_str());
+      total_ms[k] = kv.second.first;
+      launches[k] = kv.second.second;
+    }
+    ++k;
+  }
+  return k;
+}
+
+int rq_release(rq_ctx* c, void* h) {
+  if (check_ctx(c) || !h) return RQ_E_INVALID;
+  uint32_t magic;
+  std::memcpy(&magic, h, 4);
+  if (magic == SCENE_MAGIC) {
+    auto* s = (rq_scene*)h;
+    s->magic = 0;
+    ctx_free(c, s->base);
+    delete s;
+    return RQ_OK;
+  }
+  if (magic == BVH_MAGIC) {
+    auto* b = (rq_bvh*)h;
+    b->magic = 0;
+    ctx_free(c, b->base);
+    delete b;
+    return RQ_OK;
+  }
+  return set_err(c, RQ_E_INVALID, "unknown handle");
+}
+
+}  // extern "C"

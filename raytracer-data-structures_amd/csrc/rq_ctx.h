@@ -1,0 +1,50 @@
+// Host-side handle types behind the opaque ABI pointers.
+#pragma once
+#include <string>
+#include <unordered_map>
+
+#include "rq_internal.h"
+
+struct rq_ctx {
+  int device = 0;
+  size_t budget = 0;
+  size_t used = 0;
+  std::string err;
+  std::unordered_map<void*, size_t> sizes;  // live device allocations
+  rq::Profiler* prof = nullptr;
+  bool prof_on = false;
+  uint32_t trace_grid = 0;
+  uint32_t* counters = nullptr;  // COUNTER_SLOTS x {fetch, exit}
+  uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
+  uint32_t counter_seq = 0;
+  int32_t* gstack = nullptr;  // traversal stack overflow
+  uint32_t gdepth = 0;
+};
+
+struct rq_scene {
+  uint32_t magic = 0;  // first member: handle type tag
+  uint32_t nv = 0, ntri = 0;
+  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
+  uint32_t* idx = nullptr;
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
+struct rq_bvh {
+  uint32_t magic = 0;  // first member: handle type tag
+  const rq_scene* scene = nullptr;
+  int mode = 0;
+  uint32_t n = 0;
+  rq::BuildBuffers B;
+  void* base = nullptr;
+  size_t bytes = 0;
+};
+
+namespace rq {
+int set_err(rq_ctx* c, int code, const char* fmt, ...);
+int hip_err(rq_ctx* c, hipError_t e, const char* what);
+int ctx_alloc(rq_ctx* c, size_t bytes, void** p);
+void ctx_free(rq_ctx* c, void* p);
+LaunchCtx launch_ctx(rq_ctx* c, void* stream);
+uint32_t* next_counter_slot(rq_ctx* c);
+}  // namespace rq

@@ -1,0 +1,137 @@
+// Internal definitions shared by the HIP kernels and the host side of librq.
+// Not part of the public ABI (include/rq/*.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "rq/rq.h"
+
+namespace rq {
+
+// ---------------------------------------------------------------------------
+// Device data layout
+// ---------------------------------------------------------------------------
+// Internal node: the boxes of BOTH children plus the two child links, so one
+// visit is one 64-byte record (four 16-byte loads). Child link >= 0 is an
+// internal node index, < 0 is leaf ~k (k = sorted position).
+struct alignas(16) NodeRec {
+  float4 lo0;  // lo of child 0, .w = left link (int bits)
+  float4 hi0;  // hi of child 0, .w = right link (int bits)
+  float4 lo1;  // lo of child 1, .w unused
+  float4 hi1;  // hi of child 1, .w unused
+};
+static_assert(sizeof(NodeRec) == 64, "node record is 64 bytes");
+
+// Leaf triangle in sorted order: v0, e1 = v1 - v0, e2 = v2 - v0 (48 bytes).
+struct alignas(16) TriRec {
+  float4 v0;  // .w = primitive id (int bits)
+  float4 e1;
+  float4 e2;
+};
+static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
+
+// Radix sort tiling (onesweep): WAVES*64 threads, ITEMS keys per thread.
+constexpr int SORT_WAVES = 4;
+constexpr int SORT_ITEMS = 16;
+constexpr int SORT_BLOCK = SORT_WAVES * 64;
+constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int SORT_PASSES = 4;  // 30-bit codes, 8-bit digits
+
+// Lookback status word: [31:30] flag, [29:0] count.
+constexpr uint32_t LB_AGG = 1u << 30;
+constexpr uint32_t LB_INC = 2u << 30;
+constexpr uint32_t LB_MASK = (1u << 30) - 1;
+constexpr uint32_t MAX_PRIMS = (1u << 30) - 1;
+
+// Traversal: per-lane LDS stack depth; deeper entries go to a global overflow
+// area laid out [depth][thread] for coalescing.
+constexpr int TRACE_BLOCK = 128;
+constexpr int TRACE_LDS_DEPTH = 24;
+constexpr int COUNTER_SLOTS = 64;
+
+// Build workspace (device), all sized for n primitives.
+struct BuildBuffers {
+  uint32_t* codes = nullptr;      // n, primitive order
+  uint32_t* keys_a = nullptr;     // n
+  uint32_t* keys_b = nullptr;     // n
+  uint32_t* vals_a = nullptr;     // n  (final sorted_idx)
+  uint32_t* vals_b = nullptr;     // n
+  uint32_t* status = nullptr;     // SORT_PASSES * ntiles * 256
+  uint32_t* misc = nullptr;       // see MiscLayout
+  int32_t* parent = nullptr;      // 2n-1: (node << 1) | side
+  uint32_t* flags = nullptr;      // n-1 refit arrival counters
+  uint32_t* slot_h = nullptr;     // 2(n-1) child heights per node side
+  NodeRec* nodes = nullptr;       // n-1
+  TriRec* tris = nullptr;         // n
+  uint32_t ntiles = 0;
+};
+
+// misc[] layout (uint32 words)
+constexpr int MISC_HIST = 0;                          // SORT_PASSES * 256
+constexpr int MISC_TILECTR = MISC_HIST + SORT_PASSES * 256;  // SORT_PASSES
+constexpr int MISC_BOUNDS = MISC_TILECTR + SORT_PASSES;      // 6 ordered-uint (lo xyz, hi xyz)
+constexpr int MISC_ROOT = MISC_BOUNDS + 6;                   // 6 floats root box
+constexpr int MISC_HEIGHT = MISC_ROOT + 6;                   // 1
+constexpr int MISC_ERR = MISC_HEIGHT + 1;                    // 1: build error bits
+constexpr int MISC_WORDS = MISC_ERR + 1;
+
+// Build error bits (never expected; they turn a logic error into a reported
+// failure instead of an out-of-range access or an endless wait).
+constexpr uint32_t ERR_LOOKBACK_SPIN = 1u;
+constexpr uint32_t ERR_SCATTER_RANGE = 2u;
+constexpr uint32_t ERR_TOPOLOGY = 4u;
+constexpr uint32_t ERR_REFIT_DEPTH = 8u;
+constexpr uint32_t LOOKBACK_SPIN_LIMIT = 1u << 24;
+
+// ---------------------------------------------------------------------------
+// Host-side launchers (defined in the .hip files)
+// ---------------------------------------------------------------------------
+struct Profiler;
+struct LaunchCtx {
+  hipStream_t stream;
+  Profiler* prof;  // may be null
+};
+
+hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, const float* vz,
+                        const uint32_t* idx, uint32_t n, int morton_mode, BuildBuffers& B);
+
+struct TraceArgs {
+  const NodeRec* nodes;
+  const TriRec* tris;
+  uint32_t nprims;
+  rq_rays rays;
+  uint32_t nrays;
+  int32_t* prim;
+  float *t, *u, *v;
+  uint8_t* occ;
+  uint32_t* visits;
+  uint32_t* tests;
+  int32_t* gstack;
+  uint32_t gdepth;      // overflow entries per thread
+  uint32_t* counters;   // 2 words: fetch, exit (reset by the last wave)
+  uint32_t* err;        // set to 1 if a ray needed a deeper stack than provided
+  uint32_t grid;        // blocks
+};
+hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted);
+uint32_t trace_grid_blocks(int device);
+
+// Profiler hooks (rq_api.cpp)
+void prof_begin(Profiler* p, hipStream_t s, const char* name, hipEvent_t* ev);
+void prof_end(Profiler* p, hipStream_t s, const char* name, hipEvent_t ev);
+
+struct ProfScope {
+  Profiler* p;
+  hipStream_t s;
+  const char* name;
+  hipEvent_t ev = nullptr;
+  ProfScope(const LaunchCtx& lc, const char* n) : p(lc.prof), s(lc.stream), name(n) {
+    if (p) prof_begin(p, s, name, &ev);
+  }
+  ~ProfScope() {
+    if (p) prof_end(p, s, name, ev);
+  }
+};
+
+}  // namespace rq

@@ -1,0 +1,43 @@
+// Internal types of the wavefront renderer (kernels <-> host orchestration).
+#pragma once
+#include "rq_internal.h"
+
+namespace rq {
+
+struct CamParams {
+  uint32_t w, h, spp;
+  float invW, invH, aspect, angle;  // as main.cpp:544-546 (angle = tan(pi*0.5*30/180) in float)
+  uint64_t seed;
+};
+
+struct ShadeParams {
+  float light[3], emission[3], surface[3], sky[3];
+  float bias;
+  int32_t bounce;
+};
+
+struct SceneView {
+  const float *vx, *vy, *vz;
+  const uint32_t* idx;
+};
+
+struct RaySoA {
+  float *ox, *oy, *oz, *dx, *dy, *dz, *tmin, *tmax;
+};
+
+hipError_t launch_raygen_camera(const LaunchCtx& lc, const CamParams& cp, const double* jit,
+                                const RaySoA& R, uint32_t* counters);
+hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed, const float lo[3],
+                                 const float hi[3], const RaySoA& R);
+hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
+                                uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
+                                const RaySoA& shad, const RaySoA& bnc, uint32_t* counters);
+hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
+                               uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
+                               const RaySoA& shad, uint32_t* counters);
+hipError_t launch_shade_final(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
+                              uint32_t npix, uint32_t spp, const RaySoA& prim, const RaySoA& bnc,
+                              const int32_t* h0, const float* t0, const int32_t* h1,
+                              const float* t1, const uint8_t* occ, float* srgb, float* img);
+
+}  // namespace rq

@@ -1,0 +1,242 @@
+// Closest-hit and any-hit traversal of the LBVH on gfx950.
+//
+// Replaces the per-ray recursion of boxIntersect (accelerators.h:668-690)
+// and the candidate loop of castRay (main.cpp:343-358): instead of collecting
+// every candidate and then testing them, one lane owns one ray and walks the
+// tree with a stack, nearer child first, culling boxes against the best hit
+// so far. The result is the (t, primitive id) lexicographic minimum, i.e. the
+// NONE loop of main.cpp:373-387 (lowest index wins ties).
+//
+// Kernel shape: persistent waves pull rays from a global counter 64 at a
+// time; when enough lanes have finished (wave ballot), the wave refills the
+// idle lanes from the counter (ray compaction), so long rays do not hold a
+// wave of finished lanes. The per-lane stack lives in LDS ([depth][lane],
+// conflict-free) and spills to a global [depth][thread] area past
+// TRACE_LDS_DEPTH entries.
+#include <algorithm>
+
+#include "rq_internal.h"
+
+namespace rq {
+namespace {
+
+constexpr int REFILL_MIN_IDLE = 16;  // lanes idle before a wave refills
+
+struct RayState {
+  float ox, oy, oz, dx, dy, dz;
+  float ix, iy, iz;  // 1/d
+  bool sx, sy, sz;   // sign bit of 1/d: the near plane of that axis is hi
+  float tmin, tmax;
+};
+
+// Relative slack on the far bound: a triangle's rounded Möller–Trumbore t can
+// fall a few ulps outside the exact box of its vertices (hits on edges and
+// vertices); culling only beyond far * (1 + 2^-15) keeps every box that holds
+// the brute-force closest hit.
+constexpr float SLAB_SLACK = 1.000030517578125f;
+
+// Slab test (same operations and order as the oracle): near/far planes by
+// the sign of 1/d, t = (plane - o) * (1/d). A zero direction component gives
+// +-inf, or NaN with the origin on the plane, which the NaN-ignoring max
+// (near) / min (far) read as "inside". Hit when near <= far * SLAB_SLACK.
+__device__ inline bool slab(const float4& lo, const float4& hi, const RayState& r, float thi,
+                            float& tn) {
+  const float nx = ((r.sx ? hi.x : lo.x) - r.ox) * r.ix, fx = ((r.sx ? lo.x : hi.x) - r.ox) * r.ix;
+  const float ny = ((r.sy ? hi.y : lo.y) - r.oy) * r.iy, fy = ((r.sy ? lo.y : hi.y) - r.oy) * r.iy;
+  const float nz = ((r.sz ? hi.z : lo.z) - r.oz) * r.iz, fz = ((r.sz ? lo.z : hi.z) - r.oz) * r.iz;
+  tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
+  const float tf = fminf(fminf(fx, fy), fminf(fz, thi));
+  return tn <= tf * SLAB_SLACK;
+}
+
+// Möller–Trumbore, main.cpp:137-162 (v_0 read as v0), t restricted to
+// (tmin, tmax). Dot products (x*x' + y*y') + z*z', no contraction.
+__device__ inline bool mt(const TriRec& T, const RayState& r, float& t, float& u, float& v) {
+  const float px = r.dy * T.e2.z - r.dz * T.e2.y;
+  const float py = r.dz * T.e2.x - r.dx * T.e2.z;
+  const float pz = r.dx * T.e2.y - r.dy * T.e2.x;
+  const float det = (T.e1.x * px + T.e1.y * py) + T.e1.z * pz;
+  if (fabsf(det) < 1e-6f) return false;
+  const float inv = 1.0f / det;
+  const float tx = r.ox - T.v0.x, ty = r.oy - T.v0.y, tz = r.oz - T.v0.z;
+  const float uu = ((tx * px + ty * py) + tz * pz) * inv;
+  if (uu < 0.0f || uu > 1.0f) return false;
+  const float qx = ty * T.e1.z - tz * T.e1.y;
+  const float qy = tz * T.e1.x - tx * T.e1.z;
+  const float qz = tx * T.e1.y - ty * T.e1.x;
+  const float vv = ((r.dx * qx + r.dy * qy) + r.dz * qz) * inv;
+  if (vv < 0.0f || uu + vv > 1.0f) return false;
+  const float tt = ((T.e2.x * qx + T.e2.y * qy) + T.e2.z * qz) * inv;
+  if (!(tt > r.tmin && tt < r.tmax)) return false;
+  t = tt, u = uu, v = vv;
+  return true;
+}
+
+template <bool ANY, bool COUNT>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(TraceArgs a) {
+  __shared__ int32_t s_stack[TRACE_LDS_DEPTH * TRACE_BLOCK];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const uint32_t gtid = blockIdx.x * TRACE_BLOCK + tid;
+  const uint32_t gstride = gridDim.x * TRACE_BLOCK;
+  const uint64_t lt = (1ull << lane) - 1ull;
+
+  RayState r;
+  bool active = false, exhausted = false;
+  uint32_t ray = 0;
+  int32_t node = 0, best = -1;
+  int sp = 0;
+  float bt = 0.0f, bu = 0.0f, bv = 0.0f;
+  uint32_t vis = 0, tst = 0;
+
+  auto finish = [&]() {
+    if (ANY) {
+      a.occ[ray] = best >= 0 ? 1 : 0;
+    } else {
+      a.prim[ray] = best;
+      if (a.t) a.t[ray] = best >= 0 ? bt : INFINITY;
+      if (a.u) a.u[ray] = bu;
+      if (a.v) a.v[ray] = bv;
+    }
+    if (COUNT) {
+      a.visits[ray] = vis;
+      a.tests[ray] = tst;
+    }
+    active = false;
+  };
+  const int max_sp = TRACE_LDS_DEPTH + (int)a.gdepth;
+  auto push = [&](int32_t c) {
+    if (sp < TRACE_LDS_DEPTH)
+      s_stack[sp * TRACE_BLOCK + tid] = c;
+    else if (sp < max_sp)
+      a.gstack[(size_t)(sp - TRACE_LDS_DEPTH) * gstride + gtid] = c;
+    else
+      atomicOr(a.err, 1u);  // deeper than any LBVH of 30-bit codes: reported
+    sp = min(sp + 1, max_sp);
+  };
+  auto pop = [&]() -> bool {
+    if (sp == 0) return false;
+    --sp;
+    node = sp < TRACE_LDS_DEPTH ? s_stack[sp * TRACE_BLOCK + tid]
+                                : a.gstack[(size_t)(sp - TRACE_LDS_DEPTH) * gstride + gtid];
+    return true;
+  };
+
+  for (;;) {
+    // ---- refill idle lanes from the global ray counter -------------------
+    const bool need = !active && !exhausted;
+    const uint64_t m = __ballot(need);
+    if (m) {
+      const int first = __ffsll((unsigned long long)m) - 1;
+      uint32_t base = 0;
+      if (lane == first) base = atomicAdd(&a.counters[0], (uint32_t)__popcll(m));
+      base = __shfl(base, first, 64);
+      if (need) {
+        const uint32_t rr = base + (uint32_t)__popcll(m & lt);
+        if (rr >= a.nrays) {
+          exhausted = true;
+        } else {
+          ray = rr;
+          r.ox = a.rays.ox[rr], r.oy = a.rays.oy[rr], r.oz = a.rays.oz[rr];
+          r.dx = a.rays.dx[rr], r.dy = a.rays.dy[rr], r.dz = a.rays.dz[rr];
+          r.tmin = a.rays.tmin[rr], r.tmax = a.rays.tmax[rr];
+          r.ix = 1.0f / r.dx, r.iy = 1.0f / r.dy, r.iz = 1.0f / r.dz;
+          r.sx = signbit(r.ix), r.sy = signbit(r.iy), r.sz = signbit(r.iz);
+          best = -1, bt = r.tmax, bu = 0.0f, bv = 0.0f, vis = 0, tst = 0, sp = 0;
+          node = a.nprims == 1 ? ~0 : 0;
+          active = true;
+          if (!(r.tmin < r.tmax) || a.nprims == 0) finish();
+        }
+      }
+    }
+    if (__ballot(active) == 0ull && __ballot(!exhausted) == 0ull) break;
+
+    // ---- traverse until enough lanes are idle to be worth a refill -------
+    for (;;) {
+      if (active) {
+        if (node >= 0) {
+          const float4* q = reinterpret_cast<const float4*>(a.nodes + node);
+          const float4 lo0 = q[0], hi0 = q[1], lo1 = q[2], hi1 = q[3];
+          if (COUNT) vis += 2;
+          float tn0, tn1;
+          const bool h0 = slab(lo0, hi0, r, bt, tn0);
+          const bool h1 = slab(lo1, hi1, r, bt, tn1);
+          const int32_t c0 = __float_as_int(lo0.w), c1 = __float_as_int(hi0.w);
+          if (h0 && h1) {
+            const bool swap = tn1 < tn0;
+            push(swap ? c0 : c1);
+            node = swap ? c1 : c0;
+          } else if (h0) {
+            node = c0;
+          } else if (h1) {
+            node = c1;
+          } else if (!pop()) {
+            finish();
+          }
+        } else {
+          const TriRec T = a.tris[~node];
+          if (COUNT) ++tst;
+          float t, u, v;
+          bool done = false;
+          if (mt(T, r, t, u, v)) {
+            const int32_t prim = __float_as_int(T.v0.w);
+            if (ANY) {
+              best = prim, bt = t;
+              done = true;
+            } else if (best < 0 || t < bt || (t == bt && prim < best)) {
+              best = prim, bt = t, bu = u, bv = v;
+            }
+          }
+          if (done || !pop()) finish();
+        }
+      }
+      const uint64_t idle = __ballot(!active && !exhausted);
+      if (__popcll(idle) >= REFILL_MIN_IDLE || __ballot(active) == 0ull) break;
+    }
+  }
+
+  // last wave out resets the counters for the next launch on this slot
+  if (lane == 0) {
+    const uint32_t waves = gridDim.x * (TRACE_BLOCK / 64);
+    const uint32_t old = atomicAdd(&a.counters[1], 1u);
+    if (old == waves - 1) {
+      atomicExch(&a.counters[0], 0u);
+      atomicExch(&a.counters[1], 0u);
+    }
+  }
+}
+
+}  // namespace
+
+uint32_t trace_grid_blocks(int device) {
+  int cus = 0, per = 0;
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<false, false>, TRACE_BLOCK, 0);
+  if (cus <= 0) cus = 256;
+  if (per <= 0) per = 4;
+  return (uint32_t)(cus * per);
+}
+
+hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted) {
+  if (a.nrays == 0) return hipSuccess;
+  uint32_t need = (uint32_t)((a.nrays + TRACE_BLOCK - 1) / TRACE_BLOCK);
+  uint32_t grid = std::max(1u, std::min(a.grid, need));
+  TraceArgs b = a;
+  dim3 g(grid), blk(TRACE_BLOCK);
+  hipStream_t s = lc.stream;
+  if (any) {
+    ProfScope ps(lc, counted ? "trace_any_counted" : "trace_any");
+    if (counted)
+      hipLaunchKernelGGL((k_trace<true, true>), g, blk, 0, s, b);
+    else
+      hipLaunchKernelGGL((k_trace<true, false>), g, blk, 0, s, b);
+  } else {
+    ProfScope ps(lc, counted ? "trace_closest_counted" : "trace_closest");
+    if (counted)
+      hipLaunchKernelGGL((k_trace<false, true>), g, blk, 0, s, b);
+    else
+      hipLaunchKernelGGL((k_trace<false, false>), g, blk, 0, s, b);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace rq

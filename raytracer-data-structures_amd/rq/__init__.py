@@ -1,0 +1,564 @@
+"""Python bindings (ctypes) for librq, the MI355X ray-query engine.
+
+The product is the C ABI in include/rq/*.h implemented by librq.so (HIP
+kernels for gfx950). This module only marshals numpy arrays through that ABI;
+it has no compute path of its own, and it raises if librq.so is missing.
+
+Reference interface mirrored (Alhajras/Raytracer-Data-structures):
+  Settings.dataStructure / AccType  settings.h:7-17, accelerators.h:21
+  constructLBVHTree                 accelerators.h:570-586  -> Context.build_lbvh
+  boxIntersect + candidate loop     accelerators.h:668-690, main.cpp:343-358
+                                                             -> Context.closest_hit
+  kdtreeIntersect (any hit)         accelerators.h:997-1086 -> Context.any_hit
+  render                            main.cpp:541-566         -> Renderer
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent.parent
+LIB_PATH = PKG / "librq.so"
+
+
+class AccType(enum.IntEnum):
+    """settings.h dataStructure values (accelerators.h:21) plus GPU_LBVH."""
+
+    BVH = 0
+    KDTREE = 1
+    UNIFORM_GRID = 2
+    LBVH = 3
+    NONE = 4
+    GPU_LBVH = 5
+
+
+MORTON_REF_AFFINE = 0
+MORTON_SCENE_AABB = 1
+CAMERA_REF_JITTER = 0
+CAMERA_PHILOX = 1
+
+RQ_OK = 0
+ERRORS = {-1: "RQ_E_INVALID", -2: "RQ_E_NOMEM", -3: "RQ_E_HIP", -4: "RQ_E_IO",
+          -5: "RQ_E_NODEVICE", -6: "RQ_E_UNSUPPORTED"}
+
+
+class RQError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class NoMemory(RQError):
+    pass
+
+
+# ---------------------------------------------------------------- structs
+class Rays(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")]
+
+
+class Hits(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("prim", "t", "u", "v")]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("n_prims", C.c_uint32), ("n_internal", C.c_uint32), ("height", C.c_uint32),
+                ("morton_mode", C.c_int32), ("root_lo", C.c_float * 3),
+                ("root_hi", C.c_float * 3), ("device_bytes", C.c_size_t)]
+
+
+class Mesh(C.Structure):
+    _fields_ = [("vx", C.POINTER(C.c_float)), ("vy", C.POINTER(C.c_float)),
+                ("vz", C.POINTER(C.c_float)), ("nv", C.c_uint32),
+                ("idx", C.POINTER(C.c_uint32)), ("ntri", C.c_uint32)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
+                ("camera", C.c_int32), ("seed", C.c_uint64), ("light", C.c_float * 3),
+                ("emission", C.c_float * 3), ("surface", C.c_float * 3),
+                ("sky", C.c_float * 3), ("bias", C.c_float), ("bounce", C.c_int32)]
+
+
+class FrameCounts(C.Structure):
+    _fields_ = [("primary", C.c_uint64), ("bounce", C.c_uint64), ("shadow", C.c_uint64)]
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is not built; run raytracer-data-structures_amd/build.py")
+    lib = C.CDLL(str(LIB_PATH))
+    vp, u32, u64, i32, f32p = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int, C.POINTER(C.c_float)
+    sig = {
+        "rq_create": (i32, [C.POINTER(vp), i32, C.c_size_t]),
+        "rq_destroy": (i32, [vp]),
+        "rq_last_error": (C.c_char_p, [vp]),
+        "rq_version": (C.c_char_p, []),
+        "rq_bytes_in_use": (C.c_size_t, [vp]),
+        "rq_scene_upload": (i32, [vp, vp, vp, vp, u32, vp, u32, i32, C.POINTER(vp), vp]),
+        "rq_scene_update_vertices": (i32, [vp, vp, vp, vp, vp, vp]),
+        "rq_build_lbvh": (i32, [vp, vp, i32, C.POINTER(vp), vp]),
+        "rq_rebuild_lbvh": (i32, [vp, vp, vp]),
+        "rq_bvh_info_get": (i32, [vp, vp, C.POINTER(BvhInfo)]),
+        "rq_bvh_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "rq_closest_hit": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp]),
+        "rq_any_hit": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp]),
+        "rq_closest_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp, vp,
+                                         vp]),
+        "rq_any_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp, vp, vp]),
+        "rq_device_alloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
+        "rq_device_free": (i32, [vp, vp]),
+        "rq_memcpy_h2d": (i32, [vp, vp, vp, C.c_size_t, vp]),
+        "rq_memcpy_d2h": (i32, [vp, vp, vp, C.c_size_t, vp]),
+        "rq_memset_d": (i32, [vp, vp, i32, C.c_size_t, vp]),
+        "rq_sync": (i32, [vp, vp]),
+        "rq_device_errors": (i32, [vp, C.POINTER(u32)]),
+        "rq_stream_create": (i32, [vp, C.POINTER(vp)]),
+        "rq_stream_destroy": (i32, [vp, vp]),
+        "rq_profile_enable": (i32, [vp, i32]),
+        "rq_profile_reset": (i32, [vp]),
+        "rq_profile_read": (i32, [vp, vp, vp, vp, i32]),
+        "rq_release": (i32, [vp, vp]),
+        "rq_gen_icosphere": (i32, [u32, u32, C.c_float, C.c_float, f32p, C.c_float,
+                                   C.POINTER(Mesh)]),
+        "rq_gen_icosphere_field": (i32, [u32, u32, u32, u64, C.c_float, C.c_float, C.c_float,
+                                         C.c_float, C.POINTER(Mesh)]),
+        "rq_gen_soup": (i32, [u32, u64, C.c_float, C.c_float, C.c_float, C.POINTER(Mesh)]),
+        "rq_obj_read": (i32, [C.c_char_p, C.POINTER(Mesh)]),
+        "rq_obj_write": (i32, [C.c_char_p, C.POINTER(Mesh)]),
+        "rq_mesh_bounds": (None, [C.POINTER(Mesh), f32p, f32p]),
+        "rq_mesh_free": (None, [C.POINTER(Mesh)]),
+        "rq_render_defaults": (None, [C.POINTER(RenderParams)]),
+        "rq_renderer_create": (i32, [vp, C.POINTER(RenderParams), C.POINTER(vp)]),
+        "rq_renderer_destroy": (i32, [vp, vp]),
+        "rq_render_frame": (i32, [vp, vp, vp, vp]),
+        "rq_render_counts": (i32, [vp, vp, C.POINTER(FrameCounts)]),
+        "rq_render_read_image": (i32, [vp, vp, vp, vp]),
+        "rq_render_read_samples": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "rq_render_primary_rays": (i32, [vp, vp, C.POINTER(Rays), C.POINTER(u32)]),
+        "rq_raygen_camera": (i32, [vp, C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp, vp, vp,
+                                   vp, vp]),
+        "rq_raygen_uniform": (i32, [vp, u32, u64, f32p, f32p, vp, vp, vp, vp, vp, vp, vp, vp,
+                                    vp]),
+        "rq_write_ppm": (i32, [C.c_char_p, u32, u32, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB: C.CDLL | None = None
+
+
+def lib() -> C.CDLL:
+    global _LIB
+    if _LIB is None:
+        _LIB = _load()
+    return _LIB
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+# ---------------------------------------------------------------- meshes (host)
+@dataclass
+class MeshArrays:
+    vx: np.ndarray
+    vy: np.ndarray
+    vz: np.ndarray
+    idx: np.ndarray  # (ntri*3,) uint32
+
+    @property
+    def nv(self) -> int:
+        return int(self.vx.size)
+
+    @property
+    def ntri(self) -> int:
+        return int(self.idx.size // 3)
+
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        v = np.stack([self.vx, self.vy, self.vz])
+        return v.min(axis=1), v.max(axis=1)
+
+
+def _take_mesh(m: Mesh) -> MeshArrays:
+    L = lib()
+    try:
+        out = MeshArrays(
+            np.ctypeslib.as_array(m.vx, (m.nv,)).copy() if m.nv else np.zeros(0, np.float32),
+            np.ctypeslib.as_array(m.vy, (m.nv,)).copy() if m.nv else np.zeros(0, np.float32),
+            np.ctypeslib.as_array(m.vz, (m.nv,)).copy() if m.nv else np.zeros(0, np.float32),
+            np.ctypeslib.as_array(m.idx, (3 * m.ntri,)).copy() if m.ntri
+            else np.zeros(0, np.uint32))
+    finally:
+        L.rq_mesh_free(C.byref(m))
+    return out
+
+
+def _check_gen(rc: int, what: str) -> None:
+    if rc != RQ_OK:
+        raise RQError(rc, what)
+
+
+def icosphere(level: int, seed: int, amp: float = 0.1, freq: float = 4.0,
+              center=(0.0, 0.0, 0.0), scale: float = 1.0) -> MeshArrays:
+    m = Mesh()
+    c = (C.c_float * 3)(*center)
+    _check_gen(lib().rq_gen_icosphere(level, seed, amp, freq, c, scale, C.byref(m)), "icosphere")
+    return _take_mesh(m)
+
+
+def icosphere_field(count: int, level: int, seed0: int, placement_seed: int, box: float,
+                    rmin: float, amp: float = 0.1, freq: float = 4.0) -> MeshArrays:
+    m = Mesh()
+    _check_gen(lib().rq_gen_icosphere_field(count, level, seed0, placement_seed, box, rmin, amp,
+                                            freq, C.byref(m)), "icosphere_field")
+    return _take_mesh(m)
+
+
+def soup(ntri: int, seed: int, mean_edge=0.02, sd_edge=0.005, min_edge=0.002) -> MeshArrays:
+    m = Mesh()
+    _check_gen(lib().rq_gen_soup(ntri, seed, mean_edge, sd_edge, min_edge, C.byref(m)), "soup")
+    return _take_mesh(m)
+
+
+def obj_read(path: str | Path) -> MeshArrays:
+    m = Mesh()
+    _check_gen(lib().rq_obj_read(str(path).encode(), C.byref(m)), f"obj_read {path}")
+    return _take_mesh(m)
+
+
+def obj_write(path: str | Path, mesh: MeshArrays) -> None:
+    vx, vy, vz = (np.ascontiguousarray(a, np.float32) for a in (mesh.vx, mesh.vy, mesh.vz))
+    idx = np.ascontiguousarray(mesh.idx, np.uint32)
+    m = Mesh(vx.ctypes.data_as(C.POINTER(C.c_float)), vy.ctypes.data_as(C.POINTER(C.c_float)),
+             vz.ctypes.data_as(C.POINTER(C.c_float)), vx.size,
+             idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size // 3)
+    _check_gen(lib().rq_obj_write(str(path).encode(), C.byref(m)), f"obj_write {path}")
+
+
+def render_defaults() -> RenderParams:
+    p = RenderParams()
+    lib().rq_render_defaults(C.byref(p))
+    return p
+
+
+# ---------------------------------------------------------------- device
+RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
+
+
+class DeviceArray:
+    """A device allocation owned by a Context (charged to its HBM budget)."""
+
+    owned = True
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        ctx._check(lib().rq_device_alloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray, stream=None) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        if a.nbytes:
+            self.ctx._check(lib().rq_memcpy_h2d(self.ctx.h, self.ptr, _ptr(a), a.nbytes, stream))
+        return self
+
+    def download(self, dtype, count: int, stream=None) -> np.ndarray:
+        out = np.empty(count, dtype)
+        assert out.nbytes <= self.nbytes
+        if out.nbytes:
+            self.ctx._check(lib().rq_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes,
+                                                stream))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.owned:
+            lib().rq_device_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+
+class DeviceRays:
+    """Device SoA ray batch (8 float32 arrays)."""
+
+    def __init__(self, ctx: "Context", n: int):
+        self.n = int(n)
+        self.arrays = {f: DeviceArray(ctx, 4 * self.n) for f in RAY_FIELDS}
+
+    @classmethod
+    def from_host(cls, ctx: "Context", rays: dict) -> "DeviceRays":
+        n = len(rays["ox"])
+        r = cls(ctx, n)
+        for f in RAY_FIELDS:
+            r.arrays[f].upload(np.asarray(rays[f], np.float32))
+        return r
+
+    def struct(self) -> Rays:
+        return Rays(*(self.arrays[f].ptr for f in RAY_FIELDS))
+
+    def download(self) -> dict:
+        return {f: self.arrays[f].download(np.float32, self.n) for f in RAY_FIELDS}
+
+    def free(self) -> None:
+        for a in self.arrays.values():
+            a.free()
+
+
+class Context:
+    """rq_ctx: one device, an HBM budget, a host thread."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0):
+        h = C.c_void_p()
+        rc = lib().rq_create(C.byref(h), device, hbm_budget)
+        if rc != RQ_OK:
+            raise RQError(rc, f"rq_create(device={device}) failed (no gfx950 device?)")
+        self.h = h.value
+
+    def close(self) -> None:
+        if self.h:
+            lib().rq_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != RQ_OK:
+            msg = lib().rq_last_error(self.h).decode()
+            raise (NoMemory if rc == -2 else RQError)(rc, msg)
+
+    @property
+    def bytes_in_use(self) -> int:
+        return int(lib().rq_bytes_in_use(self.h))
+
+    # scene + build
+    def upload_scene(self, mesh: MeshArrays, stream=None) -> int:
+        vx, vy, vz = (np.ascontiguousarray(a, np.float32) for a in (mesh.vx, mesh.vy, mesh.vz))
+        idx = np.ascontiguousarray(mesh.idx, np.uint32)
+        s = C.c_void_p()
+        self._check(lib().rq_scene_upload(self.h, _ptr(vx), _ptr(vy), _ptr(vz), vx.size,
+                                          _ptr(idx), idx.size // 3, 0, C.byref(s), stream))
+        return s.value
+
+    def build_lbvh(self, scene: int, morton_mode: int = MORTON_SCENE_AABB, stream=None) -> int:
+        b = C.c_void_p()
+        self._check(lib().rq_build_lbvh(self.h, scene, morton_mode, C.byref(b), stream))
+        return b.value
+
+    def rebuild(self, bvh: int, stream=None) -> None:
+        self._check(lib().rq_rebuild_lbvh(self.h, bvh, stream))
+
+    def bvh_info(self, bvh: int) -> BvhInfo:
+        info = BvhInfo()
+        self._check(lib().rq_bvh_info_get(self.h, bvh, C.byref(info)))
+        return info
+
+    def export(self, bvh: int, n: int) -> dict:
+        codes = np.empty(n, np.uint32)
+        sidx = np.empty(n, np.uint32)
+        left = np.empty(max(n - 1, 0), np.int32)
+        right = np.empty(max(n - 1, 0), np.int32)
+        boxes = np.empty(max(2 * n - 1, 0) * 6, np.float32)
+        self._check(lib().rq_bvh_export(self.h, bvh, _ptr(codes), _ptr(sidx), _ptr(left),
+                                        _ptr(right), _ptr(boxes)))
+        return dict(codes=codes, sorted_idx=sidx, left=left, right=right,
+                    boxes=boxes.reshape(-1, 6))
+
+    def release(self, handle: int) -> None:
+        self._check(lib().rq_release(self.h, handle))
+
+    # queries (device buffers)
+    def closest_hit_dev(self, bvh: int, rays: DeviceRays, hits: dict, counted=None,
+                        stream=None) -> None:
+        r = rays.struct()
+        h = Hits(hits["prim"].ptr, hits["t"].ptr if "t" in hits else None,
+                 hits["u"].ptr if "u" in hits else None, hits["v"].ptr if "v" in hits else None)
+        if counted:
+            self._check(lib().rq_closest_hit_counted(self.h, bvh, C.byref(r), rays.n, C.byref(h),
+                                                     counted[0].ptr, counted[1].ptr, stream))
+        else:
+            self._check(lib().rq_closest_hit(self.h, bvh, C.byref(r), rays.n, C.byref(h), stream))
+
+    def any_hit_dev(self, bvh: int, rays: DeviceRays, occ: DeviceArray, counted=None,
+                    stream=None) -> None:
+        r = rays.struct()
+        if counted:
+            self._check(lib().rq_any_hit_counted(self.h, bvh, C.byref(r), rays.n, occ.ptr,
+                                                 counted[0].ptr, counted[1].ptr, stream))
+        else:
+            self._check(lib().rq_any_hit(self.h, bvh, C.byref(r), rays.n, occ.ptr, stream))
+
+    # queries (host round trip, for tests)
+    def closest_hit(self, bvh: int, rays: dict, counted: bool = False) -> dict:
+        n = len(rays["ox"])
+        dr = DeviceRays.from_host(self, rays)
+        hits = {k: DeviceArray(self, 4 * max(n, 1)) for k in ("prim", "t", "u", "v")}
+        cnt = (DeviceArray(self, 4 * max(n, 1)), DeviceArray(self, 4 * max(n, 1))) \
+            if counted else None
+        try:
+            self.closest_hit_dev(None if bvh is None else bvh, dr, hits, cnt)
+            self.sync()
+            out = {"prim": hits["prim"].download(np.int32, n)}
+            for k in ("t", "u", "v"):
+                out[k] = hits[k].download(np.float32, n)
+            if counted:
+                out["visits"] = cnt[0].download(np.uint32, n)
+                out["tests"] = cnt[1].download(np.uint32, n)
+            return out
+        finally:
+            dr.free()
+            for a in hits.values():
+                a.free()
+            if cnt:
+                cnt[0].free(), cnt[1].free()
+
+    def any_hit(self, bvh: int, rays: dict, counted: bool = False) -> dict:
+        n = len(rays["ox"])
+        dr = DeviceRays.from_host(self, rays)
+        occ = DeviceArray(self, max(n, 1))
+        cnt = (DeviceArray(self, 4 * max(n, 1)), DeviceArray(self, 4 * max(n, 1))) \
+            if counted else None
+        try:
+            self.any_hit_dev(bvh, dr, occ, cnt)
+            self.sync()
+            out = {"occ": occ.download(np.uint8, n)}
+            if counted:
+                out["visits"] = cnt[0].download(np.uint32, n)
+                out["tests"] = cnt[1].download(np.uint32, n)
+            return out
+        finally:
+            dr.free()
+            occ.free()
+            if cnt:
+                cnt[0].free(), cnt[1].free()
+
+    def raygen_uniform(self, n: int, seed: int, lo, hi, stream=None) -> DeviceRays:
+        r = DeviceRays(self, n)
+        a = r.arrays
+        lo3 = (C.c_float * 3)(*lo)
+        hi3 = (C.c_float * 3)(*hi)
+        self._check(lib().rq_raygen_uniform(self.h, n, seed, lo3, hi3,
+                                            *(a[f].ptr for f in RAY_FIELDS), stream))
+        return r
+
+    def raygen_camera(self, params: RenderParams, jitter: np.ndarray | None = None,
+                      stream=None) -> DeviceRays:
+        n = params.width * params.height * params.spp
+        r = DeviceRays(self, n)
+        a = r.arrays
+        jd = None
+        if jitter is not None:
+            jd = DeviceArray(self, 8 * jitter.size).upload(np.asarray(jitter, np.float64))
+        try:
+            self._check(lib().rq_raygen_camera(self.h, C.byref(params), jd.ptr if jd else None,
+                                               *(a[f].ptr for f in RAY_FIELDS), stream))
+            self.sync()
+        finally:
+            if jd:
+                jd.free()
+        return r
+
+    def sync(self, stream=None) -> None:
+        self._check(lib().rq_sync(self.h, stream))
+
+    def device_errors(self) -> int:
+        bits = C.c_uint32()
+        self._check(lib().rq_device_errors(self.h, C.byref(bits)))
+        return bits.value
+
+    def stream_create(self):
+        s = C.c_void_p()
+        self._check(lib().rq_stream_create(self.h, C.byref(s)))
+        return s.value
+
+    def stream_destroy(self, s) -> None:
+        self._check(lib().rq_stream_destroy(self.h, s))
+
+    # per-kernel timing
+    def profile(self, on: bool = True) -> None:
+        self._check(lib().rq_profile_enable(self.h, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        self._check(lib().rq_profile_reset(self.h))
+
+    def profile_read(self) -> dict:
+        cap = 64
+        names = (C.c_char * 48 * cap)()
+        ms = (C.c_double * cap)()
+        cnt = (C.c_uint64 * cap)()
+        k = lib().rq_profile_read(self.h, names, ms, cnt, cap)
+        return {names[i].value.decode(): (ms[i], cnt[i]) for i in range(min(k, cap))}
+
+
+class Renderer:
+    """rq_renderer: the GPU_LBVH wavefront replacement of render() (main.cpp:541-566)."""
+
+    def __init__(self, ctx: Context, params: RenderParams):
+        self.ctx = ctx
+        self.params = params
+        h = C.c_void_p()
+        ctx._check(lib().rq_renderer_create(ctx.h, C.byref(params), C.byref(h)))
+        self.h = h.value
+        self.n = params.width * params.height * params.spp
+
+    def frame(self, bvh: int, stream=None) -> None:
+        self.ctx._check(lib().rq_render_frame(self.ctx.h, self.h, bvh, stream))
+
+    def counts(self) -> FrameCounts:
+        c = FrameCounts()
+        self.ctx._check(lib().rq_render_counts(self.ctx.h, self.h, C.byref(c)))
+        return c
+
+    def image(self) -> tuple[np.ndarray, np.ndarray]:
+        npix = self.params.width * self.params.height
+        rgb = np.empty(3 * npix, np.float32)
+        ppm = np.empty(3 * npix, np.uint8)
+        self.ctx._check(lib().rq_render_read_image(self.ctx.h, self.h, _ptr(rgb), _ptr(ppm)))
+        return rgb, ppm
+
+    def samples(self) -> dict:
+        n = self.n
+        out = dict(hit0=np.empty(n, np.int32), hit1=np.empty(n, np.int32),
+                   shadow0=np.empty(n, np.uint8), shadow1=np.empty(n, np.uint8),
+                   rgb=np.empty(3 * n, np.float32))
+        self.ctx._check(lib().rq_render_read_samples(
+            self.ctx.h, self.h, _ptr(out["hit0"]), _ptr(out["hit1"]), _ptr(out["shadow0"]),
+            _ptr(out["shadow1"]), _ptr(out["rgb"])))
+        return out
+
+    def primary_rays(self) -> DeviceRays:
+        r = Rays()
+        n = C.c_uint32()
+        self.ctx._check(lib().rq_render_primary_rays(self.ctx.h, self.h, C.byref(r), C.byref(n)))
+        view = DeviceRays.__new__(DeviceRays)
+        view.n = n.value
+        view.arrays = {}
+        for f in RAY_FIELDS:
+            a = DeviceArray.__new__(DeviceArray)
+            a.ctx, a.nbytes, a.ptr, a.owned = self.ctx, 4 * n.value, getattr(r, f), False
+            view.arrays[f] = a
+        return view
+
+    def close(self) -> None:
+        if self.h:
+            lib().rq_renderer_destroy(self.ctx.h, self.h)
+            self.h = None
+
+
+def write_ppm(path: str | Path, w: int, h: int, data: np.ndarray) -> None:
+    data = np.ascontiguousarray(data, np.uint8)
+    rc = lib().rq_write_ppm(str(path).encode(), w, h, _ptr(data))
+    if rc != RQ_OK:
+        raise RQError(rc, f"write_ppm {path}")

@@ -1,0 +1,135 @@
+"""GPU closest-hit / any-hit parity with the CPU oracle: identical hit ids
+(lowest index on ties), t/u/v bit-exact, identical traversal counts, and
+equality with brute force (the reference's NONE loop, main.cpp:373-387)."""
+import numpy as np
+import pytest
+
+from conftest import log
+from helpers import (concat_rays, duplicate_mesh, edge_rays, mesh_configs, random_rays,
+                     random_soup, rays_toward)
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.asarray(a, np.float32).view(np.uint32)
+
+
+def _query_parity(ctx, O, mesh, rays, mode=1, brute=True):
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, mode)
+    try:
+        ob = O.OracleBVH(mesh, mode)
+        g = ctx.closest_hit(b, rays, counted=True)
+        o = ob.closest(rays)
+        assert np.array_equal(g["prim"], o["prim"]), "closest prim ids"
+        assert np.array_equal(_bits(g["t"]), _bits(o["t"])), "closest t"
+        assert np.array_equal(_bits(g["u"]), _bits(o["u"])), "closest u"
+        assert np.array_equal(_bits(g["v"]), _bits(o["v"])), "closest v"
+        assert np.array_equal(g["visits"], o["visits"]), "node visits"
+        assert np.array_equal(g["tests"], o["tests"]), "triangle tests"
+        g2 = ctx.closest_hit(b, rays)  # uncounted kernel, same answers
+        assert np.array_equal(g2["prim"], o["prim"])
+        assert np.array_equal(_bits(g2["t"]), _bits(o["t"]))
+        ga = ctx.any_hit(b, rays, counted=True)
+        oa = ob.any(rays)
+        assert np.array_equal(ga["occ"], oa["occ"]), "any-hit"
+        assert np.array_equal(ga["visits"], oa["visits"])
+        assert np.array_equal(ga["tests"], oa["tests"])
+        ga2 = ctx.any_hit(b, rays)
+        assert np.array_equal(ga2["occ"], oa["occ"])
+        if brute:
+            bf = O.closest_brute(mesh, rays)
+            assert np.array_equal(o["prim"], bf["prim"]), "oracle BVH vs brute force"
+            assert np.array_equal(_bits(o["t"]), _bits(bf["t"]))
+            ab = O.any_brute(mesh, rays)
+            assert np.array_equal(oa["occ"], ab)
+        return g
+    finally:
+        ctx.release(b)
+        ctx.release(s)
+
+
+@pytest.mark.parametrize("name", list(mesh_configs()))
+def test_query_small(ctx, O, name):
+    mesh = mesh_configs()[name]()
+    rays = concat_rays(rays_toward(3000, 11), random_rays(2000, 12), edge_rays())
+    log(f"query {name} n={len(rays['ox'])}")
+    g = _query_parity(ctx, O, mesh, rays)
+    assert (g["prim"] >= 0).any()
+
+
+def test_query_ties_lowest_index(ctx, O):
+    """Identical triangles: every hit is a tie, the lowest index must win."""
+    mesh = duplicate_mesh(300)
+    rays = rays_toward(2000, 5, spread=0.5)
+    log("query ties")
+    g = _query_parity(ctx, O, mesh, rays)
+    hit = g["prim"] >= 0
+    assert hit.sum() > 1000
+    assert (g["prim"][hit] == 0).all()
+
+
+def test_query_ref_affine_mode(ctx, O, rq):
+    """REF_AFFINE Morton map leaves few distinct codes (accelerators.h:577):
+    the tree degenerates towards index order, results must still be exact."""
+    mesh = rq.icosphere(4, 3, center=(0, 0, -5))
+    rays = rays_toward(4000, 21)
+    log("query ref-affine")
+    _query_parity(ctx, O, mesh, rays, mode=0)
+
+
+def test_query_zero_rays(ctx, O, rq):
+    mesh = rq.icosphere(1, 1, center=(0, 0, -5))
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    rays = {k: np.zeros(0, np.float32) for k in ("ox", "oy", "oz", "dx", "dy", "dz", "tmin",
+                                                    "tmax")}
+    log("query zero rays")
+    g = ctx.closest_hit(b, rays)
+    assert g["prim"].size == 0
+    ctx.release(b)
+    ctx.release(s)
+
+
+def test_query_empty_scene_misses(ctx, rq):
+    m = rq.MeshArrays(np.zeros(0, np.float32), np.zeros(0, np.float32),
+                      np.zeros(0, np.float32), np.zeros(0, np.uint32))
+    s = ctx.upload_scene(m)
+    b = ctx.build_lbvh(s, 1)
+    log("query empty scene")
+    g = ctx.closest_hit(b, rays_toward(500, 1))
+    assert (g["prim"] == -1).all() and np.isinf(g["t"]).all()
+    a = ctx.any_hit(b, rays_toward(500, 1))
+    assert (a["occ"] == 0).all()
+    ctx.release(b)
+    ctx.release(s)
+
+
+def test_query_c1_camera(ctx, O, rq):
+    """C1 mesh (81,920 triangles) with the reference camera rays at 640x480."""
+    mesh = rq.icosphere(6, 1, center=(0, 0, -5))
+    jit = O.mt19937_jitter(2 * 640 * 480)
+    rays = O.camera_rays_ref(640, 480, 1, jit)
+    log("query c1 camera")
+    g = _query_parity(ctx, O, mesh, rays, brute=False)
+    assert (g["prim"] >= 0).mean() > 0.2
+    # brute force on a strided sample
+    sub = {k: v[::97].copy() for k, v in rays.items()}
+    bf = O.closest_brute(mesh, sub)
+    assert np.array_equal(g["prim"][::97], bf["prim"])
+
+
+def test_query_soup_random(ctx, O):
+    mesh = random_soup(20000, 9, spread=1.0, size=0.03)
+    rays = random_rays(20000, 4, lo=(-1.2, -1.2, -1.2), hi=(1.2, 1.2, 1.2))
+    log("query soup random")
+    _query_parity(ctx, O, mesh, rays, brute=False)
+    sub = {k: v[::50].copy() for k, v in rays.items()}
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    g = ctx.closest_hit(b, sub)
+    bf = O.closest_brute(mesh, sub)
+    assert np.array_equal(g["prim"], bf["prim"])
+    ctx.release(b)
+    ctx.release(s)
