@@ -1,0 +1,233 @@
+#!/usr/bin/env python3
+"""Benchmark of the GPU_LBVH ray-query path on one MI355X.
+
+Default workload (incoherent large mesh): displaced icosphere, subdivision 8
+(1,310,720 FP32 triangles) centred at (0,0,-2); 1920x1080, 4 jittered samples
+per pixel (Philox); one mirror bounce per primary hit plus a shadow ray from
+every hit; the 30-bit Morton LBVH is rebuilt every frame. One step = one
+frame: LBVH rebuild + raygen + closest hit (primary) + shade + closest hit
+(bounce) + shade + any hit (shadow) + resolve, inputs resident in HBM.
+
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3]
+
+Prints one JSON line: Mrays/s over all rays traced per frame, the live
+HIP-event roofline of the dominant kernel, and the reference CPU path
+(median-split BVH + collect-then-test query, restated in oracle/port.cpp)
+timed on this host on a bounded sample of the same rays.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "raytracer-data-structures_amd"))
+sys.path.insert(0, str(ROOT))
+
+import rq  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- workloads
+def c2_setup(ctx):
+    mesh = rq.icosphere(8, 2, center=(0.0, 0.0, -2.0))
+    p = rq.render_defaults()
+    p.width, p.height, p.spp = 1920, 1080, 4
+    p.camera, p.seed, p.bounce = rq.CAMERA_PHILOX, 2, 1
+    return mesh, p, "c2_incoherent_large_mesh"
+
+
+def c1_setup(ctx):
+    mesh = rq.icosphere(6, 1, center=(0.0, 0.0, -5.0))
+    p = rq.render_defaults()
+    return mesh, p, "c1_plain_640x480"
+
+
+def counted_batch(ctx, bvh, rays, any_hit: bool):
+    """Traversal counts of the canonical traversal for one ray batch."""
+    n = rays.n
+    vis = rq.DeviceArray(ctx, 4 * n)
+    tst = rq.DeviceArray(ctx, 4 * n)
+    try:
+        if any_hit:
+            occ = rq.DeviceArray(ctx, n)
+            ctx.any_hit_dev(bvh, rays, occ, counted=(vis, tst))
+            ctx.sync()
+            occ.free()
+        else:
+            hits = {k: rq.DeviceArray(ctx, 4 * n) for k in ("prim", "t")}
+            ctx.closest_hit_dev(bvh, rays, hits, counted=(vis, tst))
+            ctx.sync()
+            for a in hits.values():
+                a.free()
+        v = vis.download(np.uint32, n).astype(np.int64)
+        t = tst.download(np.uint32, n).astype(np.int64)
+    finally:
+        vis.free()
+        tst.free()
+    return int(v.sum()), int(t.sum())
+
+
+def cpu_baseline(mesh, batches, sample_every: int, threads: int):
+    """The reference CPU path restated on triangles (oracle/port.cpp), timed
+    on this host: median-split BVH build + collect-then-test closest hit on
+    every `sample_every`-th active ray of each batch."""
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    port = O.PortBVH(mesh)
+    build_s = time.perf_counter() - t0
+    sample = {k: [] for k in O.RAY_FIELDS}
+    for host in batches:
+        active = host["tmin"] < host["tmax"]
+        idx = np.nonzero(active)[0][::sample_every]
+        for k in O.RAY_FIELDS:
+            sample[k].append(host[k][idx])
+    sample = {k: np.concatenate(v) for k, v in sample.items()}
+    n = len(sample["ox"])
+    t0 = time.perf_counter()
+    port.closest(sample, threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=round(n / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
+                sample=f"every {sample_every}th active ray of each batch of one frame "
+                       f"({n} rays: primary, bounce, shadow as closest-hit), "
+                       f"median-split BVH built in {build_s * 1e3:.0f} ms",
+                build_ms=round(build_s * 1e3, 1), rays=n, seconds=round(dt, 2))
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--cpu-sample", type=int, default=96, help="1/k of each ray batch")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the path on a single MI355X "
+              "(rays split across GPUs would need no exchange; see DESIGN.md). "
+              f"--gpus {args.gpus} is not supported.", file=sys.stderr)
+        return 2
+
+    ctx = rq.Context(0)
+    mesh, params, workload = (c2_setup if args.config == "c2" else c1_setup)(ctx)
+    log(f"{workload}: {mesh.ntri} triangles")
+    scene = ctx.upload_scene(mesh)
+    bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
+    ren = rq.Renderer(ctx, params)
+
+    for _ in range(args.warmup):
+        ctx.rebuild(bvh)
+        ren.frame(bvh)
+    ctx.sync()
+    cnt = ren.counts()
+    rays_per_frame = int(cnt.primary + cnt.bounce + cnt.shadow)
+    info = ctx.bvh_info(bvh)
+
+    # canonical traversal counts of this frame's batches (outside the timed region)
+    batches = {"primary": ren.rays(0), "shadow": ren.rays(2)}
+    if params.bounce:
+        batches["bounce"] = ren.rays(1)
+    counts = {k: counted_batch(ctx, bvh, r, any_hit=(k == "shadow")) for k, r in batches.items()}
+
+    # ---- timed region: K frames, per-kernel HIP events on the launch stream
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.rebuild(bvh)
+        ren.frame(bvh)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    prof = ctx.profile_read()
+    assert ctx.device_errors() == 0
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = rays_per_frame * args.steps / elapsed / 1e6
+
+    # ---- roofline of the dominant kernel (algorithmic bytes / event time)
+    n = ren.n
+    closest_rays = [("primary", n)] + ([("bounce", n)] if params.bounce else [])
+    closest_bytes = sum(r * (32 + 8) + 32 * counts[k][0] + 36 * counts[k][1]
+                        for k, r in closest_rays) / len(closest_rays)
+    nshadow = batches["shadow"].n
+    any_bytes = nshadow * (32 + 1) + 32 * counts["shadow"][0] + 36 * counts["shadow"][1]
+    per_kernel = {k: (ms, c) for k, (ms, c) in prof.items()}
+    dom = max(per_kernel, key=lambda k: per_kernel[k][0])
+    ms_tot, launches = per_kernel[dom]
+    avg_ms = ms_tot / launches
+    alg = {"trace_closest": closest_bytes, "trace_any": any_bytes}.get(dom)
+    roofline = None
+    if alg is not None:
+        achieved = alg / (avg_ms * 1e-3) / 1e9
+        roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1),
+                        peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
+                        traffic=None, bytes_per_launch=int(alg),
+                        avg_launch_ms=round(avg_ms, 4))
+    build_kernels = ("build_init", "centroid_bounds", "morton_hist", "radix_onesweep", "karras",
+                     "refit", "refit_top")
+    build_ms = sum(per_kernel[k][0] for k in build_kernels if k in per_kernel) / args.steps
+    closest_ms = per_kernel.get("trace_closest", (0, 1))[0] / args.steps
+    closest_rays_n = sum(r for _, r in closest_rays)
+    cpu = None
+    if not args.no_cpu:
+        host = [batches[k].download() for k in batches]
+        threads = max(1, min(16, os.cpu_count() or 1))
+        log(f"cpu baseline on {threads} threads")
+        cpu = cpu_baseline(mesh, host, args.cpu_sample, threads)
+
+    line = {
+        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": workload,
+            "triangles": mesh.ntri,
+            "width": params.width, "height": params.height, "spp": params.spp,
+            "bounce": int(params.bounce),
+            "rays_per_frame": rays_per_frame,
+            "rays_primary": int(cnt.primary), "rays_bounce": int(cnt.bounce),
+            "rays_shadow": int(cnt.shadow),
+            "lbvh_rebuilt_per_frame": True,
+            "bvh_height": int(info.height),
+        },
+        "lbvh_build_ms": round(build_ms, 4),
+        "closest_hit_mrays_s": round(closest_rays_n / (closest_ms * 1e-3) / 1e6, 1)
+        if closest_ms else None,
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in per_kernel.items()},
+        "traversal_counts": {k: {"node_visits": v[0], "tri_tests": v[1]} for k, v in
+                             counts.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ren.close()
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

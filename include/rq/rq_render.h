@@ -66,6 +66,9 @@ int rq_render_read_samples(rq_ctx* ctx, rq_renderer* r, int32_t* hit0, int32_t* 
                            uint8_t* shadow0, uint8_t* shadow1, float* sample_rgb);
 /* Device SoA of the primary rays of the last frame (valid until destroy). */
 int rq_render_primary_rays(rq_ctx* ctx, rq_renderer* r, rq_rays* out, uint32_t* n);
+/* Device SoA of one ray batch of the last frame: which = 0 primary, 1 bounce,
+ * 2 shadow (n or 2n slots; unused slots are inactive rays). */
+int rq_render_rays(rq_ctx* ctx, rq_renderer* r, int which, rq_rays* out, uint32_t* n);
 
 /* Batch ray generators into caller-owned device SoA buffers. */
 int rq_raygen_camera(rq_ctx* ctx, const rq_render_params* p, const double* jitter_dev,

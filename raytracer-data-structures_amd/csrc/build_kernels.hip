@@ -107,12 +107,22 @@ __global__ __launch_bounds__(256) void k_centroid_bounds(const float* __restrict
       hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off, 64));
     }
   }
+  __shared__ uint32_t s_red[6][4];
+  const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      atomicMin(&misc[MISC_BOUNDS + k], lo[k]);
-      atomicMax(&misc[MISC_BOUNDS + 3 + k], hi[k]);
-    }
+    for (int k = 0; k < 3; ++k) s_red[k][wave] = lo[k], s_red[3 + k][wave] = hi[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {  // one device atomic per bound per block
+    const int k = threadIdx.x;
+    uint32_t v = s_red[k][0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); ++w)
+      v = k < 3 ? min(v, s_red[k][w]) : max(v, s_red[k][w]);
+    if (k < 3)
+      atomicMin(&misc[MISC_BOUNDS + k], v);
+    else
+      atomicMax(&misc[MISC_BOUNDS + k], v);
   }
 }
 
@@ -303,6 +313,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
                                                  NodeRec* __restrict__ nodes,
                                                  int32_t* __restrict__ parent,
                                                  uint32_t* __restrict__ flags,
+                                                 uint2* __restrict__ range,
                                                  uint32_t* __restrict__ misc) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t N = n;
@@ -330,6 +341,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
   const int64_t ul = left >= 0 ? left : (N - 1) + ~left;
   const int64_t ur = right >= 0 ? right : (N - 1) + ~right;
   flags[i] = 0u;
+  range[i] = make_uint2((uint32_t)min(i, j), (uint32_t)max(i, j));
   if (g < 0 || g + 1 > N - 1 || ul >= 2 * N - 1 || ur >= 2 * N - 1) {
     atomicOr(&misc[MISC_ERR], ERR_TOPOLOGY);
     return;
@@ -338,73 +350,141 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ key
   parent[ur] = (int32_t)((i << 1) | 1);
 }
 
-// Bottom-up refit: one thread per leaf writes its triangle record and box
-// into the parent's child slot; the second arriver at a node (atomic counter)
-// unions the two child boxes and climbs. min/max is order-independent and
-// zeros are canonicalised, so the boxes are bit-exact whatever the order.
-__global__ __launch_bounds__(256) void k_refit(const float* __restrict__ vx,
-                                                const float* __restrict__ vy,
-                                                const float* __restrict__ vz,
-                                                const uint32_t* __restrict__ idx,
-                                                const uint32_t* __restrict__ sorted_idx, uint32_t n,
-                                                NodeRec* nodes, TriRec* __restrict__ tris,
-                                                const int32_t* __restrict__ parent,
-                                                uint32_t* flags, uint32_t* slot_h,
-                                                uint32_t* __restrict__ misc) {
-  uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const uint32_t p = sorted_idx[k];
-  const uint32_t a = idx[3 * p], b = idx[3 * p + 1], c = idx[3 * p + 2];
-  const float3 v0 = make_float3(vx[a], vy[a], vz[a]);
-  const float3 v1 = make_float3(vx[b], vy[b], vz[b]);
-  const float3 v2 = make_float3(vx[c], vy[c], vz[c]);
-  TriRec tr;
-  tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
-  tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
-  tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
-  tris[k] = tr;
-  float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, fminf(fminf(v0.y, v1.y), v2.y) + 0.0f,
-                 fminf(fminf(v0.z, v1.z), v2.z) + 0.0f};
-  float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f,
-                 fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f};
-  uint32_t h = 0;
-  if (n == 1) {
-    float* root = reinterpret_cast<float*>(misc + MISC_ROOT);
-    for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];
-    misc[MISC_HEIGHT] = 0u;
-    return;
-  }
-  uint32_t cur = (n - 1) + k;
-  for (int level = 0;; ++level) {
-    const int32_t pe = parent[cur];
-    const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
-    if (pe < 0 || node >= n - 1 || level > 64) {  // height <= 63 for 30-bit codes
-      atomicOr(&misc[MISC_ERR], ERR_REFIT_DEPTH);
-      return;
-    }
-    float* slo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
-    float* shi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
-    slo[0] = lo[0], slo[1] = lo[1], slo[2] = lo[2];
-    shi[0] = hi[0], shi[1] = hi[1], shi[2] = hi[2];
-    slot_h[2 * node + side] = h;
-    const uint32_t old =
-        __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0u) return;
-    const float* olo = side ? &nodes[node].lo0.x : &nodes[node].lo1.x;
-    const float* ohi = side ? &nodes[node].hi0.x : &nodes[node].hi1.x;
+// Bottom-up refit, pass 1 (one block per REFIT_CHUNK leaves): each leaf
+// writes its triangle record and its box into the parent's child slot; the
+// second arriver at a node (atomic counter) unions the two child boxes and
+// climbs. A node whose leaf range lies inside the block's chunk is completed
+// by two threads of this block, so its counter needs only work-group scope.
+// A node spanning chunks is left to pass 2: the arriving thread deposits its
+// box, counts itself with a relaxed device atomic, and the second of two
+// arrivals lists the node in the block's frontier. min/max is
+// order-independent and zeros are canonicalised, so the boxes are bit-exact
+// whatever the order.
+__device__ inline void put_slot(NodeRec* nodes, uint32_t node, uint32_t side, const float* lo,
+                                const float* hi) {
+  float* slo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
+  float* shi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
+  slo[0] = lo[0], slo[1] = lo[1], slo[2] = lo[2];
+  shi[0] = hi[0], shi[1] = hi[1], shi[2] = hi[2];
+}
+__device__ inline void union_slot(const NodeRec* nodes, uint32_t node, uint32_t side, float* lo,
+                                  float* hi) {
+  const float* olo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
+  const float* ohi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      lo[q] = fminf(lo[q], olo[q]);
-      hi[q] = fmaxf(hi[q], ohi[q]);
+  for (int q = 0; q < 3; ++q) {
+    lo[q] = fminf(lo[q], olo[q]);
+    hi[q] = fmaxf(hi[q], ohi[q]);
+  }
+}
+__device__ inline void put_root(uint32_t* misc, const float* lo, const float* hi, uint32_t h) {
+  float* root = reinterpret_cast<float*>(misc + MISC_ROOT);
+  for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];
+  misc[MISC_HEIGHT] = h;
+}
+
+__global__ __launch_bounds__(REFIT_BLOCK) void k_refit_local(
+    const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+    const uint32_t* __restrict__ idx, const uint32_t* __restrict__ sorted_idx, uint32_t n,
+    NodeRec* nodes, TriRec* __restrict__ tris, const int32_t* __restrict__ parent,
+    uint32_t* flags, uint32_t* slot_h, const uint2* __restrict__ range,
+    uint32_t* __restrict__ frontier, uint32_t* __restrict__ fcount, uint32_t* __restrict__ misc) {
+  __shared__ uint32_t s_nf;
+  if (threadIdx.x == 0) s_nf = 0u;
+  __syncthreads();
+  const uint32_t chunk_lo = blockIdx.x * REFIT_CHUNK;
+  const uint32_t chunk_hi = min(chunk_lo + REFIT_CHUNK, n) - 1;
+  for (int j = 0; j < REFIT_CHUNK / REFIT_BLOCK; ++j) {
+    const uint32_t k = chunk_lo + j * REFIT_BLOCK + threadIdx.x;
+    if (k >= n) break;
+    const uint32_t p = sorted_idx[k];
+    const uint32_t a = idx[3 * p], b = idx[3 * p + 1], c = idx[3 * p + 2];
+    const float3 v0 = make_float3(vx[a], vy[a], vz[a]);
+    const float3 v1 = make_float3(vx[b], vy[b], vz[b]);
+    const float3 v2 = make_float3(vx[c], vy[c], vz[c]);
+    TriRec tr;
+    tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
+    tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
+    tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
+    tris[k] = tr;
+    float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, fminf(fminf(v0.y, v1.y), v2.y) + 0.0f,
+                   fminf(fminf(v0.z, v1.z), v2.z) + 0.0f};
+    float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f,
+                   fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f};
+    if (n == 1) {
+      put_root(misc, lo, hi, 0u);
+      break;
     }
-    h = 1u + max(h, slot_h[2 * node + (side ^ 1u)]);
-    if (node == 0u) {
-      float* root = reinterpret_cast<float*>(misc + MISC_ROOT);
-      for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];
-      misc[MISC_HEIGHT] = h;
-      return;
+    uint32_t h = 0, cur = (n - 1) + k;
+    for (int level = 0;; ++level) {
+      const int32_t pe = parent[cur];
+      const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
+      if (pe < 0 || node >= n - 1 || level > 64) {  // height <= 63 for 30-bit codes
+        atomicOr(&misc[MISC_ERR], ERR_REFIT_DEPTH);
+        break;
+      }
+      put_slot(nodes, node, side, lo, hi);
+      slot_h[2 * node + side] = h;
+      const uint2 rg = range[node];
+      if (rg.x < chunk_lo || rg.y > chunk_hi) {
+        // spans chunks: finished by the top pass once both children are in
+        const uint32_t old =
+            __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 1u) frontier[chunk_lo + atomicAdd(&s_nf, 1u)] = node;
+        break;
+      }
+      const uint32_t old =
+          __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      union_slot(nodes, node, side ^ 1u, lo, hi);
+      h = 1u + max(h, slot_h[2 * node + (side ^ 1u)]);
+      if (node == 0u) {
+        put_root(misc, lo, hi, h);
+        break;
+      }
+      cur = node;
     }
-    cur = node;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) fcount[blockIdx.x] = s_nf;
+}
+
+// Refit pass 2, one work-group: the nodes spanning chunks, bottom-up from the
+// frontier lists of pass 1 (their child slots were written before the kernel
+// boundary); counters shared inside the work-group need only its scope.
+__global__ __launch_bounds__(REFIT_TOP_BLOCK) void k_refit_top(
+    NodeRec* nodes, const int32_t* __restrict__ parent, uint32_t* flags, uint32_t* slot_h,
+    const uint32_t* __restrict__ frontier, const uint32_t* __restrict__ fcount, uint32_t nchunks,
+    uint32_t n, uint32_t* __restrict__ misc) {
+  for (uint32_t bch = threadIdx.x; bch < nchunks; bch += REFIT_TOP_BLOCK) {
+    const uint32_t cnt = fcount[bch];
+    for (uint32_t e = 0; e < cnt; ++e) {
+      uint32_t node = frontier[bch * REFIT_CHUNK + e];
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      union_slot(nodes, node, 0u, lo, hi);
+      union_slot(nodes, node, 1u, lo, hi);
+      uint32_t h = 1u + max(slot_h[2 * node], slot_h[2 * node + 1]);
+      for (int level = 0;; ++level) {
+        if (node == 0u) {
+          put_root(misc, lo, hi, h);
+          break;
+        }
+        const int32_t pe = parent[node];
+        const uint32_t q = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
+        if (pe < 0 || q >= n - 1 || level > 64) {
+          atomicOr(&misc[MISC_ERR], ERR_REFIT_DEPTH);
+          break;
+        }
+        put_slot(nodes, q, side, lo, hi);
+        slot_h[2 * q + side] = h;
+        const uint32_t old =
+            __hip_atomic_fetch_add(&flags[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) break;
+        union_slot(nodes, q, side ^ 1u, lo, hi);
+        h = 1u + max(h, slot_h[2 * q + (side ^ 1u)]);
+        node = q;
+      }
+    }
   }
 }
 
@@ -426,8 +506,8 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
   const uint32_t gblocks = std::min<uint32_t>(2048u, cdiv(n, 256));
   if (mode == RQ_MORTON_SCENE_AABB) {
     ProfScope ps(lc, "centroid_bounds");
-    hipLaunchKernelGGL(k_centroid_bounds, dim3(gblocks), dim3(256), 0, s, vx, vy, vz, idx, n,
-                       B.misc);
+    hipLaunchKernelGGL(k_centroid_bounds, dim3(std::min<uint32_t>(gblocks, 512u)), dim3(256), 0, s,
+                       vx, vy, vz, idx, n, B.misc);
   }
   {
     ProfScope ps(lc, "morton_hist");
@@ -448,12 +528,19 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
   if (n >= 2) {
     ProfScope ps(lc, "karras");
     hipLaunchKernelGGL(k_karras, dim3(cdiv(n - 1, 256)), dim3(256), 0, s, B.keys_a, n, B.nodes,
-                       B.parent, B.flags, B.misc);
+                       B.parent, B.flags, B.range, B.misc);
   }
+  const uint32_t nchunks = cdiv(n, REFIT_CHUNK);
   {
     ProfScope ps(lc, "refit");
-    hipLaunchKernelGGL(k_refit, dim3(cdiv(n, 256)), dim3(256), 0, s, vx, vy, vz, idx, B.vals_a,
-                       n, B.nodes, B.tris, B.parent, B.flags, B.slot_h, B.misc);
+    hipLaunchKernelGGL(k_refit_local, dim3(nchunks), dim3(REFIT_BLOCK), 0, s, vx, vy, vz, idx,
+                       B.vals_a, n, B.nodes, B.tris, B.parent, B.flags, B.slot_h, B.range,
+                       B.frontier, B.fcount, B.misc);
+  }
+  if (nchunks > 1) {
+    ProfScope ps(lc, "refit_top");
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, B.nodes, B.parent,
+                       B.flags, B.slot_h, B.frontier, B.fcount, nchunks, n, B.misc);
   }
   return hipGetLastError();
 }

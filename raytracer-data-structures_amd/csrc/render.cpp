@@ -154,21 +154,20 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   LaunchCtx lc = launch_ctx(c, stream);
   const rq_scene* s = b->scene;
   SceneView sv{s->vx, s->vy, s->vz, s->idx};
-  hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, r->prim, r->counters);
+  hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, r->prim);
   if (e != hipSuccess) return hip_err(c, e, "raygen");
   rq_rays pr = as_rays(r->prim);
   rq_hits h0{r->h0, r->t0, nullptr, nullptr};
   int rc = rq_closest_hit(c, b, &pr, r->n, &h0, stream);
   if (rc) return rc;
-  e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc,
-                           r->counters);
+  e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc);
   if (e != hipSuccess) return hip_err(c, e, "shade primary");
   if (r->p.bounce) {
     rq_rays br = as_rays(r->bnc);
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
     rc = rq_closest_hit(c, b, &br, r->n, &h1, stream);
     if (rc) return rc;
-    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad, r->counters);
+    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad);
     if (e != hipSuccess) return hip_err(c, e, "shade bounce");
   }
   rq_rays sr = as_rays(r->shad);
@@ -184,9 +183,15 @@ int rq_render_counts(rq_ctx* c, rq_renderer* r, rq_frame_counts* out) {
   if (!c || !r || !out) return RQ_E_INVALID;
   uint32_t k[4];
   hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemset(r->counters, 0, 16);
+  if (e == hipSuccess)
+    e = launch_count_hits(LaunchCtx{nullptr, nullptr}, r->h0, r->p.bounce ? r->h1 : nullptr, r->n,
+                          r->counters);
   if (e == hipSuccess) e = hipMemcpy(k, r->counters, 16, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_err(c, e, "counts");
-  out->primary = k[0], out->bounce = k[1], out->shadow = k[2];
+  out->primary = r->n;
+  out->bounce = r->p.bounce ? k[0] : 0;
+  out->shadow = (uint64_t)k[0] + k[1];
   return RQ_OK;
 }
 
@@ -225,21 +230,6 @@ int rq_render_read_samples(rq_ctx* c, rq_renderer* r, int32_t* hit0, int32_t* hi
   if (e == hipSuccess && sample_rgb)
     e = hipMemcpy(sample_rgb, r->srgb, n * 12, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_err(c, e, "sample read");
-  // shadow flags of rays never spawned are reported as 0
-  if (sh0 || sh1) {
-    std::vector<int32_t> a(n), b2;
-    if (hipMemcpy(a.data(), r->h0, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-      return set_err(c, RQ_E_HIP, "sample read");
-    if (r->p.bounce) {
-      b2.resize(n);
-      if (hipMemcpy(b2.data(), r->h1, n * 4, hipMemcpyDeviceToHost) != hipSuccess)
-        return set_err(c, RQ_E_HIP, "sample read");
-    }
-    for (size_t i = 0; i < n; ++i) {
-      if (sh0 && a[i] < 0) sh0[i] = 0;
-      if (sh1 && r->p.bounce && b2[i] < 0) sh1[i] = 0;
-    }
-  }
   return RQ_OK;
 }
 
@@ -247,6 +237,23 @@ int rq_render_primary_rays(rq_ctx* c, rq_renderer* r, rq_rays* out, uint32_t* n)
   if (!c || !r || !out) return RQ_E_INVALID;
   *out = as_rays(r->prim);
   if (n) *n = r->n;
+  return RQ_OK;
+}
+
+int rq_render_rays(rq_ctx* c, rq_renderer* r, int which, rq_rays* out, uint32_t* n) {
+  if (!c || !r || !out) return RQ_E_INVALID;
+  if (which == 0) {
+    *out = as_rays(r->prim);
+    if (n) *n = r->n;
+  } else if (which == 1 && r->p.bounce) {
+    *out = as_rays(r->bnc);
+    if (n) *n = r->n;
+  } else if (which == 2) {
+    *out = as_rays(r->shad);
+    if (n) *n = r->p.bounce ? 2 * r->n : r->n;
+  } else {
+    return set_err(c, RQ_E_INVALID, "no ray batch %d", which);
+  }
   return RQ_OK;
 }
 
@@ -258,8 +265,7 @@ int rq_raygen_camera(rq_ctx* c, const rq_render_params* p, const double* jit, fl
     return set_err(c, RQ_E_INVALID, "REF_JITTER camera needs the jitter array");
   RaySoA R{ox, oy, oz, dx, dy, dz, tmin, tmax};
   hipError_t e = launch_raygen_camera(launch_ctx(c, stream), cam_params(p),
-                                      p->camera == RQ_CAMERA_REF_JITTER ? jit : nullptr, R,
-                                      nullptr);
+                                      p->camera == RQ_CAMERA_REF_JITTER ? jit : nullptr, R);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "raygen camera");
 }
 

@@ -59,10 +59,9 @@ __device__ inline float u24(uint32_t a) { return (float)(a >> 8) * (1.0f / 16777
 __global__ __launch_bounds__(256) void k_raygen_camera(CamParams cp, const double* __restrict__ jit,
                                                         float* ox, float* oy, float* oz, float* dx,
                                                         float* dy, float* dz, float* tmin,
-                                                        float* tmax, uint32_t* counters) {
+                                                        float* tmax) {
   const uint64_t n = (uint64_t)cp.w * cp.h * cp.spp;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (counters && i < 4) counters[i] = 0u;
   if (i >= n) return;
   const uint64_t pix = i / cp.spp;
   const uint32_t x = (uint32_t)(pix % cp.w), y = (uint32_t)(pix / cp.w);
@@ -183,23 +182,37 @@ __device__ inline void shadow_ray(const RaySoA& R, uint64_t i, V3 d, const Geo& 
 __device__ inline V3 get_o(const RaySoA& R, uint64_t i) { return {R.ox[i], R.oy[i], R.oz[i]}; }
 __device__ inline V3 get_d(const RaySoA& R, uint64_t i) { return {R.dx[i], R.dy[i], R.dz[i]}; }
 
-__device__ inline void count_wave(uint32_t* ctr, bool pred) {
-  const uint64_t m = __ballot(pred);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(ctr, (uint32_t)__popcll(m));
+// Frame ray counts on demand: primary = n, bounce = primary hits (when
+// bouncing), shadow = primary hits + bounce hits. One atomic per block.
+__global__ __launch_bounds__(256) void k_count_hits(const int32_t* __restrict__ h0,
+                                                     const int32_t* __restrict__ h1, uint32_t n,
+                                                     uint32_t* counters) {
+  __shared__ uint32_t s[2];
+  if (threadIdx.x < 2) s[threadIdx.x] = 0u;
+  __syncthreads();
+  uint32_t c0 = 0, c1 = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    c0 += h0[i] >= 0;
+    if (h1) c1 += h1[i] >= 0;
+  }
+  atomicAdd(&s[0], c0);
+  atomicAdd(&s[1], c1);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&counters[0], s[0]);
+    atomicAdd(&counters[1], s[1]);
+  }
 }
 
 // Stage 1: from primary hits spawn the shadow ray (slot i) and the bounce ray.
 __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams sp, uint32_t n,
                                                         RaySoA prim, const int32_t* __restrict__ h0,
                                                         const float* __restrict__ t0, RaySoA shad,
-                                                        RaySoA bnc, uint32_t* counters) {
+                                                        RaySoA bnc) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
-  const int32_t p = in ? h0[i] : -1;
-  count_wave(&counters[0], in);
-  count_wave(&counters[2], p >= 0);
-  if (sp.bounce) count_wave(&counters[1], p >= 0);
   if (!in) return;
+  const int32_t p = h0[i];
   if (p < 0) {
     put_inactive(shad, i);
     if (sp.bounce) put_inactive(bnc, i);
@@ -219,13 +232,11 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
 // Stage 2: from bounce hits spawn their shadow ray (slot n + i).
 __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams sp, uint32_t n,
                                                        RaySoA bnc, const int32_t* __restrict__ h1,
-                                                       const float* __restrict__ t1, RaySoA shad,
-                                                       uint32_t* counters) {
+                                                       const float* __restrict__ t1, RaySoA shad) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
-  const int32_t p = in ? h1[i] : -1;
-  count_wave(&counters[2], p >= 0);
   if (!in) return;
+  const int32_t p = h1[i];
   if (p < 0) {
     put_inactive(shad, (uint64_t)n + i);
     return;
@@ -283,12 +294,12 @@ inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b
 }  // namespace
 
 hipError_t launch_raygen_camera(const LaunchCtx& lc, const CamParams& cp, const double* jit,
-                                const RaySoA& R, uint32_t* counters) {
+                                const RaySoA& R) {
   const uint64_t n = (uint64_t)cp.w * cp.h * cp.spp;
   if (n == 0) return hipSuccess;
   ProfScope ps(lc, "raygen_camera");
   hipLaunchKernelGGL(k_raygen_camera, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, cp, jit, R.ox,
-                     R.oy, R.oz, R.dx, R.dy, R.dz, R.tmin, R.tmax, counters);
+                     R.oy, R.oz, R.dx, R.dy, R.dz, R.tmin, R.tmax);
   return hipGetLastError();
 }
 
@@ -304,19 +315,26 @@ hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed,
 
 hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                 uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
-                                const RaySoA& shad, const RaySoA& bnc, uint32_t* counters) {
+                                const RaySoA& shad, const RaySoA& bnc) {
   ProfScope ps(lc, "shade_primary");
   hipLaunchKernelGGL(k_shade_primary, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, prim,
-                     h0, t0, shad, bnc, counters);
+                     h0, t0, shad, bnc);
   return hipGetLastError();
 }
 
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
-                               const RaySoA& shad, uint32_t* counters) {
+                               const RaySoA& shad) {
   ProfScope ps(lc, "shade_bounce");
   hipLaunchKernelGGL(k_shade_bounce, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, bnc,
-                     h1, t1, shad, counters);
+                     h1, t1, shad);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_hits(const LaunchCtx& lc, const int32_t* h0, const int32_t* h1, uint32_t n,
+                             uint32_t* counters) {
+  hipLaunchKernelGGL(k_count_hits, dim3(std::min<uint32_t>(cdiv(n, 256), 1024u)), dim3(256), 0,
+                     lc.stream, h0, h1, n, counters);
   return hipGetLastError();
 }
 

@@ -128,8 +128,11 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
                  4 * (2 * nn),
                  4 * ni,
                  8 * ni,
+                 8 * ni,
                  64 * ni,
-                 48 * nn};
+                 48 * nn,
+                 4 * nn,
+                 4 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK)};
   size_t total = 0;
   for (size_t s : sz) total += align256(s);
   void* p;
@@ -138,7 +141,8 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
   char* q = (char*)p;
   void** dst[] = {(void**)&B.codes, (void**)&B.keys_a, (void**)&B.keys_b, (void**)&B.vals_a,
                   (void**)&B.vals_b, (void**)&B.status, (void**)&B.misc, (void**)&B.parent,
-                  (void**)&B.flags, (void**)&B.slot_h, (void**)&B.nodes, (void**)&B.tris};
+                  (void**)&B.flags, (void**)&B.slot_h, (void**)&B.range, (void**)&B.nodes,
+                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.fcount};
   for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); ++i) {
     *dst[i] = q;
     q += align256(sz[i]);

@@ -39,6 +39,12 @@ constexpr int SORT_BLOCK = SORT_WAVES * 64;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int SORT_PASSES = 4;  // 30-bit codes, 8-bit digits
 
+// Refit: a block owns REFIT_CHUNK consecutive leaves (REFIT_BLOCK threads,
+// REFIT_CHUNK / REFIT_BLOCK leaves each); the top pass runs in one work-group.
+constexpr int REFIT_BLOCK = 256;
+constexpr int REFIT_CHUNK = 1024;
+constexpr int REFIT_TOP_BLOCK = 1024;
+
 // Lookback status word: [31:30] flag, [29:0] count.
 constexpr uint32_t LB_AGG = 1u << 30;
 constexpr uint32_t LB_INC = 2u << 30;
@@ -63,6 +69,9 @@ struct BuildBuffers {
   int32_t* parent = nullptr;      // 2n-1: (node << 1) | side
   uint32_t* flags = nullptr;      // n-1 refit arrival counters
   uint32_t* slot_h = nullptr;     // 2(n-1) child heights per node side
+  uint2* range = nullptr;         // n-1 leaf range [first, last] of each internal node
+  uint32_t* frontier = nullptr;   // n: per refit chunk, spanning nodes ready for the top pass
+  uint32_t* fcount = nullptr;     // cdiv(n, REFIT_CHUNK) entries per chunk
   NodeRec* nodes = nullptr;       // n-1
   TriRec* tris = nullptr;         // n
   uint32_t ntiles = 0;

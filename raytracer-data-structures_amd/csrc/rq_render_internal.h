@@ -26,15 +26,17 @@ struct RaySoA {
 };
 
 hipError_t launch_raygen_camera(const LaunchCtx& lc, const CamParams& cp, const double* jit,
-                                const RaySoA& R, uint32_t* counters);
+                                const RaySoA& R);
 hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed, const float lo[3],
                                  const float hi[3], const RaySoA& R);
 hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                 uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
-                                const RaySoA& shad, const RaySoA& bnc, uint32_t* counters);
+                                const RaySoA& shad, const RaySoA& bnc);
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
-                               const RaySoA& shad, uint32_t* counters);
+                               const RaySoA& shad);
+hipError_t launch_count_hits(const LaunchCtx& lc, const int32_t* h0, const int32_t* h1, uint32_t n,
+                             uint32_t* counters);
 hipError_t launch_shade_final(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                               uint32_t npix, uint32_t spp, const RaySoA& prim, const RaySoA& bnc,
                               const int32_t* h0, const float* t0, const int32_t* h1,

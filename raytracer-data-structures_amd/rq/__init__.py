@@ -140,6 +140,7 @@ def _load() -> C.CDLL:
         "rq_render_read_image": (i32, [vp, vp, vp, vp]),
         "rq_render_read_samples": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "rq_render_primary_rays": (i32, [vp, vp, C.POINTER(Rays), C.POINTER(u32)]),
+        "rq_render_rays": (i32, [vp, vp, i32, C.POINTER(Rays), C.POINTER(u32)]),
         "rq_raygen_camera": (i32, [vp, C.POINTER(RenderParams), vp, vp, vp, vp, vp, vp, vp, vp,
                                    vp, vp]),
         "rq_raygen_uniform": (i32, [vp, u32, u64, f32p, f32p, vp, vp, vp, vp, vp, vp, vp, vp,
@@ -538,10 +539,11 @@ class Renderer:
             _ptr(out["shadow1"]), _ptr(out["rgb"])))
         return out
 
-    def primary_rays(self) -> DeviceRays:
+    def rays(self, which: int = 0) -> DeviceRays:
+        """Non-owning view of a ray batch of the last frame (0 primary, 1 bounce, 2 shadow)."""
         r = Rays()
         n = C.c_uint32()
-        self.ctx._check(lib().rq_render_primary_rays(self.ctx.h, self.h, C.byref(r), C.byref(n)))
+        self.ctx._check(lib().rq_render_rays(self.ctx.h, self.h, which, C.byref(r), C.byref(n)))
         view = DeviceRays.__new__(DeviceRays)
         view.n = n.value
         view.arrays = {}
@@ -550,6 +552,9 @@ class Renderer:
             a.ctx, a.nbytes, a.ptr, a.owned = self.ctx, 4 * n.value, getattr(r, f), False
             view.arrays[f] = a
         return view
+
+    def primary_rays(self) -> DeviceRays:
+        return self.rays(0)
 
     def close(self) -> None:
         if self.h:
