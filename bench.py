@@ -79,6 +79,42 @@ def counted_batch(ctx, bvh, rays, any_hit: bool):
     return int(v.sum()), int(t.sum())
 
 
+def sweep(ctx, bvh, batches) -> None:
+    """Time each traversal variant / refill threshold on this frame's batches."""
+    import ctypes as C
+
+    saved = ctx.trace_config()
+    for v in ctx.trace_variants():
+        for refill in (8, 16, 32, 48):
+            row = []
+            for name, rays in batches.items():
+                anyh = name == "shadow"
+                if anyh and v["tstack"]:
+                    row.append(f"{name}=   -   ")
+                    continue
+                ctx.trace_config(v["index"], v["index"], refill)
+                out = rq.DeviceArray(ctx, 4 * rays.n)
+                tt = rq.DeviceArray(ctx, 4 * rays.n)
+                ctx.profile(True)
+                ctx.profile_reset()
+                for _ in range(4):
+                    if anyh:
+                        ctx.any_hit_dev(bvh, rays, out)
+                    else:
+                        ctx.closest_hit_dev(bvh, rays, {"prim": out, "t": tt})
+                ctx.sync()
+                prof = ctx.profile_read()
+                ctx.profile(False)
+                key = "trace_any" if anyh else "trace_closest"
+                ms, cnt = prof[key]
+                row.append(f"{name}={ms / cnt:7.3f}")
+                out.free()
+                tt.free()
+            log(f"variant {v['index']} b{v['block']} d{v['lds_depth']} t{v['tstack']} "
+                f"refill {refill:2d}: " + " ".join(row))
+    ctx.trace_config(*saved)
+
+
 def cpu_baseline(mesh, batches, sample_every: int, threads: int):
     """The reference CPU path restated on triangles (oracle/port.cpp), timed
     on this host: median-split BVH build + collect-then-test closest hit on
@@ -114,6 +150,11 @@ def main() -> int:
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--cpu-sample", type=int, default=96, help="1/k of each ray batch")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep", action="store_true",
+                    help="time every traversal variant on this frame's batches (stderr)")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run the library copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
@@ -122,6 +163,14 @@ def main() -> int:
         return 2
 
     ctx = rq.Context(0)
+    if args.copy_calibration:
+        nb = args.copy_calibration
+        src, dst = rq.DeviceArray(ctx, nb), rq.DeviceArray(ctx, nb)
+        for _ in range(3):
+            ctx.copy(dst, src, nb)
+        ctx.sync()
+        print(json.dumps({"copy_bytes": nb, "launches": 3}), flush=True)
+        return 0
     mesh, params, workload = (c2_setup if args.config == "c2" else c1_setup)(ctx)
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)
@@ -141,6 +190,20 @@ def main() -> int:
     if params.bounce:
         batches["bounce"] = ren.rays(1)
     counts = {k: counted_batch(ctx, bvh, r, any_hit=(k == "shadow")) for k, r in batches.items()}
+    if args.sweep:
+        sweep(ctx, bvh, batches)
+    tuned = None
+    if not args.no_autotune:
+        vc, rc_, msc = ctx.autotune(bvh, batches["primary"], any_hit=False, repeats=3)
+        va, ra, msa = ctx.autotune(bvh, batches["shadow"], any_hit=True, repeats=3)
+        ctx.trace_config(vc, va, rc_)
+        tuned = {"closest_variant": ctx.trace_variants()[vc], "any_variant":
+                 ctx.trace_variants()[va], "refill_idle": rc_}
+        log(f"autotune: {tuned}")
+        for _ in range(2):
+            ctx.rebuild(bvh)
+            ren.frame(bvh)
+        ctx.sync()
 
     # ---- timed region: K frames, per-kernel HIP events on the launch stream
     ctx.profile(True)
@@ -220,6 +283,7 @@ def main() -> int:
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in per_kernel.items()},
         "traversal_counts": {k: {"node_visits": v[0], "tri_tests": v[1]} for k, v in
                              counts.items()},
+        "autotune": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -115,11 +115,31 @@ int rq_closest_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, ui
 int rq_any_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
                        uint8_t* occluded, uint32_t* visits, uint32_t* tests, void* stream);
 
+/* ---- traversal kernel variants and the autotuner ----------------------
+ * Variants differ in threads per block, LDS stack entries per lane, and
+ * whether stack entries carry their entry distance (closest hit skips
+ * entries beyond the best hit when popped). Results are identical for all. */
+typedef struct {
+  int32_t block, lds_depth, tstack;
+} rq_trace_variant;
+int rq_trace_variant_count(void);
+int rq_trace_variant_get(int index, rq_trace_variant* out);
+int rq_trace_config_set(rq_ctx* ctx, int closest_variant, int any_variant, uint32_t refill_idle);
+int rq_trace_config_get(rq_ctx* ctx, int* closest_variant, int* any_variant,
+                        uint32_t* refill_idle);
+/* Times every variant (and refill threshold 8/16/32/48 idle lanes) on the given
+ * device rays (default stream, `repeats` launches each after one warm-up),
+ * keeps the fastest in the context and reports it. */
+int rq_autotune_trace(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, int any,
+                      int repeats, int* best_variant, uint32_t* best_refill, double* best_ms);
+
 /* ---- device buffers (for callers without their own allocator) ---------- */
 int rq_device_alloc(rq_ctx* ctx, size_t bytes, void** dptr);
 int rq_device_free(rq_ctx* ctx, void* dptr);
 int rq_memcpy_h2d(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 int rq_memcpy_d2h(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
+/* Device-to-device copy by a library kernel (16-byte aligned pointers/size). */
+int rq_memcpy_d2d(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 int rq_memset_d(rq_ctx* ctx, void* dst, int value, size_t bytes, void* stream);
 int rq_sync(rq_ctx* ctx, void* stream);
 /* Reads and clears the device error bits of past queries (synchronises the

@@ -291,7 +291,25 @@ __global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp
 
 inline uint32_t cdiv(uint64_t a, uint32_t b) { return (uint32_t)((a + b - 1) / b); }
 
+// Plain device-to-device copy, 16 bytes per lane (also the known-size kernel
+// that calibrates the HBM byte counters).
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src,
+                                                 uint4* __restrict__ dst, uint64_t n16) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
+
+hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64_t bytes) {
+  if (bytes == 0) return hipSuccess;
+  ProfScope ps(lc, "copy16");
+  const uint64_t n16 = bytes / 16;
+  hipLaunchKernelGGL(k_copy16, dim3(std::min<uint64_t>((n16 + 255) / 256, 8192)), dim3(256), 0,
+                     lc.stream, (const uint4*)src, (uint4*)dst, n16);
+  return hipGetLastError();
+}
 
 hipError_t launch_raygen_camera(const LaunchCtx& lc, const CamParams& cp, const double* jit,
                                 const RaySoA& R) {

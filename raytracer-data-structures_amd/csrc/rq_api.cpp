@@ -174,16 +174,20 @@ int rq_create(rq_ctx** out, int device, size_t budget) {
   c->device = device;
   c->budget = budget ? budget : (size_t)prop.totalGlobalMem;
   c->prof = new Profiler();
-  c->trace_grid = trace_grid_blocks(device);
+  size_t gmax = 0;
+  for (int v = 0; v < trace_num_variants(); ++v) {
+    c->trace_grid[v] = trace_grid_blocks(device, v);
+    int blk, depth, ts;
+    trace_variant_info(v, &blk, &depth, &ts);
+    gmax = std::max(gmax, (size_t)c->trace_grid[v] * blk * (TRACE_MAX_STACK - depth));
+  }
   // counter slots + one trailing word of traversal error bits
   const size_t cbytes = (COUNTER_SLOTS * 2 + 1) * sizeof(uint32_t);
   int rc = ctx_alloc(c, cbytes, (void**)&c->counters);
   if (rc == RQ_OK) {
     c->errword = c->counters + COUNTER_SLOTS * 2;
     hipMemset(c->counters, 0, cbytes);
-    c->gdepth = 64 - TRACE_LDS_DEPTH;  // LBVH height <= 63 (prefix lengths 2..63)
-    size_t gbytes = (size_t)c->trace_grid * TRACE_BLOCK * c->gdepth * sizeof(int32_t);
-    rc = ctx_alloc(c, gbytes, (void**)&c->gstack);
+    rc = ctx_alloc(c, gmax * 2 * sizeof(int32_t), (void**)&c->gstack);
   }
   if (rc == RQ_OK && hipDeviceSynchronize() != hipSuccess) rc = RQ_E_HIP;
   if (rc) {
@@ -403,12 +407,17 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.occ = occ;
   a.visits = visits;
   a.tests = tests;
+  const int variant = counted ? 0 : (any ? c->any_variant : c->closest_variant);
+  int blk, depth, ts;
+  trace_variant_info(variant, &blk, &depth, &ts);
   a.gstack = c->gstack;
-  a.gdepth = c->gdepth;
+  a.gdepth = TRACE_MAX_STACK - depth;
   a.counters = next_counter_slot(c);
   a.err = c->errword;
-  a.grid = c->trace_grid;
-  hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted);
+  a.grid = c->trace_grid[variant];
+  a.refill_idle = c->refill_idle;
+  a.chunk = TRACE_RAY_CHUNK;
+  hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "trace launch");
 }
 
@@ -427,6 +436,94 @@ int rq_closest_hit_counted(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint
 int rq_any_hit_counted(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, uint8_t* occ,
                        uint32_t* visits, uint32_t* tests, void* stream) {
   return query(c, b, rays, n, nullptr, occ, visits, tests, true, true, stream);
+}
+
+int rq_trace_variant_count(void) { return trace_num_variants(); }
+
+int rq_trace_variant_get(int v, rq_trace_variant* out) {
+  if (v < 0 || v >= trace_num_variants() || !out) return RQ_E_INVALID;
+  trace_variant_info(v, &out->block, &out->lds_depth, &out->tstack);
+  return RQ_OK;
+}
+
+int rq_trace_config_set(rq_ctx* c, int closest_variant, int any_variant, uint32_t refill_idle) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  const int nv = trace_num_variants();
+  if (closest_variant < 0 || closest_variant >= nv || any_variant < 0 || any_variant >= nv ||
+      refill_idle < 1 || refill_idle > 64)
+    return set_err(c, RQ_E_INVALID, "bad traversal configuration");
+  c->closest_variant = closest_variant;
+  c->any_variant = any_variant;
+  c->refill_idle = refill_idle;
+  return RQ_OK;
+}
+
+int rq_trace_config_get(rq_ctx* c, int* closest_variant, int* any_variant, uint32_t* refill_idle) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (closest_variant) *closest_variant = c->closest_variant;
+  if (any_variant) *any_variant = c->any_variant;
+  if (refill_idle) *refill_idle = c->refill_idle;
+  return RQ_OK;
+}
+
+int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, int any,
+                      int repeats, int* best_variant, uint32_t* best_refill, double* best_ms) {
+  if (check_ctx(c) || !b || !rays || n == 0 || repeats < 1) return RQ_E_INVALID;
+  // scratch outputs for the timed launches
+  void *out0 = nullptr, *out1 = nullptr;
+  int rc = ctx_alloc(c, (size_t)n * 4, &out0);
+  if (!rc) rc = ctx_alloc(c, (size_t)n * 4, &out1);
+  if (rc) {
+    ctx_free(c, out0);
+    return rc;
+  }
+  const int saved_c = c->closest_variant, saved_a = c->any_variant;
+  const uint32_t saved_r = c->refill_idle;
+  const bool saved_prof = c->prof_on;
+  c->prof_on = false;
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  double best = 1e30;
+  int bv = 0;
+  uint32_t br = 16;
+  const uint32_t refills[] = {8, 16, 32, 48};
+  for (int v = 0; v < trace_num_variants() && !rc; ++v) {
+    int blk, depth, ts;
+    trace_variant_info(v, &blk, &depth, &ts);
+    if (any && ts) continue;  // entry distances only help closest hit
+    for (uint32_t rf : refills) {
+      c->closest_variant = v, c->any_variant = v, c->refill_idle = rf;
+      rq_hits h{(int32_t*)out0, (float*)out1, nullptr, nullptr};
+      // one untimed launch, then `repeats` timed ones
+      for (int k = 0; k <= repeats && !rc; ++k) {
+        if (k == 1) (void)hipEventRecord(e0, nullptr);
+        rc = any ? rq_any_hit(c, b, rays, n, (uint8_t*)out0, nullptr)
+                 : rq_closest_hit(c, b, rays, n, &h, nullptr);
+      }
+      (void)hipEventRecord(e1, nullptr);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (!rc && ms / repeats < best) best = ms / repeats, bv = v, br = rf;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ctx_free(c, out0);
+  ctx_free(c, out1);
+  c->prof_on = saved_prof;
+  c->closest_variant = saved_c, c->any_variant = saved_a, c->refill_idle = saved_r;
+  if (rc) return rc;
+  if (any)
+    c->any_variant = bv;
+  else
+    c->closest_variant = bv;
+  c->refill_idle = br;
+  if (best_variant) *best_variant = bv;
+  if (best_refill) *best_refill = br;
+  if (best_ms) *best_ms = best;
+  return RQ_OK;
 }
 
 int rq_device_alloc(rq_ctx* c, size_t bytes, void** p) {
@@ -449,6 +546,13 @@ int rq_memcpy_d2h(rq_ctx* c, void* dst, const void* src, size_t bytes, void* str
   hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream);
   if (e == hipSuccess) e = hipStreamSynchronize((hipStream_t)stream);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "memcpy d2h");
+}
+int rq_memcpy_d2d(rq_ctx* c, void* dst, const void* src, size_t bytes, void* stream) {
+  if (check_ctx(c) || (bytes && (!dst || !src))) return RQ_E_INVALID;
+  if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+    return set_err(c, RQ_E_INVALID, "device copy needs 16-byte aligned pointers and size");
+  hipError_t e = launch_copy16(launch_ctx(c, stream), src, dst, bytes);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "device copy");
 }
 int rq_memset_d(rq_ctx* c, void* dst, int value, size_t bytes, void* stream) {
   hipError_t e = hipMemsetAsync(dst, value, bytes, (hipStream_t)stream);

@@ -13,12 +13,13 @@ struct rq_ctx {
   std::unordered_map<void*, size_t> sizes;  // live device allocations
   rq::Profiler* prof = nullptr;
   bool prof_on = false;
-  uint32_t trace_grid = 0;
+  uint32_t trace_grid[rq::TRACE_MAX_VARIANTS] = {};  // persistent grid per kernel variant
+  int closest_variant = 0, any_variant = 0;           // chosen traversal variants
+  uint32_t refill_idle = 16;
   uint32_t* counters = nullptr;  // COUNTER_SLOTS x {fetch, exit}
   uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
   uint32_t counter_seq = 0;
-  int32_t* gstack = nullptr;  // traversal stack overflow
-  uint32_t gdepth = 0;
+  int32_t* gstack = nullptr;  // traversal stack overflow ({node, distance} pairs)
 };
 
 struct rq_scene {

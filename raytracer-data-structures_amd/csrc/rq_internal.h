@@ -51,11 +51,15 @@ constexpr uint32_t LB_INC = 2u << 30;
 constexpr uint32_t LB_MASK = (1u << 30) - 1;
 constexpr uint32_t MAX_PRIMS = (1u << 30) - 1;
 
-// Traversal: per-lane LDS stack depth; deeper entries go to a global overflow
-// area laid out [depth][thread] for coalescing.
-constexpr int TRACE_BLOCK = 128;
-constexpr int TRACE_LDS_DEPTH = 24;
+// Traversal: the LBVH of 30-bit codes is at most 63 levels deep (prefix
+// lengths 2..63 along a path), so 64 stack entries per lane always suffice;
+// kernel variants keep 12..32 of them in LDS and the rest in a global
+// overflow area ([depth][thread] pairs of {node, entry distance}).
+constexpr int TRACE_MAX_STACK = 64;
+constexpr int TRACE_MIN_LDS_DEPTH = 12;
+constexpr int TRACE_MAX_VARIANTS = 16;
 constexpr int COUNTER_SLOTS = 64;
+constexpr uint32_t TRACE_RAY_CHUNK = 256;  // rays per wave slice of the ray stream
 
 // Build workspace (device), all sized for n primitives.
 struct BuildBuffers {
@@ -121,10 +125,16 @@ struct TraceArgs {
   uint32_t gdepth;      // overflow entries per thread
   uint32_t* counters;   // 2 words: fetch, exit (reset by the last wave)
   uint32_t* err;        // set to 1 if a ray needed a deeper stack than provided
-  uint32_t grid;        // blocks
+  uint32_t grid;        // blocks (persistent grid of the chosen variant)
+  uint32_t refill_idle; // idle lanes in a wave before it refills from its slice
+  uint32_t chunk;       // rays per slice taken from the global counter
 };
-hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted);
-uint32_t trace_grid_blocks(int device);
+hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
+                        int variant);
+int trace_num_variants();
+void trace_variant_info(int v, int* block, int* depth, int* tstack);
+uint32_t trace_grid_blocks(int device, int variant);
+hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64_t bytes);
 
 // Profiler hooks (rq_api.cpp)
 void prof_begin(Profiler* p, hipStream_t s, const char* name, hipEvent_t* ev);

@@ -7,10 +7,10 @@
 // so far. The result is the (t, primitive id) lexicographic minimum, i.e. the
 // NONE loop of main.cpp:373-387 (lowest index wins ties).
 //
-// Kernel shape: persistent waves pull rays from a global counter 64 at a
-// time; when enough lanes have finished (wave ballot), the wave refills the
-// idle lanes from the counter (ray compaction), so long rays do not hold a
-// wave of finished lanes. The per-lane stack lives in LDS ([depth][lane],
+// Kernel shape: persistent waves take slices of the ray stream from a global
+// counter; when enough lanes have finished (wave ballot), the wave refills the
+// idle lanes from its slice (ray compaction), so long rays do not hold a wave
+// of finished lanes. The per-lane stack lives in LDS ([depth][lane],
 // conflict-free) and spills to a global [depth][thread] area past
 // TRACE_LDS_DEPTH entries.
 #include <algorithm>
@@ -72,16 +72,29 @@ __device__ inline bool mt(const TriRec& T, const RayState& r, float& t, float& u
   return true;
 }
 
-template <bool ANY, bool COUNT>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(TraceArgs a) {
-  __shared__ int32_t s_stack[TRACE_LDS_DEPTH * TRACE_BLOCK];
+// Persistent traversal kernel. BLOCK threads per block; DEPTH stack entries
+// per lane in LDS ([depth][lane], conflict-free), deeper entries in the global
+// overflow area ([depth][thread]); TSTACK: entries carry their entry distance
+// and a closest-hit pop skips entries now beyond the best hit (the box test
+// redone against the current best). Leaves reached from a node are tested in
+// the same loop iteration; the order of node visits and triangle tests is the
+// canonical one unless TSTACK skips an entry.
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+  __shared__ int32_t s_node[DEPTH * BLOCK];
+  __shared__ float s_tn[TSTACK ? DEPTH * BLOCK : 1];
   const int tid = threadIdx.x, lane = tid & 63;
-  const uint32_t gtid = blockIdx.x * TRACE_BLOCK + tid;
-  const uint32_t gstride = gridDim.x * TRACE_BLOCK;
+  const uint32_t gtid = blockIdx.x * BLOCK + tid;
+  const uint32_t gstride = gridDim.x * BLOCK;
   const uint64_t lt = (1ull << lane) - 1ull;
+  const int max_sp = DEPTH + (int)a.gdepth;
 
   RayState r;
-  bool active = false, exhausted = false;
+  bool active = false;
+  // wave-private slice of the ray stream [cnext, cend), refilled CHUNK rays at
+  // a time from the global counter (one atomic per slice, not per refill)
+  uint32_t cnext = 0, cend = 0;
+  bool wave_out = false;  // the global stream is exhausted for this wave
   uint32_t ray = 0;
   int32_t node = 0, best = -1;
   int sp = 0;
@@ -103,100 +116,119 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(TraceArgs a) {
     }
     active = false;
   };
-  const int max_sp = TRACE_LDS_DEPTH + (int)a.gdepth;
-  auto push = [&](int32_t c) {
-    if (sp < TRACE_LDS_DEPTH)
-      s_stack[sp * TRACE_BLOCK + tid] = c;
-    else if (sp < max_sp)
-      a.gstack[(size_t)(sp - TRACE_LDS_DEPTH) * gstride + gtid] = c;
-    else
+  auto push = [&](int32_t c, float tn) {
+    if (sp < DEPTH) {
+      s_node[sp * BLOCK + tid] = c;
+      if (TSTACK) s_tn[sp * BLOCK + tid] = tn;
+    } else if (sp < max_sp) {
+      const size_t g = (size_t)(sp - DEPTH) * gstride + gtid;
+      a.gstack[2 * g] = c;
+      if (TSTACK) a.gstack[2 * g + 1] = __float_as_int(tn);
+    } else {
       atomicOr(a.err, 1u);  // deeper than any LBVH of 30-bit codes: reported
+    }
     sp = min(sp + 1, max_sp);
   };
+  // Pops the next entry into `node`; false when the stack is empty.
   auto pop = [&]() -> bool {
-    if (sp == 0) return false;
-    --sp;
-    node = sp < TRACE_LDS_DEPTH ? s_stack[sp * TRACE_BLOCK + tid]
-                                : a.gstack[(size_t)(sp - TRACE_LDS_DEPTH) * gstride + gtid];
-    return true;
+    while (sp > 0) {
+      --sp;
+      float tn = 0.0f;
+      if (sp < DEPTH) {
+        node = s_node[sp * BLOCK + tid];
+        if (TSTACK) tn = s_tn[sp * BLOCK + tid];
+      } else {
+        const size_t g = (size_t)(sp - DEPTH) * gstride + gtid;
+        node = a.gstack[2 * g];
+        if (TSTACK) tn = __int_as_float(a.gstack[2 * g + 1]);
+      }
+      if (!TSTACK || ANY || tn <= bt * SLAB_SLACK) return true;
+    }
+    return false;
   };
 
   for (;;) {
-    // ---- refill idle lanes from the global ray counter -------------------
-    const bool need = !active && !exhausted;
-    const uint64_t m = __ballot(need);
-    if (m) {
-      const int first = __ffsll((unsigned long long)m) - 1;
-      uint32_t base = 0;
-      if (lane == first) base = atomicAdd(&a.counters[0], (uint32_t)__popcll(m));
-      base = __shfl(base, first, 64);
-      if (need) {
-        const uint32_t rr = base + (uint32_t)__popcll(m & lt);
-        if (rr >= a.nrays) {
-          exhausted = true;
-        } else {
-          ray = rr;
-          r.ox = a.rays.ox[rr], r.oy = a.rays.oy[rr], r.oz = a.rays.oz[rr];
-          r.dx = a.rays.dx[rr], r.dy = a.rays.dy[rr], r.dz = a.rays.dz[rr];
-          r.tmin = a.rays.tmin[rr], r.tmax = a.rays.tmax[rr];
-          r.ix = 1.0f / r.dx, r.iy = 1.0f / r.dy, r.iz = 1.0f / r.dz;
-          r.sx = signbit(r.ix), r.sy = signbit(r.iy), r.sz = signbit(r.iz);
-          best = -1, bt = r.tmax, bu = 0.0f, bv = 0.0f, vis = 0, tst = 0, sp = 0;
-          node = a.nprims == 1 ? ~0 : 0;
-          active = true;
-          if (!(r.tmin < r.tmax) || a.nprims == 0) finish();
+    // ---- refill idle lanes from the wave's slice of the ray stream -------
+    uint64_t m = __ballot(!active);
+    for (int round = 0; round < 2 && m && !wave_out; ++round) {
+      if (cnext >= cend) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&a.counters[0], a.chunk);
+        base = __shfl(base, 0, 64);
+        if (base >= a.nrays) {
+          wave_out = true;
+          break;
         }
+        cnext = base;
+        cend = min(base + a.chunk, a.nrays);
       }
+      const uint32_t take = min((uint32_t)__popcll(m), cend - cnext);
+      const uint32_t rank = (uint32_t)__popcll(m & lt);
+      if (!active && rank < take) {
+        const uint32_t rr = cnext + rank;
+        ray = rr;
+        r.ox = a.rays.ox[rr], r.oy = a.rays.oy[rr], r.oz = a.rays.oz[rr];
+        r.dx = a.rays.dx[rr], r.dy = a.rays.dy[rr], r.dz = a.rays.dz[rr];
+        r.tmin = a.rays.tmin[rr], r.tmax = a.rays.tmax[rr];
+        r.ix = 1.0f / r.dx, r.iy = 1.0f / r.dy, r.iz = 1.0f / r.dz;
+        r.sx = signbit(r.ix), r.sy = signbit(r.iy), r.sz = signbit(r.iz);
+        best = -1, bt = r.tmax, bu = 0.0f, bv = 0.0f, vis = 0, tst = 0, sp = 0;
+        node = a.nprims == 1 ? ~0 : 0;
+        active = true;
+        if (!(r.tmin < r.tmax) || a.nprims == 0) finish();
+      }
+      cnext += take;
+      m = __ballot(!active);
     }
-    if (__ballot(active) == 0ull && __ballot(!exhausted) == 0ull) break;
+    if (__ballot(active) == 0ull && wave_out) break;
 
     // ---- traverse until enough lanes are idle to be worth a refill -------
     for (;;) {
-      if (active) {
-        if (node >= 0) {
-          const float4* q = reinterpret_cast<const float4*>(a.nodes + node);
-          const float4 lo0 = q[0], hi0 = q[1], lo1 = q[2], hi1 = q[3];
-          if (COUNT) vis += 2;
-          float tn0, tn1;
-          const bool h0 = slab(lo0, hi0, r, bt, tn0);
-          const bool h1 = slab(lo1, hi1, r, bt, tn1);
-          const int32_t c0 = __float_as_int(lo0.w), c1 = __float_as_int(hi0.w);
-          if (h0 && h1) {
-            const bool swap = tn1 < tn0;
-            push(swap ? c0 : c1);
-            node = swap ? c1 : c0;
-          } else if (h0) {
-            node = c0;
-          } else if (h1) {
-            node = c1;
-          } else if (!pop()) {
-            finish();
-          }
-        } else {
-          const TriRec T = a.tris[~node];
-          if (COUNT) ++tst;
-          float t, u, v;
-          bool done = false;
-          if (mt(T, r, t, u, v)) {
-            const int32_t prim = __float_as_int(T.v0.w);
-            if (ANY) {
-              best = prim, bt = t;
-              done = true;
-            } else if (best < 0 || t < bt || (t == bt && prim < best)) {
-              best = prim, bt = t, bu = u, bv = v;
-            }
-          }
-          if (done || !pop()) finish();
+      if (active && node >= 0) {
+        const float4* q = reinterpret_cast<const float4*>(a.nodes + node);
+        const float4 lo0 = q[0], hi0 = q[1], lo1 = q[2], hi1 = q[3];
+        if (COUNT) vis += 2;
+        float tn0, tn1;
+        const bool h0 = slab(lo0, hi0, r, bt, tn0);
+        const bool h1 = slab(lo1, hi1, r, bt, tn1);
+        const int32_t c0 = __float_as_int(lo0.w), c1 = __float_as_int(hi0.w);
+        if (h0 && h1) {
+          const bool swap = tn1 < tn0;
+          push(swap ? c0 : c1, swap ? tn0 : tn1);
+          node = swap ? c1 : c0;
+        } else if (h0) {
+          node = c0;
+        } else if (h1) {
+          node = c1;
+        } else if (!pop()) {
+          finish();
         }
       }
-      const uint64_t idle = __ballot(!active && !exhausted);
-      if (__popcll(idle) >= REFILL_MIN_IDLE || __ballot(active) == 0ull) break;
+      // leaves reached from here are tested before the next node fetch
+      while (active && node < 0) {
+        const TriRec T = a.tris[~node];
+        if (COUNT) ++tst;
+        float t, u, v;
+        bool done = false;
+        if (mt(T, r, t, u, v)) {
+          const int32_t prim = __float_as_int(T.v0.w);
+          if (ANY) {
+            best = prim, bt = t;
+            done = true;
+          } else if (best < 0 || t < bt || (t == bt && prim < best)) {
+            best = prim, bt = t, bu = u, bv = v;
+          }
+        }
+        if (done || !pop()) finish();
+      }
+      const uint64_t idle = __ballot(!active);
+      if (__ballot(active) == 0ull || (!wave_out && __popcll(idle) >= a.refill_idle)) break;
     }
   }
 
   // last wave out resets the counters for the next launch on this slot
   if (lane == 0) {
-    const uint32_t waves = gridDim.x * (TRACE_BLOCK / 64);
+    const uint32_t waves = gridDim.x * (BLOCK / 64);
     const uint32_t old = atomicAdd(&a.counters[1], 1u);
     if (old == waves - 1) {
       atomicExch(&a.counters[0], 0u);
@@ -205,36 +237,63 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(TraceArgs a) {
   }
 }
 
+typedef void (*TraceFn)(TraceArgs);
+struct Variant {
+  int block, depth;
+  bool tstack;
+  TraceFn closest, any;
+};
+
+#define RQ_VARIANT(B, D, T) \
+  { B, D, T, k_trace<false, false, B, D, T>, k_trace<true, false, B, D, false> }
+const Variant kVariants[] = {
+    RQ_VARIANT(128, 24, false), RQ_VARIANT(128, 16, false), RQ_VARIANT(64, 24, false),
+    RQ_VARIANT(256, 16, false), RQ_VARIANT(128, 24, true),  RQ_VARIANT(128, 16, true),
+    RQ_VARIANT(64, 16, true),   RQ_VARIANT(64, 32, false),  RQ_VARIANT(64, 12, true),
+};
+#undef RQ_VARIANT
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
 }  // namespace
 
-uint32_t trace_grid_blocks(int device) {
+int trace_num_variants() { return kNumVariants; }
+
+void trace_variant_info(int v, int* block, int* depth, int* tstack) {
+  const Variant& x = kVariants[v];
+  *block = x.block, *depth = x.depth, *tstack = x.tstack ? 1 : 0;
+}
+
+uint32_t trace_grid_blocks(int device, int v) {
   int cus = 0, per = 0;
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_trace<false, false>, TRACE_BLOCK, 0);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kVariants[v].closest,
+                                                     kVariants[v].block, 0);
   if (cus <= 0) cus = 256;
-  if (per <= 0) per = 4;
+  if (per <= 0) per = 2;
   return (uint32_t)(cus * per);
 }
 
-hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted) {
+hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
+                        int variant) {
   if (a.nrays == 0) return hipSuccess;
-  uint32_t need = (uint32_t)((a.nrays + TRACE_BLOCK - 1) / TRACE_BLOCK);
+  // the instrumented kernels always run the canonical configuration (variant 0)
+  const Variant& V = kVariants[counted ? 0 : variant];
+  uint32_t need = (uint32_t)((a.nrays + V.block - 1) / V.block);
   uint32_t grid = std::max(1u, std::min(a.grid, need));
-  TraceArgs b = a;
-  dim3 g(grid), blk(TRACE_BLOCK);
+  dim3 g(grid), blk(V.block);
   hipStream_t s = lc.stream;
   if (any) {
     ProfScope ps(lc, counted ? "trace_any_counted" : "trace_any");
     if (counted)
-      hipLaunchKernelGGL((k_trace<true, true>), g, blk, 0, s, b);
+      hipLaunchKernelGGL((k_trace<true, true, 128, 24, false>), g, blk, 0, s, a);
     else
-      hipLaunchKernelGGL((k_trace<true, false>), g, blk, 0, s, b);
+      hipLaunchKernelGGL(V.any, g, blk, 0, s, a);
   } else {
     ProfScope ps(lc, counted ? "trace_closest_counted" : "trace_closest");
     if (counted)
-      hipLaunchKernelGGL((k_trace<false, true>), g, blk, 0, s, b);
+      hipLaunchKernelGGL((k_trace<false, true, 128, 24, false>), g, blk, 0, s, a);
     else
-      hipLaunchKernelGGL((k_trace<false, false>), g, blk, 0, s, b);
+      hipLaunchKernelGGL(V.closest, g, blk, 0, s, a);
   }
   return hipGetLastError();
 }

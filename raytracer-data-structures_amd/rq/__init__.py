@@ -71,6 +71,10 @@ class BvhInfo(C.Structure):
                 ("root_hi", C.c_float * 3), ("device_bytes", C.c_size_t)]
 
 
+class TraceVariant(C.Structure):
+    _fields_ = [("block", C.c_int32), ("lds_depth", C.c_int32), ("tstack", C.c_int32)]
+
+
 class Mesh(C.Structure):
     _fields_ = [("vx", C.POINTER(C.c_float)), ("vy", C.POINTER(C.c_float)),
                 ("vz", C.POINTER(C.c_float)), ("nv", C.c_uint32),
@@ -110,10 +114,17 @@ def _load() -> C.CDLL:
         "rq_closest_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp, vp,
                                          vp]),
         "rq_any_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp, vp, vp]),
+        "rq_trace_variant_count": (i32, []),
+        "rq_trace_variant_get": (i32, [i32, C.POINTER(TraceVariant)]),
+        "rq_trace_config_set": (i32, [vp, i32, i32, u32]),
+        "rq_trace_config_get": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(u32)]),
+        "rq_autotune_trace": (i32, [vp, vp, C.POINTER(Rays), u32, i32, i32, C.POINTER(i32),
+                                    C.POINTER(u32), C.POINTER(C.c_double)]),
         "rq_device_alloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
         "rq_device_free": (i32, [vp, vp]),
         "rq_memcpy_h2d": (i32, [vp, vp, vp, C.c_size_t, vp]),
         "rq_memcpy_d2h": (i32, [vp, vp, vp, C.c_size_t, vp]),
+        "rq_memcpy_d2d": (i32, [vp, vp, vp, C.c_size_t, vp]),
         "rq_memset_d": (i32, [vp, vp, i32, C.c_size_t, vp]),
         "rq_sync": (i32, [vp, vp]),
         "rq_device_errors": (i32, [vp, C.POINTER(u32)]),
@@ -471,6 +482,9 @@ class Context:
                 jd.free()
         return r
 
+    def copy(self, dst: "DeviceArray", src: "DeviceArray", nbytes: int, stream=None) -> None:
+        self._check(lib().rq_memcpy_d2d(self.h, dst.ptr, src.ptr, nbytes, stream))
+
     def sync(self, stream=None) -> None:
         self._check(lib().rq_sync(self.h, stream))
 
@@ -486,6 +500,34 @@ class Context:
 
     def stream_destroy(self, s) -> None:
         self._check(lib().rq_stream_destroy(self.h, s))
+
+    # traversal variants + autotuner
+    @staticmethod
+    def trace_variants() -> list[dict]:
+        out = []
+        for i in range(lib().rq_trace_variant_count()):
+            v = TraceVariant()
+            lib().rq_trace_variant_get(i, C.byref(v))
+            out.append(dict(index=i, block=v.block, lds_depth=v.lds_depth, tstack=v.tstack))
+        return out
+
+    def trace_config(self, closest: int | None = None, any_hit: int | None = None,
+                     refill_idle: int | None = None) -> tuple[int, int, int]:
+        c, a, r = C.c_int(), C.c_int(), C.c_uint32()
+        self._check(lib().rq_trace_config_get(self.h, C.byref(c), C.byref(a), C.byref(r)))
+        nc = c.value if closest is None else closest
+        na = a.value if any_hit is None else any_hit
+        nr = r.value if refill_idle is None else refill_idle
+        self._check(lib().rq_trace_config_set(self.h, nc, na, nr))
+        return nc, na, nr
+
+    def autotune(self, bvh: int, rays: DeviceRays, any_hit: bool = False,
+                 repeats: int = 3) -> tuple[int, int, float]:
+        v, rf, ms = C.c_int(), C.c_uint32(), C.c_double()
+        r = rays.struct()
+        self._check(lib().rq_autotune_trace(self.h, bvh, C.byref(r), rays.n, 1 if any_hit else 0,
+                                            repeats, C.byref(v), C.byref(rf), C.byref(ms)))
+        return v.value, rf.value, ms.value
 
     # per-kernel timing
     def profile(self, on: bool = True) -> None:

@@ -133,3 +133,52 @@ def test_query_soup_random(ctx, O):
     assert np.array_equal(g["prim"], bf["prim"])
     ctx.release(b)
     ctx.release(s)
+
+
+def test_all_trace_variants_identical(ctx, O, rq):
+    """Every traversal kernel variant (block size, LDS stack depth, entry
+    distances on the stack, refill threshold) returns the oracle's answers."""
+    mesh = rq.icosphere(5, 4, center=(0, 0, -5))
+    rays = concat_rays(rays_toward(6000, 31), random_rays(3000, 32), edge_rays())
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    ob = O.OracleBVH(mesh, 1)
+    o = ob.closest(rays)
+    oa = ob.any(rays)
+    saved = ctx.trace_config()
+    try:
+        for v in ctx.trace_variants():
+            for refill in (1, 16, 64):
+                log(f"variant {v} refill {refill}")
+                ctx.trace_config(v["index"], v["index"] if not v["tstack"] else 0, refill)
+                g = ctx.closest_hit(b, rays)
+                assert np.array_equal(g["prim"], o["prim"]), v
+                assert np.array_equal(_bits(g["t"]), _bits(o["t"])), v
+                assert np.array_equal(_bits(g["u"]), _bits(o["u"])), v
+                ga = ctx.any_hit(b, rays)
+                assert np.array_equal(ga["occ"], oa["occ"]), v
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.trace_config(*saved)
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_autotune_picks_a_valid_variant(ctx, rq):
+    mesh = rq.icosphere(5, 4, center=(0, 0, -5))
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    saved = ctx.trace_config()
+    rays = rq.DeviceRays.from_host(ctx, rays_toward(100000, 3))
+    try:
+        log("autotune")
+        v, refill, ms = ctx.autotune(b, rays, any_hit=False, repeats=2)
+        assert 0 <= v < len(ctx.trace_variants()) and refill in (8, 16, 32, 48) and ms > 0
+        assert ctx.trace_config()[0] == v
+        va, _, _ = ctx.autotune(b, rays, any_hit=True, repeats=2)
+        assert not ctx.trace_variants()[va]["tstack"]
+    finally:
+        rays.free()
+        ctx.trace_config(*saved)
+        ctx.release(b)
+        ctx.release(s)

@@ -1,0 +1,126 @@
+"""CPU: the oracle's Möller–Trumbore test (closed-form known answers), its
+canonical BVH traversal against brute force, the reference's own sphere
+test, and the CPU baseline port."""
+import numpy as np
+import pytest
+
+from helpers import (RAY_FIELDS, concat_rays, duplicate_mesh, edge_rays, mesh_configs,
+                     random_rays, random_soup, rays_toward)
+
+V0, V1, V2 = (0.0, 0.0, -5.0), (1.0, 0.0, -5.0), (0.0, 1.0, -5.0)
+
+
+@pytest.mark.parametrize("o,d,expect", [
+    ((0.25, 0.25, 0.0), (0, 0, -1), (True, 5.0, 0.25, 0.25)),
+    ((0.5, 0.0, 0.0), (0, 0, -1), (True, 5.0, 0.5, 0.0)),     # on an edge: v = 0 counts
+    ((0.0, 0.0, 0.0), (0, 0, -1), (True, 5.0, 0.0, 0.0)),     # on a vertex
+    ((0.75, 0.75, 0.0), (0, 0, -1), (False,)),                # u + v > 1
+    ((-0.25, 0.25, 0.0), (0, 0, -1), (False,)),               # u < 0
+    ((0.25, 0.25, 0.0), (1, 0, 0), (False,)),                 # parallel: |det| < EPS
+    ((0.25, 0.25, -10.0), (0, 0, 1), (True, 5.0, 0.25, 0.25)),  # from behind (no culling)
+    ((0.25, 0.25, 0.0), (0, 0, 1), (False,)),                 # t = -5 is not in (0, inf)
+])
+def test_mt_known_answers(O, o, d, expect):
+    hit, t, u, v = O.mt(V0, V1, V2, o, d)
+    assert hit == expect[0]
+    if hit:
+        assert (t, u, v) == expect[1:]
+
+
+def test_mt_interval_is_open(O):
+    o, d = (0.25, 0.25, 0.0), (0, 0, -1)
+    assert not O.mt(V0, V1, V2, o, d, 0.0, 5.0)[0]   # t == tmax excluded
+    assert not O.mt(V0, V1, V2, o, d, 5.0, 9.0)[0]   # t == tmin excluded
+    assert O.mt(V0, V1, V2, o, d, 4.999, 5.001)[0]
+
+
+def test_mt_tiny_determinant(O):
+    s = 1e-4  # edges 1e-4: det = 1e-8 < EPS (main.cpp:59) -> rejected
+    hit = O.mt((0, 0, -5), (s, 0, -5), (0, s, -5), (s / 4, s / 4, 0), (0, 0, -1))[0]
+    assert not hit
+
+
+@pytest.mark.parametrize("name", list(mesh_configs()))
+@pytest.mark.parametrize("mode", [0, 1])
+def test_bvh_equals_brute_force(O, name, mode):
+    mesh = mesh_configs()[name]()
+    rays = concat_rays(rays_toward(3000, 11), random_rays(2000, 12), edge_rays())
+    b = O.OracleBVH(mesh, mode)
+    o = b.closest(rays)
+    bf = O.closest_brute(mesh, rays)
+    assert np.array_equal(o["prim"], bf["prim"])
+    assert np.array_equal(o["t"].view(np.uint32), bf["t"].view(np.uint32))
+    assert np.array_equal(o["u"].view(np.uint32), bf["u"].view(np.uint32))
+    assert np.array_equal(b.any(rays)["occ"], O.any_brute(mesh, rays))
+
+
+def test_bvh_equals_brute_force_vertex_aimed(O, rq):
+    """Rays aimed exactly at mesh vertices (hits on edges and vertices)."""
+    mesh = rq.icosphere(5, 2, center=(0, 0, -5))
+    v = np.stack([mesh.vx, mesh.vy, mesh.vz], 1)
+    rng = np.random.default_rng(0)
+    rows = []
+    for org in [(0, 0, 0), (0, 0, -5), (0.3, -0.2, 1.0)]:
+        o = np.array(org, np.float32)
+        d = (v[rng.choice(len(v), 4000)] - o).astype(np.float32)
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rows.append(np.column_stack([np.tile(o, (len(d), 1)), d, np.zeros(len(d)),
+                                     np.full(len(d), np.inf)]).astype(np.float32))
+    a = np.concatenate(rows)
+    rays = {k: a[:, i].copy() for i, k in enumerate(RAY_FIELDS)}
+    b = O.OracleBVH(mesh, 1)
+    o = b.closest(rays)
+    bf = O.closest_brute(mesh, rays)
+    assert np.array_equal(o["prim"], bf["prim"])
+    assert np.array_equal(o["t"].view(np.uint32), bf["t"].view(np.uint32))
+
+
+def test_ties_lowest_index(O):
+    mesh = duplicate_mesh(50)
+    rays = rays_toward(500, 5, spread=0.5)
+    o = O.OracleBVH(mesh, 1).closest(rays)
+    assert (o["prim"][o["prim"] >= 0] == 0).all() and (o["prim"] >= 0).sum() > 100
+
+
+def test_canonical_counts(O, rq):
+    """Visits count two node records per internal node; a miss of the root's
+    children costs exactly 2 visits and no test."""
+    mesh = rq.icosphere(2, 1, center=(0, 0, -5))
+    rays = rays_toward(1, 0, target=(0, 0, 50), spread=0.0)  # points away from the mesh
+    o = O.OracleBVH(mesh, 1).closest(rays)
+    assert o["prim"][0] == -1 and o["visits"][0] == 2 and o["tests"][0] == 0
+    o = O.OracleBVH(mesh, 1).closest(rays_toward(500, 1))
+    assert (o["visits"] % 2 == 0).all() and (o["tests"] >= (o["prim"] >= 0)).all()
+
+
+def test_sphere_test_matches_reference(O):
+    ref = O.ref()
+    if ref is None:
+        pytest.skip("reference build (oracle/_ref) not present")
+    import ctypes as C
+
+    rng = np.random.default_rng(3)
+    for _ in range(3000):
+        c = rng.uniform(-5, 5, 3).astype(np.float32)
+        r = np.float32(rng.uniform(0.05, 3))
+        o = rng.uniform(-5, 5, 3).astype(np.float32)
+        d = rng.normal(size=3)
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        a0, a1, b0, b1 = C.c_float(), C.c_float(), C.c_float(), C.c_float()
+        h1 = O.lib().or_ray_sphere(O._p(c), float(r), O._p(o), O._p(d), C.byref(a0),
+                                   C.byref(a1))
+        h2 = ref.ref_ray_sphere(O._p(c), float(r), O._p(o), O._p(d), C.byref(b0), C.byref(b1))
+        assert h1 == h2
+        if h1:
+            assert (a0.value, a1.value) == (b0.value, b1.value)
+
+
+def test_port_baseline_matches_brute_force(O):
+    """The CPU baseline (reference median-split BVH + collect-then-test) finds
+    the same closest triangle wherever the closest t is unique."""
+    mesh = random_soup(3000, 21, spread=1.0, size=0.05)
+    rays = random_rays(3000, 22, lo=(-1.2, -1.2, -1.2), hi=(1.2, 1.2, 1.2))
+    p = O.PortBVH(mesh).closest(rays, threads=4)
+    bf = O.closest_brute(mesh, rays)
+    assert np.array_equal(p["prim"], bf["prim"])
+    assert O.PortBVH(mesh).n_nodes == 2 * 3000 - 1
