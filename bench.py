@@ -79,6 +79,170 @@ def counted_batch(ctx, bvh, rays, any_hit: bool):
     return int(v.sum()), int(t.sum())
 
 
+COPY_PROBE_BYTES = 1 << 30
+KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<true, false"}
+
+
+def copy_probe(ctx) -> None:
+    """A library copy kernel over a known byte count: the calibration of the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    src = rq.DeviceArray(ctx, COPY_PROBE_BYTES)
+    dst = rq.DeviceArray(ctx, COPY_PROBE_BYTES)
+    ctx.copy(dst, src, COPY_PROBE_BYTES)
+    ctx.sync()
+    src.free()
+    dst.free()
+
+
+def _pmc_run(counter: str, config: str) -> dict | None:
+    """One rocprofv3 --pmc child run of this script in probe mode; returns the
+    counter's mean per dispatch (KiB) for each kernel name."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    import tempfile
+
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not Path(prof).exists():
+        return None
+    out = tempfile.mkdtemp(prefix="rq_pmc_", dir="/tmp")
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "probe", "--",
+           sys.executable, str(ROOT / "bench.py"), "--pmc-probe", "--config", config,
+           "--no-cpu", "--traffic", "off", "--warmup", "1"]
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        subprocess.run(cmd, cwd="/tmp", env=env, timeout=240, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+    except (subprocess.SubprocessError, OSError) as e:
+        log(f"pmc {counter}: {e}")
+        return None
+    files = glob.glob(f"{out}/**/*counter_collection.csv", recursive=True)
+    if not files:
+        return None
+    sums, counts = {}, {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            if row.get("Counter_Name") != counter:
+                continue
+            k = row["Kernel_Name"]
+            sums[k] = sums.get(k, 0.0) + float(row["Counter_Value"])
+            counts[k] = counts.get(k, 0) + 1
+    shutil.rmtree(out, ignore_errors=True)
+    return {k: sums[k] / counts[k] for k in sums}
+
+
+def measure_traffic(config: str, kernel: str):
+    """HBM bytes per launch of `kernel`: FETCH_SIZE and WRITE_SIZE per dispatch,
+    each scaled by known bytes / counter reading of the copy probe."""
+    sym = KERNEL_SYMBOL.get(kernel)
+    if sym is None:
+        return None
+    total = 0.0
+    detail = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        res = _pmc_run(counter, config)
+        if not res:
+            return None
+        kv = [v for k, v in res.items() if sym in k]
+        cv = [v for k, v in res.items() if "k_copy16" in k]
+        if not kv or not cv or cv[0] <= 0:
+            return None
+        scale = (COPY_PROBE_BYTES / 1024.0) / cv[0]
+        nbytes = kv[0] * 1024.0 * scale
+        detail[counter] = {"kib_per_dispatch": round(kv[0], 1), "copy_scale": round(scale, 4),
+                           "bytes": int(nbytes)}
+        total += nbytes
+    log(f"traffic {kernel}: {detail}")
+    return int(total)
+
+
+def run_c3(ctx, args) -> int:
+    """Raw ray batch: 256 displaced icospheres of 20,480 triangles (5,242,880)
+    with Poisson-disk centres in [0,100]^3; 2^24 rays with origins uniform in
+    the scene box inflated by 10% per side and directions uniform on the
+    sphere; one step = closest hit + any hit over the same rays."""
+    mesh = rq.icosphere_field(256, 5, 3, 3, 100.0, 2.5)
+    log(f"c3_raw_ray_batch: {mesh.ntri} triangles")
+    lo, hi = mesh.bounds()
+    ext = hi - lo
+    lo = (lo - ext * np.float32(0.1)).astype(np.float32)
+    hi = (hi + ext * np.float32(0.1)).astype(np.float32)
+    n = 1 << 24
+    scene = ctx.upload_scene(mesh)
+    bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
+    rays = ctx.raygen_uniform(n, 3, lo, hi)
+    hits = {k: rq.DeviceArray(ctx, 4 * n) for k in ("prim", "t")}
+    occ = rq.DeviceArray(ctx, n)
+    vc, tc = counted_batch(ctx, bvh, rays, any_hit=False)
+    va, ta = counted_batch(ctx, bvh, rays, any_hit=True)
+    tuned = None
+    if not args.no_autotune:
+        c_v, c_r, _ = ctx.autotune(bvh, rays, any_hit=False, repeats=2)
+        a_v, a_r, _ = ctx.autotune(bvh, rays, any_hit=True, repeats=2)
+        ctx.trace_config(c_v, a_v, c_r)
+        tuned = {"closest_variant": c_v, "any_variant": a_v, "refill_idle": c_r}
+    for _ in range(args.warmup):
+        ctx.closest_hit_dev(bvh, rays, hits)
+        ctx.any_hit_dev(bvh, rays, occ)
+    ctx.sync()
+    if args.pmc_probe:
+        ctx.closest_hit_dev(bvh, rays, hits)
+        ctx.any_hit_dev(bvh, rays, occ)
+        ctx.sync()
+        copy_probe(ctx)
+        return 0
+    ctx.profile(True)
+    ctx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.closest_hit_dev(bvh, rays, hits)
+        ctx.any_hit_dev(bvh, rays, occ)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    prof = ctx.profile_read()
+    assert ctx.device_errors() == 0
+    elapsed = t1 - t0
+    prim = hits["prim"].download(np.int32, n)
+    occ_h = occ.download(np.uint8, n)
+    cl_ms = prof["trace_closest"][0] / prof["trace_closest"][1]
+    an_ms = prof["trace_any"][0] / prof["trace_any"][1]
+    b_closest = n * (32 + 8) + 32 * vc + 36 * tc
+    b_any = n * (32 + 1) + 32 * va + 36 * ta
+    dom = "trace_closest" if cl_ms >= an_ms else "trace_any"
+    alg, dms = (b_closest, cl_ms) if dom == "trace_closest" else (b_any, an_ms)
+    achieved = alg / (dms * 1e-3) / 1e9
+    roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    bytes_per_launch=int(alg), avg_launch_ms=round(dms, 4))
+    if args.traffic == "auto":
+        roofline["traffic"] = measure_traffic("c3", dom)
+    line = {
+        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
+        "value": round(2 * n * args.steps / elapsed / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c3_raw_ray_batch", "triangles": mesh.ntri, "rays": n,
+                   "queries": "closest+any", "closest_hit_rate": round(float((prim >= 0).mean()), 4),
+                   "occluded_rate": round(float(occ_h.mean()), 4)},
+        "closest_hit_mrays_s": round(n / (cl_ms * 1e-3) / 1e6, 1),
+        "any_hit_mrays_s": round(n / (an_ms * 1e-3) / 1e6, 1),
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
+        "traversal_counts": {"closest": {"node_visits": vc, "tri_tests": tc},
+                             "any": {"node_visits": va, "tri_tests": ta}},
+        "autotune": tuned, "roofline": roofline, "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        host = rays.download()
+        threads = max(1, min(16, os.cpu_count() or 1))
+        line["cpu_baseline"] = cpu_baseline(mesh, [host], 64, threads)
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
 def sweep(ctx, bvh, batches) -> None:
     """Time each traversal variant / refill threshold on this frame's batches."""
     import ctypes as C
@@ -147,7 +311,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--traffic", default="auto", choices=["auto", "off"],
+                    help="measure HBM bytes per launch with two rocprofv3 --pmc child runs")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="(child of --traffic) run the workload briefly, then a 1 GiB copy")
     ap.add_argument("--cpu-sample", type=int, default=96, help="1/k of each ray batch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
@@ -171,6 +339,8 @@ def main() -> int:
         ctx.sync()
         print(json.dumps({"copy_bytes": nb, "launches": 3}), flush=True)
         return 0
+    if args.config == "c3":
+        return run_c3(ctx, args)
     mesh, params, workload = (c2_setup if args.config == "c2" else c1_setup)(ctx)
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)
@@ -204,6 +374,14 @@ def main() -> int:
             ctx.rebuild(bvh)
             ren.frame(bvh)
         ctx.sync()
+
+    if args.pmc_probe:
+        for _ in range(2):
+            ctx.rebuild(bvh)
+            ren.frame(bvh)
+        ctx.sync()
+        copy_probe(ctx)
+        return 0
 
     # ---- timed region: K frames, per-kernel HIP events on the launch stream
     ctx.profile(True)
@@ -246,6 +424,8 @@ def main() -> int:
     build_ms = sum(per_kernel[k][0] for k in build_kernels if k in per_kernel) / args.steps
     closest_ms = per_kernel.get("trace_closest", (0, 1))[0] / args.steps
     closest_rays_n = sum(r for _, r in closest_rays)
+    if roofline is not None and args.traffic == "auto":
+        roofline["traffic"] = measure_traffic(args.config, roofline["kernel"])
     cpu = None
     if not args.no_cpu:
         host = [batches[k].download() for k in batches]
