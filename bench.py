@@ -237,7 +237,8 @@ def run_c3(ctx, args) -> int:
     if not args.no_cpu:
         host = rays.download()
         threads = max(1, min(16, os.cpu_count() or 1))
-        line["cpu_baseline"] = cpu_baseline(mesh, [host], 64, threads)
+        line["cpu_baseline"] = cpu_baseline(mesh, [host], 64, threads,
+                                            what="of the 2^24 uniform rays (closest hit)")
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
@@ -279,7 +280,8 @@ def sweep(ctx, bvh, batches) -> None:
     ctx.trace_config(*saved)
 
 
-def cpu_baseline(mesh, batches, sample_every: int, threads: int):
+def cpu_baseline(mesh, batches, sample_every: int, threads: int,
+                 what: str = "of each batch of one frame (primary, bounce, shadow as closest hit)"):
     """The reference CPU path restated on triangles (oracle/port.cpp), timed
     on this host: median-split BVH build + collect-then-test closest hit on
     every `sample_every`-th active ray of each batch."""
@@ -300,8 +302,7 @@ def cpu_baseline(mesh, batches, sample_every: int, threads: int):
     port.closest(sample, threads=threads)
     dt = time.perf_counter() - t0
     return dict(value=round(n / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
-                sample=f"every {sample_every}th active ray of each batch of one frame "
-                       f"({n} rays: primary, bounce, shadow as closest-hit), "
+                sample=f"every {sample_every}th active ray {what}: {n} rays, "
                        f"median-split BVH built in {build_s * 1e3:.0f} ms",
                 build_ms=round(build_s * 1e3, 1), rays=n, seconds=round(dt, 2))
 

@@ -1,0 +1,60 @@
+// Internal types of the persistent batch kernel (batch_kernels.hip <-> batch.cpp).
+#pragma once
+#include "build_device.h"
+
+namespace rq {
+
+enum BatchPhase {
+  PH_INIT = 0,
+  PH_BOUNDS,
+  PH_MORTON,
+  PH_SORT0,
+  PH_SORT1,
+  PH_SORT2,
+  PH_SORT3,
+  PH_KARRAS,
+  PH_REFIT,
+  PH_TOP,
+  PH_TRACE,
+  BATCH_PHASES
+};
+
+// control words of one batch
+constexpr int BATCH_CTL_TICKET = 0;                        // BATCH_PHASES queue tickets
+constexpr int BATCH_CTL_BARRIER = BATCH_PHASES;            // grid barrier arrivals
+constexpr int BATCH_CTL_BARRIER_ERR = BATCH_PHASES + 1;    // sticky: barrier wait gave up
+constexpr int BATCH_CTL_TRACE_ERR = BATCH_PHASES + 2;      // sticky: stack deeper than provided
+constexpr int BATCH_CTL_WORDS = BATCH_PHASES + 3;
+constexpr uint32_t BATCH_ANY_BIT = 0x80000000u;            // trace item tile: any-hit query
+
+struct BatchMeshDev {
+  BuildView v;
+  rq_rays rays;
+  uint32_t nrays;
+  int32_t* prim;
+  float *t, *u, *v_;
+  uint8_t* occ;
+};
+
+struct BatchWork {
+  uint32_t mesh;
+  uint32_t tile;  // unit of the phase; trace: ray tile, BATCH_ANY_BIT for any-hit
+};
+
+struct BatchArgs {
+  const BatchMeshDev* meshes;
+  const BatchWork* items;               // every phase's items, phase after phase
+  uint32_t phase_begin[BATCH_PHASES + 1];
+  uint32_t* ctl;                        // BATCH_CTL_WORDS
+  int32_t* gstack;                      // traversal stack overflow of the persistent grid
+  uint32_t gdepth;
+  uint32_t ray_tile;
+  uint32_t refill_idle;
+};
+
+int batch_num_variants();
+void batch_variant_info(int v, int* depth, int* tstack);
+uint32_t batch_grid_blocks(int device, int v);
+hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid);
+
+}  // namespace rq

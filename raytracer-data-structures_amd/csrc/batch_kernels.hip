@@ -1,0 +1,243 @@
+// Persistent batch kernel: builds the LBVH of many meshes and traces their
+// ray batches in one launch. Blocks pull work items (mesh, tile) from a device
+// work queue, one queue segment per phase (init, bounds, Morton, four sort
+// passes, Karras, refit, refit top, trace) holding the items of every mesh;
+// a grid-wide barrier separates phases, so a mesh's phase k + 1 reads what
+// any block wrote in phase k. Within a phase the queue balances uneven meshes:
+// a block takes the next item as soon as it finishes one. Items of a sort pass
+// are queued mesh by mesh in tile order, so a tile's look-back only waits on
+// tiles already taken by running blocks.
+#include <algorithm>
+
+#include "batch_internal.h"
+#include "build_device.h"
+#include "trace_device.h"
+
+namespace rq {
+namespace {
+
+// About 4M polls (a few seconds): far beyond any phase of a resident grid.
+constexpr uint32_t BARRIER_SPIN_LIMIT = 1u << 22;
+
+// Every thread releases its writes at agent scope, the block arrives, thread 0
+// waits for all blocks and acquires before the block goes on. The wait is
+// bounded: a grid that is not fully resident sets the sticky error word, and
+// once it is set no later barrier of the launch waits at all, so such a
+// launch ends after one time-out with the error reported.
+__device__ inline void grid_barrier(uint32_t* bar, uint32_t target, uint32_t* err) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(bar, 1u);
+    uint32_t spins = 0;
+    while (ld_agent(bar) < target) {
+      if (++spins > BARRIER_SPIN_LIMIT || ((spins & 1023u) == 0u && ld_agent(err) != 0u)) {
+        atomicOr(err, 1u);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(2);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+}
+
+// Loads a block-uniform descriptor into scalar registers: every dword goes
+// through readfirstlane, so the struct is provably uniform and is never copied
+// to per-lane scratch.
+template <typename T>
+__device__ inline T load_uniform(const T* p) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized descriptor");
+  constexpr int W = sizeof(T) / 4;
+  const uint32_t* src = reinterpret_cast<const uint32_t*>(p);
+  uint32_t w[W];
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  T t;
+  __builtin_memcpy(&t, w, sizeof(T));
+  return t;
+}
+
+template <int DEPTH, bool TSTACK>
+__device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* err,
+                                  uint32_t ray_tile, uint32_t refill_idle, const BatchMeshDev& m,
+                                  uint32_t tile, uint32_t* lds, uint32_t* s_ctr) {
+  const bool any = (tile & BATCH_ANY_BIT) != 0;
+  const uint32_t r0 = (tile & ~BATCH_ANY_BIT) * ray_tile;
+  const uint32_t nr = min(ray_tile, m.nrays - r0);
+  TraceArgs t{};
+  t.nodes = m.v.B.nodes;
+  t.tris = m.v.B.tris;
+  t.nprims = m.v.n;
+  t.rays = rq_rays{m.rays.ox + r0, m.rays.oy + r0, m.rays.oz + r0, m.rays.dx + r0,
+                   m.rays.dy + r0, m.rays.dz + r0, m.rays.tmin + r0, m.rays.tmax + r0};
+  t.nrays = nr;
+  t.gstack = gstack;
+  t.gdepth = gdepth;
+  t.err = err;
+  t.refill_idle = refill_idle;
+  t.chunk = 64;
+  if (threadIdx.x == 0) *s_ctr = 0u;
+  __syncthreads();
+  int32_t* s_node = reinterpret_cast<int32_t*>(lds);
+  float* s_tn = reinterpret_cast<float*>(lds + DEPTH * BUILD_BLOCK);
+  const uint32_t gtid = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  const uint32_t gstride = gridDim.x * BUILD_BLOCK;
+  if (any) {
+    t.occ = m.occ + r0;
+    trace_loop<true, false, BUILD_BLOCK, DEPTH, false>(t, s_ctr, s_node, s_tn, gtid, gstride);
+  } else {
+    t.prim = m.prim + r0;
+    t.t = m.t ? m.t + r0 : nullptr;
+    t.u = m.u ? m.u + r0 : nullptr;
+    t.v = m.v_ ? m.v_ + r0 : nullptr;
+    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK>(t, s_ctr, s_node, s_tn, gtid, gstride);
+  }
+  __syncthreads();
+}
+
+// LDS words of the batch kernel: the largest of the lean sort tile, the
+// Morton histograms and the traversal stack.
+constexpr int cmax(int x, int y) { return x > y ? x : y; }
+constexpr int batch_lds_words(int depth, bool tstack) {
+  return cmax(cmax(sort_lds_words(SORT_ITEMS_LEAN), SORT_PASSES * 256),
+              depth * BUILD_BLOCK * (tstack ? 2 : 1));
+}
+
+// Occupancy is set for the traversal phase (the longest): WAVES waves per
+// SIMD caps the registers of every phase, and the build phases are written to
+// fit that budget (lean 8-key sort tiles, descriptors loaded per phase).
+template <int DEPTH, bool TSTACK, int WAVES>
+__global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_batch(
+    BatchArgs a) {
+  __shared__ uint32_t lds[batch_lds_words(DEPTH, TSTACK)];
+  __shared__ uint32_t s_item, s_ctr;
+  uint32_t* const ctl = a.ctl;
+  const BatchWork* const items = a.items;
+  const BatchMeshDev* const meshes = a.meshes;
+  for (int ph = 0; ph < BATCH_PHASES; ++ph) {
+    const uint32_t b0 = a.phase_begin[ph], b1 = a.phase_begin[ph + 1];  // kernarg, uniform
+    for (;;) {
+      if (b1 == b0) break;
+      if (threadIdx.x == 0) s_item = b0 + atomicAdd(ctl + BATCH_CTL_TICKET + ph, 1u);
+      __syncthreads();
+      // block-uniform: keep the item and its mesh descriptor in scalar registers
+      const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+      __syncthreads();
+      if (it >= b1) break;
+      const BatchWork w = load_uniform(items + it);
+      const BatchMeshDev* const mp = meshes + w.mesh;
+      // each phase loads the descriptor itself: fields it does not read are
+      // never loaded and hold no scalar registers
+#define VIEW const BuildView v = load_uniform(&mp->v)
+      switch (ph) {
+        case PH_INIT: {
+          VIEW;
+          build_init_part(v, threadIdx.x, BUILD_BLOCK);
+          break;
+        }
+        case PH_BOUNDS: {
+          VIEW;
+          bounds_tile(v, w.tile, lds);
+          break;
+        }
+        case PH_MORTON: {
+          VIEW;
+          morton_tile(v, w.tile, lds);
+          break;
+        }
+        case PH_SORT0: {
+          VIEW;
+          onesweep_tile<0, SORT_ITEMS_LEAN>(v, w.tile, lds);
+          break;
+        }
+        case PH_SORT1: {
+          VIEW;
+          onesweep_tile<1, SORT_ITEMS_LEAN>(v, w.tile, lds);
+          break;
+        }
+        case PH_SORT2: {
+          VIEW;
+          onesweep_tile<2, SORT_ITEMS_LEAN>(v, w.tile, lds);
+          break;
+        }
+        case PH_SORT3: {
+          VIEW;
+          onesweep_tile<3, SORT_ITEMS_LEAN>(v, w.tile, lds);
+          break;
+        }
+        case PH_KARRAS: {
+          VIEW;
+          karras_tile(v, w.tile);
+          break;
+        }
+        case PH_REFIT: {
+          VIEW;
+          refit_local_tile(v, w.tile, lds);
+          break;
+        }
+        case PH_TOP: {
+          VIEW;
+          refit_top_block(v);
+          break;
+        }
+        default: {
+          const BatchMeshDev m = load_uniform(mp);
+          trace_item<DEPTH, TSTACK>(a.gstack, a.gdepth, ctl + BATCH_CTL_TRACE_ERR, a.ray_tile,
+                                    a.refill_idle, m, w.tile, lds, &s_ctr);
+          break;
+        }
+      }
+#undef VIEW
+      __syncthreads();
+    }
+    if (ph + 1 < BATCH_PHASES)
+      grid_barrier(ctl + BATCH_CTL_BARRIER, (uint32_t)(ph + 1) * gridDim.x,
+                   ctl + BATCH_CTL_BARRIER_ERR);
+  }
+}
+
+// Clears the queue tickets, the barrier and the error words before a run (a
+// kernel, so that the reset lives in the same captured graph as the run).
+__global__ void k_batch_reset(uint32_t* ctl) {
+  if (threadIdx.x < BATCH_CTL_WORDS) ctl[threadIdx.x] = 0u;
+}
+
+typedef void (*BatchFn)(BatchArgs);
+struct BatchVariant {
+  int depth;
+  bool tstack;
+  BatchFn fn;
+};
+const BatchVariant kBatchVariants[] = {
+    {16, false, k_batch<16, false, 7>}, {20, false, k_batch<20, false, 7>},
+    {8, true, k_batch<8, true, 7>},     {10, true, k_batch<10, true, 7>},
+    {16, false, k_batch<16, false, 8>}, {8, true, k_batch<8, true, 8>},
+};
+constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+
+}  // namespace
+
+int batch_num_variants() { return kNumBatchVariants; }
+void batch_variant_info(int v, int* depth, int* tstack) {
+  *depth = kBatchVariants[v].depth;
+  *tstack = kBatchVariants[v].tstack ? 1 : 0;
+}
+
+uint32_t batch_grid_blocks(int device, int v) {
+  int cus = 0, per = 0;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kBatchVariants[v].fn, BUILD_BLOCK, 0);
+  if (cus <= 0) cus = 256;
+  if (per <= 0) per = 1;
+  return (uint32_t)(cus * per);
+}
+
+hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid) {
+  ProfScope ps(lc, "batch");
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, lc.stream, a.ctl);
+  hipLaunchKernelGGL(kBatchVariants[variant].fn, dim3(grid), dim3(BUILD_BLOCK), 0, lc.stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace rq

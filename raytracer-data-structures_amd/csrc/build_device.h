@@ -1,0 +1,551 @@
+// Block-level device functions of the LBVH build (Morton -> onesweep radix
+// sort -> Karras emission -> two-pass refit). Each function is one unit of
+// work for one block of BUILD_BLOCK (256) threads: the standalone kernels in
+// build_kernels.hip launch one block per unit, the persistent batch kernel in
+// batch_kernels.hip pulls the same units from its work queue. LDS scratch is
+// passed in so that a kernel calling several phases holds one buffer.
+//
+// Replaces constructLBVHTree (accelerators.h:570-586): its Morton step
+// (morton3D :385-394 on (centre + 30)/1000, :577), its sort (the decimal LSD
+// radixSort :348-369, which sorts a discarded copy) and its tree step (the
+// median split :452-564); results are bit-exact with the CPU restatement in
+// oracle/.
+#pragma once
+#include "rq_internal.h"
+
+namespace rq {
+
+constexpr int BUILD_BLOCK = 256;
+constexpr int TRI_TILE = 4096;   // triangles per bounds / Morton unit (16 per thread)
+constexpr int NODE_TILE = 4096;  // internal nodes per Karras unit
+// LDS words needed by the largest phase (the onesweep tile)
+__host__ __device__ constexpr int sort_lds_words(int items) {
+  return SORT_WAVES * 256 + SORT_BLOCK * items + 256 + 256 + 2 * SORT_WAVES;
+}
+constexpr int BUILD_LDS_WORDS = sort_lds_words(SORT_ITEMS);
+
+// Everything one mesh build reads and writes.
+struct BuildView {
+  const float *vx, *vy, *vz;
+  const uint32_t* idx;
+  uint32_t n;
+  int mode;
+  BuildBuffers B;
+};
+
+// ---- helpers ------------------------------------------------------------------
+// Element i of a per-primitive array (i < MAX_PRIMS < 2^30): the byte offset
+// fits 32 bits, so the access takes a scalar base and a 32-bit lane offset
+// (one VGPR per address instead of a 64-bit pair).
+template <typename T>
+__device__ inline T& el32(T* base, uint32_t i) {
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ inline const T& el32(const T* base, uint32_t i) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                     (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+__device__ inline uint32_t f2ord(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Spread the 10 low bits of v to bit positions 3k (shift-or-mask form).
+__device__ inline uint32_t spread3(uint32_t v) {
+  v &= 0x3FFu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// morton3D semantics (accelerators.h:387-393): clamp(a*1024, 0, 1023) with
+// the std::max/std::min comparisons, truncate, x is the high bit of a triple.
+__device__ inline uint32_t quant10(float a) {
+  a = a * 1024.0f;
+  a = (a < 0.0f) ? 0.0f : a;
+  a = (1023.0f < a) ? 1023.0f : a;
+  return (a != a) ? 0u : (uint32_t)a;
+}
+__device__ inline uint32_t morton30(float x, float y, float z) {
+  return (spread3(quant10(x)) << 2) | (spread3(quant10(y)) << 1) | spread3(quant10(z));
+}
+
+__device__ inline float3 centroid(const BuildView& v, uint32_t t) {
+  const uint32_t a = v.idx[3 * t], b = v.idx[3 * t + 1], c = v.idx[3 * t + 2];
+  float3 r;
+  r.x = ((v.vx[a] + v.vx[b]) + v.vx[c]) / 3.0f + 0.0f;
+  r.y = ((v.vy[a] + v.vy[b]) + v.vy[c]) / 3.0f + 0.0f;
+  r.z = ((v.vz[a] + v.vz[b]) + v.vz[c]) / 3.0f + 0.0f;
+  return r;
+}
+
+__device__ inline uint32_t ld_agent(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void st_agent(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Inclusive scan across the 64 lanes of a wave.
+__device__ inline uint32_t wave_incl_scan(uint32_t v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+__host__ __device__ inline uint32_t tiles_of(uint32_t n, uint32_t tile) {
+  return (n + tile - 1) / tile;
+}
+
+// ---- phase 0: reset the build's counters, histograms and look-back words --------
+// Threads [first, first + stride) of the caller's grid share the work.
+__device__ inline void build_init_part(const BuildView& v, uint32_t first, uint32_t stride) {
+  uint32_t* misc = v.B.misc;
+  const uint32_t nstatus = SORT_PASSES * tiles_of(v.n, SORT_TILE_LEAN) * 256;  // both tilings
+  for (uint32_t k = first; k < nstatus; k += stride) v.B.status[k] = 0u;
+  for (uint32_t k = first; k < (uint32_t)MISC_BOUNDS; k += stride) misc[k] = 0u;  // hist, tickets
+  if (first == 0) misc[MISC_ERR] = 0u;
+  if (first < 3) {
+    misc[MISC_BOUNDS + first] = 0xFFFFFFFFu;  // lo (ordered min)
+    misc[MISC_BOUNDS + 3 + first] = 0u;        // hi (ordered max)
+  }
+}
+
+// ---- phase 1: centroid bounds of one tile (SCENE_AABB map) ------------------------
+__device__ inline void bounds_tile(const BuildView& v, uint32_t tile, uint32_t* lds) {
+  uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
+  const uint32_t end = min((tile + 1) * (uint32_t)TRI_TILE, v.n);
+  for (uint32_t t = tile * TRI_TILE + threadIdx.x; t < end; t += BUILD_BLOCK) {
+    const float3 c = centroid(v, t);
+    const uint32_t o[3] = {f2ord(c.x), f2ord(c.y), f2ord(c.z)};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lo[k] = min(lo[k], o[k]), hi[k] = max(hi[k], o[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      lo[k] = min(lo[k], (uint32_t)__shfl_xor((int)lo[k], off, 64));
+      hi[k] = max(hi[k], (uint32_t)__shfl_xor((int)hi[k], off, 64));
+    }
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) lds[k * 4 + wave] = lo[k], lds[(3 + k) * 4 + wave] = hi[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {  // one device atomic per bound per tile
+    const int k = threadIdx.x;
+    uint32_t m = lds[k * 4];
+    for (int w = 1; w < BUILD_BLOCK / 64; ++w) m = k < 3 ? min(m, lds[k * 4 + w]) : max(m, lds[k * 4 + w]);
+    if (k < 3)
+      atomicMin(&v.B.misc[MISC_BOUNDS + k], m);
+    else
+      atomicMax(&v.B.misc[MISC_BOUNDS + k], m);
+  }
+  __syncthreads();
+}
+
+// ---- phase 2: Morton codes of one tile + the four 8-bit digit histograms ------
+__device__ inline void morton_tile(const BuildView& v, uint32_t tile, uint32_t* lds) {
+  uint32_t* s_hist = lds;  // SORT_PASSES * 256
+  for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BUILD_BLOCK) s_hist[i] = 0u;
+  float lo[3] = {0, 0, 0}, ext[3] = {0, 0, 0};
+  if (v.mode == RQ_MORTON_SCENE_AABB) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = ord2f(v.B.misc[MISC_BOUNDS + k]);
+      ext[k] = ord2f(v.B.misc[MISC_BOUNDS + 3 + k]) - lo[k];
+    }
+  }
+  __syncthreads();
+  const uint32_t end = min((tile + 1) * (uint32_t)TRI_TILE, v.n);
+  for (uint32_t t = tile * TRI_TILE + threadIdx.x; t < end; t += BUILD_BLOCK) {
+    const float3 c = centroid(v, t);
+    float q[3] = {c.x, c.y, c.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (v.mode == RQ_MORTON_SCENE_AABB)
+        q[k] = ext[k] > 0.0f ? (q[k] - lo[k]) / ext[k] : 0.0f;
+      else
+        q[k] = (q[k] + 30.0f) / 1000.0f;
+    }
+    const uint32_t code = morton30(q[0], q[1], q[2]);
+    v.B.codes[t] = code;
+#pragma unroll
+    for (int p = 0; p < SORT_PASSES; ++p)
+      atomicAdd(&s_hist[p * 256 + ((code >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BUILD_BLOCK) {
+    const uint32_t c = s_hist[i];
+    if (c) atomicAdd(&v.B.misc[MISC_HIST + i], c);
+  }
+  __syncthreads();
+}
+
+// ---- phases 3-6: one stable 8-bit LSD pass over one tile ------------------------
+// onesweep: wave w of the tile owns keys [w*64*ITEMS, (w+1)*64*ITEMS), lane l
+// its keys i*64 + l; ranking wave by wave, item by item (8 ballots give the
+// peers with the same digit) keeps input order among equal digits. Tile digit
+// counts go to a decoupled look-back (flag in the top 2 bits of one word,
+// agent-scope relaxed atomics, bounded wait), keys and then values are staged
+// in one LDS tile buffer so the global writes of each digit are contiguous.
+// Every earlier tile of the pass must already be running (ticket order).
+template <int PASS, int ITEMS = SORT_ITEMS>
+__device__ inline void onesweep_tile(const BuildView& v, uint32_t tile, uint32_t* lds) {
+  constexpr int pass = PASS;  // static: buffer selection never indexes the view
+  constexpr int TILE = SORT_BLOCK * ITEMS;
+  static_assert(BUILD_BLOCK == SORT_BLOCK && SORT_BLOCK == 256, "one thread per digit");
+  uint32_t* s_cnt = lds;                        // [SORT_WAVES][256]
+  uint32_t* s_stage = s_cnt + SORT_WAVES * 256;  // [TILE] keys, then values
+  uint32_t* s_excl = s_stage + TILE;         // [256]
+  uint32_t* s_base = s_excl + 256;              // [256]
+  uint32_t* s_wsum = s_base + 256;              // [2][SORT_WAVES]
+
+  const uint32_t n = v.n;
+  const uint32_t* kin = pass == 0 ? v.B.codes : ((pass & 1) ? v.B.keys_b : v.B.keys_a);
+  const uint32_t* vin = pass == 0 ? nullptr : ((pass & 1) ? v.B.vals_b : v.B.vals_a);
+  uint32_t* kout = (pass & 1) ? v.B.keys_a : v.B.keys_b;
+  uint32_t* vout = (pass & 1) ? v.B.vals_a : v.B.vals_b;
+  static_assert(PASS >= 0 && PASS < SORT_PASSES, "30-bit codes: four 8-bit passes");
+  uint32_t* status = v.B.status + (size_t)pass * tiles_of(n, TILE) * 256;
+  uint32_t* misc = v.B.misc;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int shift = 8 * pass;
+  for (int i = tid; i < SORT_WAVES * 256; i += SORT_BLOCK) s_cnt[i] = 0u;
+  __syncthreads();
+  const uint32_t base = tile * TILE;
+  const uint32_t wbase = base + wave * (64 * ITEMS);
+  const uint64_t lt = (1ull << lane) - 1ull;
+
+  // values are read only when they are staged, after the keys went out
+  uint32_t key[ITEMS], rank[ITEMS];
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t g = wbase + i * 64 + lane;
+    key[i] = g < n ? el32(kin, g) : 0u;
+  }
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const bool ok = wbase + i * 64 + lane < n;
+    const uint32_t d = (key[i] >> shift) & 255u;
+    uint64_t peers = __ballot(ok);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bal = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t before = (uint32_t)__popcll(peers & lt);
+    const uint32_t c = ok ? s_cnt[wave * 256 + d] : 0u;
+    rank[i] = c + before;
+    if (ok && before == 0) s_cnt[wave * 256 + d] = c + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  // Only key[] and rank[] stay live across the scans below: hiding the keys
+  // and the lane id from the optimiser makes it recompute digits, indices and
+  // LDS addresses after the scans instead of holding them in registers.
+  uint32_t lane_r = (uint32_t)lane;
+  asm volatile("" : "+v"(lane_r));
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) asm volatile("" : "+v"(key[i]));
+
+  // per digit (thread = digit): wave-exclusive prefix and tile total
+  const uint32_t d = (uint32_t)tid;
+  uint32_t total = 0;
+#pragma unroll
+  for (int w = 0; w < SORT_WAVES; ++w) {
+    const uint32_t c = s_cnt[w * 256 + d];
+    s_cnt[w * 256 + d] = total;
+    total += c;
+  }
+  uint32_t* st = status + (size_t)tile * 256 + d;
+  st_agent(st, (tile == 0 ? LB_INC : LB_AGG) | total);
+
+  // block-exclusive scans of the tile totals and of the global histogram
+  const uint32_t hcount = misc[MISC_HIST + pass * 256 + d];
+  const uint32_t inc_t = wave_incl_scan(total, lane);
+  const uint32_t inc_h = wave_incl_scan(hcount, lane);
+  if (lane == 63) s_wsum[wave] = inc_t, s_wsum[SORT_WAVES + wave] = inc_h;
+  __syncthreads();
+  uint32_t pre_t = 0, pre_h = 0;
+  for (int w = 0; w < wave; ++w) pre_t += s_wsum[w], pre_h += s_wsum[SORT_WAVES + w];
+  const uint32_t blk_excl = pre_t + inc_t - total;
+  const uint32_t glb_excl = pre_h + inc_h - hcount;
+
+  // decoupled look-back over earlier tiles for this digit
+  uint32_t prefix = 0;
+  if (tile > 0) {
+    int64_t j = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    for (;;) {
+      const uint32_t s = ld_agent(status + (size_t)j * 256 + d);
+      const uint32_t f = s & ~LB_MASK;
+      if (f == 0u) {
+        if (++spins > LOOKBACK_SPIN_LIMIT) {  // bounded wait: report, never hang
+          atomicOr(&misc[MISC_ERR], ERR_LOOKBACK_SPIN);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      prefix += s & LB_MASK;
+      if (f == LB_INC) break;
+      --j;
+    }
+    st_agent(st, LB_INC | (prefix + total));
+  }
+  s_excl[d] = blk_excl;
+  s_base[d] = glb_excl + prefix - blk_excl;
+  __syncthreads();
+
+  // keys: rank[i] becomes the tile position, dst[j] the global position of
+  // staged slot tid + j * SORT_BLOCK (reused for the values)
+  static_assert(TILE % SORT_BLOCK == 0, "whole rounds");
+  constexpr int ROUNDS = TILE / SORT_BLOCK;
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t dd = (key[i] >> shift) & 255u;
+    rank[i] += s_excl[dd] + s_cnt[wave * 256 + dd];
+    if (wbase + i * 64 + lane_r < n) {
+      if (rank[i] < (uint32_t)TILE)
+        s_stage[rank[i]] = key[i];
+      else
+        atomicOr(&misc[MISC_ERR], ERR_SCATTER_RANGE);
+    }
+  }
+  __syncthreads();
+  const uint32_t cnt = min((uint32_t)TILE, n - base);
+  uint32_t dst[ROUNDS];
+#pragma unroll
+  for (int j = 0; j < ROUNDS; ++j) {
+    const uint32_t p = tid + j * SORT_BLOCK;
+    dst[j] = n;
+    if (p < cnt) {
+      const uint32_t k = s_stage[p];
+      const uint32_t g = s_base[(k >> shift) & 255u] + p;
+      if (g < n) {
+        el32(kout, g) = k;
+        dst[j] = g;
+      } else {
+        atomicOr(&misc[MISC_ERR], ERR_SCATTER_RANGE);
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ITEMS; ++i) {
+    const uint32_t g = wbase + i * 64 + lane_r;
+    if (g < n && rank[i] < (uint32_t)TILE) s_stage[rank[i]] = vin ? el32(vin, g) : g;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < ROUNDS; ++j)
+    if (dst[j] < n) el32(vout, dst[j]) = s_stage[tid + j * SORT_BLOCK];
+  __syncthreads();
+}
+
+// ---- phase 7: Karras 2012 emission for a tile of internal nodes ----------------
+// Internal node i in [0, n-2] from the sorted codes; equal codes fall back to
+// the index (augmented key code<<32 | i).
+__device__ inline int kdelta(const uint32_t* __restrict__ k, int64_t n, int64_t i, uint32_t ki,
+                             int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t kj = k[j];
+  return ki != kj ? __clz(ki ^ kj) : 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+__device__ inline void karras_node(const BuildView& v, int64_t i) {
+  const uint32_t* keys = v.B.keys_a;
+  const int64_t N = v.n;
+  const uint32_t ki = keys[i];
+  const int d = kdelta(keys, N, i, ki, i + 1) > kdelta(keys, N, i, ki, i - 1) ? 1 : -1;
+  const int dmin = kdelta(keys, N, i, ki, i - d);
+  int64_t lmax = 2;
+  while (kdelta(keys, N, i, ki, i + lmax * d) > dmin) lmax <<= 1;
+  int64_t l = 0;
+  for (int64_t t = lmax >> 1; t >= 1; t >>= 1)
+    if (kdelta(keys, N, i, ki, i + (l + t) * d) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int dnode = kdelta(keys, N, i, ki, j);
+  int64_t s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (kdelta(keys, N, i, ki, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int64_t g = i + s * d + min(d, 0);
+  const int32_t left = (min(i, j) == g) ? ~(int32_t)g : (int32_t)g;
+  const int32_t right = (max(i, j) == g + 1) ? ~(int32_t)(g + 1) : (int32_t)(g + 1);
+  reinterpret_cast<int32_t*>(&v.B.nodes[i].lo0)[3] = left;
+  reinterpret_cast<int32_t*>(&v.B.nodes[i].hi0)[3] = right;
+  const int64_t ul = left >= 0 ? left : (N - 1) + ~left;
+  const int64_t ur = right >= 0 ? right : (N - 1) + ~right;
+  v.B.flags[i] = 0u;
+  v.B.range[i] = make_uint2((uint32_t)min(i, j), (uint32_t)max(i, j));
+  if (g < 0 || g + 1 > N - 1 || ul >= 2 * N - 1 || ur >= 2 * N - 1) {
+    atomicOr(&v.B.misc[MISC_ERR], ERR_TOPOLOGY);
+    return;
+  }
+  v.B.parent[ul] = (int32_t)(i << 1);
+  v.B.parent[ur] = (int32_t)((i << 1) | 1);
+}
+
+__device__ inline void karras_tile(const BuildView& v, uint32_t tile) {
+  if (v.n < 2) return;
+  const int64_t end = min((int64_t)(tile + 1) * NODE_TILE, (int64_t)v.n - 1);
+  for (int64_t i = (int64_t)tile * NODE_TILE + threadIdx.x; i < end; i += BUILD_BLOCK)
+    karras_node(v, i);
+}
+
+// ---- phases 8-9: bottom-up refit ---------------------------------------------------
+// Pass 1 (one unit per REFIT_CHUNK leaves): each leaf writes its triangle
+// record and its box into the parent's child slot; the second arriver at a
+// node (atomic counter) unions the two child boxes and climbs. A node whose
+// leaf range lies inside the chunk is completed by two threads of this block,
+// so its counter needs only work-group scope (on a part with one L2 per XCD an
+// agent-scope acquire/release costs an L2 write-back and invalidate). A node
+// spanning chunks is left to pass 2: the arriving thread deposits its box,
+// counts itself with a relaxed device atomic, and the second of two arrivals
+// lists the node in the chunk's frontier. min/max is order-independent and
+// zeros are canonicalised, so the boxes are bit-exact whatever the order.
+__device__ inline void put_slot(NodeRec* nodes, uint32_t node, uint32_t side, const float* lo,
+                                const float* hi) {
+  float* slo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
+  float* shi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
+  slo[0] = lo[0], slo[1] = lo[1], slo[2] = lo[2];
+  shi[0] = hi[0], shi[1] = hi[1], shi[2] = hi[2];
+}
+__device__ inline void union_slot(const NodeRec* nodes, uint32_t node, uint32_t side, float* lo,
+                                  float* hi) {
+  const float* olo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
+  const float* ohi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    lo[q] = fminf(lo[q], olo[q]);
+    hi[q] = fmaxf(hi[q], ohi[q]);
+  }
+}
+__device__ inline void put_root(uint32_t* misc, const float* lo, const float* hi, uint32_t h) {
+  float* root = reinterpret_cast<float*>(misc + MISC_ROOT);
+  for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];
+  misc[MISC_HEIGHT] = h;
+}
+
+__device__ inline void refit_local_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
+  uint32_t* s_nf = lds;
+  if (threadIdx.x == 0) *s_nf = 0u;
+  __syncthreads();
+  const BuildBuffers& B = v.B;
+  const uint32_t n = v.n;
+  const uint32_t chunk_lo = chunk * REFIT_CHUNK;
+  const uint32_t chunk_hi = min(chunk_lo + REFIT_CHUNK, n) - 1;
+  for (int jj = 0; jj < REFIT_CHUNK / BUILD_BLOCK; ++jj) {
+    const uint32_t k = chunk_lo + jj * BUILD_BLOCK + threadIdx.x;
+    if (k >= n) break;
+    const uint32_t p = B.vals_a[k];
+    const uint32_t a = v.idx[3 * p], b = v.idx[3 * p + 1], c = v.idx[3 * p + 2];
+    const float3 v0 = make_float3(v.vx[a], v.vy[a], v.vz[a]);
+    const float3 v1 = make_float3(v.vx[b], v.vy[b], v.vz[b]);
+    const float3 v2 = make_float3(v.vx[c], v.vy[c], v.vz[c]);
+    TriRec tr;
+    tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
+    tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
+    tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
+    B.tris[k] = tr;
+    float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, fminf(fminf(v0.y, v1.y), v2.y) + 0.0f,
+                   fminf(fminf(v0.z, v1.z), v2.z) + 0.0f};
+    float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f,
+                   fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f};
+    if (n == 1) {
+      put_root(B.misc, lo, hi, 0u);
+      break;
+    }
+    uint32_t h = 0, cur = (n - 1) + k;
+    for (int level = 0;; ++level) {
+      const int32_t pe = B.parent[cur];
+      const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
+      if (pe < 0 || node >= n - 1 || level > 64) {  // height <= 63 for 30-bit codes
+        atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
+        break;
+      }
+      put_slot(B.nodes, node, side, lo, hi);
+      B.slot_h[2 * node + side] = h;
+      const uint2 rg = B.range[node];
+      if (rg.x < chunk_lo || rg.y > chunk_hi) {
+        // spans chunks: finished by pass 2 once both children are in
+        const uint32_t old =
+            __hip_atomic_fetch_add(&B.flags[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (old == 1u) B.frontier[chunk_lo + atomicAdd(s_nf, 1u)] = node;
+        break;
+      }
+      const uint32_t old =
+          __hip_atomic_fetch_add(&B.flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      union_slot(B.nodes, node, side ^ 1u, lo, hi);
+      h = 1u + max(h, B.slot_h[2 * node + (side ^ 1u)]);
+      if (node == 0u) {
+        put_root(B.misc, lo, hi, h);
+        break;
+      }
+      cur = node;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) B.fcount[chunk] = *s_nf;
+  __syncthreads();
+}
+
+// Pass 2 (one block): the nodes spanning chunks, bottom-up from the frontier
+// lists of pass 1 (their child slots were written before this block started
+// reading: a kernel boundary or an acquire on the chunk counters); counters
+// shared inside the block need only work-group scope.
+__device__ inline void refit_top_block(const BuildView& v) {
+  const BuildBuffers& B = v.B;
+  const uint32_t n = v.n;
+  const uint32_t nchunks = tiles_of(n, REFIT_CHUNK);
+  if (nchunks <= 1) return;
+  for (uint32_t bch = threadIdx.x; bch < nchunks; bch += blockDim.x) {
+    const uint32_t cnt = B.fcount[bch];
+    for (uint32_t e = 0; e < cnt; ++e) {
+      uint32_t node = B.frontier[bch * REFIT_CHUNK + e];
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      union_slot(B.nodes, node, 0u, lo, hi);
+      union_slot(B.nodes, node, 1u, lo, hi);
+      uint32_t h = 1u + max(B.slot_h[2 * node], B.slot_h[2 * node + 1]);
+      for (int level = 0;; ++level) {
+        if (node == 0u) {
+          put_root(B.misc, lo, hi, h);
+          break;
+        }
+        const int32_t pe = B.parent[node];
+        const uint32_t q = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
+        if (pe < 0 || q >= n - 1 || level > 64) {
+          atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
+          break;
+        }
+        put_slot(B.nodes, q, side, lo, hi);
+        B.slot_h[2 * q + side] = h;
+        const uint32_t old =
+            __hip_atomic_fetch_add(&B.flags[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) break;
+        union_slot(B.nodes, q, side ^ 1u, lo, hi);
+        h = 1u + max(h, B.slot_h[2 * q + (side ^ 1u)]);
+        node = q;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+}  // namespace rq

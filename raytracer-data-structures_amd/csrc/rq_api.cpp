@@ -107,15 +107,11 @@ uint32_t* next_counter_slot(rq_ctx* c) {
 
 using namespace rq;
 
-namespace {
-constexpr uint32_t SCENE_MAGIC = 0x53435131u;  // "SCQ1"
-constexpr uint32_t BVH_MAGIC = 0x42565131u;    // "BVQ1"
-
-size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+namespace rq {
 
 int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
   BuildBuffers& B = b->B;
-  B.ntiles = (n + SORT_TILE - 1) / SORT_TILE;
+  B.ntiles = (n + SORT_TILE_LEAN - 1) / SORT_TILE_LEAN;
   size_t nn = std::max<uint32_t>(n, 1);
   size_t ni = n > 1 ? n - 1 : 1;
   size_t sz[] = {4 * nn,
@@ -152,8 +148,10 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
   return RQ_OK;
 }
 
-int check_ctx(rq_ctx* c) { return c ? RQ_OK : RQ_E_INVALID; }
+}  // namespace rq
 
+namespace {
+int check_ctx(rq_ctx* c) { return c ? RQ_OK : RQ_E_INVALID; }
 }  // namespace
 
 extern "C" {
@@ -625,6 +623,7 @@ int rq_release(rq_ctx* c, void* h) {
   }
   if (magic == BVH_MAGIC) {
     auto* b = (rq_bvh*)h;
+    if (!b->owned) return set_err(c, RQ_E_INVALID, "BVH belongs to a batch");
     b->magic = 0;
     ctx_free(c, b->base);
     delete b;

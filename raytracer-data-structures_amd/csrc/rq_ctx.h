@@ -22,6 +22,10 @@ struct rq_ctx {
   int32_t* gstack = nullptr;  // traversal stack overflow ({node, distance} pairs)
 };
 
+constexpr uint32_t SCENE_MAGIC = 0x53435131u;  // "SCQ1"
+constexpr uint32_t BVH_MAGIC = 0x42565131u;    // "BVQ1"
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
 struct rq_scene {
   uint32_t magic = 0;  // first member: handle type tag
   uint32_t nv = 0, ntri = 0;
@@ -39,6 +43,7 @@ struct rq_bvh {
   rq::BuildBuffers B;
   void* base = nullptr;
   size_t bytes = 0;
+  bool owned = true;  // false: part of a batch, released with it
 };
 
 namespace rq {
@@ -48,4 +53,5 @@ int ctx_alloc(rq_ctx* c, size_t bytes, void** p);
 void ctx_free(rq_ctx* c, void* p);
 LaunchCtx launch_ctx(rq_ctx* c, void* stream);
 uint32_t* next_counter_slot(rq_ctx* c);
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n);  // build buffers for n primitives
 }  // namespace rq

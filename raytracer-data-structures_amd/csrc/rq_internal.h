@@ -33,10 +33,15 @@ struct alignas(16) TriRec {
 static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
 
 // Radix sort tiling (onesweep): WAVES*64 threads, ITEMS keys per thread.
+// The standalone build sorts 16 keys per thread; the batch kernel, whose
+// register budget is set by traversal, sorts 8 (SORT_ITEMS_LEAN). Look-back
+// status words are allocated for the smaller tile, so either fits.
 constexpr int SORT_WAVES = 4;
 constexpr int SORT_ITEMS = 16;
+constexpr int SORT_ITEMS_LEAN = 8;
 constexpr int SORT_BLOCK = SORT_WAVES * 64;
 constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
+constexpr int SORT_TILE_LEAN = SORT_BLOCK * SORT_ITEMS_LEAN;
 constexpr int SORT_PASSES = 4;  // 30-bit codes, 8-bit digits
 
 // Refit: a block owns REFIT_CHUNK consecutive leaves (REFIT_BLOCK threads,
@@ -68,7 +73,7 @@ struct BuildBuffers {
   uint32_t* keys_b = nullptr;     // n
   uint32_t* vals_a = nullptr;     // n  (final sorted_idx)
   uint32_t* vals_b = nullptr;     // n
-  uint32_t* status = nullptr;     // SORT_PASSES * ntiles * 256
+  uint32_t* status = nullptr;     // SORT_PASSES * ntiles * 256 (tiles of SORT_TILE_LEAN)
   uint32_t* misc = nullptr;       // see MiscLayout
   int32_t* parent = nullptr;      // 2n-1: (node << 1) | side
   uint32_t* flags = nullptr;      // n-1 refit arrival counters
@@ -78,7 +83,7 @@ struct BuildBuffers {
   uint32_t* fcount = nullptr;     // cdiv(n, REFIT_CHUNK) entries per chunk
   NodeRec* nodes = nullptr;       // n-1
   TriRec* tris = nullptr;         // n
-  uint32_t ntiles = 0;
+  uint32_t ntiles = 0;            // tiles of SORT_TILE_LEAN keys
 };
 
 // misc[] layout (uint32 words)

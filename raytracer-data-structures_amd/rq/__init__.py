@@ -92,6 +92,20 @@ class FrameCounts(C.Structure):
     _fields_ = [("primary", C.c_uint64), ("bounce", C.c_uint64), ("shadow", C.c_uint64)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("nrays", C.c_uint32), ("rays", Rays), ("hits", Hits),
+                ("occluded", C.c_void_p)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("lds_depth", C.c_int32), ("tstack", C.c_int32), ("ray_tile", C.c_uint32),
+                ("refill_idle", C.c_uint32)]
+
+
+BATCH_PHASES = ("init", "bounds", "morton", "sort0", "sort1", "sort2", "sort3", "karras", "refit",
+                "refit_top", "trace")
+
+
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise RuntimeError(f"{LIB_PATH} is not built; run raytracer-data-structures_amd/build.py")
@@ -157,6 +171,18 @@ def _load() -> C.CDLL:
         "rq_raygen_uniform": (i32, [vp, u32, u64, f32p, f32p, vp, vp, vp, vp, vp, vp, vp, vp,
                                     vp]),
         "rq_write_ppm": (i32, [C.c_char_p, u32, u32, vp]),
+        "rq_batch_create": (i32, [vp, C.POINTER(BatchItem), u32, i32, C.POINTER(vp)]),
+        "rq_batch_destroy": (i32, [vp, vp]),
+        "rq_batch_run": (i32, [vp, vp, vp]),
+        "rq_batch_run_graph": (i32, [vp, vp, vp]),
+        "rq_batch_bvh": (i32, [vp, vp, u32, C.POINTER(vp)]),
+        "rq_batch_config_count": (i32, []),
+        "rq_batch_config_get": (i32, [i32, C.POINTER(BatchConfig)]),
+        "rq_batch_set_config": (i32, [vp, vp, i32, u32, u32]),
+        "rq_batch_autotune": (i32, [vp, vp, i32, C.POINTER(i32), C.POINTER(u32), C.POINTER(u32),
+                                    C.POINTER(C.c_double)]),
+        "rq_batch_phase_items": (i32, [vp, vp, vp, i32]),
+        "rq_batch_errors": (i32, [vp, vp, C.POINTER(u32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -292,6 +318,10 @@ class DeviceArray:
             self.ctx._check(lib().rq_memcpy_d2h(self.ctx.h, _ptr(out), self.ptr, out.nbytes,
                                                 stream))
         return out
+
+    def fill(self, byte: int, stream=None) -> "DeviceArray":
+        self.ctx._check(lib().rq_memset_d(self.ctx.h, self.ptr, byte, self.nbytes, stream))
+        return self
 
     def free(self) -> None:
         if self.ptr and self.owned:
@@ -543,6 +573,87 @@ class Context:
         cnt = (C.c_uint64 * cap)()
         k = lib().rq_profile_read(self.h, names, ms, cnt, cap)
         return {names[i].value.decode(): (ms[i], cnt[i]) for i in range(min(k, cap))}
+
+
+class Batch:
+    """rq_batch: many meshes built and traced in one persistent launch.
+
+    items: dicts with keys scene (rq_scene handle), rays (DeviceRays or None),
+    hits (dict of DeviceArray: prim required, t/u/v optional; None skips the
+    closest-hit query) and occ (DeviceArray of uint8 or None). The buffers are
+    kept referenced here and must not be freed before close()."""
+
+    def __init__(self, ctx: Context, items: list[dict], morton_mode: int = MORTON_SCENE_AABB):
+        self.ctx = ctx
+        self.items = list(items)
+        arr = (BatchItem * max(len(self.items), 1))()
+        for i, it in enumerate(self.items):
+            rays = it.get("rays")
+            hits = it.get("hits")
+            occ = it.get("occ")
+            arr[i].scene = it["scene"]
+            arr[i].nrays = rays.n if rays is not None else 0
+            if rays is not None:
+                arr[i].rays = rays.struct()
+            if hits is not None:
+                arr[i].hits = Hits(hits["prim"].ptr, hits["t"].ptr if "t" in hits else None,
+                                   hits["u"].ptr if "u" in hits else None,
+                                   hits["v"].ptr if "v" in hits else None)
+            arr[i].occluded = occ.ptr if occ is not None else None
+        h = C.c_void_p()
+        ctx._check(lib().rq_batch_create(ctx.h, arr, len(self.items), morton_mode, C.byref(h)))
+        self.h = h.value
+
+    def close(self) -> None:
+        if self.h:
+            self.ctx._check(lib().rq_batch_destroy(self.ctx.h, self.h))
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def run(self, stream=None) -> None:
+        self.ctx._check(lib().rq_batch_run(self.ctx.h, self.h, stream))
+
+    def run_graph(self, stream=None) -> None:
+        self.ctx._check(lib().rq_batch_run_graph(self.ctx.h, self.h, stream))
+
+    def bvh(self, i: int) -> int:
+        b = C.c_void_p()
+        self.ctx._check(lib().rq_batch_bvh(self.ctx.h, self.h, i, C.byref(b)))
+        return b.value
+
+    @staticmethod
+    def configs() -> list[dict]:
+        out = []
+        for i in range(lib().rq_batch_config_count()):
+            c = BatchConfig()
+            lib().rq_batch_config_get(i, C.byref(c))
+            out.append(dict(index=i, lds_depth=c.lds_depth, tstack=c.tstack))
+        return out
+
+    def set_config(self, index: int, ray_tile: int, refill_idle: int) -> None:
+        self.ctx._check(lib().rq_batch_set_config(self.ctx.h, self.h, index, ray_tile,
+                                                  refill_idle))
+
+    def autotune(self, repeats: int = 2) -> tuple[int, int, int, float]:
+        v, t, r, ms = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_double()
+        self.ctx._check(lib().rq_batch_autotune(self.ctx.h, self.h, repeats, C.byref(v),
+                                                C.byref(t), C.byref(r), C.byref(ms)))
+        return v.value, t.value, r.value, ms.value
+
+    def phase_items(self) -> dict:
+        counts = np.zeros(len(BATCH_PHASES), np.uint32)
+        lib().rq_batch_phase_items(self.ctx.h, self.h, _ptr(counts), counts.size)
+        return dict(zip(BATCH_PHASES, counts.tolist()))
+
+    def errors(self) -> int:
+        bits = C.c_uint32()
+        self.ctx._check(lib().rq_batch_errors(self.ctx.h, self.h, C.byref(bits)))
+        return bits.value
 
 
 class Renderer:
