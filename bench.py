@@ -80,7 +80,8 @@ def counted_batch(ctx, bvh, rays, any_hit: bool):
 
 
 COPY_PROBE_BYTES = 1 << 30
-KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<true, false"}
+KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<true, false",
+                 "batch": "k_batch<"}
 
 
 def copy_probe(ctx) -> None:
@@ -244,6 +245,175 @@ def run_c3(ctx, args) -> int:
     return 0
 
 
+# Algorithmic bytes of one LBVH build per triangle (SCENE_AABB Morton map):
+# bounds 48 (3 indices + 9 vertex floats), Morton 52 (same + code), sort 60
+# (pass 0: key in, key + index out; passes 1-3: key + index in and out),
+# Karras 28 (key, two child links, two parent words, leaf range), refit 176
+# (sorted index, 3 indices, 9 floats, 48 B triangle record, 64 B node boxes,
+# arrival and height words).
+BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
+
+
+def c4_workload(ctx, count: int = 512):
+    """Batched build and traverse: `count` independent meshes with
+    F_m = round(exp(U[ln 1e3, ln 5e5])) triangles (numpy seed 4); even m a
+    displaced icosphere at the largest level with 20*4^L <= F_m, odd m a
+    random soup of F_m triangles (centroids uniform in the unit cube, edge
+    length ~N(0.02, 0.005) clipped at 0.002). R_m = round(exp(U[ln 2^16,
+    ln 2^20])) rays per mesh (numpy seed 5), origins uniform in the mesh box
+    inflated by 10% per side, directions uniform on the sphere."""
+    rng = np.random.default_rng(4)
+    tris = np.rint(np.exp(rng.uniform(np.log(1e3), np.log(5e5), count))).astype(np.int64)
+    nrays = np.rint(np.exp(np.random.default_rng(5).uniform(np.log(65536.0), np.log(1048576.0),
+                                                             count))).astype(np.int64)
+    meshes, scenes, rays = [], [], []
+    for m in range(count):
+        if m % 2 == 0:
+            level = int(np.floor(np.log(tris[m] / 20.0) / np.log(4.0)))
+            while 20 * 4 ** (level + 1) <= tris[m]:
+                level += 1
+            while level > 0 and 20 * 4 ** level > tris[m]:
+                level -= 1
+            mesh = rq.icosphere(level, 1000 + m)
+        else:
+            mesh = rq.soup(int(tris[m]), 2000 + m)
+        lo, hi = mesh.bounds()
+        ext = hi - lo
+        lo = (lo - ext * np.float32(0.1)).astype(np.float32)
+        hi = (hi + ext * np.float32(0.1)).astype(np.float32)
+        scenes.append(ctx.upload_scene(mesh))
+        rays.append(ctx.raygen_uniform(int(nrays[m]), 3000 + m, lo, hi))
+        meshes.append(mesh)
+    return meshes, scenes, rays
+
+
+def run_c4(ctx, args) -> int:
+    """512 meshes built and traced (closest + any hit) in one persistent launch
+    replayed from a hipGraph, against the plain per-request path (per mesh:
+    LBVH rebuild, closest-hit launch, any-hit launch, wait) on the same
+    device-resident meshes, rays and output buffers."""
+    t_setup = time.perf_counter()
+    meshes, scenes, rays = c4_workload(ctx)
+    ntri = sum(m.ntri for m in meshes)
+    nray = sum(r.n for r in rays)
+    log(f"c4: {len(meshes)} meshes, {ntri} triangles, {nray} rays "
+        f"(setup {time.perf_counter() - t_setup:.1f} s)")
+    hits = [{k: rq.DeviceArray(ctx, 4 * r.n) for k in ("prim", "t")} for r in rays]
+    occ = [rq.DeviceArray(ctx, r.n) for r in rays]
+    items = [dict(scene=s, rays=r, hits=h, occ=o) for s, r, h, o in zip(scenes, rays, hits, occ)]
+    batch = rq.Batch(ctx, items, rq.MORTON_SCENE_AABB)
+    tuned = None
+    if not args.no_autotune:
+        v, tile, refill, ms = batch.autotune(1)
+        tuned = {"variant": rq.Batch.configs()[v], "ray_tile": tile, "refill_idle": refill,
+                 "ms": round(ms, 3)}
+        log(f"batch autotune: {tuned}")
+    for _ in range(max(args.warmup, 1)):
+        batch.run_graph()
+    assert batch.errors() == 0, "batch errors"
+    if args.pmc_probe:
+        batch.run_graph()
+        copy_probe(ctx)
+        return 0
+
+    # ---- batched: K replays of the captured persistent launch
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run_graph()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    prof = ctx.profile_read()
+    assert batch.errors() == 0, "batch errors"
+    batch_s = (t1 - t0) / args.steps
+    check = [0, 1, len(meshes) // 2, len(meshes) - 1]
+    ref = {m: (hits[m]["prim"].download(np.int32, rays[m].n), occ[m].download(np.uint8, rays[m].n))
+           for m in check}
+
+    # ---- plain per-request path on the batch's own BVH buffers
+    bvhs = [batch.bvh(m) for m in range(len(meshes))]
+    med = int(np.argsort([r.n for r in rays])[len(rays) // 2])
+    if not args.no_autotune:
+        vc, rc_, _ = ctx.autotune(bvhs[med], rays[med], any_hit=False, repeats=2)
+        va, _, _ = ctx.autotune(bvhs[med], rays[med], any_hit=True, repeats=2)
+        ctx.trace_config(vc, va, rc_)
+
+    def per_request():
+        for m in range(len(meshes)):
+            ctx.rebuild(bvhs[m])
+            ctx.closest_hit_dev(bvhs[m], rays[m], hits[m])
+            ctx.any_hit_dev(bvhs[m], rays[m], occ[m])
+            ctx.sync()
+
+    per_request()
+    base_steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(base_steps):
+        per_request()
+    t1 = time.perf_counter()
+    plain_s = (t1 - t0) / base_steps
+    assert ctx.device_errors() == 0
+    for m in check:
+        assert np.array_equal(hits[m]["prim"].download(np.int32, rays[m].n), ref[m][0]), m
+        assert np.array_equal(occ[m].download(np.uint8, rays[m].n), ref[m][1]), m
+
+    # ---- roofline of the batch kernel: build bytes + traversal bytes
+    vis_c = tst_c = vis_a = tst_a = 0
+    for m in range(len(meshes)):
+        a, b = counted_batch(ctx, bvhs[m], rays[m], any_hit=False)
+        c, d = counted_batch(ctx, bvhs[m], rays[m], any_hit=True)
+        vis_c, tst_c, vis_a, tst_a = vis_c + a, tst_c + b, vis_a + c, tst_a + d
+    alg = (BUILD_BYTES_PER_TRI * ntri + nray * (32 + 8) + 32 * vis_c + 36 * tst_c +
+           nray * (32 + 1) + 32 * vis_a + 36 * tst_a)
+    ms_tot, launches = prof["batch_graph"]
+    avg_ms = ms_tot / launches
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    roofline = dict(kernel="k_batch", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                    bytes_per_launch=int(alg), avg_launch_ms=round(avg_ms, 4))
+    if args.traffic == "auto":
+        roofline["traffic"] = measure_traffic("c4", "batch")
+    line = {
+        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
+        "value": round(2 * nray / batch_s / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batch_s * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c4_batched_build_and_trace", "meshes": len(meshes),
+                   "triangles": ntri, "rays": nray, "queries": "build+closest+any",
+                   "launch": "one persistent kernel per step, hipGraph replay"},
+        "per_request_ms": round(plain_s * 1e3, 3),
+        "per_request_mrays_s": round(2 * nray / plain_s / 1e6, 2),
+        "speedup_vs_per_request": round(plain_s / batch_s, 3),
+        "traversal_counts": {"closest": {"node_visits": vis_c, "tri_tests": tst_c},
+                             "any": {"node_visits": vis_a, "tri_tests": tst_a}},
+        "autotune": tuned, "roofline": roofline, "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        sample = list(range(0, len(meshes), 61))  # icospheres and soups
+        total_rays = 0
+        total_s = 0.0
+        build_ms = 0.0
+        for m in sample:
+            host = rays[m].download()
+            cb = cpu_baseline(meshes[m], [host], 1, threads, what="")
+            total_rays += cb["rays"]
+            total_s += cb["seconds"]
+            build_ms += cb["build_ms"]
+        line["cpu_baseline"] = dict(
+            value=round(total_rays / total_s / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
+            sample=f"meshes {sample}: every active ray (closest hit), {total_rays} rays; "
+                   f"median-split BVH builds {build_ms:.0f} ms in total")
+    print(json.dumps(line), flush=True)
+    batch.close()
+    ctx.close()
+    return 0
+
+
 def sweep(ctx, bvh, batches) -> None:
     """Time each traversal variant / refill threshold on this frame's batches."""
     import ctypes as C
@@ -304,7 +474,7 @@ def cpu_baseline(mesh, batches, sample_every: int, threads: int,
     return dict(value=round(n / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
                 sample=f"every {sample_every}th active ray {what}: {n} rays, "
                        f"median-split BVH built in {build_s * 1e3:.0f} ms",
-                build_ms=round(build_s * 1e3, 1), rays=n, seconds=round(dt, 2))
+                build_ms=round(build_s * 1e3, 1), rays=n, seconds=round(dt, 4))
 
 
 def main() -> int:
@@ -312,7 +482,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--traffic", default="auto", choices=["auto", "off"],
                     help="measure HBM bytes per launch with two rocprofv3 --pmc child runs")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -342,6 +512,8 @@ def main() -> int:
         return 0
     if args.config == "c3":
         return run_c3(ctx, args)
+    if args.config == "c4":
+        return run_c4(ctx, args)
     mesh, params, workload = (c2_setup if args.config == "c2" else c1_setup)(ctx)
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)

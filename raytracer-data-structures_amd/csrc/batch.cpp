@@ -14,8 +14,8 @@
 using namespace rq;
 
 namespace {
-constexpr uint32_t kRayTiles[] = {8192, 32768, 131072};
-constexpr uint32_t kRefills[] = {16, 32, 48};
+constexpr uint32_t kRayTiles[] = {2048, 4096, 8192, 32768};
+constexpr uint32_t kRefills[] = {16, 32};
 constexpr uint32_t kMinRayTile = 1024;
 }  // namespace
 
@@ -31,8 +31,9 @@ struct rq_batch {
   uint32_t gdepth = 0;
   std::vector<uint32_t> grid;
   int variant = 0;
-  uint32_t ray_tile = 32768, refill = 32;
+  uint32_t ray_tile = 2048, refill = 16;  // the autotuned choice on 512 mixed meshes
   uint32_t phase_begin[BATCH_PHASES + 1] = {};
+  uint32_t trace_seg[BATCH_XCDS + 1] = {};
   uint32_t built_ray_tile = 0;
   hipStream_t own_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -70,12 +71,48 @@ void free_batch(rq_ctx* c, rq_batch* b) {
   delete b;
 }
 
+// Trace items of mesh m: its closest-hit ray tiles, then its any-hit tiles.
+void push_trace(std::vector<BatchWork>& w, const rq_batch_item& it, uint32_t m, uint32_t ray_tile) {
+  const uint32_t nt = tiles_of(it.nrays, ray_tile);
+  if (it.hits.prim)
+    for (uint32_t t = 0; t < nt; ++t) w.push_back(BatchWork{m, t});
+  if (it.occluded)
+    for (uint32_t t = 0; t < nt; ++t) w.push_back(BatchWork{m, t | BATCH_ANY_BIT});
+}
+
 // The work queue: the items of every mesh, phase after phase, tiles in order.
+// Trace items go to BATCH_XCDS segments: meshes are dealt to the segment with
+// the least ray work so far, largest first (LPT), and keep their index order
+// inside a segment.
 int make_work(rq_ctx* c, rq_batch* b, uint32_t ray_tile) {
   std::vector<BatchWork> w;
   const uint32_t nm = (uint32_t)b->items.size();
   for (int ph = 0; ph < BATCH_PHASES; ++ph) {
     b->phase_begin[ph] = (uint32_t)w.size();
+    if (ph == PH_TRACE) {
+      std::vector<uint64_t> work(nm);
+      std::vector<uint32_t> order(nm), seg(nm);
+      for (uint32_t m = 0; m < nm; ++m) {
+        const rq_batch_item& it = b->items[m];
+        work[m] = (uint64_t)it.nrays * ((it.hits.prim ? 1 : 0) + (it.occluded ? 1 : 0));
+        order[m] = m;
+      }
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint32_t x, uint32_t y) { return work[x] > work[y]; });
+      uint64_t load[BATCH_XCDS] = {};
+      for (uint32_t m : order) {
+        const int q = (int)(std::min_element(load, load + BATCH_XCDS) - load);
+        seg[m] = (uint32_t)q;
+        load[q] += work[m];
+      }
+      for (int q = 0; q < BATCH_XCDS; ++q) {
+        b->trace_seg[q] = (uint32_t)w.size();
+        for (uint32_t m = 0; m < nm; ++m)
+          if (seg[m] == (uint32_t)q) push_trace(w, b->items[m], m, ray_tile);
+      }
+      b->trace_seg[BATCH_XCDS] = (uint32_t)w.size();
+      continue;
+    }
     for (uint32_t m = 0; m < nm; ++m) {
       const uint32_t n = b->bvhs[m]->n;
       uint32_t k = 0;
@@ -92,14 +129,6 @@ int make_work(rq_ctx* c, rq_batch* b, uint32_t ray_tile) {
         default: break;
       }
       for (uint32_t t = 0; t < k; ++t) w.push_back(BatchWork{m, t});
-      if (ph == PH_TRACE) {
-        const rq_batch_item& it = b->items[m];
-        const uint32_t nt = tiles_of(it.nrays, ray_tile);
-        if (it.hits.prim)
-          for (uint32_t t = 0; t < nt; ++t) w.push_back(BatchWork{m, t});
-        if (it.occluded)
-          for (uint32_t t = 0; t < nt; ++t) w.push_back(BatchWork{m, t | BATCH_ANY_BIT});
-      }
     }
   }
   b->phase_begin[BATCH_PHASES] = (uint32_t)w.size();
@@ -117,6 +146,7 @@ BatchArgs batch_args(const rq_batch* b) {
   a.meshes = b->meshes;
   a.items = b->work;
   std::memcpy(a.phase_begin, b->phase_begin, sizeof a.phase_begin);
+  std::memcpy(a.trace_seg, b->trace_seg, sizeof a.trace_seg);
   a.ctl = b->ctl;
   a.gstack = b->gstack;
   int depth, ts;

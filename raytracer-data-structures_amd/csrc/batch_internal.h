@@ -19,12 +19,18 @@ enum BatchPhase {
   BATCH_PHASES
 };
 
+// The trace items are split into one queue segment per XCD (each XCD has its
+// own L2): a mesh's rays are traced on one XCD so its BVH stays in that L2;
+// a block drains its own XCD's segment first, then takes from the others.
+constexpr int BATCH_XCDS = 8;
+
 // control words of one batch
 constexpr int BATCH_CTL_TICKET = 0;                        // BATCH_PHASES queue tickets
 constexpr int BATCH_CTL_BARRIER = BATCH_PHASES;            // grid barrier arrivals
-constexpr int BATCH_CTL_BARRIER_ERR = BATCH_PHASES + 1;    // sticky: barrier wait gave up
-constexpr int BATCH_CTL_TRACE_ERR = BATCH_PHASES + 2;      // sticky: stack deeper than provided
-constexpr int BATCH_CTL_WORDS = BATCH_PHASES + 3;
+constexpr int BATCH_CTL_BARRIER_ERR = BATCH_PHASES + 1;    // barrier wait gave up (this run)
+constexpr int BATCH_CTL_TRACE_ERR = BATCH_PHASES + 2;      // stack deeper than provided (this run)
+constexpr int BATCH_CTL_XTICKET = BATCH_PHASES + 3;        // BATCH_XCDS trace segment tickets
+constexpr int BATCH_CTL_WORDS = BATCH_CTL_XTICKET + BATCH_XCDS;
 constexpr uint32_t BATCH_ANY_BIT = 0x80000000u;            // trace item tile: any-hit query
 
 struct BatchMeshDev {
@@ -45,6 +51,7 @@ struct BatchArgs {
   const BatchMeshDev* meshes;
   const BatchWork* items;               // every phase's items, phase after phase
   uint32_t phase_begin[BATCH_PHASES + 1];
+  uint32_t trace_seg[BATCH_XCDS + 1];   // trace items of XCD q: [trace_seg[q], trace_seg[q + 1])
   uint32_t* ctl;                        // BATCH_CTL_WORDS
   int32_t* gstack;                      // traversal stack overflow of the persistent grid
   uint32_t gdepth;

@@ -16,6 +16,9 @@
 namespace rq {
 namespace {
 
+// s_getreg operand of HW_REG_XCC_ID (register 20, bits [3:0]): the XCD id.
+constexpr int XCC_ID_HWREG = 20 | (0 << 6) | ((4 - 1) << 11);
+
 // About 4M polls (a few seconds): far beyond any phase of a resident grid.
 constexpr uint32_t BARRIER_SPIN_LIMIT = 1u << 22;
 
@@ -115,11 +118,21 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
   uint32_t* const ctl = a.ctl;
   const BatchWork* const items = a.items;
   const BatchMeshDev* const meshes = a.meshes;
+  // XCD this block runs on (hardware register; any value is safe: blocks
+  // drain every segment before the launch ends)
+  const uint32_t xcd = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) % BATCH_XCDS;
   for (int ph = 0; ph < BATCH_PHASES; ++ph) {
-    const uint32_t b0 = a.phase_begin[ph], b1 = a.phase_begin[ph + 1];  // kernarg, uniform
+   const int nq = ph == PH_TRACE ? BATCH_XCDS : 1;
+   for (int k = 0; k < nq; ++k) {
+    const uint32_t q = (xcd + (uint32_t)k) % BATCH_XCDS;
+    // kernarg words, block-uniform
+    const uint32_t b0 = ph == PH_TRACE ? a.trace_seg[q] : a.phase_begin[ph];
+    const uint32_t b1 = ph == PH_TRACE ? a.trace_seg[q + 1] : a.phase_begin[ph + 1];
+    uint32_t* const ticket =
+        ph == PH_TRACE ? ctl + BATCH_CTL_XTICKET + q : ctl + BATCH_CTL_TICKET + ph;
     for (;;) {
       if (b1 == b0) break;
-      if (threadIdx.x == 0) s_item = b0 + atomicAdd(ctl + BATCH_CTL_TICKET + ph, 1u);
+      if (threadIdx.x == 0) s_item = b0 + atomicAdd(ticket, 1u);
       __syncthreads();
       // block-uniform: keep the item and its mesh descriptor in scalar registers
       const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
@@ -191,6 +204,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
 #undef VIEW
       __syncthreads();
     }
+   }
     if (ph + 1 < BATCH_PHASES)
       grid_barrier(ctl + BATCH_CTL_BARRIER, (uint32_t)(ph + 1) * gridDim.x,
                    ctl + BATCH_CTL_BARRIER_ERR);

@@ -230,12 +230,16 @@ __device__ inline void onesweep_tile(const BuildView& v, uint32_t tile, uint32_t
   const uint32_t wbase = base + wave * (64 * ITEMS);
   const uint64_t lt = (1ull << lane) - 1ull;
 
-  // values are read only when they are staged, after the keys went out
-  uint32_t key[ITEMS], rank[ITEMS];
+  // Full tiles load the values with the keys (their latency hides behind
+  // the ranking); lean tiles, whose register budget is tight, read them only
+  // when they are staged, after the keys went out.
+  constexpr bool PREFETCH = ITEMS >= 16;
+  uint32_t key[ITEMS], rank[ITEMS], val[PREFETCH ? ITEMS : 1];
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t g = wbase + i * 64 + lane;
     key[i] = g < n ? el32(kin, g) : 0u;
+    if (PREFETCH) val[i] = g < n ? (vin ? el32(vin, g) : g) : 0u;
   }
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
@@ -347,7 +351,8 @@ __device__ inline void onesweep_tile(const BuildView& v, uint32_t tile, uint32_t
 #pragma unroll
   for (int i = 0; i < ITEMS; ++i) {
     const uint32_t g = wbase + i * 64 + lane_r;
-    if (g < n && rank[i] < (uint32_t)TILE) s_stage[rank[i]] = vin ? el32(vin, g) : g;
+    if (g < n && rank[i] < (uint32_t)TILE)
+      s_stage[rank[i]] = PREFETCH ? val[i] : (vin ? el32(vin, g) : g);
   }
   __syncthreads();
 #pragma unroll
@@ -401,10 +406,14 @@ __device__ inline void karras_node(const BuildView& v, int64_t i) {
   v.B.parent[ur] = (int32_t)((i << 1) | 1);
 }
 
+// TILE internal nodes per unit: the batch kernel queues NODE_TILE, the
+// standalone kernel one node per thread (each node is an independent chain of
+// dependent key loads, so more units hide more latency).
+template <int TILE = NODE_TILE>
 __device__ inline void karras_tile(const BuildView& v, uint32_t tile) {
   if (v.n < 2) return;
-  const int64_t end = min((int64_t)(tile + 1) * NODE_TILE, (int64_t)v.n - 1);
-  for (int64_t i = (int64_t)tile * NODE_TILE + threadIdx.x; i < end; i += BUILD_BLOCK)
+  const int64_t end = min((int64_t)(tile + 1) * TILE, (int64_t)v.n - 1);
+  for (int64_t i = (int64_t)tile * TILE + threadIdx.x; i < end; i += blockDim.x)
     karras_node(v, i);
 }
 
@@ -450,9 +459,9 @@ __device__ inline void refit_local_tile(const BuildView& v, uint32_t chunk, uint
   const uint32_t n = v.n;
   const uint32_t chunk_lo = chunk * REFIT_CHUNK;
   const uint32_t chunk_hi = min(chunk_lo + REFIT_CHUNK, n) - 1;
-  for (int jj = 0; jj < REFIT_CHUNK / BUILD_BLOCK; ++jj) {
-    const uint32_t k = chunk_lo + jj * BUILD_BLOCK + threadIdx.x;
-    if (k >= n) break;
+  // blockDim.x threads share the chunk's leaves (the standalone kernel runs
+  // REFIT_CHUNK threads, one leaf each; the batch kernel BUILD_BLOCK)
+  for (uint32_t k = chunk_lo + threadIdx.x; k <= chunk_hi; k += blockDim.x) {
     const uint32_t p = B.vals_a[k];
     const uint32_t a = v.idx[3 * p], b = v.idx[3 * p + 1], c = v.idx[3 * p + 2];
     const float3 v0 = make_float3(v.vx[a], v.vy[a], v.vz[a]);

@@ -35,9 +35,9 @@ typedef void (*OnesweepFn)(BuildView);
 const OnesweepFn kOnesweep[SORT_PASSES] = {k_onesweep<0>, k_onesweep<1>, k_onesweep<2>,
                                            k_onesweep<3>};
 
-__global__ __launch_bounds__(BUILD_BLOCK) void k_karras(BuildView v) { karras_tile(v, blockIdx.x); }
+__global__ __launch_bounds__(BUILD_BLOCK) void k_karras(BuildView v) { karras_tile<BUILD_BLOCK>(v, blockIdx.x); }
 
-__global__ __launch_bounds__(BUILD_BLOCK) void k_refit_local(BuildView v) {
+__global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(BuildView v) {
   __shared__ uint32_t lds[1];
   refit_local_tile(v, blockIdx.x, lds);
 }
@@ -73,12 +73,12 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
   }
   if (n >= 2) {
     ProfScope ps(lc, "karras");
-    hipLaunchKernelGGL(k_karras, dim3(tiles_of(n - 1, NODE_TILE)), dim3(BUILD_BLOCK), 0, s, v);
+    hipLaunchKernelGGL(k_karras, dim3(tiles_of(n - 1, BUILD_BLOCK)), dim3(BUILD_BLOCK), 0, s, v);
   }
   const uint32_t nchunks = tiles_of(n, REFIT_CHUNK);
   {
     ProfScope ps(lc, "refit");
-    hipLaunchKernelGGL(k_refit_local, dim3(nchunks), dim3(BUILD_BLOCK), 0, s, v);
+    hipLaunchKernelGGL(k_refit_local, dim3(nchunks), dim3(REFIT_CHUNK), 0, s, v);
   }
   if (nchunks > 1) {
     ProfScope ps(lc, "refit_top");

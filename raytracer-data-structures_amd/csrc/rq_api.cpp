@@ -414,7 +414,7 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.err = c->errword;
   a.grid = c->trace_grid[variant];
   a.refill_idle = c->refill_idle;
-  a.chunk = TRACE_RAY_CHUNK;
+  a.chunk = trace_chunk(n, a.grid * (uint32_t)blk / 64u);
   hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "trace launch");
 }

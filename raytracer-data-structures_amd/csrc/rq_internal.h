@@ -44,9 +44,9 @@ constexpr int SORT_TILE = SORT_BLOCK * SORT_ITEMS;
 constexpr int SORT_TILE_LEAN = SORT_BLOCK * SORT_ITEMS_LEAN;
 constexpr int SORT_PASSES = 4;  // 30-bit codes, 8-bit digits
 
-// Refit: a block owns REFIT_CHUNK consecutive leaves (REFIT_BLOCK threads,
-// REFIT_CHUNK / REFIT_BLOCK leaves each); the top pass runs in one work-group.
-constexpr int REFIT_BLOCK = 256;
+// Refit: a block owns REFIT_CHUNK consecutive leaves (the standalone kernel
+// one leaf per thread, the batch kernel four); the top pass runs in one
+// work-group.
 constexpr int REFIT_CHUNK = 1024;
 constexpr int REFIT_TOP_BLOCK = 1024;
 
@@ -64,7 +64,17 @@ constexpr int TRACE_MAX_STACK = 64;
 constexpr int TRACE_MIN_LDS_DEPTH = 12;
 constexpr int TRACE_MAX_VARIANTS = 16;
 constexpr int COUNTER_SLOTS = 64;
-constexpr uint32_t TRACE_RAY_CHUNK = 256;  // rays per wave slice of the ray stream
+constexpr uint32_t TRACE_RAY_CHUNK = 256;  // rays per wave slice of the ray stream (largest)
+
+// Rays per wave slice for a launch of n rays over `waves` persistent waves:
+// 256 when every wave gets at least two full slices, halved down to one ray
+// per lane (64) for smaller launches, so that a small batch still spreads
+// over as many waves as it can fill.
+inline uint32_t trace_chunk(uint32_t n, uint32_t waves) {
+  uint32_t c = TRACE_RAY_CHUNK;
+  while (c > 64 && (uint64_t)n < 2ull * c * waves) c >>= 1;
+  return c;
+}
 
 // Build workspace (device), all sized for n primitives.
 struct BuildBuffers {
