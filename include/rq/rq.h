@@ -33,6 +33,7 @@ extern "C" {
 #define RQ_E_IO (-4)          /* file could not be read or parsed            */
 #define RQ_E_NODEVICE (-5)    /* no usable gfx950 device                     */
 #define RQ_E_UNSUPPORTED (-6) /* feature not available for this input       */
+#define RQ_E_DEVICE (-7)      /* a kernel reported an error (stack, barrier)  */
 
 /* Morton normalisation (accelerators.h:577 vs Karras 2012). */
 #define RQ_MORTON_REF_AFFINE 0 /* (centroid + 30) / 1000, bit-exact with the reference */

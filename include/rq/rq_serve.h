@@ -117,12 +117,21 @@ int rq_server_submit(rq_server* s, const rq_request* r, uint64_t* ticket);
 int rq_server_poll(rq_server* s, uint64_t ticket, rq_request_status* st);
 /* Blocks until the request is done or failed. */
 int rq_server_wait(rq_server* s, uint64_t ticket, rq_request_status* st);
+/* Up to `cap` requests finished since the last call (done or failed), in
+ * completion order: their tickets and statuses; *count is the number taken. */
+int rq_server_take_done(rq_server* s, uint64_t* tickets, rq_request_status* st, uint32_t cap,
+                        uint32_t* count);
 /* Forgets a finished request's record. */
 int rq_server_forget(rq_server* s, uint64_t ticket);
 int rq_server_stats_get(rq_server* s, rq_server_stats* st);
 const char* rq_server_last_error(rq_server* s);
 /* Seconds on the server's monotonic clock (the clock of rq_request_status). */
 double rq_server_clock(void);
+/* The rays of an RQ_RAYS_PINHOLE request, into caller device arrays (for
+ * checking a request against rq_closest_hit / rq_any_hit on the same rays). */
+int rq_raygen_pinhole(rq_ctx* ctx, const rq_pinhole* cam, uint32_t n, uint64_t seed, float* ox,
+                      float* oy, float* oz, float* dx, float* dy, float* dz, float* tmin,
+                      float* tmax, void* stream);
 /* Page-locked host memory for request arrays (fast asynchronous copies). */
 void* rq_host_alloc(size_t bytes);
 void rq_host_free(void* p);

@@ -61,7 +61,10 @@ struct BatchArgs {
 
 int batch_num_variants();
 void batch_variant_info(int v, int* depth, int* tstack);
-uint32_t batch_grid_blocks(int device, int v);
-hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid);
+uint32_t batch_grid_blocks(int device, int v, bool trace_only = false);
+// Resets the control words, then one persistent launch (all phases, or the
+// trace phase alone).
+hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid,
+                        bool trace_only = false);
 
 }  // namespace rq

@@ -110,7 +110,9 @@ constexpr int batch_lds_words(int depth, bool tstack) {
 // Occupancy is set for the traversal phase (the longest): WAVES waves per
 // SIMD caps the registers of every phase, and the build phases are written to
 // fit that budget (lean 8-key sort tiles, descriptors loaded per phase).
-template <int DEPTH, bool TSTACK, int WAVES>
+// TRACE_ONLY: the trace phase alone (ray tiles over resident BVHs, no grid
+// barrier), the continuous-batching step of the serving loop.
+template <int DEPTH, bool TSTACK, int WAVES, bool TRACE_ONLY>
 __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_batch(
     BatchArgs a) {
   __shared__ uint32_t lds[batch_lds_words(DEPTH, TSTACK)];
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
   // XCD this block runs on (hardware register; any value is safe: blocks
   // drain every segment before the launch ends)
   const uint32_t xcd = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) % BATCH_XCDS;
-  for (int ph = 0; ph < BATCH_PHASES; ++ph) {
+  for (int ph = TRACE_ONLY ? (int)PH_TRACE : 0; ph < BATCH_PHASES; ++ph) {
    const int nq = ph == PH_TRACE ? BATCH_XCDS : 1;
    for (int k = 0; k < nq; ++k) {
     const uint32_t q = (xcd + (uint32_t)k) % BATCH_XCDS;
@@ -223,11 +225,13 @@ struct BatchVariant {
   bool tstack;
   BatchFn fn;
 };
-const BatchVariant kBatchVariants[] = {
-    {16, false, k_batch<16, false, 7>}, {20, false, k_batch<20, false, 7>},
-    {8, true, k_batch<8, true, 7>},     {10, true, k_batch<10, true, 7>},
-    {16, false, k_batch<16, false, 8>}, {8, true, k_batch<8, true, 8>},
-};
+#define RQ_BATCH_VARIANTS(T)                                                               \
+  {16, false, k_batch<16, false, 7, T>}, {20, false, k_batch<20, false, 7, T>},              \
+      {8, true, k_batch<8, true, 7, T>}, {10, true, k_batch<10, true, 7, T>},                \
+      {16, false, k_batch<16, false, 8, T>}, {8, true, k_batch<8, true, 8, T>}
+const BatchVariant kBatchVariants[] = {RQ_BATCH_VARIANTS(false)};
+const BatchVariant kTraceVariants[] = {RQ_BATCH_VARIANTS(true)};
+#undef RQ_BATCH_VARIANTS
 constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 
 }  // namespace
@@ -238,19 +242,22 @@ void batch_variant_info(int v, int* depth, int* tstack) {
   *tstack = kBatchVariants[v].tstack ? 1 : 0;
 }
 
-uint32_t batch_grid_blocks(int device, int v) {
+uint32_t batch_grid_blocks(int device, int v, bool trace_only) {
   int cus = 0, per = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kBatchVariants[v].fn, BUILD_BLOCK, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per, (trace_only ? kTraceVariants : kBatchVariants)[v].fn, BUILD_BLOCK, 0);
   if (cus <= 0) cus = 256;
   if (per <= 0) per = 1;
   return (uint32_t)(cus * per);
 }
 
-hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid) {
-  ProfScope ps(lc, "batch");
+hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid,
+                        bool trace_only) {
+  ProfScope ps(lc, trace_only ? "batch_trace" : "batch");
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, lc.stream, a.ctl);
-  hipLaunchKernelGGL(kBatchVariants[variant].fn, dim3(grid), dim3(BUILD_BLOCK), 0, lc.stream, a);
+  hipLaunchKernelGGL((trace_only ? kTraceVariants : kBatchVariants)[variant].fn, dim3(grid),
+                     dim3(BUILD_BLOCK), 0, lc.stream, a);
   return hipGetLastError();
 }
 

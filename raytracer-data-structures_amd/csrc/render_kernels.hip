@@ -110,6 +110,26 @@ __global__ __launch_bounds__(256) void k_raygen_uniform(uint32_t n, uint64_t see
   tmin[i] = 0.0f, tmax[i] = INFINITY;
 }
 
+// Pinhole camera of the serving requests: ray i through pixel (i % w, i / w)
+// of a w x h image, jittered by Philox(seed, i); d = normalize(f + sx*r + sy*u)
+// with the camera basis computed on the host.
+__global__ __launch_bounds__(256) void k_raygen_pinhole(PinholeParams p, float* ox, float* oy,
+                                                         float* oz, float* dx, float* dy,
+                                                         float* dz, float* tmin, float* tmax) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= p.n) return;
+  const uint32_t x = i % p.w, y = i / p.w;
+  uint32_t o[4];
+  philox(i, 0u, 0u, 0x50494E48u, p.seed, o);
+  const float sx = (2.0f * (((float)x + u24(o[0])) * p.inv_w) - 1.0f) * p.tan_half * p.aspect;
+  const float sy = (1.0f - 2.0f * (((float)y + u24(o[1])) * p.inv_h)) * p.tan_half;
+  const V3 d = normalize({(p.f[0] + sx * p.r[0]) + sy * p.u[0], (p.f[1] + sx * p.r[1]) + sy * p.u[1],
+                          (p.f[2] + sx * p.r[2]) + sy * p.u[2]});
+  ox[i] = p.eye[0], oy[i] = p.eye[1], oz[i] = p.eye[2];
+  dx[i] = d.x, dy[i] = d.y, dz[i] = d.z;
+  tmin[i] = 0.0f, tmax[i] = INFINITY;
+}
+
 // ---- shading ---------------------------------------------------------------
 struct Geo {
   V3 p, n;
@@ -328,6 +348,14 @@ hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed,
   const float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
   hipLaunchKernelGGL(k_raygen_uniform, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, n, seed, lo[0],
                      lo[1], lo[2], ex, ey, ez, R.ox, R.oy, R.oz, R.dx, R.dy, R.dz, R.tmin, R.tmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_raygen_pinhole(const LaunchCtx& lc, const PinholeParams& p, const RaySoA& R) {
+  if (p.n == 0) return hipSuccess;
+  ProfScope ps(lc, "raygen_pinhole");
+  hipLaunchKernelGGL(k_raygen_pinhole, dim3(cdiv(p.n, 256)), dim3(256), 0, lc.stream, p, R.ox, R.oy,
+                     R.oz, R.dx, R.dy, R.dz, R.tmin, R.tmax);
   return hipGetLastError();
 }
 

@@ -10,6 +10,14 @@ struct CamParams {
   uint64_t seed;
 };
 
+// Pinhole camera (serving requests): basis f (forward), r (right), u (up).
+struct PinholeParams {
+  float eye[3], f[3], r[3], u[3];
+  float tan_half, aspect, inv_w, inv_h;
+  uint32_t w, n;
+  uint64_t seed;
+};
+
 struct ShadeParams {
   float light[3], emission[3], surface[3], sky[3];
   float bias;
@@ -29,6 +37,7 @@ hipError_t launch_raygen_camera(const LaunchCtx& lc, const CamParams& cp, const 
                                 const RaySoA& R);
 hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed, const float lo[3],
                                  const float hi[3], const RaySoA& R);
+hipError_t launch_raygen_pinhole(const LaunchCtx& lc, const PinholeParams& p, const RaySoA& R);
 hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                 uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
                                 const RaySoA& shad, const RaySoA& bnc);

@@ -102,6 +102,41 @@ class BatchConfig(C.Structure):
                 ("refill_idle", C.c_uint32)]
 
 
+class Pinhole(C.Structure):
+    _fields_ = [("eye", C.c_float * 3), ("look_at", C.c_float * 3), ("up", C.c_float * 3),
+                ("fov_deg", C.c_float), ("width", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("mesh", C.c_uint32), ("new_mesh", C.POINTER(Mesh)), ("nrays", C.c_uint32),
+                ("source", C.c_int32), ("host_rays", Rays), ("pinhole", Pinhole),
+                ("lo", C.c_float * 3), ("hi", C.c_float * 3), ("seed", C.c_uint64),
+                ("query", C.c_int32), ("prim", C.c_void_p), ("t", C.c_void_p),
+                ("occluded", C.c_void_p)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("mesh", C.c_uint32), ("steps", C.c_uint32),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("mesh_slots", C.c_uint32), ("slot_tris", C.c_uint32),
+                ("arena_bytes", C.c_size_t), ("max_inflight", C.c_uint32),
+                ("step_rays", C.c_uint32), ("chunk_rays", C.c_uint32), ("ray_tile", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("steps", C.c_uint64), ("rays", C.c_uint64), ("builds", C.c_uint64),
+                ("max_step_requests", C.c_uint32), ("resident", C.c_uint32),
+                ("arena_peak", C.c_size_t), ("busy_s", C.c_double)]
+
+
+RAYS_HOST, RAYS_PINHOLE, RAYS_UNIFORM = 0, 1, 2
+Q_CLOSEST, Q_ANY = 1, 2
+REQ_QUEUED, REQ_ADMITTED, REQ_DONE = 0, 1, 2
+
 BATCH_PHASES = ("init", "bounds", "morton", "sort0", "sort1", "sort2", "sort3", "karras", "refit",
                 "refit_top", "trace")
 
@@ -183,6 +218,22 @@ def _load() -> C.CDLL:
                                     C.POINTER(C.c_double)]),
         "rq_batch_phase_items": (i32, [vp, vp, vp, i32]),
         "rq_batch_errors": (i32, [vp, vp, C.POINTER(u32)]),
+        "rq_server_defaults": (None, [C.POINTER(ServerConfig)]),
+        "rq_server_create": (i32, [i32, C.POINTER(ServerConfig), C.POINTER(vp)]),
+        "rq_server_destroy": (i32, [vp]),
+        "rq_server_load_mesh": (i32, [vp, C.POINTER(Mesh), C.POINTER(u32)]),
+        "rq_server_submit": (i32, [vp, C.POINTER(Request), C.POINTER(u64)]),
+        "rq_server_poll": (i32, [vp, u64, C.POINTER(RequestStatus)]),
+        "rq_server_wait": (i32, [vp, u64, C.POINTER(RequestStatus)]),
+        "rq_server_forget": (i32, [vp, u64]),
+        "rq_server_take_done": (i32, [vp, vp, vp, u32, C.POINTER(u32)]),
+        "rq_server_stats_get": (i32, [vp, C.POINTER(ServerStats)]),
+        "rq_server_last_error": (C.c_char_p, [vp]),
+        "rq_server_clock": (C.c_double, []),
+        "rq_host_alloc": (vp, [C.c_size_t]),
+        "rq_host_free": (None, [vp]),
+        "rq_raygen_pinhole": (i32, [vp, C.POINTER(Pinhole), u32, u64, vp, vp, vp, vp, vp, vp, vp,
+                                    vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -495,6 +546,16 @@ class Context:
                                             *(a[f].ptr for f in RAY_FIELDS), stream))
         return r
 
+    def raygen_pinhole(self, n: int, seed: int, eye, look_at, up=(0.0, 1.0, 0.0),
+                       fov_deg: float = 40.0, width: int = 1024, stream=None) -> DeviceRays:
+        r = DeviceRays(self, n)
+        a = r.arrays
+        p = Pinhole((C.c_float * 3)(*eye), (C.c_float * 3)(*look_at), (C.c_float * 3)(*up),
+                    fov_deg, width)
+        self._check(lib().rq_raygen_pinhole(self.h, C.byref(p), n, seed,
+                                            *(a[f].ptr for f in RAY_FIELDS), stream))
+        return r
+
     def raygen_camera(self, params: RenderParams, jitter: np.ndarray | None = None,
                       stream=None) -> DeviceRays:
         n = params.width * params.height * params.spp
@@ -654,6 +715,150 @@ class Batch:
         bits = C.c_uint32()
         self.ctx._check(lib().rq_batch_errors(self.ctx.h, self.h, C.byref(bits)))
         return bits.value
+
+
+class HostArray:
+    """Page-locked host memory (rq_host_alloc) viewed as a numpy array."""
+
+    def __init__(self, count: int, dtype):
+        self.dtype = np.dtype(dtype)
+        self.count = int(count)
+        nbytes = max(self.count * self.dtype.itemsize, 1)
+        self.ptr = lib().rq_host_alloc(nbytes)
+        if not self.ptr:
+            raise RQError(-2, f"rq_host_alloc({nbytes}) failed")
+        buf = (C.c_char * nbytes).from_address(self.ptr)
+        self.array = np.frombuffer(buf, self.dtype, self.count)
+
+    def free(self) -> None:
+        if self.ptr:
+            self.array = None
+            lib().rq_host_free(self.ptr)
+            self.ptr = None
+
+
+def _mesh_struct(m: MeshArrays):
+    vx, vy, vz = (np.ascontiguousarray(a, np.float32) for a in (m.vx, m.vy, m.vz))
+    idx = np.ascontiguousarray(m.idx, np.uint32)
+    st = Mesh(vx.ctypes.data_as(C.POINTER(C.c_float)), vy.ctypes.data_as(C.POINTER(C.c_float)),
+              vz.ctypes.data_as(C.POINTER(C.c_float)), vx.size,
+              idx.ctypes.data_as(C.POINTER(C.c_uint32)), idx.size // 3)
+    return st, (vx, vy, vz, idx)
+
+
+def server_clock() -> float:
+    return lib().rq_server_clock()
+
+
+class Server:
+    """rq_server: resident mesh slots and a continuous-batching request loop.
+
+    submit() keeps references to every array a request reads or writes until
+    wait() / poll() reports it finished."""
+
+    def __init__(self, device: int = 0, **cfg):
+        c = ServerConfig()
+        lib().rq_server_defaults(C.byref(c))
+        for k, v in cfg.items():
+            setattr(c, k, v)
+        self.config = c
+        h = C.c_void_p()
+        rc = lib().rq_server_create(device, C.byref(c), C.byref(h))
+        if rc != RQ_OK:
+            raise (NoMemory if rc == -2 else RQError)(rc, f"rq_server_create failed ({rc})")
+        self.h = h.value
+        self._keep = {}
+
+    def close(self) -> None:
+        if self.h:
+            lib().rq_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != RQ_OK:
+            msg = lib().rq_server_last_error(self.h).decode()
+            raise (NoMemory if rc == -2 else RQError)(rc, msg)
+
+    def load_mesh(self, mesh: MeshArrays) -> int:
+        st, keep = _mesh_struct(mesh)
+        slot = C.c_uint32()
+        self._check(lib().rq_server_load_mesh(self.h, C.byref(st), C.byref(slot)))
+        return slot.value
+
+    def submit(self, nrays: int, query: int, prim=None, t=None, occ=None, mesh: int = 0,
+               new_mesh: MeshArrays | None = None, rays: dict | None = None,
+               pinhole: dict | None = None, box=None, seed: int = 0) -> int:
+        """prim / t / occ: numpy arrays (int32, float32, uint8) of >= nrays
+        elements, ideally HostArray views. rays: host SoA dict (RAYS_HOST);
+        pinhole: dict(eye, look_at, up, fov_deg, width); box: (lo, hi)."""
+        r = Request()
+        r.mesh = mesh
+        r.nrays = nrays
+        r.query = query
+        keep = [prim, t, occ]
+        if new_mesh is not None:
+            st, arrays = _mesh_struct(new_mesh)
+            keep += [st, arrays]
+            r.new_mesh = C.pointer(st)
+        if rays is not None:
+            arrs = [np.ascontiguousarray(rays[f], np.float32) for f in RAY_FIELDS]
+            keep.append(arrs)
+            r.source = RAYS_HOST
+            r.host_rays = Rays(*(a.ctypes.data_as(C.c_void_p) for a in arrs))
+        elif pinhole is not None:
+            r.source = RAYS_PINHOLE
+            r.pinhole = Pinhole((C.c_float * 3)(*pinhole["eye"]),
+                                (C.c_float * 3)(*pinhole["look_at"]),
+                                (C.c_float * 3)(*pinhole.get("up", (0.0, 1.0, 0.0))),
+                                pinhole.get("fov_deg", 40.0), pinhole.get("width", 1024))
+        elif box is not None:
+            r.source = RAYS_UNIFORM
+            r.lo = (C.c_float * 3)(*box[0])
+            r.hi = (C.c_float * 3)(*box[1])
+        r.seed = seed
+        r.prim = _ptr(prim)
+        r.t = _ptr(t)
+        r.occluded = _ptr(occ)
+        tk = C.c_uint64()
+        self._check(lib().rq_server_submit(self.h, C.byref(r), C.byref(tk)))
+        self._keep[tk.value] = keep
+        return tk.value
+
+    def poll(self, ticket: int) -> RequestStatus:
+        st = RequestStatus()
+        self._check(lib().rq_server_poll(self.h, ticket, C.byref(st)))
+        return st
+
+    def wait(self, ticket: int) -> RequestStatus:
+        st = RequestStatus()
+        rc = lib().rq_server_wait(self.h, ticket, C.byref(st))
+        if rc not in (RQ_OK,) and st.state >= 0:
+            self._check(rc)
+        return st
+
+    def take_done(self, cap: int = 256) -> list[tuple[int, RequestStatus]]:
+        """Requests finished since the last call: (ticket, status), completion order."""
+        tk = (C.c_uint64 * cap)()
+        st = (RequestStatus * cap)()
+        n = C.c_uint32()
+        self._check(lib().rq_server_take_done(self.h, tk, st, cap, C.byref(n)))
+        return [(tk[i], st[i]) for i in range(n.value)]
+
+    def forget(self, ticket: int) -> None:
+        self._check(lib().rq_server_forget(self.h, ticket))
+        self._keep.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        self._check(lib().rq_server_stats_get(self.h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
 
 
 class Renderer:

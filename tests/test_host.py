@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol(rq):
 def test_headers_compile_as_c_and_cpp(tmp_path):
     src = tmp_path / "t.c"
     src.write_text('#include "rq/rq.h"\n#include "rq/rq_scene.h"\n#include "rq/rq_render.h"\n'
-                   '#include "rq/settings.h"\n'
+                   '#include "rq/settings.h"\n#include "rq/rq_batch.h"\n#include "rq/rq_serve.h"\n'
                    "_Static_assert(RQ_BVH == 0 && RQ_KDTREE == 1 && RQ_UNIFORM_GRID == 2 && "
                    "RQ_LBVH == 3 && RQ_NONE == 4 && RQ_GPU_LBVH == 5, \"AccType\");\n"
                    "int main(void) { rq_settings s = rq_settings_default();\n"
