@@ -414,6 +414,211 @@ def run_c4(ctx, args) -> int:
     return 0
 
 
+def c5_meshes(count: int, seed: int, lo: float = 1e4, hi: float = 2e6):
+    """F ~ round(exp(U[ln lo, ln hi])) (numpy seed); even m a displaced
+    icosphere at the largest level with 20*4^L <= F, odd m a random soup."""
+    rng = np.random.default_rng(seed)
+    tris = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
+    out = []
+    for m in range(count):
+        if m % 2 == 0:
+            level = 0
+            while 20 * 4 ** (level + 1) <= tris[m]:
+                level += 1
+            out.append(rq.icosphere(level, 10 * seed + m))
+        else:
+            out.append(rq.soup(int(tris[m]), 1000 * seed + m))
+    return out
+
+
+_BOUNDS: dict = {}
+
+
+def _c5_rays_spec(mesh, kind: int, seed: int) -> dict:
+    if id(mesh) not in _BOUNDS:
+        _BOUNDS[id(mesh)] = mesh.bounds()
+    lo, hi = _BOUNDS[id(mesh)]
+    c = (lo + hi) * 0.5
+    r = float(np.linalg.norm(hi - lo)) * 0.5
+    if kind == 0:
+        return dict(pinhole=dict(eye=(float(c[0]), float(c[1]), float(c[2] + 2.5 * r)),
+                                 look_at=tuple(float(x) for x in c), up=(0.0, 1.0, 0.0),
+                                 fov_deg=40.0, width=1024), seed=seed)
+    ext = hi - lo
+    return dict(box=(tuple(float(x) for x in lo - 0.1 * ext), tuple(float(x) for x in hi + 0.1 * ext)),
+                seed=seed)
+
+
+def run_c5(ctx, args) -> int:
+    """Serving: 64 resident meshes (F ~ log-U[1e4, 2e6], seed 5) in a server
+    with 64 slots of 2^21 triangles and a 200 GB HBM budget. Open loop:
+    Poisson arrivals at 2000 requests/s for 1 s of warm-up plus
+    `--serve-seconds` measured; request sizes log-U[4096, 4194304] rays (seed
+    7), alternately pinhole rays aimed at the mesh and uniform random rays
+    (generated on the device from the request's parameters), closest hit with
+    prim + t returned to page-locked host arrays; 5% of requests upload a new
+    mesh (from a pool of 8 pre-generated meshes, seed 6) that is built before
+    its rays are traced. Latency = results on the host - scheduled arrival.
+    Then a burst of the first 1000 non-upload requests measures the served
+    throughput at saturation, and the same 1000 requests through the plain
+    per-request path (ray generation, rq_closest_hit, copy back, wait)."""
+    import ctypes as C
+
+    t_setup = time.perf_counter()
+    resident = c5_meshes(64, 5)
+    uploads = c5_meshes(8, 6)
+    log(f"c5: {sum(m.ntri for m in resident)} resident triangles, "
+        f"{sum(m.ntri for m in uploads)} in the upload pool ({time.perf_counter() - t_setup:.1f} s)")
+    srv = rq.Server(0)
+    slots = [srv.load_mesh(m) for m in resident]
+    slot_mesh = {s: m for s, m in zip(slots, resident)}
+    lam, warm, meas = 2000.0, 1.0, float(args.serve_seconds)
+    rng = np.random.default_rng(7)
+    n_req = int(lam * (warm + meas) * 1.2) + 16
+    arrival = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    n_req = int(np.searchsorted(arrival, warm + meas))
+    arrival = arrival[:n_req]
+    sizes = np.rint(np.exp(rng.uniform(np.log(4096.0), np.log(4194304.0), n_req))).astype(np.int64)
+    target = rng.integers(0, len(slots), n_req)
+    upload = rng.random(n_req) < 0.05
+    max_n = int(sizes.max())
+    ring = 80  # page-locked result buffers (at most 80 requests in flight from this driver)
+    bufs = [(rq.HostArray(max_n, np.int32), rq.HostArray(max_n, np.float32)) for _ in range(ring)]
+    log(f"c5: {n_req} requests over {warm + meas:.0f} s, {int(upload.sum())} uploads, "
+        f"offered {sizes.sum() / (warm + meas) / 1e6:.0f} Mrays/s")
+
+    def spec(j):
+        if upload[j]:
+            m = uploads[j % len(uploads)]
+            sp = _c5_rays_spec(m, j % 2, 100000 + j)
+            sp["new_mesh"] = m
+        else:
+            sp = _c5_rays_spec(slot_mesh[slots[target[j]]], j % 2, 100000 + j)
+            sp["mesh"] = slots[target[j]]
+        return sp
+
+    def drive(order, t0, times):
+        """Submits order[k] at t0 + times[k] (or as soon as a buffer is free);
+        returns per-request (t_done - scheduled arrival, steps)."""
+        free = list(range(ring))
+        inflight = {}
+        lat = np.full(len(order), np.nan)
+        steps = np.zeros(len(order), np.int64)
+        k = 0
+        while k < len(order) or inflight:
+            now = rq.server_clock()
+            while k < len(order) and free and t0 + times[k] <= now:
+                j = order[k]
+                b = free.pop()
+                tk = srv.submit(int(sizes[j]), rq.Q_CLOSEST, prim=bufs[b][0].array,
+                                t=bufs[b][1].array, **spec(j))
+                inflight[tk] = (k, b)
+                k += 1
+            for tk, st in srv.take_done():
+                kk, b = inflight.pop(tk)
+                if st.state != rq.REQ_DONE:
+                    raise RuntimeError(f"request {order[kk]} failed: {st.state}")
+                lat[kk] = st.t_done - (t0 + times[kk])
+                steps[kk] = st.steps
+                free.append(b)
+                srv.forget(tk)
+        return lat, steps
+
+    # ---- open loop at lambda = 2000/s
+    s0 = srv.stats()
+    t0 = rq.server_clock() + 0.05
+    lat, steps = drive(list(range(n_req)), t0, arrival)
+    wall = rq.server_clock() - t0
+    s1 = srv.stats()
+    sel = arrival >= warm
+    lat_ms = lat[sel] * 1e3
+    up_ms = lat[sel & upload] * 1e3
+    done_rays = int(sizes[sel].sum())
+    p50, p99 = (float(np.percentile(lat_ms, q)) for q in (50, 99))
+    log(f"c5 open loop: p50 {p50:.2f} ms, p99 {p99:.2f} ms, wall {wall:.2f} s")
+
+    # ---- saturation: a burst of 1000 resident-mesh requests, served vs plain
+    burst = [j for j in range(n_req) if not upload[j]][:1000]
+    t0 = rq.server_clock() + 0.01
+    blat, _ = drive(burst, t0, np.zeros(len(burst)))
+    served_s = float(np.nanmax(blat))
+    burst_rays = int(sizes[burst].sum())
+    srv_stats = srv.stats()
+    srv.close()
+
+    lib = rq.lib()
+    bvhs = {}
+    for sl, m in slot_mesh.items():
+        sc = ctx.upload_scene(m)
+        bvhs[sl] = ctx.build_lbvh(sc, rq.MORTON_SCENE_AABB)
+    dr = rq.DeviceRays(ctx, max_n)
+    dprim, dt = rq.DeviceArray(ctx, 4 * max_n), rq.DeviceArray(ctx, 4 * max_n)
+    ptrs = [dr.arrays[f].ptr for f in rq.RAY_FIELDS]
+    rays_st = dr.struct()
+
+    def plain(j):
+        n = int(sizes[j])
+        sp = spec(j)
+        if "pinhole" in sp:
+            p = sp["pinhole"]
+            cam = rq.Pinhole((C.c_float * 3)(*p["eye"]), (C.c_float * 3)(*p["look_at"]),
+                             (C.c_float * 3)(*p["up"]), p["fov_deg"], p["width"])
+            ctx._check(lib.rq_raygen_pinhole(ctx.h, C.byref(cam), n, sp["seed"], *ptrs, None))
+        else:
+            lo3 = (C.c_float * 3)(*sp["box"][0])
+            hi3 = (C.c_float * 3)(*sp["box"][1])
+            ctx._check(lib.rq_raygen_uniform(ctx.h, n, sp["seed"], lo3, hi3, *ptrs, None))
+        h = rq.Hits(dprim.ptr, dt.ptr, None, None)
+        ctx._check(lib.rq_closest_hit(ctx.h, bvhs[sp["mesh"]], C.byref(rays_st), n, C.byref(h),
+                                      None))
+        ctx._check(lib.rq_memcpy_d2h(ctx.h, bufs[0][0].ptr, dprim.ptr, 4 * n, None))
+        ctx._check(lib.rq_memcpy_d2h(ctx.h, bufs[0][1].ptr, dt.ptr, 4 * n, None))
+
+    for j in burst[:20]:
+        plain(j)
+    ctx.sync()
+    t_a = time.perf_counter()
+    for j in burst:
+        plain(j)
+    ctx.sync()
+    plain_s = time.perf_counter() - t_a
+    served_mrays = burst_rays / served_s / 1e6
+    plain_mrays = burst_rays / plain_s / 1e6
+    line = {
+        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms; serving p50/p99 latency",
+        "value": round(done_rays / meas / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
+        "steps": int(sel.sum()), "warmup": int((~sel).sum()),
+        "ms_per_step": round(float(np.mean(lat_ms)), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "c5_serving", "resident_meshes": len(slots),
+                   "resident_triangles": int(sum(m.ntri for m in resident)),
+                   "arrival_rate_per_s": lam, "measured_s": meas, "requests": int(sel.sum()),
+                   "uploads": int((sel & upload).sum()), "hbm_budget_bytes": 200 * 10 ** 9,
+                   "offered_mrays_s": round(float(sizes[sel].sum()) / meas / 1e6, 1),
+                   "query": "closest (prim, t) to host", "rays": "half pinhole, half uniform"},
+        "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3),
+                       "max": round(float(np.max(lat_ms)), 3),
+                       "upload_p50": round(float(np.percentile(up_ms, 50)), 3) if len(up_ms) else None,
+                       "upload_p99": round(float(np.percentile(up_ms, 99)), 3) if len(up_ms) else None},
+        "trace_launches": int(s1["steps"] - s0["steps"]),
+        "mean_launches_per_request": round(float(steps[sel].mean()), 2),
+        "max_requests_per_launch": int(s1["max_step_requests"]),
+        "gpu_trace_busy_fraction": round((s1["busy_s"] - s0["busy_s"]) / wall, 3),
+        "saturation": {"requests": len(burst), "rays": burst_rays,
+                       "served_mrays_s": round(served_mrays, 1),
+                       "plain_per_request_mrays_s": round(plain_mrays, 1),
+                       "speedup": round(served_mrays / plain_mrays, 3)},
+        "server_stats": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in srv_stats.items()},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    for a, b in bufs:
+        a.free()
+        b.free()
+    ctx.close()
+    return 0
+
+
 def sweep(ctx, bvh, batches) -> None:
     """Time each traversal variant / refill threshold on this frame's batches."""
     import ctypes as C
@@ -482,7 +687,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--serve-seconds", type=float, default=10.0,
+                    help="c5: measured seconds of arrivals (after 1 s of warm-up)")
     ap.add_argument("--traffic", default="auto", choices=["auto", "off"],
                     help="measure HBM bytes per launch with two rocprofv3 --pmc child runs")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -514,6 +721,8 @@ def main() -> int:
         return run_c3(ctx, args)
     if args.config == "c4":
         return run_c4(ctx, args)
+    if args.config == "c5":
+        return run_c5(ctx, args)
     mesh, params, workload = (c2_setup if args.config == "c2" else c1_setup)(ctx)
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)

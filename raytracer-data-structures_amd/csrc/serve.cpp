@@ -127,7 +127,8 @@ struct Step {
   uint64_t seq = 0;
 };
 
-constexpr int kStepBufs = 2;  // steps in flight (double-buffered tables)
+constexpr int kStepBufs = 2;           // steps in flight (double-buffered tables)
+constexpr uint32_t kMinStepTile = 256;  // smallest ray tile of a step (one ray per thread)
 
 size_t request_bytes(const rq_request& r) {
   const size_t n = r.nrays;
@@ -357,19 +358,25 @@ bool launch_step(rq_server* s) {
   if (!st) return false;
   st->chunks.clear();
   uint64_t total = 0;
-  size_t items = 0;
-  const uint32_t tile = s->cfg.ray_tile;
   for (Req* q : s->admitted) {
     if (!q->ready || q->err || q->assigned >= q->r.nrays) continue;
     if (st->chunks.size() >= s->cfg.max_inflight || total >= s->cfg.step_rays) break;
     const uint32_t c = std::min<uint32_t>(q->r.nrays - q->assigned, s->cfg.chunk_rays);
-    const size_t ci = (size_t)((c + tile - 1) / tile) * ((q->prim ? 1 : 0) + (q->occ ? 1 : 0));
-    if (items + ci > s->max_items) break;  // table capacity (not reached with the sizing above)
     st->chunks.push_back(Chunk{q, q->assigned, c});
     total += c;
-    items += ci;
   }
   if (st->chunks.empty()) return false;
+  // Ray tile of this step: halved from the configured one (down to 256 rays,
+  // one per thread) until the step has work items for 4 blocks per grid
+  // slot, so that a small step still spreads over the whole device.
+  auto count_items = [&](uint32_t tl) {
+    size_t k = 0;
+    for (const Chunk& c : st->chunks)
+      k += (size_t)((c.count + tl - 1) / tl) * ((c.q->prim ? 1 : 0) + (c.q->occ ? 1 : 0));
+    return k;
+  };
+  uint32_t tile = s->cfg.ray_tile;
+  while (tile > kMinStepTile && count_items(tile) < 4ull * s->grid) tile >>= 1;
   // mesh table and work items (ray tiles dealt to XCD segments, LPT by rays)
   BatchMeshDev* md = s->host_meshes[buf];
   const uint32_t ne = (uint32_t)st->chunks.size();
@@ -680,7 +687,8 @@ int rq_server_create(int device, const rq_server_config* cfg_in, rq_server** out
   }
   if (!rc) rc = ctx_alloc(c, std::max<size_t>(cfg.arena_bytes, 256), (void**)&s->arena_base);
   s->arena.init(cfg.arena_bytes);
-  s->max_items = 2 * ((size_t)(cfg.step_rays + cfg.chunk_rays) / cfg.ray_tile + cfg.max_inflight) + 16;
+  s->max_items =
+      2 * ((size_t)(cfg.step_rays + cfg.chunk_rays) / kMinStepTile + cfg.max_inflight) + 16;
   for (int b = 0; b < kStepBufs && !rc; ++b) {
     rc = ctx_alloc(c, cfg.max_inflight * sizeof(BatchMeshDev), (void**)&s->dev_meshes[b]);
     if (!rc) rc = ctx_alloc(c, s->max_items * sizeof(BatchWork), (void**)&s->dev_items[b]);
