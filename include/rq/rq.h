@@ -40,6 +40,7 @@ extern "C" {
 #define RQ_MORTON_SCENE_AABB 1 /* (centroid - lo) / (hi - lo) over centroid bounds     */
 
 #define RQ_PRIM_TRIANGLE 0
+#define RQ_PRIM_SPHERE 1 /* the reference's primitive (accelerators.h:54-92) */
 
 typedef struct rq_ctx rq_ctx;
 typedef struct rq_scene rq_scene;
@@ -79,6 +80,15 @@ size_t rq_bytes_in_use(const rq_ctx* ctx);
 int rq_scene_upload(rq_ctx* ctx, const float* vx, const float* vy, const float* vz, uint32_t nv,
                     const uint32_t* idx, uint32_t ntri, int prim_kind, rq_scene** out,
                     void* stream);
+/* Spheres (the reference's scene objects, main.cpp:677-689): centres, radii
+ * and an optional surface colour per sphere (3 floats each, added by the
+ * renderer's shading as sphere->surfaceColor at main.cpp:490; NULL uses the
+ * render parameters' colour). Built and queried like triangle scenes; a
+ * sphere hit is the reference's geometric test (accelerators.h:79-92) with
+ * castRay's far-root rule (main.cpp:349-353), t in [tmin, tmax). */
+int rq_scene_upload_spheres(rq_ctx* ctx, const float* cx, const float* cy, const float* cz,
+                            const float* radius, const float* rgb, uint32_t n, rq_scene** out,
+                            void* stream);
 /* Replace the vertex positions of an uploaded scene (same counts); the next
  * rq_rebuild_lbvh sees them. */
 int rq_scene_update_vertices(rq_ctx* ctx, rq_scene* s, const float* vx, const float* vy,

@@ -41,6 +41,8 @@ typedef struct {
   float sky[3];      /* miss colour (0.6, 0.8, 1) (main.cpp:312, :341) */
   float bias;        /* 1e-4 (main.cpp:404) */
   int32_t bounce;    /* 0: primary + shadow; 1: + mirror bounce and its shadow ray */
+  int32_t shadows;   /* 1: a shadow ray per hit (any hit); 0: the reference's trace_more
+                        stub (main.cpp:235-245): lights never occluded, no shadow rays */
 } rq_render_params;
 
 typedef struct {

@@ -46,6 +46,23 @@ int rq_gen_soup(uint32_t ntri, uint64_t seed, float mean_edge, float sd_edge, fl
 int rq_obj_read(const char* path, rq_mesh* out);
 int rq_obj_write(const char* path, const rq_mesh* m);
 
+/* Spheres (host SoA): centres, radii and a surface colour (3 floats) each. */
+typedef struct {
+  float *cx, *cy, *cz, *r, *rgb;
+  uint32_t n;
+} rq_spheres;
+
+/* The reference's scene (createScene_new, main.cpp:599-721): the `v` records
+ * at the start of an OBJ file, read up to the first record of another kind
+ * (or `max_vertices`, 0 = no limit), each one a sphere of radius 0.05 at
+ * v * 100 + clone * 20, then y - 10, z - 60 (float arithmetic in that order),
+ * colour (0.8, 0.7, 0), repeated for `clones` copies of the file (the
+ * reference's NUMBER_OF_CLONES, 1); then the ground sphere of radius 100 at
+ * (0.93591022, -105.47120094, -43.2363205), colour (0, 0, 0). */
+int rq_gen_reference_scene(const char* obj_path, uint32_t clones, uint32_t max_vertices,
+                           rq_spheres* out);
+void rq_spheres_free(rq_spheres* s);
+
 /* Axis-aligned bounds of the vertices (lo xyz, hi xyz). */
 void rq_mesh_bounds(const rq_mesh* m, float lo[3], float hi[3]);
 void rq_mesh_free(rq_mesh* m);

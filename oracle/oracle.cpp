@@ -30,11 +30,14 @@ inline float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline V3 cross(V3 a, V3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
-// geometry.h:125-134 (normalize: factor = 1/sqrt(norm), then multiply)
+// geometry.h:125-134 (normalize: factor = 1/sqrt(norm), then multiply). The
+// template's unqualified sqrt is the C double sqrt (geometry.h includes
+// <cmath> without std:: overloads in scope), so the factor is computed in
+// double and rounded once to float.
 inline V3 normalize(V3 a) {
   float n = a.x * a.x + a.y * a.y + a.z * a.z;
   if (n > 0) {
-    float f = 1 / std::sqrt(n);
+    float f = (float)(1.0 / ::sqrt((double)n));
     a.x *= f, a.y *= f, a.z *= f;
   }
   return a;
@@ -107,14 +110,12 @@ extern "C" void or_centroid(const float* vx, const float* vy, const float* vz,
   c[2] = ((a.z + b.z) + d.z) / 3.0f;
 }
 
-extern "C" int or_morton_codes(const float* vx, const float* vy, const float* vz, uint32_t nv,
-                               const uint32_t* idx, uint32_t ntri, int mode, uint32_t* codes) {
-  (void)nv;
-  std::vector<float> cs((size_t)ntri * 3);
+namespace {
+// Morton codes of n centroids (3 floats each, zeros canonicalised in place).
+void codes_from_centroids(std::vector<float>& cs, uint32_t ntri, int mode, uint32_t* codes) {
   float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   for (uint32_t t = 0; t < ntri; ++t) {
     float* c = &cs[3 * (size_t)t];
-    or_centroid(vx, vy, vz, idx, t, c);
     for (int k = 0; k < 3; ++k) {
       c[k] = c[k] + 0.0f;  // canonical zero
       lo[k] = std::min(lo[k], c[k]);
@@ -134,6 +135,15 @@ extern "C" int or_morton_codes(const float* vx, const float* vy, const float* vz
     }
     codes[t] = or_morton3d(n[0], n[1], n[2]);
   }
+}
+}  // namespace
+
+extern "C" int or_morton_codes(const float* vx, const float* vy, const float* vz, uint32_t nv,
+                               const uint32_t* idx, uint32_t ntri, int mode, uint32_t* codes) {
+  (void)nv;
+  std::vector<float> cs((size_t)ntri * 3);
+  for (uint32_t t = 0; t < ntri; ++t) or_centroid(vx, vy, vz, idx, t, &cs[3 * (size_t)t]);
+  codes_from_centroids(cs, ntri, mode, codes);
   return 0;
 }
 
@@ -231,6 +241,8 @@ extern "C" void or_topdown(const uint32_t* sc, uint32_t n, int32_t* left, int32_
   }
 }
 
+void refit_internal(uint32_t n, const int32_t* left, const int32_t* right, float* boxes);
+
 // Bottom-up AABB refit: leaf box = min/max of the triangle's vertices
 // (JoinBoxPopintBounds, accelerators.h:216-229), internal box = union of the
 // children (JoinBounds, :201-214). Zeros are canonicalised to +0.
@@ -249,7 +261,13 @@ extern "C" void or_refit(const float* vx, const float* vy, const float* vz, cons
       b[3 + a] = std::max(std::max(x0, x1), x2) + 0.0f;
     }
   }
+  refit_internal(n, left, right, boxes);
+}
+
+// Unions child boxes bottom-up (leaf boxes already in place).
+void refit_internal(uint32_t n, const int32_t* left, const int32_t* right, float* boxes) {
   if (n < 2) return;
+  auto leafbox = [&](uint32_t k) { return boxes + 6 * ((size_t)(n - 1) + k); };
   auto box = [&](int32_t c) -> float* { return c >= 0 ? boxes + 6 * (size_t)c : leafbox(~c); };
   // post-order over internal nodes
   std::vector<std::pair<int32_t, int>> st;
@@ -339,6 +357,24 @@ extern "C" int or_ray_sphere(const float c[3], float r, const float o[3], const 
   return 1;
 }
 
+namespace {
+// A sphere leaf as castRay uses it (main.cpp:349-353): the reference's test,
+// then t = t0, or t1 when t0 < 0; accepted here when t is in [tmin, tmax).
+inline bool sphere_e(V3 c, float r2, V3 o, V3 d, float tmin, float tmax, float* t) {
+  V3 l = sub(c, o);
+  float tca = dot(l, d);
+  if (tca < 0) return false;
+  float d2 = dot(l, l) - tca * tca;
+  if (d2 > r2) return false;
+  float thc = std::sqrt(r2 - d2);
+  float t0 = tca - thc, t1 = tca + thc;
+  float tt = t0 < 0 ? t1 : t0;
+  if (!(tt >= tmin && tt < tmax)) return false;
+  *t = tt;
+  return true;
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Brute-force queries: the NONE branch of castRay (main.cpp:373-387) walks
 // every primitive in index order and keeps strictly smaller t, so the lowest
@@ -405,6 +441,22 @@ extern "C" void or_any_brute(const float* vx, const float* vy, const float* vz,
   });
 }
 
+extern "C" void or_any_brute_spheres(const float* cx, const float* cy, const float* cz,
+                                     const float* r, uint32_t n, const or_rays* rays,
+                                     uint32_t nrays, uint8_t* occ, int nthreads) {
+  parallel_for(nrays, nthreads, [&](uint32_t i) {
+    V3 o, d;
+    float tmin, tmax;
+    load_ray(rays, i, &o, &d, &tmin, &tmax);
+    uint8_t hit = 0;
+    for (uint32_t p = 0; p < n && !hit && tmin < tmax; ++p) {
+      float t;
+      if (sphere_e({cx[p], cy[p], cz[p]}, r[p] * r[p], o, d, tmin, tmax, &t)) hit = 1;
+    }
+    occ[i] = hit;
+  });
+}
+
 // ---------------------------------------------------------------------------
 // Oracle LBVH and its canonical traversal.
 // ---------------------------------------------------------------------------
@@ -413,8 +465,9 @@ struct or_bvh {
   std::vector<uint32_t> codes, sorted_codes, sorted_idx;
   std::vector<int32_t> left, right;
   std::vector<float> boxes;  // (2n-1)*6
-  std::vector<float> tri;    // n*9 in leaf order: v0, e1, e2
+  std::vector<float> tri;    // n*9 in leaf order: v0, e1, e2 (spheres: centre, r, r*r)
   uint32_t height = 0;
+  int kind = 0;              // 0 triangles, 1 spheres
 };
 
 extern "C" or_bvh* or_bvh_build(const float* vx, const float* vy, const float* vz, uint32_t nv,
@@ -444,6 +497,38 @@ extern "C" or_bvh* or_bvh_build(const float* vx, const float* vy, const float* v
     T[7] = e2.y, T[8] = e2.z;
   }
   b->height = or_tree_height(b->left.data(), b->right.data(), ntri);
+  return b;
+}
+
+extern "C" or_bvh* or_bvh_build_spheres(const float* cx, const float* cy, const float* cz,
+                                        const float* r, uint32_t n, int morton_mode) {
+  auto* b = new or_bvh();
+  b->kind = 1;
+  b->n = n;
+  b->codes.resize(n);
+  b->sorted_codes.resize(n);
+  b->sorted_idx.resize(n);
+  b->left.assign(n > 1 ? n - 1 : 0, 0);
+  b->right.assign(n > 1 ? n - 1 : 0, 0);
+  b->boxes.assign(n ? (2 * (size_t)n - 1) * 6 : 0, 0.0f);
+  b->tri.assign((size_t)n * 9, 0.0f);
+  if (n == 0) return b;
+  // centroid = centre (main.cpp:680-683 / accelerators.h:576)
+  std::vector<float> cs((size_t)n * 3);
+  for (uint32_t p = 0; p < n; ++p) cs[3 * p] = cx[p], cs[3 * p + 1] = cy[p], cs[3 * p + 2] = cz[p];
+  codes_from_centroids(cs, n, morton_mode, b->codes.data());
+  or_sort_pairs(b->codes.data(), n, b->sorted_codes.data(), b->sorted_idx.data());
+  or_karras(b->sorted_codes.data(), n, b->left.data(), b->right.data());
+  for (uint32_t k = 0; k < n; ++k) {
+    uint32_t p = b->sorted_idx[k];
+    float* bx = &b->boxes[6 * ((size_t)(n - 1) + k)];
+    const float c[3] = {cx[p], cy[p], cz[p]};
+    for (int a = 0; a < 3; ++a) bx[a] = (c[a] - r[p]) + 0.0f, bx[3 + a] = (c[a] + r[p]) + 0.0f;
+    float* T = &b->tri[9 * (size_t)k];
+    T[0] = c[0], T[1] = c[1], T[2] = c[2], T[3] = r[p], T[4] = r[p] * r[p];
+  }
+  refit_internal(n, b->left.data(), b->right.data(), b->boxes.data());
+  b->height = or_tree_height(b->left.data(), b->right.data(), n);
   return b;
 }
 
@@ -481,20 +566,64 @@ inline RayPre prep(V3 o, V3 d, float tmin, float tmax) {
 // vertices); culling only beyond far * (1 + 2^-15) keeps every box that holds
 // the brute-force closest hit.
 constexpr float SLAB_SLACK = 1.000030517578125f;
+// Sphere scenes: the reference's sphere test computes d2 = l.l - tca^2, whose
+// cancellation at a distance L moves a grazing hit's t by up to about
+// L * 2^-11 (and can accept a ray just outside the box); a 2^-9 relative
+// slack keeps every box whose sphere's computed t can still beat the best hit.
+constexpr float SPHERE_SLACK = 1.001953125f;
 // Near/far planes chosen by the sign of 1/d, t = (plane - o) * (1/d). A zero
 // direction component gives +-inf, or NaN when the origin lies on the plane;
 // the NaN-ignoring max (near) and min (far) then treat that axis as "inside",
 // which is exact for a ray parallel to the slab.
-inline bool slab(const RayPre& r, const float* b, float thi, float* tnear) {
+inline bool slab(const RayPre& r, const float* b, float thi, float* tnear,
+                 float slack = SLAB_SLACK) {
   float nx = ((r.nx ? b[3] : b[0]) - r.o.x) * r.inv.x, fx = ((r.nx ? b[0] : b[3]) - r.o.x) * r.inv.x;
   float ny = ((r.ny ? b[4] : b[1]) - r.o.y) * r.inv.y, fy = ((r.ny ? b[1] : b[4]) - r.o.y) * r.inv.y;
   float nz = ((r.nz ? b[5] : b[2]) - r.o.z) * r.inv.z, fz = ((r.nz ? b[2] : b[5]) - r.o.z) * r.inv.z;
   float tn = fmx(fmx(nx, ny), fmx(nz, r.tmin));
   float tf = fmn(fmn(fx, fy), fmn(fz, thi));
   *tnear = tn;
-  return tn <= tf * SLAB_SLACK;
+  return tn <= tf * slack;
 }
 
+}  // namespace
+
+// NONE loop over spheres (main.cpp:376-386): index order, strictly smaller t.
+// box_filter != 0 keeps only spheres whose box (centre -/+ r) the ray hits by
+// the slab test (sphere slack) with the far bound at tmax: the candidates of the reference's
+// BVH path (boxIntersect, main.cpp:343-358). The two differ only where the
+// sphere test's d2 = l.l - tca^2 loses its precision to cancellation and
+// accepts a ray that misses the sphere's box.
+extern "C" void or_closest_brute_spheres(const float* cx, const float* cy, const float* cz,
+                                         const float* r, uint32_t n, const or_rays* rays,
+                                         uint32_t nrays, or_hits* hits, int box_filter,
+                                         int nthreads) {
+  parallel_for(nrays, nthreads, [&](uint32_t i) {
+    V3 o, d;
+    float tmin, tmax;
+    load_ray(rays, i, &o, &d, &tmin, &tmax);
+    const RayPre rp = prep(o, d, tmin, tmax);
+    int32_t best = -1;
+    float bt = tmax;
+    for (uint32_t p = 0; p < n && tmin < tmax; ++p) {
+      float t;
+      if (!sphere_e({cx[p], cy[p], cz[p]}, r[p] * r[p], o, d, tmin, tmax, &t)) continue;
+      if (box_filter) {
+        const float b[6] = {(cx[p] - r[p]) + 0.0f, (cy[p] - r[p]) + 0.0f, (cz[p] - r[p]) + 0.0f,
+                            (cx[p] + r[p]) + 0.0f, (cy[p] + r[p]) + 0.0f, (cz[p] + r[p]) + 0.0f};
+        float tn;
+        if (!slab(rp, b, tmax, &tn, SPHERE_SLACK)) continue;
+      }
+      if (best < 0 || t < bt) best = (int32_t)p, bt = t;
+    }
+    hits->prim[i] = best;
+    if (hits->t) hits->t[i] = best >= 0 ? bt : INFINITY;
+    if (hits->u) hits->u[i] = 0.0f;
+    if (hits->v) hits->v[i] = 0.0f;
+  });
+}
+
+namespace {
 // Canonical traversal: start at internal node 0; for an internal node load
 // both child records (2 node visits), cull each against [tmin, best t], go to
 // the nearer hit child and push the other (ties go left); a leaf is one
@@ -521,7 +650,8 @@ int32_t trace(const or_bvh* b, const RayPre& r, float* bt, float* bu, float* bv,
       const float* b0 = &b->boxes[6 * (size_t)(c0 >= 0 ? c0 : (int32_t)(n - 1) + ~c0)];
       const float* b1 = &b->boxes[6 * (size_t)(c1 >= 0 ? c1 : (int32_t)(n - 1) + ~c1)];
       float tn0, tn1;
-      bool h0 = slab(r, b0, t_best, &tn0), h1 = slab(r, b1, t_best, &tn1);
+      const float slack = b->kind == 1 ? SPHERE_SLACK : SLAB_SLACK;
+      bool h0 = slab(r, b0, t_best, &tn0, slack), h1 = slab(r, b1, t_best, &tn1, slack);
       if (h0 && h1) {
         if (tn1 < tn0) {
           stack[sp++] = c0;
@@ -542,9 +672,12 @@ int32_t trace(const or_bvh* b, const RayPre& r, float* bt, float* bu, float* bv,
       uint32_t k = (uint32_t)~node;
       ++tests;
       const float* T = &b->tri[9 * (size_t)k];
-      float t, u, v;
-      if (mt_e({T[0], T[1], T[2]}, {T[3], T[4], T[5]}, {T[6], T[7], T[8]}, r.o, r.d, r.tmin,
-               r.tmax, &t, &u, &v)) {
+      float t, u = 0, v = 0;
+      const bool hit = b->kind == 1
+                           ? sphere_e({T[0], T[1], T[2]}, T[4], r.o, r.d, r.tmin, r.tmax, &t)
+                           : mt_e({T[0], T[1], T[2]}, {T[3], T[4], T[5]}, {T[6], T[7], T[8]}, r.o,
+                                  r.d, r.tmin, r.tmax, &t, &u, &v);
+      if (hit) {
         int32_t prim = (int32_t)b->sorted_idx[k];
         if (ANY) {
           best = prim;
@@ -758,7 +891,10 @@ inline HitGeo hit_geo(const or_bvh* b, uint32_t leaf, V3 o, V3 d, float t) {
   const float* T = &b->tri[9 * (size_t)leaf];
   HitGeo g;
   g.p = add(o, mul(d, t));
-  g.n = normalize(cross({T[3], T[4], T[5]}, {T[6], T[7], T[8]}));
+  if (b->kind == 1)
+    g.n = normalize(sub(g.p, {T[0], T[1], T[2]}));  // main.cpp:397-398
+  else
+    g.n = normalize(cross({T[3], T[4], T[5]}, {T[6], T[7], T[8]}));
   if (dot(d, g.n) > 0) g.n = {-g.n.x, -g.n.y, -g.n.z};
   return g;
 }
@@ -767,7 +903,8 @@ inline V3 shadow_origin(V3 d, const HitGeo& g, float bias) {
 }
 // Direct light at a hit: one point light, Lambert + Phong^25, diffuse
 // constant getDiffuseColor(0.2) = (0.815, 0.235, 0.031) (accelerators.h:94-99).
-inline V3 direct(V3 d, const HitGeo& g, const or_shade_params* sp, bool in_shadow) {
+inline V3 direct(V3 d, const HitGeo& g, const or_shade_params* sp, int32_t prim,
+                 bool in_shadow) {
   V3 L{sp->light[0], sp->light[1], sp->light[2]}, E{sp->emission[0], sp->emission[1],
                                                     sp->emission[2]};
   V3 ld = sub(L, g.p);
@@ -779,9 +916,11 @@ inline V3 direct(V3 d, const HitGeo& g, const or_shade_params* sp, bool in_shado
   float sp25 = std::pow(std::max(0.f, -dot(R, d)), 25.0f);
   const float diff[3] = {0.815f * 0.8f, 0.235f * 0.8f, 0.031f * 0.8f};
   V3 c;
-  c.x = ((amt.x * diff[0]) / 2.0f + (sp25 * E.x) * 0.5f) + sp->surface[0];
-  c.y = ((amt.y * diff[1]) / 2.0f + (sp25 * E.y) * 0.5f) + sp->surface[1];
-  c.z = ((amt.z * diff[2]) / 2.0f + (sp25 * E.z) * 0.5f) + sp->surface[2];
+  // sphere->surfaceColor (main.cpp:490): per primitive when given
+  const float* S = sp->prim_rgb ? sp->prim_rgb + 3 * (size_t)prim : sp->surface;
+  c.x = ((amt.x * diff[0]) / 2.0f + (sp25 * E.x) * 0.5f) + S[0];
+  c.y = ((amt.y * diff[1]) / 2.0f + (sp25 * E.y) * 0.5f) + S[1];
+  c.z = ((amt.z * diff[2]) / 2.0f + (sp25 * E.z) * 0.5f) + S[2];
   return c;
 }
 inline RayPre shadow_ray(V3 d, const HitGeo& g, const or_shade_params* sp) {
@@ -823,11 +962,13 @@ extern "C" void or_render_samples(const or_bvh* b, const float* vx, const float*
       V3 col{sp->sky[0], sp->sky[1], sp->sky[2]};
       if (p0 >= 0) {
         HitGeo g = hit_geo(b, leaf[p0], o, d, t);
-        RayPre sr = shadow_ray(d, g, sp);
         float st, su, sv;
-        s0 = trace<true>(b, sr, &st, &su, &sv, &vi, &te) >= 0;
-        C[0]++, C[3] += vi, C[4] += te, C[6]++;
-        col = direct(d, g, sp, s0);
+        if (sp->shadows) {
+          RayPre sr = shadow_ray(d, g, sp);
+          s0 = trace<true>(b, sr, &st, &su, &sv, &vi, &te) >= 0;
+          C[0]++, C[3] += vi, C[4] += te, C[6]++;
+        }
+        col = direct(d, g, sp, p0, s0);
         if (sp->bounce) {
           V3 rd = normalize(reflect(d, g.n));
           V3 ro = dot(rd, g.n) < 0 ? sub(g.p, mul(g.n, sp->bias)) : add(g.p, mul(g.n, sp->bias));
@@ -838,10 +979,12 @@ extern "C" void or_render_samples(const or_bvh* b, const float* vx, const float*
           V3 bc{sp->sky[0], sp->sky[1], sp->sky[2]};
           if (p1 >= 0) {
             HitGeo g1 = hit_geo(b, leaf[p1], ro, rd, t1);
-            RayPre sr1 = shadow_ray(rd, g1, sp);
-            s1 = trace<true>(b, sr1, &st, &su, &sv, &vi, &te) >= 0;
-            C[0]++, C[3] += vi, C[4] += te, C[6]++;
-            bc = direct(rd, g1, sp, s1);
+            if (sp->shadows) {
+              RayPre sr1 = shadow_ray(rd, g1, sp);
+              s1 = trace<true>(b, sr1, &st, &su, &sv, &vi, &te) >= 0;
+              C[0]++, C[3] += vi, C[4] += te, C[6]++;
+            }
+            bc = direct(rd, g1, sp, p1, s1);
           }
           col = {col.x + kr * bc.x, col.y + kr * bc.y, col.z + kr * bc.z};
         }

@@ -21,6 +21,9 @@
  *   or_mt                          closed-form known answers (axis triangles)
  *   or_closest_* / or_any_*        brute force with lowest-index tie rule
  *   or_ray_sphere                  compiled reference Sphere::raySphereIntersect
+ *   or_*_brute_spheres             compiled reference NONE loop (castRay, fixture + live)
+ *   or_bvh_build_spheres           box-filtered brute force; reference castRay colours
+ *   normalize (camera, shading)    the reference's Vec3f::normalize and render() camera
  *   or_mt19937_jitter              C++ standard known answer + numpy MT19937
  *   or_philox4x32_10               Random123 published known-answer vectors
  *   or_shade_*                     closed-form single-light cases
@@ -76,10 +79,25 @@ void or_closest_brute(const float* vx, const float* vy, const float* vz, const u
 void or_any_brute(const float* vx, const float* vy, const float* vz, const uint32_t* idx,
                   uint32_t ntri, const or_rays* rays, uint32_t nrays, uint8_t* occ, int nthreads);
 
+/* NONE loop over spheres (main.cpp:376-386) with the sphere leaf of castRay:
+ * t0, or t1 when t0 < 0 (main.cpp:349-353), accepted in [tmin, tmax).
+ * box_filter: only spheres whose box the ray hits (the reference BVH path's
+ * candidates, main.cpp:343-358), the semantics of any box hierarchy. */
+void or_closest_brute_spheres(const float* cx, const float* cy, const float* cz, const float* r,
+                              uint32_t n, const or_rays* rays, uint32_t nrays, or_hits* hits,
+                              int box_filter, int nthreads);
+void or_any_brute_spheres(const float* cx, const float* cy, const float* cz, const float* r,
+                          uint32_t n, const or_rays* rays, uint32_t nrays, uint8_t* occ,
+                          int nthreads);
+
 /* Canonical LBVH traversal over an oracle-built tree. visits/tests may be NULL. */
 typedef struct or_bvh or_bvh;
 or_bvh* or_bvh_build(const float* vx, const float* vy, const float* vz, uint32_t nv,
                      const uint32_t* idx, uint32_t ntri, int morton_mode);
+/* Sphere scene (accelerators.h:54-92 objects): centroid = centre, leaf box
+ * centre -/+ r (main.cpp:687-689), the same Morton / sort / Karras / refit. */
+or_bvh* or_bvh_build_spheres(const float* cx, const float* cy, const float* cz, const float* r,
+                             uint32_t n, int morton_mode);
 void or_bvh_free(or_bvh* b);
 void or_bvh_arrays(const or_bvh* b, uint32_t* codes, uint32_t* sorted_idx, int32_t* left,
                    int32_t* right, float* boxes);
@@ -111,7 +129,9 @@ typedef struct {
   float surface[3];
   float sky[3];
   float bias;
-  int bounce; /* 0: primary + shadow (Lambert config); 1: + mirror bounce */
+  int bounce;  /* 0: primary + shadow (Lambert config); 1: + mirror bounce */
+  int shadows; /* 1: shadow ray per hit; 0: trace_more stub (main.cpp:235-245), never shadowed */
+  const float* prim_rgb; /* surface colour per primitive (3 floats), or NULL for `surface` */
 } or_shade_params;
 /* Full frame on the CPU: rays given, returns per-sample colour (3 floats) and the
    primary/bounce hits plus the shadow booleans. Any output pointer may be NULL. */

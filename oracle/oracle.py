@@ -31,7 +31,7 @@ class Hits(C.Structure):
 class ShadeParams(C.Structure):
     _fields_ = [("light", C.c_float * 3), ("emission", C.c_float * 3),
                 ("surface", C.c_float * 3), ("sky", C.c_float * 3), ("bias", C.c_float),
-                ("bounce", C.c_int)]
+                ("bounce", C.c_int), ("shadows", C.c_int), ("prim_rgb", C.c_void_p)]
 
 
 RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
@@ -65,6 +65,10 @@ def lib() -> C.CDLL:
                                         C.POINTER(Hits), i32]),
             "or_any_brute": (None, [vp, vp, vp, vp, u32, C.POINTER(Rays), u32, vp, i32]),
             "or_bvh_build": (vp, [vp, vp, vp, u32, vp, u32, i32]),
+            "or_bvh_build_spheres": (vp, [vp, vp, vp, vp, u32, i32]),
+            "or_closest_brute_spheres": (None, [vp, vp, vp, vp, u32, C.POINTER(Rays), u32,
+                                                C.POINTER(Hits), i32, i32]),
+            "or_any_brute_spheres": (None, [vp, vp, vp, vp, u32, C.POINTER(Rays), u32, vp, i32]),
             "or_bvh_free": (None, [vp]),
             "or_bvh_arrays": (None, [vp, vp, vp, vp, vp, vp]),
             "or_bvh_height": (u32, [vp]),
@@ -105,6 +109,8 @@ def ref() -> C.CDLL | None:
             ("ref_box_hit", i32, [vp, vp, vp, vp]),
             ("ref_sphere_closest", i32, [vp] * 4 + [u32] + [vp] * 6 + [u32, i32, vp, vp]),
             ("ref_random_double", None, [u32, vp]),
+            ("ref_sphere_scene_colors", None, [vp] * 5 + [u32] + [vp] * 6 + [u32, vp]),
+            ("ref_camera_dirs", None, [u32, u32, u32, vp, vp, vp, vp]),
         ]:
             fn = getattr(L, name)
             fn.restype = res
@@ -169,10 +175,21 @@ def mt(v0, v1, v2, o, d, tmin=0.0, tmax=np.inf):
     return bool(hit), t.value, u.value, v.value
 
 
+def sphere_ptrs(sph) -> tuple[list, list]:
+    keep = [_f32(sph.cx), _f32(sph.cy), _f32(sph.cz), _f32(sph.r)]
+    return [_p(a) for a in keep], keep
+
+
 class OracleBVH:
-    """The oracle's LBVH: Morton -> stable sort -> Karras -> refit."""
+    """The oracle's LBVH: Morton -> stable sort -> Karras -> refit. `mesh` is a
+    triangle mesh (vx, vy, vz, idx) or a sphere set (cx, cy, cz, r)."""
 
     def __init__(self, mesh, mode: int = 1):
+        if hasattr(mesh, "cx"):
+            (cx, cy, cz, r), self._keep = sphere_ptrs(mesh)
+            self.n = len(self._keep[0])
+            self.h = lib().or_bvh_build_spheres(cx, cy, cz, r, self.n, mode)
+            return
         (vx, vy, vz, idx), self._keep = mesh_ptrs(mesh)
         self.n = mesh.ntri
         self.h = lib().or_bvh_build(vx, vy, vz, mesh.nv, idx, mesh.ntri, mode)
@@ -218,22 +235,32 @@ class OracleBVH:
         return out
 
 
-def closest_brute(mesh, rays: dict, threads: int = 8) -> dict:
-    (vx, vy, vz, idx), keep = mesh_ptrs(mesh)
+def closest_brute(mesh, rays: dict, threads: int = 8, box_filter: bool = False) -> dict:
+    """NONE loop. Spheres: box_filter keeps only spheres whose box the ray hits."""
     rs, rk = rays_struct(rays)
     n = len(rk[0])
     out = dict(prim=np.empty(n, np.int32), t=np.empty(n, np.float32),
                u=np.empty(n, np.float32), v=np.empty(n, np.float32))
     h = Hits(_p(out["prim"]), _p(out["t"]), _p(out["u"]), _p(out["v"]))
+    if hasattr(mesh, "cx"):
+        (cx, cy, cz, r), keep = sphere_ptrs(mesh)
+        lib().or_closest_brute_spheres(cx, cy, cz, r, len(keep[0]), C.byref(rs), n, C.byref(h),
+                                       1 if box_filter else 0, threads)
+        return out
+    (vx, vy, vz, idx), keep = mesh_ptrs(mesh)
     lib().or_closest_brute(vx, vy, vz, idx, mesh.ntri, C.byref(rs), n, C.byref(h), threads)
     return out
 
 
 def any_brute(mesh, rays: dict, threads: int = 8) -> np.ndarray:
-    (vx, vy, vz, idx), keep = mesh_ptrs(mesh)
     rs, rk = rays_struct(rays)
     n = len(rk[0])
     occ = np.empty(n, np.uint8)
+    if hasattr(mesh, "cx"):
+        (cx, cy, cz, r), keep = sphere_ptrs(mesh)
+        lib().or_any_brute_spheres(cx, cy, cz, r, len(keep[0]), C.byref(rs), n, _p(occ), threads)
+        return occ
+    (vx, vy, vz, idx), keep = mesh_ptrs(mesh)
     lib().or_any_brute(vx, vy, vz, idx, mesh.ntri, C.byref(rs), n, _p(occ), threads)
     return occ
 
@@ -283,13 +310,22 @@ def philox(ctr, key) -> np.ndarray:
 
 
 def shade_params(light=(0, 3, 30), emission=(1, 1, 1), surface=(0, 0, 0), sky=(0.6, 0.8, 1.0),
-                 bias=1e-4, bounce=0) -> ShadeParams:
-    return ShadeParams((C.c_float * 3)(*light), (C.c_float * 3)(*emission),
-                       (C.c_float * 3)(*surface), (C.c_float * 3)(*sky), bias, bounce)
+                 bias=1e-4, bounce=0, shadows=1, prim_rgb=None) -> ShadeParams:
+    """prim_rgb: float32 array (3 per primitive) kept alive by the returned object."""
+    sp = ShadeParams((C.c_float * 3)(*light), (C.c_float * 3)(*emission),
+                     (C.c_float * 3)(*surface), (C.c_float * 3)(*sky), bias, bounce, shadows,
+                     None)
+    if prim_rgb is not None:
+        sp._rgb = _f32(prim_rgb)
+        sp.prim_rgb = _p(sp._rgb)
+    return sp
 
 
 def render_samples(bvh: OracleBVH, mesh, rays: dict, sp: ShadeParams, threads: int = 8) -> dict:
-    (vx, vy, vz, idx), keep = mesh_ptrs(mesh)
+    if hasattr(mesh, "cx"):
+        (vx, vy, vz, idx), keep = [None] * 4, None  # sphere leaves carry their centres
+    else:
+        (vx, vy, vz, idx), keep = mesh_ptrs(mesh)
     rs, rk = rays_struct(rays)
     n = len(rk[0])
     out = dict(rgb=np.empty(3 * n, np.float32), hit0=np.empty(n, np.int32),
@@ -329,3 +365,19 @@ class PortBVH:
         t = np.empty(n, np.float32)
         lib().port_closest(self.h, *(_p(a) for a in rk), n, _p(prim), _p(t), threads)
         return dict(prim=prim, t=t)
+
+
+def ref_sphere_scene_colors(sph, rays: dict) -> np.ndarray | None:
+    """castRay of the reference itself (NONE loop, DIFFUSE_AND_GLOSSY shading,
+    trace_more stub) over a sphere set: one colour per ray (3 floats), or
+    None when the reference build is absent."""
+    L = ref()
+    if L is None:
+        return None
+    keep = [_f32(sph.cx), _f32(sph.cy), _f32(sph.cz), _f32(sph.r), _f32(sph.rgb)]
+    rk = [_f32(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
+    n = len(rk[0])
+    out = np.empty(3 * n, np.float32)
+    L.ref_sphere_scene_colors(*(_p(a) for a in keep), len(keep[0]), *(_p(a) for a in rk), n,
+                              _p(out))
+    return out

@@ -91,6 +91,62 @@ int ref_sphere_closest(const float* cx, const float* cy, const float* cz, const 
   return totalNodes;
 }
 
+// castRay itself (main.cpp:291-500) over a sphere scene built like
+// createScene_new's objects (main.cpp:677-689, per-sphere surface colour),
+// with the render loop's light (main.cpp:775), the NONE data structure and
+// depth 1 (main.cpp:558): one colour per ray.
+void ref_sphere_scene_colors(const float* cx, const float* cy, const float* cz, const float* rad,
+                             const float* rgb, unsigned n, const float* ox, const float* oy,
+                             const float* oz, const float* dx, const float* dy, const float* dz,
+                             unsigned nrays, float* out) {
+  sceneFixed.clear();
+  std::vector<SceneObject> scene;
+  for (unsigned i = 0; i < n; ++i) {
+    SceneObject s;
+    s.objId = (int)i;
+    s.radius = rad[i];
+    s.center = Vec3f(cx[i], cy[i], cz[i]);
+    s.position = s.center;
+    s.shininess = 64;
+    s.isSphere = true;
+    s.boxBoundries = BoxBoundries(s.center - s.radius, s.center + s.radius);
+    s.sphere = Sphere((int)i, s.center, s.radius, Vec3f(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]),
+                      0, 0.0);
+    scene.push_back(s);
+    sceneFixed.push_back(s);
+  }
+  std::vector<Sphere> spheres;
+  std::vector<Sphere> lights{Sphere(0, Vec3f(0, 3, 30), 10, Vec3f(1, 1, 1), 0, 0.0, Vec3f(1))};
+  Settings settings;
+  settings.dataStructure = NONE;
+  std::shared_ptr<Node> root;
+  const int depth = 1;
+  for (unsigned r = 0; r < nrays; ++r) {
+    Vec3f c = castRay(Vec3f(ox[r], oy[r], oz[r]), Vec3f(dx[r], dy[r], dz[r]), spheres, lights,
+                      scene, root, depth, settings);
+    out[3 * r] = c.x, out[3 * r + 1] = c.y, out[3 * r + 2] = c.z;
+  }
+}
+
+// render()'s primary ray directions (main.cpp:542-556) through the
+// reference's own Vec3f::normalize, for given jitter pairs.
+void ref_camera_dirs(unsigned w, unsigned h, unsigned spp, const double* jitter, float* dx,
+                     float* dy, float* dz) {
+  float invWidth = 1 / float(w), invHeight = 1 / float(h);
+  float fov = 30, aspectratio = w / float(h);
+  float angle = tan(M_PI * 0.5 * fov / 180.);
+  size_t i = 0;
+  for (unsigned y = 0; y < h; ++y)
+    for (unsigned x = 0; x < w; ++x)
+      for (unsigned k = 0; k < spp; ++k, ++i) {
+        float xx = (2 * ((x + jitter[2 * i]) * invWidth) - 1) * angle * aspectratio;
+        float yy = (1 - 2 * ((y + jitter[2 * i + 1]) * invHeight)) * angle;
+        Vec3f raydir(xx, yy, -1);
+        raydir.normalize();
+        dx[i] = raydir.x, dy[i] = raydir.y, dz[i] = raydir.z;
+      }
+}
+
 // The reference's mt19937 + uniform_real_distribution jitter stream
 // (main.cpp:503-508): n successive random_double() values.
 void ref_random_double(unsigned n, double* out) {

@@ -85,6 +85,8 @@ void push_trace(std::vector<BatchWork>& w, const rq_batch_item& it, uint32_t m, 
 // the least ray work so far, largest first (LPT), and keep their index order
 // inside a segment.
 int make_work(rq_ctx* c, rq_batch* b, uint32_t ray_tile) {
+  // a captured graph holds this queue's phase bounds in its kernel arguments
+  drop_graph(b);
   std::vector<BatchWork> w;
   const uint32_t nm = (uint32_t)b->items.size();
   for (int ph = 0; ph < BATCH_PHASES; ++ph) {
@@ -182,6 +184,8 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     const rq_batch_item& it = items[i];
     if (!it.scene || it.scene->magic != SCENE_MAGIC)
       return set_err(c, RQ_E_INVALID, "item %u: bad scene", i);
+    if (it.scene->kind != RQ_PRIM_TRIANGLE)
+      return set_err(c, RQ_E_UNSUPPORTED, "item %u: batches hold triangle scenes", i);
     if (it.nrays && (!it.rays.ox || !it.rays.oy || !it.rays.oz || !it.rays.dx || !it.rays.dy ||
                      !it.rays.dz || !it.rays.tmin || !it.rays.tmax))
       return set_err(c, RQ_E_INVALID, "item %u: null ray array", i);
@@ -209,7 +213,7 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     const rq_bvh* v = b->bvhs[i];
     const rq_scene* s = it.scene;
     BatchMeshDev& d = md[i];
-    d.v = BuildView{s->vx, s->vy, s->vz, s->idx, v->n, mode, v->B};
+    d.v = BuildView{s->vx, s->vy, s->vz, s->idx, v->n, mode, v->B, s->r, s->r != nullptr};
     d.rays = it.rays;
     d.nrays = it.nrays;
     d.prim = it.hits.prim, d.t = it.hits.t, d.u = it.hits.u, d.v_ = it.hits.v;
@@ -272,13 +276,13 @@ int rq_batch_run(rq_ctx* c, rq_batch* b, void* stream) {
 int rq_batch_run_graph(rq_ctx* c, rq_batch* b, void* stream) {
   if (!c || !b) return RQ_E_INVALID;
   hipStream_t s = stream ? (hipStream_t)stream : b->own_stream;
+  if (b->built_ray_tile != b->ray_tile) {
+    int rc = make_work(c, b, b->ray_tile);  // also drops the graph
+    if (rc) return rc;
+  }
   if (!b->exec || b->graph_variant != b->variant || b->graph_ray_tile != b->ray_tile ||
       b->graph_refill != b->refill) {
     drop_graph(b);
-    if (b->built_ray_tile != b->ray_tile) {
-      int rc = make_work(c, b, b->ray_tile);
-      if (rc) return rc;
-    }
     hipError_t e = hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hip_err(c, e, "graph capture begin");
     int rc = launch(c, b, b->own_stream, false);

@@ -80,6 +80,7 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   t.err = err;
   t.refill_idle = refill_idle;
   t.chunk = 64;
+  t.sphere = 0;  // batch meshes are triangle scenes
   if (threadIdx.x == 0) *s_ctr = 0u;
   __syncthreads();
   int32_t* s_node = reinterpret_cast<int32_t*>(lds);
@@ -88,13 +89,15 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   const uint32_t gstride = gridDim.x * BUILD_BLOCK;
   if (any) {
     t.occ = m.occ + r0;
-    trace_loop<true, false, BUILD_BLOCK, DEPTH, false>(t, s_ctr, s_node, s_tn, gtid, gstride);
+    trace_loop<true, false, BUILD_BLOCK, DEPTH, false, false>(t, s_ctr, s_node, s_tn, gtid,
+                                                              gstride);
   } else {
     t.prim = m.prim + r0;
     t.t = m.t ? m.t + r0 : nullptr;
     t.u = m.u ? m.u + r0 : nullptr;
     t.v = m.v_ ? m.v_ + r0 : nullptr;
-    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK>(t, s_ctr, s_node, s_tn, gtid, gstride);
+    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false>(t, s_ctr, s_node, s_tn, gtid,
+                                                                gstride);
   }
   __syncthreads();
 }

@@ -25,12 +25,17 @@ __host__ __device__ constexpr int sort_lds_words(int items) {
 constexpr int BUILD_LDS_WORDS = sort_lds_words(SORT_ITEMS);
 
 // Everything one mesh build reads and writes.
+// Triangles: vx/vy/vz vertices, idx 3 per primitive, vr null. Spheres
+// (the reference's primitive, accelerators.h:54-92): vx/vy/vz centres, vr
+// radii, idx unused.
 struct BuildView {
   const float *vx, *vy, *vz;
   const uint32_t* idx;
   uint32_t n;
   int mode;
   BuildBuffers B;
+  const float* vr = nullptr;
+  int32_t sphere = 0;  // vr is set (an integer flag: device code tests it, not the pointer)
 };
 
 // ---- helpers ------------------------------------------------------------------
@@ -76,7 +81,10 @@ __device__ inline uint32_t morton30(float x, float y, float z) {
   return (spread3(quant10(x)) << 2) | (spread3(quant10(y)) << 1) | spread3(quant10(z));
 }
 
+// Triangle centroid ((v0 + v1) + v2) / 3 per axis; a sphere's centre
+// (main.cpp:680-683 places each object at its centre). Zeros canonicalised.
 __device__ inline float3 centroid(const BuildView& v, uint32_t t) {
+  if (v.sphere) return make_float3(v.vx[t] + 0.0f, v.vy[t] + 0.0f, v.vz[t] + 0.0f);
   const uint32_t a = v.idx[3 * t], b = v.idx[3 * t + 1], c = v.idx[3 * t + 2];
   float3 r;
   r.x = ((v.vx[a] + v.vx[b]) + v.vx[c]) / 3.0f + 0.0f;
@@ -463,19 +471,31 @@ __device__ inline void refit_local_tile(const BuildView& v, uint32_t chunk, uint
   // REFIT_CHUNK threads, one leaf each; the batch kernel BUILD_BLOCK)
   for (uint32_t k = chunk_lo + threadIdx.x; k <= chunk_hi; k += blockDim.x) {
     const uint32_t p = B.vals_a[k];
-    const uint32_t a = v.idx[3 * p], b = v.idx[3 * p + 1], c = v.idx[3 * p + 2];
-    const float3 v0 = make_float3(v.vx[a], v.vy[a], v.vz[a]);
-    const float3 v1 = make_float3(v.vx[b], v.vy[b], v.vz[b]);
-    const float3 v2 = make_float3(v.vx[c], v.vy[c], v.vz[c]);
     TriRec tr;
-    tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
-    tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
-    tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
+    float lo[3], hi[3];
+    if (v.sphere) {
+      // sphere leaf: centre and radius, box centre -/+ radius (main.cpp:687-689);
+      // the record keeps r and r*r (the reference's radius2)
+      const float cx = v.vx[p], cy = v.vy[p], cz = v.vz[p], r = v.vr[p];
+      tr.v0 = make_float4(cx, cy, cz, __int_as_float((int32_t)p));
+      tr.e1 = make_float4(r, r * r, 0.0f, 0.0f);
+      tr.e2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      lo[0] = (cx - r) + 0.0f, lo[1] = (cy - r) + 0.0f, lo[2] = (cz - r) + 0.0f;
+      hi[0] = (cx + r) + 0.0f, hi[1] = (cy + r) + 0.0f, hi[2] = (cz + r) + 0.0f;
+    } else {
+      const uint32_t a = v.idx[3 * p], b = v.idx[3 * p + 1], c = v.idx[3 * p + 2];
+      const float3 v0 = make_float3(v.vx[a], v.vy[a], v.vz[a]);
+      const float3 v1 = make_float3(v.vx[b], v.vy[b], v.vz[b]);
+      const float3 v2 = make_float3(v.vx[c], v.vy[c], v.vz[c]);
+      tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
+      tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
+      tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
+      lo[0] = fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, lo[1] = fminf(fminf(v0.y, v1.y), v2.y) + 0.0f;
+      lo[2] = fminf(fminf(v0.z, v1.z), v2.z) + 0.0f;
+      hi[0] = fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, hi[1] = fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f;
+      hi[2] = fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f;
+    }
     B.tris[k] = tr;
-    float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, fminf(fminf(v0.y, v1.y), v2.y) + 0.0f,
-                   fminf(fminf(v0.z, v1.z), v2.z) + 0.0f};
-    float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f,
-                   fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f};
     if (n == 1) {
       put_root(B.misc, lo, hi, 0u);
       break;

@@ -47,10 +47,10 @@ __global__ __launch_bounds__(REFIT_TOP_BLOCK) void k_refit_top(BuildView v) { re
 }  // namespace
 
 hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, const float* vz,
-                        const uint32_t* idx, uint32_t n, int mode, BuildBuffers& B) {
+                        const uint32_t* idx, uint32_t n, int mode, BuildBuffers& B, const float* vr) {
   if (n == 0) return hipSuccess;
   hipStream_t s = lc.stream;
-  const BuildView v{vx, vy, vz, idx, n, mode, B};
+  const BuildView v{vx, vy, vz, idx, n, mode, B, vr, vr != nullptr};
   const uint32_t ntiles = tiles_of(n, SORT_TILE);
   const uint32_t nstatus = SORT_PASSES * ntiles * 256;
   {

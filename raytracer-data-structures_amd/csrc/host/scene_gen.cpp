@@ -1,12 +1,14 @@
 // Host-side scene generators and OBJ I/O (include/rq/rq_scene.h).
 // The reference loads meshes in createScene_new (main.cpp:599-721), reading
 // only `v` lines (it stops at the first other token, :694-697) and making a
-// sphere per vertex. The build's scenes are triangle meshes: generated here
-// deterministically, or read from OBJ with both `v` and `f` records.
+// sphere per vertex: rq_gen_reference_scene rebuilds that sphere scene. The
+// triangle scenes are generated here deterministically, or read from OBJ with
+// both `v` and `f` records.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -306,6 +308,63 @@ void rq_mesh_bounds(const rq_mesh* m, float lo[3], float hi[3]) {
       hi[k] = std::fmax(hi[k], v[k]);
     }
   }
+}
+
+int rq_gen_reference_scene(const char* path, uint32_t clones, uint32_t max_vertices,
+                           rq_spheres* out) {
+  if (!path || !out || clones == 0) return RQ_E_INVALID;
+  std::memset(out, 0, sizeof *out);
+  std::vector<float> v;  // x y z per vertex of one copy of the file
+  {
+    std::ifstream f(path);
+    if (!f.good()) return RQ_E_IO;
+    // main.cpp:665-703: whitespace-separated tokens; a `v` token is followed
+    // by three floats; the first other token (or the end) stops the reading
+    for (;;) {
+      std::string tok;
+      f >> tok;
+      if (tok != "v" || (max_vertices && v.size() / 3 >= max_vertices)) break;
+      float x = 0, y = 0, z = 0;
+      f >> x >> y >> z;
+      v.push_back(x), v.push_back(y), v.push_back(z);
+    }
+  }
+  const size_t nv = v.size() / 3, n = nv * clones + 1;
+  if (n > 0xFFFFFFFFull) return RQ_E_INVALID;
+  out->cx = (float*)std::malloc(n * 4);
+  out->cy = (float*)std::malloc(n * 4);
+  out->cz = (float*)std::malloc(n * 4);
+  out->r = (float*)std::malloc(n * 4);
+  out->rgb = (float*)std::malloc(n * 12);
+  if (!out->cx || !out->cy || !out->cz || !out->r || !out->rgb) {
+    rq_spheres_free(out);
+    return RQ_E_NOMEM;
+  }
+  size_t k = 0;
+  const float radius = (float)(0.01 * 5);
+  for (uint32_t clone = 0; clone < clones; ++clone) {
+    const float shift = (float)(clone * 20);
+    for (size_t i = 0; i < nv; ++i, ++k) {
+      // s.center = vertex * 100 + shift; s.center.y += -10; s.center.z += -60
+      out->cx[k] = v[3 * i] * 100.0f + shift;
+      out->cy[k] = (v[3 * i + 1] * 100.0f + shift) + -10.0f;
+      out->cz[k] = (v[3 * i + 2] * 100.0f + shift) + -60.0f;
+      out->r[k] = radius;
+      out->rgb[3 * k] = 0.8f, out->rgb[3 * k + 1] = 0.7f, out->rgb[3 * k + 2] = 0.0f;
+    }
+  }
+  // the reference's double literals narrowed to float (Vec3f constructor)
+  out->cx[k] = (float)0.93591022, out->cy[k] = (float)-105.47120094, out->cz[k] = (float)-43.2363205;
+  out->r[k] = 100.0f;
+  out->rgb[3 * k] = out->rgb[3 * k + 1] = out->rgb[3 * k + 2] = 0.0f;
+  out->n = (uint32_t)n;
+  return 0;
+}
+
+void rq_spheres_free(rq_spheres* s) {
+  if (!s) return;
+  std::free(s->cx), std::free(s->cy), std::free(s->cz), std::free(s->r), std::free(s->rgb);
+  std::memset(s, 0, sizeof *s);
 }
 
 void rq_mesh_free(rq_mesh* m) {

@@ -84,6 +84,7 @@ void rq_render_defaults(rq_render_params* p) {
   p->sky[0] = 0.6f, p->sky[1] = 0.8f, p->sky[2] = 1.0f;
   p->bias = 1e-4f;
   p->bounce = 0;
+  p->shadows = 1;
 }
 
 int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) {
@@ -105,6 +106,7 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
   }
   r->sh.bias = p->bias;
   r->sh.bounce = p->bounce;
+  r->sh.shadows = p->shadows;
   const size_t ns = p->bounce ? 2 * n : n;
   int rc = alloc_rays(c, r, n, r->prim);
   if (!rc && p->bounce) rc = alloc_rays(c, r, n, r->bnc);
@@ -153,7 +155,7 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   if (!c || !r || !b) return RQ_E_INVALID;
   LaunchCtx lc = launch_ctx(c, stream);
   const rq_scene* s = b->scene;
-  SceneView sv{s->vx, s->vy, s->vz, s->idx};
+  SceneView sv{s->vx, s->vy, s->vz, s->idx, s->r, s->rgb};
   hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, r->prim);
   if (e != hipSuccess) return hip_err(c, e, "raygen");
   rq_rays pr = as_rays(r->prim);
@@ -170,9 +172,11 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad);
     if (e != hipSuccess) return hip_err(c, e, "shade bounce");
   }
-  rq_rays sr = as_rays(r->shad);
-  rc = rq_any_hit(c, b, &sr, r->p.bounce ? 2 * r->n : r->n, r->occ, stream);
-  if (rc) return rc;
+  if (r->p.shadows) {
+    rq_rays sr = as_rays(r->shad);
+    rc = rq_any_hit(c, b, &sr, r->p.bounce ? 2 * r->n : r->n, r->occ, stream);
+    if (rc) return rc;
+  }
   e = launch_shade_final(lc, sv, r->sh, r->npix, r->p.spp, r->prim, r->bnc, r->h0, r->t0, r->h1,
                          r->t1, r->occ, r->srgb, r->img);
   if (e != hipSuccess) return hip_err(c, e, "shade final");
@@ -191,7 +195,7 @@ int rq_render_counts(rq_ctx* c, rq_renderer* r, rq_frame_counts* out) {
   if (e != hipSuccess) return hip_err(c, e, "counts");
   out->primary = r->n;
   out->bounce = r->p.bounce ? k[0] : 0;
-  out->shadow = (uint64_t)k[0] + k[1];
+  out->shadow = r->p.shadows ? (uint64_t)k[0] + k[1] : 0;
   return RQ_OK;
 }
 
@@ -220,9 +224,14 @@ int rq_render_read_samples(rq_ctx* c, rq_renderer* r, int32_t* hit0, int32_t* hi
     else
       for (size_t i = 0; i < n; ++i) hit1[i] = -1;
   }
-  if (e == hipSuccess && sh0) e = hipMemcpy(sh0, r->occ, n, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && sh0) {
+    if (r->p.shadows)
+      e = hipMemcpy(sh0, r->occ, n, hipMemcpyDeviceToHost);
+    else
+      std::memset(sh0, 0, n);
+  }
   if (e == hipSuccess && sh1) {
-    if (r->p.bounce)
+    if (r->p.bounce && r->p.shadows)
       e = hipMemcpy(sh1, r->occ + n, n, hipMemcpyDeviceToHost);
     else
       std::memset(sh1, 0, n);

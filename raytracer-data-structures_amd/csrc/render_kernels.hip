@@ -22,10 +22,13 @@ __device__ inline float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z *
 __device__ inline V3 cross(V3 a, V3 b) {
   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+// Vec3::normalize (geometry.h:125-134): its unqualified sqrt inside the
+// template resolves to the C double sqrt, so factor = (float)(1.0 / sqrt(n))
+// is computed in double (one rounding to float), then multiplied in float.
 __device__ inline V3 normalize(V3 a) {
   float n = (a.x * a.x + a.y * a.y) + a.z * a.z;
   if (n > 0.0f) {
-    float f = 1.0f / sqrtf(n);
+    const float f = (float)(1.0 / sqrt((double)n));
     a.x *= f, a.y *= f, a.z *= f;
   }
   return a;
@@ -135,16 +138,26 @@ struct Geo {
   V3 p, n;
 };
 __device__ inline V3 vtx(const SceneView& s, uint32_t i) { return {s.vx[i], s.vy[i], s.vz[i]}; }
-// Hit point o + d*t and the geometric normal normalize(e1 x e2) turned to face
-// against the ray (main.cpp:396-406 with the triangle normal for the sphere's).
+// Hit point o + d*t and the normal turned to face against the ray
+// (main.cpp:396-406): a sphere's normalize(p - centre), a triangle's
+// geometric normal normalize(e1 x e2).
 __device__ inline Geo hit_geo(const SceneView& s, int32_t prim, V3 o, V3 d, float t) {
-  const uint32_t a = s.idx[3 * prim], b = s.idx[3 * prim + 1], c = s.idx[3 * prim + 2];
-  const V3 v0 = vtx(s, a), v1 = vtx(s, b), v2 = vtx(s, c);
   Geo g;
   g.p = add(o, mul(d, t));
-  g.n = normalize(cross(sub(v1, v0), sub(v2, v0)));
+  if (s.r) {
+    g.n = normalize(sub(g.p, vtx(s, (uint32_t)prim)));
+  } else {
+    const uint32_t a = s.idx[3 * prim], b = s.idx[3 * prim + 1], c = s.idx[3 * prim + 2];
+    const V3 v0 = vtx(s, a), v1 = vtx(s, b), v2 = vtx(s, c);
+    g.n = normalize(cross(sub(v1, v0), sub(v2, v0)));
+  }
   if (dot(d, g.n) > 0.0f) g.n = {-g.n.x, -g.n.y, -g.n.z};
   return g;
+}
+// sphere->surfaceColor (main.cpp:490): per primitive when the scene has colours.
+__device__ inline V3 surface_of(const SceneView& s, const ShadeParams& sp, int32_t prim) {
+  if (s.rgb) return {s.rgb[3 * prim], s.rgb[3 * prim + 1], s.rgb[3 * prim + 2]};
+  return {sp.surface[0], sp.surface[1], sp.surface[2]};
 }
 __device__ inline V3 reflect(V3 I, V3 N) { return sub(I, mul(N, 2.0f * dot(I, N))); }
 // main.cpp:85-103
@@ -166,7 +179,7 @@ __device__ inline float fresnel(V3 I, V3 N, float ior) {
   return (Rs * Rs + Rp * Rp) / 2.0f;
 }
 // main.cpp:463-490 for one light; diffuse = getDiffuseColor(0.2) * 0.8.
-__device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, bool shadow) {
+__device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, V3 surf, bool shadow) {
   const V3 L{sp.light[0], sp.light[1], sp.light[2]};
   const V3 ld = normalize(sub(L, g.p));
   const float LdotN = max0(dot(ld, g.n));
@@ -177,9 +190,9 @@ __device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, bool shad
   const float s25 = powf(max0(-dot(R, d)), 25.0f);
   const float k0 = 0.815f * 0.8f, k1 = 0.235f * 0.8f, k2 = 0.031f * 0.8f;
   V3 c;
-  c.x = ((amt.x * k0) / 2.0f + (s25 * sp.emission[0]) * 0.5f) + sp.surface[0];
-  c.y = ((amt.y * k1) / 2.0f + (s25 * sp.emission[1]) * 0.5f) + sp.surface[1];
-  c.z = ((amt.z * k2) / 2.0f + (s25 * sp.emission[2]) * 0.5f) + sp.surface[2];
+  c.x = ((amt.x * k0) / 2.0f + (s25 * sp.emission[0]) * 0.5f) + surf.x;
+  c.y = ((amt.y * k1) / 2.0f + (s25 * sp.emission[1]) * 0.5f) + surf.y;
+  c.z = ((amt.z * k2) / 2.0f + (s25 * sp.emission[2]) * 0.5f) + surf.z;
   return c;
 }
 __device__ inline void put_ray(const RaySoA& R, uint64_t i, V3 o, V3 d, float tmin, float tmax) {
@@ -240,7 +253,7 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
   }
   const V3 o = get_o(prim, i), d = get_d(prim, i);
   const Geo g = hit_geo(s, p, o, d, t0[i]);
-  shadow_ray(shad, i, d, g, sp);
+  if (sp.shadows) shadow_ray(shad, i, d, g, sp);
   if (sp.bounce) {
     const V3 rd = normalize(reflect(d, g.n));
     const V3 ro =
@@ -263,7 +276,7 @@ __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams s
   }
   const V3 o = get_o(bnc, i), d = get_d(bnc, i);
   const Geo g = hit_geo(s, p, o, d, t1[i]);
-  shadow_ray(shad, (uint64_t)n + i, d, g, sp);
+  if (sp.shadows) shadow_ray(shad, (uint64_t)n + i, d, g, sp);
 }
 
 // Final: per pixel, colour every sample and sum them in sample order
@@ -289,7 +302,7 @@ __global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp
     if (p >= 0) {
       const V3 o = get_o(prim, i), d = get_d(prim, i);
       const Geo g = hit_geo(s, p, o, d, t0[i]);
-      col = direct(d, g, sp, occ[i] != 0);
+      col = direct(d, g, sp, surface_of(s, sp, p), sp.shadows && occ[i] != 0);
       if (sp.bounce) {
         const float kr = fresnel(d, g.n, 2.0f);
         V3 bc = sky;
@@ -297,7 +310,7 @@ __global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp
         if (q >= 0) {
           const V3 o1 = get_o(bnc, i), d1 = get_d(bnc, i);
           const Geo g1 = hit_geo(s, q, o1, d1, t1[i]);
-          bc = direct(d1, g1, sp, occ[n + i] != 0);
+          bc = direct(d1, g1, sp, surface_of(s, sp, q), sp.shadows && occ[n + i] != 0);
         }
         col = {col.x + kr * bc.x, col.y + kr * bc.y, col.z + kr * bc.z};
       }

@@ -262,6 +262,52 @@ int rq_scene_upload(rq_ctx* c, const float* vx, const float* vy, const float* vz
   return RQ_OK;
 }
 
+int rq_scene_upload_spheres(rq_ctx* c, const float* cx, const float* cy, const float* cz,
+                            const float* radius, const float* rgb, uint32_t n, rq_scene** out,
+                            void* stream) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (!out) return set_err(c, RQ_E_INVALID, "null output handle");
+  *out = nullptr;
+  if (n > MAX_PRIMS) return set_err(c, RQ_E_INVALID, "too many spheres (%u)", n);
+  if (n && (!cx || !cy || !cz || !radius)) return set_err(c, RQ_E_INVALID, "null sphere array");
+  hipSetDevice(c->device);
+  auto* s = new rq_scene();
+  s->magic = SCENE_MAGIC;
+  s->kind = RQ_PRIM_SPHERE;
+  s->nv = s->ntri = n;
+  const size_t fb = align256((size_t)n * 4), cb = rgb ? align256((size_t)n * 12) : 0;
+  void* p;
+  int rc = ctx_alloc(c, 4 * fb + cb, &p);
+  if (rc) {
+    delete s;
+    return rc;
+  }
+  s->base = p;
+  s->bytes = 4 * fb + cb;
+  char* q = (char*)p;
+  s->vx = (float*)q, s->vy = (float*)(q + fb), s->vz = (float*)(q + 2 * fb);
+  s->r = (float*)(q + 3 * fb);
+  if (rgb) s->rgb = (float*)(q + 4 * fb);
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipSuccess;
+  if (n) {
+    e = hipMemcpyAsync(s->vx, cx, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->vy, cy, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->vz, cz, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(s->r, radius, (size_t)n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && rgb)
+      e = hipMemcpyAsync(s->rgb, rgb, (size_t)n * 12, hipMemcpyHostToDevice, st);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // host arrays may go away after return
+  if (e != hipSuccess) {
+    ctx_free(c, p);
+    delete s;
+    return hip_err(c, e, "sphere upload");
+  }
+  *out = s;
+  return RQ_OK;
+}
+
 int rq_scene_update_vertices(rq_ctx* c, rq_scene* s, const float* vx, const float* vy,
                              const float* vz, void* stream) {
   if (check_ctx(c) || !s || s->magic != SCENE_MAGIC) return set_err(c, RQ_E_INVALID, "bad scene");
@@ -306,7 +352,7 @@ int rq_rebuild_lbvh(rq_ctx* c, rq_bvh* b, void* stream) {
   if (check_ctx(c) || !b || b->magic != BVH_MAGIC) return set_err(c, RQ_E_INVALID, "bad bvh");
   const rq_scene* s = b->scene;
   hipError_t e = launch_build(launch_ctx(c, stream), s->vx, s->vy, s->vz, s->idx, b->n, b->mode,
-                              b->B);
+                              b->B, s->r);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "lbvh build launch");
 }
 
@@ -415,6 +461,7 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.grid = c->trace_grid[variant];
   a.refill_idle = c->refill_idle;
   a.chunk = trace_chunk(n, a.grid * (uint32_t)blk / 64u);
+  a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
   hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "trace launch");
 }

@@ -28,9 +28,12 @@ inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct rq_scene {
   uint32_t magic = 0;  // first member: handle type tag
-  uint32_t nv = 0, ntri = 0;
-  float *vx = nullptr, *vy = nullptr, *vz = nullptr;
-  uint32_t* idx = nullptr;
+  uint32_t nv = 0, ntri = 0;  // spheres: nv = ntri = number of spheres
+  int kind = 0;               // RQ_PRIM_TRIANGLE / RQ_PRIM_SPHERE
+  float *vx = nullptr, *vy = nullptr, *vz = nullptr;  // vertices, or sphere centres
+  uint32_t* idx = nullptr;                            // triangles only
+  float* r = nullptr;                                 // sphere radii
+  float* rgb = nullptr;                               // optional surface colour per primitive
   void* base = nullptr;
   size_t bytes = 0;
 };

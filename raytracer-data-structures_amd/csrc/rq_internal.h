@@ -122,8 +122,10 @@ struct LaunchCtx {
   Profiler* prof;  // may be null
 };
 
+// vr: sphere radii (spheres centred at vx/vy/vz), or null for triangles.
 hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, const float* vz,
-                        const uint32_t* idx, uint32_t n, int morton_mode, BuildBuffers& B);
+                        const uint32_t* idx, uint32_t n, int morton_mode, BuildBuffers& B,
+                        const float* vr = nullptr);
 
 struct TraceArgs {
   const NodeRec* nodes;
@@ -143,6 +145,7 @@ struct TraceArgs {
   uint32_t grid;        // blocks (persistent grid of the chosen variant)
   uint32_t refill_idle; // idle lanes in a wave before it refills from its slice
   uint32_t chunk;       // rays per slice taken from the global counter
+  int32_t sphere;       // leaves are spheres (centre, r, r*r) instead of triangles
 };
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
                         int variant);

@@ -22,11 +22,16 @@ struct ShadeParams {
   float light[3], emission[3], surface[3], sky[3];
   float bias;
   int32_t bounce;
+  int32_t shadows;
 };
 
+// Triangles: vertices + indices. Spheres: centres in vx/vy/vz, radii in r.
+// rgb: optional surface colour per primitive (else ShadeParams::surface).
 struct SceneView {
   const float *vx, *vy, *vz;
   const uint32_t* idx;
+  const float* r;
+  const float* rgb;
 };
 
 struct RaySoA {

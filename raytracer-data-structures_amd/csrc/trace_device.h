@@ -25,19 +25,25 @@ struct RayState {
 // vertices); culling only beyond far * (1 + 2^-15) keeps every box that holds
 // the brute-force closest hit.
 constexpr float SLAB_SLACK = 1.000030517578125f;
+// Sphere scenes: the reference's sphere test computes d2 = l.l - tca^2, whose
+// cancellation at a distance L moves a grazing hit's t by up to about
+// L * 2^-11 (and can accept a ray just outside the box); culling at the best
+// hit with a 2^-9 relative slack keeps every box whose sphere's computed t
+// can still beat the best hit.
+constexpr float SPHERE_SLACK = 1.001953125f;
 
 // Slab test (same operations and order as the oracle): near/far planes by
 // the sign of 1/d, t = (plane - o) * (1/d). A zero direction component gives
 // +-inf, or NaN with the origin on the plane, which the NaN-ignoring max
 // (near) / min (far) read as "inside". Hit when near <= far * SLAB_SLACK.
 __device__ inline bool slab(const float4& lo, const float4& hi, const RayState& r, float thi,
-                            float& tn) {
+                            float slack, float& tn) {
   const float nx = ((r.sx ? hi.x : lo.x) - r.ox) * r.ix, fx = ((r.sx ? lo.x : hi.x) - r.ox) * r.ix;
   const float ny = ((r.sy ? hi.y : lo.y) - r.oy) * r.iy, fy = ((r.sy ? lo.y : hi.y) - r.oy) * r.iy;
   const float nz = ((r.sz ? hi.z : lo.z) - r.oz) * r.iz, fz = ((r.sz ? lo.z : hi.z) - r.oz) * r.iz;
   tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
   const float tf = fminf(fminf(fx, fy), fminf(fz, thi));
-  return tn <= tf * SLAB_SLACK;
+  return tn <= tf * slack;
 }
 
 // Möller–Trumbore, main.cpp:137-162 (v_0 read as v0), t restricted to
@@ -63,6 +69,26 @@ __device__ inline bool mt(const TriRec& T, const RayState& r, float& t, float& u
   return true;
 }
 
+// Geometric ray-sphere test of the reference (accelerators.h:79-92): a centre
+// behind the origin (tca < 0) or a miss distance beyond the radius rejects;
+// otherwise t0 = tca - thc, t1 = tca + thc, and castRay takes t1 when t0 < 0
+// (main.cpp:349-353). The reference accepts that t whatever its value (it is
+// >= 0 once tca >= 0); here it must lie in [tmin, tmax).
+__device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) {
+  const float lx = S.v0.x - r.ox, ly = S.v0.y - r.oy, lz = S.v0.z - r.oz;
+  const float tca = (lx * r.dx + ly * r.dy) + lz * r.dz;
+  if (tca < 0.0f) return false;
+  const float d2 = ((lx * lx + ly * ly) + lz * lz) - tca * tca;
+  const float r2 = S.e1.y;
+  if (d2 > r2) return false;
+  const float thc = sqrtf(r2 - d2);
+  const float t0 = tca - thc, t1 = tca + thc;
+  const float tt = t0 < 0.0f ? t1 : t0;
+  if (!(tt >= r.tmin && tt < r.tmax)) return false;
+  t = tt;
+  return true;
+}
+
 // Persistent traversal loop of one wave. BLOCK threads per block; DEPTH stack entries
 // per lane in LDS ([depth][lane], conflict-free), deeper entries in the global
 // overflow area ([depth][thread]); TSTACK: entries carry their entry distance
@@ -71,12 +97,14 @@ __device__ inline bool mt(const TriRec& T, const RayState& r, float& t, float& u
 // the same loop iteration; the order of node visits and triangle tests is the
 // canonical one unless TSTACK skips an entry. Rays are taken a.chunk at a time
 // from *ctr (global or LDS); the loop returns when *ctr passes a.nrays.
-template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK>
+// SPHERES: the leaves may be spheres (a.sphere); false compiles triangles only.
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true>
 __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_node,
                                   float* s_tn, uint32_t gtid, uint32_t gstride) {
   const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
   const int max_sp = DEPTH + (int)a.gdepth;
+  const float slack = (SPHERES && a.sphere) ? SPHERE_SLACK : SLAB_SLACK;
 
   RayState r;
   bool active = false;
@@ -131,7 +159,7 @@ __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_
         node = a.gstack[2 * g];
         if (TSTACK) tn = __int_as_float(a.gstack[2 * g + 1]);
       }
-      if (!TSTACK || ANY || tn <= bt * SLAB_SLACK) return true;
+      if (!TSTACK || ANY || tn <= bt * slack) return true;
     }
     return false;
   };
@@ -178,8 +206,8 @@ __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_
         const float4 lo0 = q[0], hi0 = q[1], lo1 = q[2], hi1 = q[3];
         if (COUNT) vis += 2;
         float tn0, tn1;
-        const bool h0 = slab(lo0, hi0, r, bt, tn0);
-        const bool h1 = slab(lo1, hi1, r, bt, tn1);
+        const bool h0 = slab(lo0, hi0, r, bt, slack, tn0);
+        const bool h1 = slab(lo1, hi1, r, bt, slack, tn1);
         const int32_t c0 = __float_as_int(lo0.w), c1 = __float_as_int(hi0.w);
         if (h0 && h1) {
           const bool swap = tn1 < tn0;
@@ -197,9 +225,9 @@ __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_
       while (active && node < 0) {
         const TriRec T = a.tris[~node];
         if (COUNT) ++tst;
-        float t, u, v;
+        float t, u = 0.0f, v = 0.0f;
         bool done = false;
-        if (mt(T, r, t, u, v)) {
+        if ((SPHERES && a.sphere) ? sphere_hit(T, r, t) : mt(T, r, t, u, v)) {
           const int32_t prim = __float_as_int(T.v0.w);
           if (ANY) {
             best = prim, bt = t;

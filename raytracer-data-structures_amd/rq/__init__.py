@@ -85,7 +85,14 @@ class RenderParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32),
                 ("camera", C.c_int32), ("seed", C.c_uint64), ("light", C.c_float * 3),
                 ("emission", C.c_float * 3), ("surface", C.c_float * 3),
-                ("sky", C.c_float * 3), ("bias", C.c_float), ("bounce", C.c_int32)]
+                ("sky", C.c_float * 3), ("bias", C.c_float), ("bounce", C.c_int32),
+                ("shadows", C.c_int32)]
+
+
+class Spheres(C.Structure):
+    _fields_ = [("cx", C.POINTER(C.c_float)), ("cy", C.POINTER(C.c_float)),
+                ("cz", C.POINTER(C.c_float)), ("r", C.POINTER(C.c_float)),
+                ("rgb", C.POINTER(C.c_float)), ("n", C.c_uint32)]
 
 
 class FrameCounts(C.Structure):
@@ -206,6 +213,9 @@ def _load() -> C.CDLL:
         "rq_raygen_uniform": (i32, [vp, u32, u64, f32p, f32p, vp, vp, vp, vp, vp, vp, vp, vp,
                                     vp]),
         "rq_write_ppm": (i32, [C.c_char_p, u32, u32, vp]),
+        "rq_scene_upload_spheres": (i32, [vp, vp, vp, vp, vp, vp, u32, C.POINTER(vp), vp]),
+        "rq_gen_reference_scene": (i32, [C.c_char_p, u32, u32, C.POINTER(Spheres)]),
+        "rq_spheres_free": (None, [C.POINTER(Spheres)]),
         "rq_batch_create": (i32, [vp, C.POINTER(BatchItem), u32, i32, C.POINTER(vp)]),
         "rq_batch_destroy": (i32, [vp, vp]),
         "rq_batch_run": (i32, [vp, vp, vp]),
@@ -275,6 +285,43 @@ class MeshArrays:
     def bounds(self) -> tuple[np.ndarray, np.ndarray]:
         v = np.stack([self.vx, self.vy, self.vz])
         return v.min(axis=1), v.max(axis=1)
+
+
+@dataclass
+class SphereArrays:
+    """Spheres: centres, radii, surface colour (3 per sphere)."""
+    cx: np.ndarray
+    cy: np.ndarray
+    cz: np.ndarray
+    r: np.ndarray
+    rgb: np.ndarray
+
+    @property
+    def n(self) -> int:
+        return int(self.cx.size)
+
+    @property
+    def ntri(self) -> int:  # primitives, as for meshes
+        return self.n
+
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        c = np.stack([self.cx, self.cy, self.cz])
+        return (c - self.r).min(axis=1), (c + self.r).max(axis=1)
+
+
+def reference_scene(obj_path: str | Path, clones: int = 1, max_vertices: int = 0) -> SphereArrays:
+    """The reference's sphere scene from an OBJ file (rq_gen_reference_scene)."""
+    s = Spheres()
+    _check_gen(lib().rq_gen_reference_scene(str(obj_path).encode(), clones, max_vertices,
+                                            C.byref(s)), f"reference_scene {obj_path}")
+    try:
+        n = s.n
+        arr = lambda p, k: np.ctypeslib.as_array(p, (k,)).copy()  # noqa: E731
+        out = SphereArrays(arr(s.cx, n), arr(s.cy, n), arr(s.cz, n), arr(s.r, n),
+                           arr(s.rgb, 3 * n))
+    finally:
+        lib().rq_spheres_free(C.byref(s))
+    return out
 
 
 def _take_mesh(m: Mesh) -> MeshArrays:
@@ -443,6 +490,14 @@ class Context:
         s = C.c_void_p()
         self._check(lib().rq_scene_upload(self.h, _ptr(vx), _ptr(vy), _ptr(vz), vx.size,
                                           _ptr(idx), idx.size // 3, 0, C.byref(s), stream))
+        return s.value
+
+    def upload_spheres(self, sph: SphereArrays, colors: bool = True, stream=None) -> int:
+        a = [np.ascontiguousarray(x, np.float32) for x in (sph.cx, sph.cy, sph.cz, sph.r, sph.rgb)]
+        s = C.c_void_p()
+        self._check(lib().rq_scene_upload_spheres(self.h, _ptr(a[0]), _ptr(a[1]), _ptr(a[2]),
+                                                  _ptr(a[3]), _ptr(a[4]) if colors else None,
+                                                  a[0].size, C.byref(s), stream))
         return s.value
 
     def build_lbvh(self, scene: int, morton_mode: int = MORTON_SCENE_AABB, stream=None) -> int:

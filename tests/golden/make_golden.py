@@ -11,6 +11,12 @@ ref_vectors.npz   outputs of the reference's own code (compiled from its
                   through constructBVHNew + boxIntersect and the NONE loop.
 oracle_small.npz  the CPU oracle's LBVH and query results on a small mesh
                   (regression vectors for oracle and GPU alike).
+ref_sphere_scene.npz  the reference's own scene (createScene_new on the
+                  first 8000 vertices of its models/bunny.obj, plus the ground
+                  sphere) seen by its camera at 128x96, one jittered sample
+                  per pixel: the reference camera's ray directions, the
+                  reference's castRay colour (NONE loop) and closest sphere /
+                  t of every primary ray.
 Each file holds plain numeric arrays (numpy .npz, no pickled objects).
 """
 from __future__ import annotations
@@ -121,7 +127,41 @@ def oracle_small() -> dict:
     return out
 
 
+REFERENCE_MODELS = Path("/root/reference/project/raytracer/models")
+
+
+def ref_sphere_scene() -> dict:
+    ref = O.ref()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libref.so missing: run make -C oracle with the reference")
+    sph = rq.reference_scene(REFERENCE_MODELS / "bunny.obj", 1, 8000)
+    w, h = 128, 96
+    n = w * h
+    jit = O.mt19937_jitter(2 * n)
+    # primary rays from the reference's own camera expression and Vec3f
+    dirs = [np.empty(n, np.float32) for _ in range(3)]
+    ref.ref_camera_dirs(w, h, 1, O._p(jit), *(O._p(a) for a in dirs))
+    rays = dict(ox=np.zeros(n, np.float32), oy=np.zeros(n, np.float32),
+                oz=np.zeros(n, np.float32), dx=dirs[0], dy=dirs[1], dz=dirs[2],
+                tmin=np.zeros(n, np.float32), tmax=np.full(n, np.inf, np.float32))
+    colors = O.ref_sphere_scene_colors(sph, rays)
+    hit = np.empty(n, np.int32)
+    t = np.empty(n, np.float32)
+    cols = [np.ascontiguousarray(a, np.float32) for a in (sph.cx, sph.cy, sph.cz, sph.r)]
+    rk = [np.ascontiguousarray(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
+    ref.ref_sphere_closest(*(O._p(a) for a in cols), sph.n, *(O._p(a) for a in rk), n, 0,
+                           O._p(hit), O._p(t))
+    return dict(cx=sph.cx, cy=sph.cy, cz=sph.cz, r=sph.r, rgb=sph.rgb, width=np.int32(w),
+                height=np.int32(h), dx=dirs[0], dy=dirs[1], dz=dirs[2], colors=colors, hit=hit,
+                t=t)
+
+
+GENERATORS = {"ref_vectors": ref_vectors, "oracle_small": oracle_small,
+              "ref_sphere_scene": ref_sphere_scene}
+
+
 if __name__ == "__main__":
-    np.savez_compressed(HERE / "ref_vectors.npz", **ref_vectors())
-    np.savez_compressed(HERE / "oracle_small.npz", **oracle_small())
-    print("wrote", HERE / "ref_vectors.npz", HERE / "oracle_small.npz")
+    names = sys.argv[1:] or list(GENERATORS)
+    for name in names:
+        np.savez_compressed(HERE / f"{name}.npz", **GENERATORS[name]())
+        print("wrote", HERE / f"{name}.npz")

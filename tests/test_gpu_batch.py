@@ -211,6 +211,26 @@ def test_batch_every_config(setup, rq, tile):
     setup.batch.set_config(0, 32768, 32)
 
 
+def test_batch_graph_after_queue_rebuild(setup):
+    """A graph captured for one ray tile, then a plain run with another tile
+    (the work queue is rebuilt), then the first configuration again: the
+    replay must see a queue built for its own tile."""
+    log("batch graph after work-queue rebuild")
+    setup.batch.set_config(0, 2048, 16)
+    setup.clear()
+    setup.batch.run_graph()
+    setup.check(setup.results(), "graph tile 2048")
+    setup.batch.set_config(0, 32768, 16)
+    setup.clear()
+    setup.batch.run()
+    setup.check(setup.results(), "run tile 32768")
+    setup.batch.set_config(0, 2048, 16)
+    setup.clear()
+    setup.batch.run_graph()
+    setup.check(setup.results(), "graph tile 2048 again")
+    assert setup.batch.errors() == 0
+
+
 def test_batch_autotune(setup):
     log("batch autotune")
     v, t, r, ms = setup.batch.autotune(1)
