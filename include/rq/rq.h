@@ -109,6 +109,21 @@ int rq_bvh_info_get(rq_ctx* ctx, const rq_bvh* b, rq_bvh_info* info);
 int rq_bvh_export(rq_ctx* ctx, const rq_bvh* b, uint32_t* codes, uint32_t* sorted_idx,
                   int32_t* left, int32_t* right, float* boxes);
 
+/* ---- cached BVH blob ----------------------------------------------------
+ * The reference rebuilds its tree on every run (main.cpp:791-843). A built
+ * LBVH can instead be saved as bytes (a 64-byte header with the primitive
+ * kind, Morton mode, counts, a fingerprint of the scene's arrays and a
+ * checksum, then codes, sorted indices, parent links, nodes, leaf records and
+ * the root box) and loaded into a new BVH of the same scene without a build.
+ * Load errors: RQ_E_IO for a damaged or foreign blob, RQ_E_INVALID when the
+ * scene does not match the fingerprint. */
+int rq_bvh_blob_size(rq_ctx* ctx, const rq_bvh* b, size_t* bytes);
+int rq_bvh_save(rq_ctx* ctx, const rq_bvh* b, void* blob, size_t capacity, size_t* written);
+int rq_bvh_load(rq_ctx* ctx, const rq_scene* s, const void* blob, size_t bytes, rq_bvh** out,
+                void* stream);
+int rq_bvh_save_file(rq_ctx* ctx, const rq_bvh* b, const char* path);
+int rq_bvh_load_file(rq_ctx* ctx, const rq_scene* s, const char* path, rq_bvh** out);
+
 /* ---- queries -----------------------------------------------------------
  * Closest hit replaces boxIntersect (accelerators.h:668-690) plus the
  * candidate loop of castRay (main.cpp:343-358); the result equals the NONE

@@ -44,6 +44,11 @@ int rq_gen_soup(uint32_t ntri, uint64_t seed, float mean_edge, float sd_edge, fl
 /* Wavefront OBJ: `v x y z` and `f a b c ...` (1-based or negative indices,
  * `a/b/c` forms, polygons fanned into triangles); other lines ignored. */
 int rq_obj_read(const char* path, rq_mesh* out);
+/* The same reader with an explicit thread count (0 = up to 16, 1 = one pass on
+ * the calling thread): the file is memory-mapped and parsed in line-aligned
+ * chunks, relative indices resolved against the vertices before each record;
+ * the result is identical for every thread count. */
+int rq_obj_read_ex(const char* path, int threads, rq_mesh* out);
 int rq_obj_write(const char* path, const rq_mesh* m);
 
 /* Spheres (host SoA): centres, radii and a surface colour (3 floats) each. */

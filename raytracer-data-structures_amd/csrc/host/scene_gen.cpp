@@ -4,12 +4,19 @@
 // sphere per vertex: rq_gen_reference_scene rebuilds that sphere scene. The
 // triangle scenes are generated here deterministically, or read from OBJ with
 // both `v` and `f` records.
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -235,56 +242,178 @@ int rq_gen_soup(uint32_t ntri, uint64_t seed, float mean_edge, float sd_edge, fl
   return export_mesh(std::move(m), out);
 }
 
-int rq_obj_read(const char* path, rq_mesh* out) {
-  if (!path || !out) return RQ_E_INVALID;
-  std::memset(out, 0, sizeof *out);
-  FILE* f = std::fopen(path, "rb");
-  if (!f) return RQ_E_IO;
-  std::fseek(f, 0, SEEK_END);
-  long len = std::ftell(f);
-  std::fseek(f, 0, SEEK_SET);
-  std::string buf((size_t)(len > 0 ? len : 0), '\0');
-  size_t got = len > 0 ? std::fread(&buf[0], 1, (size_t)len, f) : 0;
-  std::fclose(f);
-  if ((long)got != len) return RQ_E_IO;
-  Mesh m;
-  const char* p = buf.c_str();
-  const char* end = p + buf.size();
+// OBJ parsing in parallel over a memory-mapped file. Pass 1 splits the file
+// at line boundaries into chunks and counts each chunk's `v` records and
+// triangles (fan of each `f` record); prefix sums give every chunk its first
+// vertex and first triangle. Pass 2 parses the chunks into the final arrays:
+// an `f` record's relative (negative) indices and range check use the number
+// of vertices before that record, as a sequential reader would.
+namespace {
+
+struct ObjChunk {
+  const char *b, *e;
+  uint64_t nv = 0, nt = 0;    // pass 1 counts
+  uint64_t v0 = 0, t0 = 0;    // prefix offsets
+  bool bad = false;
+};
+
+inline const char* skip_ws(const char* p, const char* e) {
+  while (p < e && (*p == ' ' || *p == '\t')) ++p;
+  return p;
+}
+
+inline bool is_rec(const char* p, const char* eol, char k) {
+  return p + 1 < eol && p[0] == k && (p[1] == ' ' || p[1] == '\t');
+}
+
+// vertices of an `f` record (tokens up to the end of line)
+inline int64_t face_count(const char* q, const char* eol) {
+  int64_t k = 0;
+  while (q < eol) {
+    while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
+    if (q >= eol) break;
+    ++k;
+    while (q < eol && *q != ' ' && *q != '\t' && *q != '\r') ++q;
+  }
+  return k;
+}
+
+void count_chunk(ObjChunk& c) {
+  for (const char* p = c.b; p < c.e;) {
+    const char* eol = (const char*)std::memchr(p, '\n', (size_t)(c.e - p));
+    if (!eol) eol = c.e;
+    p = skip_ws(p, eol);
+    if (is_rec(p, eol, 'v')) {
+      ++c.nv;
+    } else if (is_rec(p, eol, 'f')) {
+      const int64_t k = face_count(p + 1, eol);
+      if (k >= 3) c.nt += (uint64_t)(k - 2);
+    }
+    p = eol + 1;
+  }
+}
+
+void parse_chunk(ObjChunk& c, const char* file_end, float* x, float* y, float* z, uint32_t* idx) {
+  uint64_t nv = c.v0, nt = c.t0;
   std::vector<int64_t> poly;
-  while (p < end) {
-    const char* eol = (const char*)std::memchr(p, '\n', (size_t)(end - p));
-    if (!eol) eol = end;
-    while (p < eol && (*p == ' ' || *p == '\t')) ++p;
-    if (p + 1 < eol && p[0] == 'v' && (p[1] == ' ' || p[1] == '\t')) {
+  for (const char* p = c.b; p < c.e && !c.bad;) {
+    const char* eol = (const char*)std::memchr(p, '\n', (size_t)(c.e - p));
+    if (!eol) eol = c.e;
+    p = skip_ws(p, eol);
+    if (is_rec(p, eol, 'v')) {
       char* q;
-      float x = std::strtof(p + 1, &q);
-      float y = std::strtof(q, &q);
-      float z = std::strtof(q, &q);
-      if (q > eol) return RQ_E_IO;
-      m.x.push_back(x), m.y.push_back(y), m.z.push_back(z);
-    } else if (p + 1 < eol && p[0] == 'f' && (p[1] == ' ' || p[1] == '\t')) {
+      const float a = std::strtof(p + 1, &q);
+      const float b = std::strtof(q, &q);
+      const float d = std::strtof(q, &q);
+      if (q > eol || q > file_end) {
+        c.bad = true;
+        break;
+      }
+      x[nv] = a, y[nv] = b, z[nv] = d;
+      ++nv;
+    } else if (is_rec(p, eol, 'f')) {
       poly.clear();
       const char* q = p + 1;
       while (q < eol) {
         while (q < eol && (*q == ' ' || *q == '\t' || *q == '\r')) ++q;
         if (q >= eol) break;
         char* r;
-        long long v = std::strtoll(q, &r, 10);
-        if (r == q) return RQ_E_IO;
-        int64_t nv = (int64_t)m.x.size();
-        int64_t i0 = v > 0 ? v - 1 : nv + v;
-        if (v == 0 || i0 < 0 || i0 >= nv) return RQ_E_IO;
+        const long long v = std::strtoll(q, &r, 10);
+        const int64_t i0 = v > 0 ? v - 1 : (int64_t)nv + v;
+        if (r == q || v == 0 || i0 < 0 || i0 >= (int64_t)nv) {
+          c.bad = true;
+          break;
+        }
         poly.push_back(i0);
         q = r;
         while (q < eol && *q != ' ' && *q != '\t' && *q != '\r') ++q;  // skip /vt/vn
       }
-      for (size_t k = 1; k + 1 < poly.size(); ++k)
-        m.idx.insert(m.idx.end(), {(uint32_t)poly[0], (uint32_t)poly[k], (uint32_t)poly[k + 1]});
+      for (size_t k = 1; k + 1 < poly.size() && !c.bad; ++k, ++nt) {
+        idx[3 * nt] = (uint32_t)poly[0];
+        idx[3 * nt + 1] = (uint32_t)poly[k];
+        idx[3 * nt + 2] = (uint32_t)poly[k + 1];
+      }
     }
     p = eol + 1;
   }
+}
+
+}  // namespace
+
+int rq_obj_read_ex(const char* path, int threads, rq_mesh* out) {
+  if (!path || !out) return RQ_E_INVALID;
+  std::memset(out, 0, sizeof *out);
+  const int fd = ::open(path, O_RDONLY);
+  if (fd < 0) return RQ_E_IO;
+  struct stat sb;
+  if (::fstat(fd, &sb) != 0) {
+    ::close(fd);
+    return RQ_E_IO;
+  }
+  const size_t len = (size_t)sb.st_size;
+  const long page = ::sysconf(_SC_PAGESIZE);
+  // strtof may look one byte past a last line without a newline: the mapping
+  // gives zero bytes after the end unless the file ends on a page boundary,
+  // in which case the file is read into a terminated buffer instead
+  std::string copy;
+  const char* data = nullptr;
+  void* map = MAP_FAILED;
+  if (len && page > 0 && len % (size_t)page != 0) {
+    map = ::mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);
+    if (map != MAP_FAILED) data = (const char*)map;
+  }
+  if (!data && len) {
+    copy.resize(len);
+    size_t got = 0;
+    while (got < len) {
+      const ssize_t r = ::pread(fd, &copy[got], len - got, (off_t)got);
+      if (r <= 0) break;
+      got += (size_t)r;
+    }
+    if (got != len) {
+      ::close(fd);
+      return RQ_E_IO;
+    }
+    data = copy.c_str();
+  }
+  ::close(fd);
+  const char* end = data + len;
+  int nt = threads > 0 ? threads : (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  nt = (int)std::max<size_t>(1, std::min<size_t>((size_t)nt, len / (256 << 10) + 1));
+  std::vector<ObjChunk> ch;
+  for (int k = 0; k < nt && len; ++k) {
+    const char* b = k == 0 ? data : ch.back().e;
+    const char* e = k == nt - 1 ? end : data + len * (size_t)(k + 1) / (size_t)nt;
+    if (e < b) e = b;
+    while (e < end && e > data && e[-1] != '\n') ++e;  // end after a newline
+    ObjChunk c;
+    c.b = b, c.e = e;
+    ch.push_back(c);
+  }
+  auto run = [&](auto&& f) {
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < ch.size(); ++k) th.emplace_back([&, k] { f(ch[k]); });
+    if (!ch.empty()) f(ch[0]);
+    for (auto& t : th) t.join();
+  };
+  run([](ObjChunk& c) { count_chunk(c); });
+  uint64_t nv = 0, ntri = 0;
+  for (auto& c : ch) c.v0 = nv, c.t0 = ntri, nv += c.nv, ntri += c.nt;
+  int rc = RQ_OK;
+  if (nv > 0xFFFFFFFFull || ntri > 0xFFFFFFFFull / 3) rc = RQ_E_INVALID;
+  Mesh m;
+  if (!rc) {
+    m.x.resize(nv), m.y.resize(nv), m.z.resize(nv), m.idx.resize(3 * ntri);
+    run([&](ObjChunk& c) { parse_chunk(c, end, m.x.data(), m.y.data(), m.z.data(), m.idx.data()); });
+    for (auto& c : ch)
+      if (c.bad) rc = RQ_E_IO;
+  }
+  if (map != MAP_FAILED) ::munmap(map, len);
+  if (rc) return rc;
   return export_mesh(std::move(m), out);
 }
+
+int rq_obj_read(const char* path, rq_mesh* out) { return rq_obj_read_ex(path, 0, out); }
 
 int rq_obj_write(const char* path, const rq_mesh* m) {
   if (!path || !m) return RQ_E_INVALID;

@@ -43,7 +43,7 @@ CAMERA_PHILOX = 1
 
 RQ_OK = 0
 ERRORS = {-1: "RQ_E_INVALID", -2: "RQ_E_NOMEM", -3: "RQ_E_HIP", -4: "RQ_E_IO",
-          -5: "RQ_E_NODEVICE", -6: "RQ_E_UNSUPPORTED"}
+          -5: "RQ_E_NODEVICE", -6: "RQ_E_UNSUPPORTED", -7: "RQ_E_DEVICE"}
 
 
 class RQError(RuntimeError):
@@ -196,6 +196,7 @@ def _load() -> C.CDLL:
                                          C.c_float, C.POINTER(Mesh)]),
         "rq_gen_soup": (i32, [u32, u64, C.c_float, C.c_float, C.c_float, C.POINTER(Mesh)]),
         "rq_obj_read": (i32, [C.c_char_p, C.POINTER(Mesh)]),
+        "rq_obj_read_ex": (i32, [C.c_char_p, i32, C.POINTER(Mesh)]),
         "rq_obj_write": (i32, [C.c_char_p, C.POINTER(Mesh)]),
         "rq_mesh_bounds": (None, [C.POINTER(Mesh), f32p, f32p]),
         "rq_mesh_free": (None, [C.POINTER(Mesh)]),
@@ -214,6 +215,11 @@ def _load() -> C.CDLL:
                                     vp]),
         "rq_write_ppm": (i32, [C.c_char_p, u32, u32, vp]),
         "rq_scene_upload_spheres": (i32, [vp, vp, vp, vp, vp, vp, u32, C.POINTER(vp), vp]),
+        "rq_bvh_blob_size": (i32, [vp, vp, C.POINTER(C.c_size_t)]),
+        "rq_bvh_save": (i32, [vp, vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
+        "rq_bvh_load": (i32, [vp, vp, vp, C.c_size_t, C.POINTER(vp), vp]),
+        "rq_bvh_save_file": (i32, [vp, vp, C.c_char_p]),
+        "rq_bvh_load_file": (i32, [vp, vp, C.c_char_p, C.POINTER(vp)]),
         "rq_gen_reference_scene": (i32, [C.c_char_p, u32, u32, C.POINTER(Spheres)]),
         "rq_spheres_free": (None, [C.POINTER(Spheres)]),
         "rq_batch_create": (i32, [vp, C.POINTER(BatchItem), u32, i32, C.POINTER(vp)]),
@@ -365,9 +371,10 @@ def soup(ntri: int, seed: int, mean_edge=0.02, sd_edge=0.005, min_edge=0.002) ->
     return _take_mesh(m)
 
 
-def obj_read(path: str | Path) -> MeshArrays:
+def obj_read(path: str | Path, threads: int = 0) -> MeshArrays:
+    """Parallel memory-mapped OBJ reader (threads 0 = automatic, 1 = sequential)."""
     m = Mesh()
-    _check_gen(lib().rq_obj_read(str(path).encode(), C.byref(m)), f"obj_read {path}")
+    _check_gen(lib().rq_obj_read_ex(str(path).encode(), threads, C.byref(m)), f"obj_read {path}")
     return _take_mesh(m)
 
 
@@ -523,6 +530,27 @@ class Context:
                                         _ptr(right), _ptr(boxes)))
         return dict(codes=codes, sorted_idx=sidx, left=left, right=right,
                     boxes=boxes.reshape(-1, 6))
+
+    def bvh_save(self, bvh: int) -> bytes:
+        """The built LBVH as a blob (rq_bvh_save)."""
+        n = C.c_size_t()
+        self._check(lib().rq_bvh_blob_size(self.h, bvh, C.byref(n)))
+        buf = (C.c_char * n.value)()
+        self._check(lib().rq_bvh_save(self.h, bvh, buf, n.value, None))
+        return bytes(buf)
+
+    def bvh_load(self, scene: int, blob: bytes) -> int:
+        b = C.c_void_p()
+        self._check(lib().rq_bvh_load(self.h, scene, blob, len(blob), C.byref(b), None))
+        return b.value
+
+    def bvh_save_file(self, bvh: int, path) -> None:
+        self._check(lib().rq_bvh_save_file(self.h, bvh, str(path).encode()))
+
+    def bvh_load_file(self, scene: int, path) -> int:
+        b = C.c_void_p()
+        self._check(lib().rq_bvh_load_file(self.h, scene, str(path).encode(), C.byref(b)))
+        return b.value
 
     def release(self, handle: int) -> None:
         self._check(lib().rq_release(self.h, handle))
