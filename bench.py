@@ -625,7 +625,7 @@ def sweep(ctx, bvh, batches) -> None:
 
     saved = ctx.trace_config()
     for v in ctx.trace_variants():
-        for refill in (8, 16, 32, 48):
+        for refill in (16, 48, 56, 64):
             row = []
             for name, rays in batches.items():
                 anyh = name == "shadow"

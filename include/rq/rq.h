@@ -108,6 +108,12 @@ int rq_bvh_info_get(rq_ctx* ctx, const rq_bvh* b, rq_bvh_info* info);
  * leaves, (lo xyz, hi xyz). */
 int rq_bvh_export(rq_ctx* ctx, const rq_bvh* b, uint32_t* codes, uint32_t* sorted_idx,
                   int32_t* left, int32_t* right, float* boxes);
+/* Download the 4-wide nodes (see rq_trace_variant): for binary internal node
+ * i, boxes[i*24 + slot*6 + (lo xyz, hi xyz)] and links[i*4 + slot] (>= 0 wide
+ * node, < 0 leaf ~k, 0 unused slot). Only the root and nodes at even depth
+ * are defined; the rest of each array is left as the device holds it.
+ * RQ_E_UNSUPPORTED for a BVH without wide nodes. */
+int rq_bvh_export_wide(rq_ctx* ctx, const rq_bvh* b, float* boxes, int32_t* links);
 
 /* ---- cached BVH blob ----------------------------------------------------
  * The reference rebuilds its tree on every run (main.cpp:791-843). A built
@@ -142,18 +148,23 @@ int rq_any_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32
                        uint8_t* occluded, uint32_t* visits, uint32_t* tests, void* stream);
 
 /* ---- traversal kernel variants and the autotuner ----------------------
- * Variants differ in threads per block, LDS stack entries per lane, and
- * whether stack entries carry their entry distance (closest hit skips
- * entries beyond the best hit when popped). Results are identical for all. */
+ * Variants differ in threads per block, LDS stack entries per lane, whether
+ * stack entries carry their entry distance (closest hit skips entries
+ * beyond the best hit when popped), and the node format walked: binary
+ * (64-byte nodes, two children) or wide (the binary tree collapsed into
+ * 128-byte 4-child nodes, written by every rq_build_lbvh / rq_rebuild_lbvh /
+ * rq_bvh_load). Results are identical for all. A BVH without wide nodes (one
+ * owned by a batch) is walked by variant 0 when a wide variant is
+ * configured. */
 typedef struct {
-  int32_t block, lds_depth, tstack;
+  int32_t block, lds_depth, tstack, wide;
 } rq_trace_variant;
 int rq_trace_variant_count(void);
 int rq_trace_variant_get(int index, rq_trace_variant* out);
 int rq_trace_config_set(rq_ctx* ctx, int closest_variant, int any_variant, uint32_t refill_idle);
 int rq_trace_config_get(rq_ctx* ctx, int* closest_variant, int* any_variant,
                         uint32_t* refill_idle);
-/* Times every variant (and refill threshold 8/16/32/48 idle lanes) on the given
+/* Times every variant (and refill threshold 16/48/56/64 idle lanes) on the given
  * device rays (default stream, `repeats` launches each after one warm-up),
  * keeps the fastest in the context and reports it. */
 int rq_autotune_trace(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, int any,

@@ -15,7 +15,7 @@ using namespace rq;
 
 namespace {
 constexpr uint32_t kRayTiles[] = {2048, 4096, 8192, 32768};
-constexpr uint32_t kRefills[] = {16, 32};
+constexpr uint32_t kRefills[] = {16, 32, 64};
 constexpr uint32_t kMinRayTile = 1024;
 }  // namespace
 
@@ -203,7 +203,7 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     v->n = items[i].scene->ntri;
     v->owned = false;
     b->bvhs.push_back(v);
-    rc = bvh_alloc(c, v, v->n);
+    rc = bvh_alloc(c, v, v->n, false);
   }
   // device mesh table
   std::vector<BatchMeshDev> md(count);

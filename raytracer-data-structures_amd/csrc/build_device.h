@@ -577,4 +577,53 @@ __device__ inline void refit_top_block(const BuildView& v) {
   __syncthreads();
 }
 
+// ---- 4-wide collapse ------------------------------------------------------------
+// The wide node of binary internal node i holds, in binary order, for each
+// child c of i: c's box if c is a leaf (one slot, the next one unused), else
+// the boxes of c's two children (two slots). Boxes are copied exactly, so a
+// slab test against a wide slot is the test the binary traversal makes
+// against the same subtree. Only nodes at even depth are written (the root,
+// and every internal slot of a written node, is at even depth); the depth
+// parity comes from the parent links, at most 64 steps.
+__device__ inline void widen_node(const BuildBuffers& B, uint32_t n, uint32_t i) {
+  uint32_t d = 0, cur = i;
+  while (cur != 0u) {
+    const int32_t pe = B.parent[cur];
+    cur = (uint32_t)pe >> 1;
+    if (pe < 0 || cur >= n - 1 || ++d > 64) {
+      atomicOr(&B.misc[MISC_ERR], ERR_WIDEN_DEPTH);
+      return;
+    }
+  }
+  if (d & 1u) return;
+  const NodeRec N = B.nodes[i];
+  const float4 empty_lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
+  const float4 empty_hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  float4 lo[4], hi[4];
+  int32_t link[4];
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int32_t c = __float_as_int(side ? N.hi0.w : N.lo0.w);
+    if (c < 0) {
+      lo[2 * side] = side ? N.lo1 : N.lo0, hi[2 * side] = side ? N.hi1 : N.hi0;
+      link[2 * side] = c;
+      lo[2 * side + 1] = empty_lo, hi[2 * side + 1] = empty_hi, link[2 * side + 1] = 0;
+    } else {
+      const NodeRec C = B.nodes[c];
+      lo[2 * side] = C.lo0, hi[2 * side] = C.hi0, link[2 * side] = __float_as_int(C.lo0.w);
+      lo[2 * side + 1] = C.lo1, hi[2 * side + 1] = C.hi1;
+      link[2 * side + 1] = __float_as_int(C.hi0.w);
+    }
+  }
+  WideNode& W = B.wide[i];
+  W.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
+  W.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
+  W.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
+  W.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
+  W.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
+  W.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
+  W.link = make_int4(link[0], link[1], link[2], link[3]);
+  W.pad = make_int4(0, 0, 0, 0);
+}
+
 }  // namespace rq

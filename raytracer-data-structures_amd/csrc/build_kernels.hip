@@ -44,6 +44,11 @@ __global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(BuildView v) {
 
 __global__ __launch_bounds__(REFIT_TOP_BLOCK) void k_refit_top(BuildView v) { refit_top_block(v); }
 
+__global__ __launch_bounds__(256) void k_widen(BuildBuffers B, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n - 1) widen_node(B, n, i);
+}
+
 }  // namespace
 
 hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, const float* vz,
@@ -84,6 +89,14 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
     ProfScope ps(lc, "refit_top");
     hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, v);
   }
+  if (B.wide) return launch_widen(lc, B, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_widen(const LaunchCtx& lc, BuildBuffers& B, uint32_t n) {
+  if (n < 2 || !B.wide) return hipGetLastError();
+  ProfScope ps(lc, "widen");
+  hipLaunchKernelGGL(k_widen, dim3(tiles_of(n - 1, 256)), dim3(256), 0, lc.stream, B, n);
   return hipGetLastError();
 }
 

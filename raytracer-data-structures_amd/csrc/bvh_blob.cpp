@@ -155,7 +155,7 @@ int rq_bvh_load(rq_ctx* c, const rq_scene* s, const void* blob, size_t bytes, rq
   b->scene = s;
   b->mode = h.mode;
   b->n = h.n;
-  rc = bvh_alloc(c, b, b->n);
+  rc = bvh_alloc(c, b, b->n, true);
   if (rc) {
     delete b;
     return rc;
@@ -177,6 +177,7 @@ int rq_bvh_load(rq_ctx* c, const rq_scene* s, const void* blob, size_t bytes, rq
   std::memcpy(misc + MISC_ROOT, p, 24), p += 24;
   std::memcpy(misc + MISC_HEIGHT, p, 4), p += 4;
   if (e == hipSuccess && n) e = hipMemcpyAsync(b->B.misc, misc, sizeof misc, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = launch_widen(launch_ctx(c, stream), b->B, n);  // derived, not stored
   if (e == hipSuccess) e = hipStreamSynchronize(st);  // the blob may go away after return
   if (e != hipSuccess) {
     ctx_free(c, b->base);

@@ -109,7 +109,7 @@ using namespace rq;
 
 namespace rq {
 
-int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide) {
   BuildBuffers& B = b->B;
   B.ntiles = (n + SORT_TILE_LEAN - 1) / SORT_TILE_LEAN;
   size_t nn = std::max<uint32_t>(n, 1);
@@ -128,7 +128,8 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
                  64 * ni,
                  48 * nn,
                  4 * nn,
-                 4 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK)};
+                 4 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK),
+                 wide ? 128 * ni : 0};
   size_t total = 0;
   for (size_t s : sz) total += align256(s);
   void* p;
@@ -138,9 +139,9 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
   void** dst[] = {(void**)&B.codes, (void**)&B.keys_a, (void**)&B.keys_b, (void**)&B.vals_a,
                   (void**)&B.vals_b, (void**)&B.status, (void**)&B.misc, (void**)&B.parent,
                   (void**)&B.flags, (void**)&B.slot_h, (void**)&B.range, (void**)&B.nodes,
-                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.fcount};
+                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.fcount, (void**)&B.wide};
   for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); ++i) {
-    *dst[i] = q;
+    *dst[i] = sz[i] ? q : nullptr;
     q += align256(sz[i]);
   }
   b->base = p;
@@ -177,7 +178,7 @@ int rq_create(rq_ctx** out, int device, size_t budget) {
     c->trace_grid[v] = trace_grid_blocks(device, v);
     int blk, depth, ts;
     trace_variant_info(v, &blk, &depth, &ts);
-    gmax = std::max(gmax, (size_t)c->trace_grid[v] * blk * (TRACE_MAX_STACK - depth));
+    gmax = std::max(gmax, (size_t)c->trace_grid[v] * blk * (trace_max_stack(v) - depth));
   }
   // counter slots + one trailing word of traversal error bits
   const size_t cbytes = (COUNTER_SLOTS * 2 + 1) * sizeof(uint32_t);
@@ -333,7 +334,7 @@ int rq_build_lbvh(rq_ctx* c, const rq_scene* s, int mode, rq_bvh** out, void* st
   b->scene = s;
   b->mode = mode;
   b->n = s->ntri;
-  int rc = bvh_alloc(c, b, b->n);
+  int rc = bvh_alloc(c, b, b->n, true);
   if (rc) {
     delete b;
     return rc;
@@ -430,6 +431,27 @@ int rq_bvh_export(rq_ctx* c, const rq_bvh* b, uint32_t* codes, uint32_t* sorted_
   return RQ_OK;
 }
 
+int rq_bvh_export_wide(rq_ctx* c, const rq_bvh* b, float* boxes, int32_t* links) {
+  if (check_ctx(c) || !b || b->magic != BVH_MAGIC) return set_err(c, RQ_E_INVALID, "bad bvh");
+  if (!b->B.wide) return set_err(c, RQ_E_UNSUPPORTED, "this BVH has no wide nodes");
+  if (b->n < 2) return RQ_OK;
+  const size_t ni = b->n - 1;
+  std::vector<WideNode> w(ni);
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(w.data(), b->B.wide, ni * sizeof(WideNode), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(c, e, "wide export");
+  for (size_t i = 0; i < ni; ++i) {
+    const float* q = &w[i].lox.x;  // lox, hix, loy, hiy, loz, hiz (4 slots each)
+    for (int k = 0; k < 4 && boxes; ++k) {
+      float* o = boxes + 24 * i + 6 * k;
+      o[0] = q[k], o[1] = q[8 + k], o[2] = q[16 + k];
+      o[3] = q[4 + k], o[4] = q[12 + k], o[5] = q[20 + k];
+    }
+    if (links) std::memcpy(links + 4 * i, &w[i].link, 16);
+  }
+  return RQ_OK;
+}
+
 static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
                  uint8_t* occ, uint32_t* visits, uint32_t* tests, bool any, bool counted,
                  void* stream) {
@@ -451,11 +473,16 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.occ = occ;
   a.visits = visits;
   a.tests = tests;
-  const int variant = counted ? 0 : (any ? c->any_variant : c->closest_variant);
-  int blk, depth, ts;
-  trace_variant_info(variant, &blk, &depth, &ts);
+  int variant = counted ? 0 : (any ? c->any_variant : c->closest_variant);
+  int blk, depth, ts, wide;
+  trace_variant_info(variant, &blk, &depth, &ts, &wide);
+  if (wide && !b->B.wide) {  // no wide nodes (a batch's BVH): the canonical binary walk
+    variant = 0;
+    trace_variant_info(variant, &blk, &depth, &ts, &wide);
+  }
+  a.wide = b->B.wide;
   a.gstack = c->gstack;
-  a.gdepth = TRACE_MAX_STACK - depth;
+  a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
   a.counters = next_counter_slot(c);
   a.err = c->errword;
   a.grid = c->trace_grid[variant];
@@ -487,7 +514,7 @@ int rq_trace_variant_count(void) { return trace_num_variants(); }
 
 int rq_trace_variant_get(int v, rq_trace_variant* out) {
   if (v < 0 || v >= trace_num_variants() || !out) return RQ_E_INVALID;
-  trace_variant_info(v, &out->block, &out->lds_depth, &out->tstack);
+  trace_variant_info(v, &out->block, &out->lds_depth, &out->tstack, &out->wide);
   return RQ_OK;
 }
 
@@ -532,11 +559,12 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
   double best = 1e30;
   int bv = 0;
   uint32_t br = 16;
-  const uint32_t refills[] = {8, 16, 32, 48};
+  const uint32_t refills[] = {16, 48, 56, 64};
   for (int v = 0; v < trace_num_variants() && !rc; ++v) {
-    int blk, depth, ts;
-    trace_variant_info(v, &blk, &depth, &ts);
+    int blk, depth, ts, wide;
+    trace_variant_info(v, &blk, &depth, &ts, &wide);
     if (any && ts) continue;  // entry distances only help closest hit
+    if (wide && !b->B.wide) continue;
     for (uint32_t rf : refills) {
       c->closest_variant = v, c->any_variant = v, c->refill_idle = rf;
       rq_hits h{(int32_t*)out0, (float*)out1, nullptr, nullptr};

@@ -15,7 +15,7 @@ struct rq_ctx {
   bool prof_on = false;
   uint32_t trace_grid[rq::TRACE_MAX_VARIANTS] = {};  // persistent grid per kernel variant
   int closest_variant = 0, any_variant = 0;           // chosen traversal variants
-  uint32_t refill_idle = 16;
+  uint32_t refill_idle = 64;  // refill a wave only once every lane is idle
   uint32_t* counters = nullptr;  // COUNTER_SLOTS x {fetch, exit}
   uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
   uint32_t counter_seq = 0;
@@ -56,5 +56,7 @@ int ctx_alloc(rq_ctx* c, size_t bytes, void** p);
 void ctx_free(rq_ctx* c, void* p);
 LaunchCtx launch_ctx(rq_ctx* c, void* stream);
 uint32_t* next_counter_slot(rq_ctx* c);
-int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n);  // build buffers for n primitives
+// Build buffers for n primitives; `wide`: also the 4-wide node array (the
+// standalone build and blob load; batch and server slots walk binary nodes).
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide);
 }  // namespace rq

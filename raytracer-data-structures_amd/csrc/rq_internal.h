@@ -32,6 +32,20 @@ struct alignas(16) TriRec {
 };
 static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
 
+// 4-wide node: the binary LBVH collapsed two levels at a time. The wide node
+// of a binary internal node i (kept at index i, so the array is sparse: only
+// the root and nodes at even depth are ever reached) holds the boxes of i's
+// grandchildren, or of a child that is a leaf, as four child slots in SoA
+// order, one 128-byte record per visit. Links follow the binary convention
+// (>= 0 wide node = binary index, < 0 leaf ~k); an unused slot has link 0
+// (the root is never a child) and an empty box (lo +inf, hi -inf).
+struct alignas(128) WideNode {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  int4 link;
+  int4 pad;
+};
+static_assert(sizeof(WideNode) == 128, "wide node record is 128 bytes");
+
 // Radix sort tiling (onesweep): WAVES*64 threads, ITEMS keys per thread.
 // The standalone build sorts 16 keys per thread; the batch kernel, whose
 // register budget is set by traversal, sorts 8 (SORT_ITEMS_LEAN). Look-back
@@ -61,6 +75,9 @@ constexpr uint32_t MAX_PRIMS = (1u << 30) - 1;
 // kernel variants keep 12..32 of them in LDS and the rest in a global
 // overflow area ([depth][thread] pairs of {node, entry distance}).
 constexpr int TRACE_MAX_STACK = 64;
+// Wide traversal keeps up to 3 entries per wide ancestor, and a path of at
+// most 64 binary levels has at most 32 wide ones.
+constexpr int TRACE_MAX_STACK_WIDE = 96;
 constexpr int TRACE_MIN_LDS_DEPTH = 12;
 constexpr int TRACE_MAX_VARIANTS = 16;
 constexpr int COUNTER_SLOTS = 64;
@@ -93,6 +110,7 @@ struct BuildBuffers {
   uint32_t* fcount = nullptr;     // cdiv(n, REFIT_CHUNK) entries per chunk
   NodeRec* nodes = nullptr;       // n-1
   TriRec* tris = nullptr;         // n
+  WideNode* wide = nullptr;       // n-1 (sparse, see WideNode), or null: binary only
   uint32_t ntiles = 0;            // tiles of SORT_TILE_LEAN keys
 };
 
@@ -111,6 +129,7 @@ constexpr uint32_t ERR_LOOKBACK_SPIN = 1u;
 constexpr uint32_t ERR_SCATTER_RANGE = 2u;
 constexpr uint32_t ERR_TOPOLOGY = 4u;
 constexpr uint32_t ERR_REFIT_DEPTH = 8u;
+constexpr uint32_t ERR_WIDEN_DEPTH = 16u;
 constexpr uint32_t LOOKBACK_SPIN_LIMIT = 1u << 24;
 
 // ---------------------------------------------------------------------------
@@ -126,9 +145,13 @@ struct LaunchCtx {
 hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, const float* vz,
                         const uint32_t* idx, uint32_t n, int morton_mode, BuildBuffers& B,
                         const float* vr = nullptr);
+// Writes B.wide from the binary nodes of a built LBVH (part of launch_build
+// when B.wide is set; called by itself after a blob load).
+hipError_t launch_widen(const LaunchCtx& lc, BuildBuffers& B, uint32_t n);
 
 struct TraceArgs {
   const NodeRec* nodes;
+  const WideNode* wide;  // wide variants only
   const TriRec* tris;
   uint32_t nprims;
   rq_rays rays;
@@ -150,7 +173,8 @@ struct TraceArgs {
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
                         int variant);
 int trace_num_variants();
-void trace_variant_info(int v, int* block, int* depth, int* tstack);
+void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide = nullptr);
+int trace_max_stack(int variant);  // total stack entries per lane (LDS + overflow)
 uint32_t trace_grid_blocks(int device, int variant);
 hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64_t bytes);
 

@@ -46,6 +46,25 @@ __device__ inline bool slab(const float4& lo, const float4& hi, const RayState& 
   return tn <= tf * slack;
 }
 
+// Slot test of a wide node on pre-selected near / far planes: the same
+// operations as slab(), so a slot culls exactly where the binary walk culls
+// the same box.
+__device__ inline bool slab_w(float nx, float ny, float nz, float fx, float fy, float fz,
+                              const RayState& r, float thi, float slack, float& tn) {
+  const float an = (nx - r.ox) * r.ix, bn = (ny - r.oy) * r.iy, cn = (nz - r.oz) * r.iz;
+  const float af = (fx - r.ox) * r.ix, bf = (fy - r.oy) * r.iy, cf = (fz - r.oz) * r.iz;
+  tn = fmaxf(fmaxf(an, bn), fmaxf(cn, r.tmin));
+  const float tf = fminf(fminf(af, bf), fminf(cf, thi));
+  return tn <= tf * slack;
+}
+
+__device__ inline void cas_pair(uint32_t& ka, int32_t& ca, uint32_t& kb, int32_t& cb) {
+  const bool sw = kb < ka;
+  const uint32_t k0 = sw ? kb : ka, k1 = sw ? ka : kb;
+  const int32_t c0 = sw ? cb : ca, c1 = sw ? ca : cb;
+  ka = k0, kb = k1, ca = c0, cb = c1;
+}
+
 // Möller–Trumbore, main.cpp:137-162 (v_0 read as v0), t restricted to
 // (tmin, tmax). Dot products (x*x' + y*y') + z*z', no contraction.
 __device__ inline bool mt(const TriRec& T, const RayState& r, float& t, float& u, float& v) {
@@ -98,9 +117,16 @@ __device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) 
 // canonical one unless TSTACK skips an entry. Rays are taken a.chunk at a time
 // from *ctr (global or LDS); the loop returns when *ctr passes a.nrays.
 // SPHERES: the leaves may be spheres (a.sphere); false compiles triangles only.
-template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true>
+// WIDE: walk the 4-wide nodes (a.wide) instead of the binary ones: the slots
+// a ray's box test accepts are visited nearest first (closest hit; any hit in
+// slot order), the others pushed farthest first. The same boxes are tested
+// with the same slack, so culling is as conservative as the binary walk and
+// the result (the lexicographic minimum) is the same; visit order differs.
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true,
+          bool WIDE = false>
 __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_node,
                                   float* s_tn, uint32_t gtid, uint32_t gstride) {
+  static_assert(!(WIDE && COUNT), "counted traversal walks the binary nodes");
   const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
   const int max_sp = DEPTH + (int)a.gdepth;
@@ -201,7 +227,59 @@ __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_
 
     // ---- traverse until enough lanes are idle to be worth a refill -------
     for (;;) {
-      if (active && node >= 0) {
+      if (WIDE && active && node >= 0) {
+        const WideNode* w = a.wide + node;
+        const float4 lx = w->lox, hx = w->hix, ly = w->loy, hy = w->hiy, lz = w->loz,
+                     hz = w->hiz;
+        const int4 ln = w->link;
+        // near / far rows by the direction's sign (hi is near when 1/d < 0)
+        const float4 nx = r.sx ? hx : lx, fx = r.sx ? lx : hx;
+        const float4 ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
+        const float4 nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
+        float t0, t1, t2, t3;
+        const bool h0 = slab_w(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, r, bt, slack, t0) & (ln.x != 0);
+        const bool h1 = slab_w(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, r, bt, slack, t1) & (ln.y != 0);
+        const bool h2 = slab_w(nx.z, ny.z, nz.z, fx.z, fy.z, fz.z, r, bt, slack, t2) & (ln.z != 0);
+        const bool h3 = slab_w(nx.w, ny.w, nz.w, fx.w, fy.w, fz.w, r, bt, slack, t3) & (ln.w != 0);
+        const int k = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+        // sort the slots by entry distance, misses last: a hit's key is its
+        // distance's bits (ordered for tn >= +0, i.e. tmin >= 0; otherwise
+        // only the visit order suffers), a miss's 0xFFFFFFFF. Any hit: hits
+        // first, distances unused.
+        auto key = [](bool h, float t) -> uint32_t {
+          return h ? (ANY ? 0u : __float_as_uint(t)) : 0xFFFFFFFFu;
+        };
+        uint32_t k0 = key(h0, t0), k1 = key(h1, t1), k2 = key(h2, t2), k3 = key(h3, t3);
+        int32_t c0 = ln.x, c1 = ln.y, c2 = ln.z, c3 = ln.w;
+        cas_pair(k0, c0, k1, c1), cas_pair(k2, c2, k3, c3);
+        cas_pair(k0, c0, k2, c2), cas_pair(k1, c1, k3, c3);
+        cas_pair(k1, c1, k2, c2);
+        if (k == 0) {
+          if (!pop()) finish();
+        } else {
+          // push the k-1 farther hits, farthest deepest: entry sp+j holds
+          // sorted slot k-1-j; with three LDS entries free all three are
+          // written (entries above the new top are dead)
+          if (k > 1 && sp + 3 <= DEPTH) {
+            const int32_t e0 = k == 4 ? c3 : (k == 3 ? c2 : c1), e1 = k == 4 ? c2 : c1;
+            s_node[sp * BLOCK + tid] = e0;
+            s_node[(sp + 1) * BLOCK + tid] = e1;
+            s_node[(sp + 2) * BLOCK + tid] = c1;
+            if (TSTACK) {
+              const uint32_t d0 = k == 4 ? k3 : (k == 3 ? k2 : k1), d1 = k == 4 ? k2 : k1;
+              s_tn[sp * BLOCK + tid] = __uint_as_float(d0);
+              s_tn[(sp + 1) * BLOCK + tid] = __uint_as_float(d1);
+              s_tn[(sp + 2) * BLOCK + tid] = __uint_as_float(k1);
+            }
+            sp += k - 1;
+          } else {
+            if (k > 3) push(c3, __uint_as_float(k3));
+            if (k > 2) push(c2, __uint_as_float(k2));
+            if (k > 1) push(c1, __uint_as_float(k1));
+          }
+          node = c0;
+        }
+      } else if (!WIDE && active && node >= 0) {
         const float4* q = reinterpret_cast<const float4*>(a.nodes + node);
         const float4 lo0 = q[0], hi0 = q[1], lo1 = q[2], hi1 = q[3];
         if (COUNT) vis += 2;

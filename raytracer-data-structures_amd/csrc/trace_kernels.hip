@@ -15,13 +15,13 @@
 namespace rq {
 namespace {
 
-template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK>
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool WIDE = false>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
   __shared__ int32_t s_node[DEPTH * BLOCK];
   __shared__ float s_tn[TSTACK ? DEPTH * BLOCK : 1];
-  trace_loop<ANY, COUNT, BLOCK, DEPTH, TSTACK>(a, &a.counters[0], s_node, s_tn,
-                                               blockIdx.x * BLOCK + threadIdx.x,
-                                               gridDim.x * BLOCK);
+  trace_loop<ANY, COUNT, BLOCK, DEPTH, TSTACK, true, WIDE>(a, &a.counters[0], s_node, s_tn,
+                                                           blockIdx.x * BLOCK + threadIdx.x,
+                                                           gridDim.x * BLOCK);
   // last wave out resets the counters for the next launch on this slot
   if ((threadIdx.x & 63) == 0) {
     const uint32_t waves = gridDim.x * (BLOCK / 64);
@@ -36,16 +36,23 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
 typedef void (*TraceFn)(TraceArgs);
 struct Variant {
   int block, depth;
-  bool tstack;
+  bool tstack, wide;
   TraceFn closest, any;
 };
 
-#define RQ_VARIANT(B, D, T) \
-  { B, D, T, k_trace<false, false, B, D, T>, k_trace<true, false, B, D, false> }
+// Variant 0 is the canonical binary configuration (the counted kernels run
+// it); the wide variants need the BVH's 4-wide nodes.
+#define RQ_VARIANT(B, D, T, W) \
+  { B, D, T, W, k_trace<false, false, B, D, T, W>, k_trace<true, false, B, D, false, W> }
 const Variant kVariants[] = {
-    RQ_VARIANT(128, 24, false), RQ_VARIANT(128, 16, false), RQ_VARIANT(64, 24, false),
-    RQ_VARIANT(256, 16, false), RQ_VARIANT(128, 24, true),  RQ_VARIANT(128, 16, true),
-    RQ_VARIANT(64, 16, true),   RQ_VARIANT(64, 32, false),  RQ_VARIANT(64, 12, true),
+    RQ_VARIANT(128, 24, false, false), RQ_VARIANT(128, 16, false, false),
+    RQ_VARIANT(64, 24, false, false),  RQ_VARIANT(256, 16, false, false),
+    RQ_VARIANT(128, 24, true, false),  RQ_VARIANT(128, 16, true, false),
+    RQ_VARIANT(64, 16, true, false),   RQ_VARIANT(64, 32, false, false),
+    RQ_VARIANT(64, 12, true, false),   RQ_VARIANT(128, 24, false, true),
+    RQ_VARIANT(128, 32, false, true), RQ_VARIANT(64, 32, false, true),
+    RQ_VARIANT(128, 24, true, true),  RQ_VARIANT(64, 24, true, true),
+    RQ_VARIANT(256, 16, false, true),
 };
 #undef RQ_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -54,10 +61,15 @@ constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 int trace_num_variants() { return kNumVariants; }
 
-void trace_variant_info(int v, int* block, int* depth, int* tstack) {
+static_assert(sizeof(kVariants) / sizeof(kVariants[0]) <= TRACE_MAX_VARIANTS, "variant table");
+
+void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide) {
   const Variant& x = kVariants[v];
   *block = x.block, *depth = x.depth, *tstack = x.tstack ? 1 : 0;
+  if (wide) *wide = x.wide ? 1 : 0;
 }
+
+int trace_max_stack(int v) { return kVariants[v].wide ? TRACE_MAX_STACK_WIDE : TRACE_MAX_STACK; }
 
 uint32_t trace_grid_blocks(int device, int v) {
   int cus = 0, per = 0;
@@ -72,7 +84,9 @@ uint32_t trace_grid_blocks(int device, int v) {
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
                         int variant) {
   if (a.nrays == 0) return hipSuccess;
-  // the instrumented kernels always run the canonical configuration (variant 0)
+  // the instrumented kernels always run the canonical configuration (variant 0);
+  // a wide variant needs the wide nodes
+  if (!counted && kVariants[variant].wide && !a.wide) return hipErrorInvalidValue;
   const Variant& V = kVariants[counted ? 0 : variant];
   uint32_t need = (uint32_t)((a.nrays + V.block - 1) / V.block);
   uint32_t grid = std::max(1u, std::min(a.grid, need));

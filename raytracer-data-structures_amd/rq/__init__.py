@@ -72,7 +72,8 @@ class BvhInfo(C.Structure):
 
 
 class TraceVariant(C.Structure):
-    _fields_ = [("block", C.c_int32), ("lds_depth", C.c_int32), ("tstack", C.c_int32)]
+    _fields_ = [("block", C.c_int32), ("lds_depth", C.c_int32), ("tstack", C.c_int32),
+                ("wide", C.c_int32)]
 
 
 class Mesh(C.Structure):
@@ -165,6 +166,7 @@ def _load() -> C.CDLL:
         "rq_rebuild_lbvh": (i32, [vp, vp, vp]),
         "rq_bvh_info_get": (i32, [vp, vp, C.POINTER(BvhInfo)]),
         "rq_bvh_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
+        "rq_bvh_export_wide": (i32, [vp, vp, vp, vp]),
         "rq_closest_hit": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp]),
         "rq_any_hit": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp]),
         "rq_closest_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp, vp,
@@ -531,6 +533,15 @@ class Context:
         return dict(codes=codes, sorted_idx=sidx, left=left, right=right,
                     boxes=boxes.reshape(-1, 6))
 
+    def export_wide(self, bvh: int, n: int) -> dict:
+        """The 4-wide nodes (rq_bvh_export_wide): boxes[n-1, 4, 6] and
+        links[n-1, 4], defined for the root and nodes at even depth."""
+        m = max(n - 1, 0)
+        boxes = np.zeros(m * 24, np.float32)
+        links = np.zeros(m * 4, np.int32)
+        self._check(lib().rq_bvh_export_wide(self.h, bvh, _ptr(boxes), _ptr(links)))
+        return dict(boxes=boxes.reshape(m, 4, 6), links=links.reshape(m, 4))
+
     def bvh_save(self, bvh: int) -> bytes:
         """The built LBVH as a blob (rq_bvh_save)."""
         n = C.c_size_t()
@@ -682,7 +693,8 @@ class Context:
         for i in range(lib().rq_trace_variant_count()):
             v = TraceVariant()
             lib().rq_trace_variant_get(i, C.byref(v))
-            out.append(dict(index=i, block=v.block, lds_depth=v.lds_depth, tstack=v.tstack))
+            out.append(dict(index=i, block=v.block, lds_depth=v.lds_depth, tstack=v.tstack,
+                            wide=v.wide))
         return out
 
     def trace_config(self, closest: int | None = None, any_hit: int | None = None,

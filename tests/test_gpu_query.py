@@ -173,7 +173,7 @@ def test_autotune_picks_a_valid_variant(ctx, rq):
     try:
         log("autotune")
         v, refill, ms = ctx.autotune(b, rays, any_hit=False, repeats=2)
-        assert 0 <= v < len(ctx.trace_variants()) and refill in (8, 16, 32, 48) and ms > 0
+        assert 0 <= v < len(ctx.trace_variants()) and refill in (16, 48, 56, 64) and ms > 0
         assert ctx.trace_config()[0] == v
         va, _, _ = ctx.autotune(b, rays, any_hit=True, repeats=2)
         assert not ctx.trace_variants()[va]["tstack"]
