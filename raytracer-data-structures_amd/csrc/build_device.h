@@ -121,7 +121,7 @@ __device__ inline void build_init_part(const BuildView& v, uint32_t first, uint3
   const uint32_t nstatus = SORT_PASSES * tiles_of(v.n, SORT_TILE_LEAN) * 256;  // both tilings
   for (uint32_t k = first; k < nstatus; k += stride) v.B.status[k] = 0u;
   for (uint32_t k = first; k < (uint32_t)MISC_BOUNDS; k += stride) misc[k] = 0u;  // hist, tickets
-  if (first == 0) misc[MISC_ERR] = 0u;
+  if (first == 0) misc[MISC_ERR] = 0u, misc[MISC_HEIGHT] = 0u;
   if (first < 3) {
     misc[MISC_BOUNDS + first] = 0xFFFFFFFFu;  // lo (ordered min)
     misc[MISC_BOUNDS + 3 + first] = 0u;        // hi (ordered max)
@@ -459,6 +459,34 @@ __device__ inline void put_root(uint32_t* misc, const float* lo, const float* hi
   misc[MISC_HEIGHT] = h;
 }
 
+// Leaf record of sorted position k and its box (the leaf's slot value).
+__device__ inline void leaf_record(const BuildView& v, uint32_t k, TriRec& tr, float* lo,
+                                   float* hi) {
+  const uint32_t p = v.B.vals_a[k];
+  if (v.sphere) {
+    // sphere leaf: centre and radius, box centre -/+ radius (main.cpp:687-689);
+    // the record keeps r and r*r (the reference's radius2)
+    const float cx = v.vx[p], cy = v.vy[p], cz = v.vz[p], r = v.vr[p];
+    tr.v0 = make_float4(cx, cy, cz, __int_as_float((int32_t)p));
+    tr.e1 = make_float4(r, r * r, 0.0f, 0.0f);
+    tr.e2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    lo[0] = (cx - r) + 0.0f, lo[1] = (cy - r) + 0.0f, lo[2] = (cz - r) + 0.0f;
+    hi[0] = (cx + r) + 0.0f, hi[1] = (cy + r) + 0.0f, hi[2] = (cz + r) + 0.0f;
+  } else {
+    const uint32_t a = v.idx[3 * p], b = v.idx[3 * p + 1], c = v.idx[3 * p + 2];
+    const float3 v0 = make_float3(v.vx[a], v.vy[a], v.vz[a]);
+    const float3 v1 = make_float3(v.vx[b], v.vy[b], v.vz[b]);
+    const float3 v2 = make_float3(v.vx[c], v.vy[c], v.vz[c]);
+    tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
+    tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
+    tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
+    lo[0] = fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, lo[1] = fminf(fminf(v0.y, v1.y), v2.y) + 0.0f;
+    lo[2] = fminf(fminf(v0.z, v1.z), v2.z) + 0.0f;
+    hi[0] = fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, hi[1] = fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f;
+    hi[2] = fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f;
+  }
+}
+
 __device__ inline void refit_local_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
   uint32_t* s_nf = lds;
   if (threadIdx.x == 0) *s_nf = 0u;
@@ -470,31 +498,9 @@ __device__ inline void refit_local_tile(const BuildView& v, uint32_t chunk, uint
   // blockDim.x threads share the chunk's leaves (the standalone kernel runs
   // REFIT_CHUNK threads, one leaf each; the batch kernel BUILD_BLOCK)
   for (uint32_t k = chunk_lo + threadIdx.x; k <= chunk_hi; k += blockDim.x) {
-    const uint32_t p = B.vals_a[k];
     TriRec tr;
     float lo[3], hi[3];
-    if (v.sphere) {
-      // sphere leaf: centre and radius, box centre -/+ radius (main.cpp:687-689);
-      // the record keeps r and r*r (the reference's radius2)
-      const float cx = v.vx[p], cy = v.vy[p], cz = v.vz[p], r = v.vr[p];
-      tr.v0 = make_float4(cx, cy, cz, __int_as_float((int32_t)p));
-      tr.e1 = make_float4(r, r * r, 0.0f, 0.0f);
-      tr.e2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-      lo[0] = (cx - r) + 0.0f, lo[1] = (cy - r) + 0.0f, lo[2] = (cz - r) + 0.0f;
-      hi[0] = (cx + r) + 0.0f, hi[1] = (cy + r) + 0.0f, hi[2] = (cz + r) + 0.0f;
-    } else {
-      const uint32_t a = v.idx[3 * p], b = v.idx[3 * p + 1], c = v.idx[3 * p + 2];
-      const float3 v0 = make_float3(v.vx[a], v.vy[a], v.vz[a]);
-      const float3 v1 = make_float3(v.vx[b], v.vy[b], v.vz[b]);
-      const float3 v2 = make_float3(v.vx[c], v.vy[c], v.vz[c]);
-      tr.v0 = make_float4(v0.x, v0.y, v0.z, __int_as_float((int32_t)p));
-      tr.e1 = make_float4(v1.x - v0.x, v1.y - v0.y, v1.z - v0.z, 0.0f);
-      tr.e2 = make_float4(v2.x - v0.x, v2.y - v0.y, v2.z - v0.z, 0.0f);
-      lo[0] = fminf(fminf(v0.x, v1.x), v2.x) + 0.0f, lo[1] = fminf(fminf(v0.y, v1.y), v2.y) + 0.0f;
-      lo[2] = fminf(fminf(v0.z, v1.z), v2.z) + 0.0f;
-      hi[0] = fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, hi[1] = fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f;
-      hi[2] = fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f;
-    }
+    leaf_record(v, k, tr, lo, hi);
     B.tris[k] = tr;
     if (n == 1) {
       put_root(B.misc, lo, hi, 0u);
@@ -584,19 +590,22 @@ __device__ inline void refit_top_block(const BuildView& v) {
 // slab test against a wide slot is the test the binary traversal makes
 // against the same subtree. Only nodes at even depth are written (the root,
 // and every internal slot of a written node, is at even depth); the depth
-// parity comes from the parent links, at most 64 steps.
-__device__ inline void widen_node(const BuildBuffers& B, uint32_t n, uint32_t i) {
+// comes from the parent links, at most 64 steps. Returns d + 1 for a node
+// with a leaf child (the depth of that leaf), else 0: the tree height is the
+// maximum over all nodes.
+__device__ inline uint32_t widen_node(const BuildBuffers& B, uint32_t n, uint32_t i) {
   uint32_t d = 0, cur = i;
   while (cur != 0u) {
     const int32_t pe = B.parent[cur];
     cur = (uint32_t)pe >> 1;
     if (pe < 0 || cur >= n - 1 || ++d > 64) {
       atomicOr(&B.misc[MISC_ERR], ERR_WIDEN_DEPTH);
-      return;
+      return 0u;
     }
   }
-  if (d & 1u) return;
   const NodeRec N = B.nodes[i];
+  const uint32_t h = (__float_as_int(N.lo0.w) < 0 || __float_as_int(N.hi0.w) < 0) ? d + 1u : 0u;
+  if (d & 1u) return h;
   const float4 empty_lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
   const float4 empty_hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
   float4 lo[4], hi[4];
@@ -624,6 +633,174 @@ __device__ inline void widen_node(const BuildBuffers& B, uint32_t n, uint32_t i)
   W.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
   W.link = make_int4(link[0], link[1], link[2], link[3]);
   W.pad = make_int4(0, 0, 0, 0);
+  return h;
 }
+
+// ---- chunk-union refit (standalone build) ----------------------------------------
+// Pass 1, one block of REFIT_CHUNK threads per chunk of REFIT_CHUNK leaves:
+// each thread writes its leaf record, the block scans the leaf boxes into
+// per-leaf prefix and suffix unions of the chunk (and the chunk's union), and
+// each leaf climbs as in the frontier refit while the node's leaf range lies
+// inside the chunk: such a node's index lies in the chunk too (a Karras node
+// i is an end of its own range), so its arrival counter lives in LDS. A node
+// spanning chunks only receives its child's box. Pass 2 gives every spanning
+// node [a, b] its box directly: suffix(a) | chunks strictly between | prefix(b)
+// (min / max are order-independent, so this is the box the climb would
+// build). The height is left to the depth walk of the widen kernel.
+__device__ inline void box_union(float* b, const float* o) {
+  b[0] = fminf(b[0], o[0]), b[1] = fminf(b[1], o[1]), b[2] = fminf(b[2], o[2]);
+  b[3] = fmaxf(b[3], o[3]), b[4] = fmaxf(b[4], o[4]), b[5] = fmaxf(b[5], o[5]);
+}
+__device__ inline void box_store(float* dst, const float* b) {
+  float2* d = reinterpret_cast<float2*>(dst);  // 24-byte records, 8-byte aligned
+  d[0] = make_float2(b[0], b[1]), d[1] = make_float2(b[2], b[3]), d[2] = make_float2(b[4], b[5]);
+}
+__device__ inline void box_load(const float* src, float* b) {
+  const float2* q = reinterpret_cast<const float2*>(src);
+  const float2 a = q[0], c = q[1], e = q[2];
+  b[0] = a.x, b[1] = a.y, b[2] = c.x, b[3] = c.y, b[4] = e.x, b[5] = e.y;
+}
+
+// LDS: REFIT_CHUNK arrival counters + 2 * 16 wave unions (6 floats each).
+constexpr int REFIT_CHUNK_LDS_WORDS = REFIT_CHUNK + 2 * (REFIT_CHUNK / 64) * 6;
+
+__device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
+  static_assert(REFIT_CHUNK == 1024, "one leaf per thread of a 1024-thread block");
+  uint32_t* s_flag = lds;
+  float* s_wpre = reinterpret_cast<float*>(lds + REFIT_CHUNK);  // [wave][6] exclusive prefix
+  float* s_wsuf = s_wpre + (REFIT_CHUNK / 64) * 6;               // [wave][6] exclusive suffix
+  const BuildBuffers& B = v.B;
+  const uint32_t n = v.n;
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+  const uint32_t chunk_lo = chunk * REFIT_CHUNK;
+  const uint32_t chunk_hi = min(chunk_lo + REFIT_CHUNK, n) - 1;
+  const uint32_t k = chunk_lo + tid;
+  const bool valid = k <= chunk_hi;
+  s_flag[tid] = 0u;
+  float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (valid) {
+    TriRec tr;
+    leaf_record(v, k, tr, box, box + 3);
+    B.tris[k] = tr;
+  }
+  // wave-level inclusive prefix (pre) and suffix (suf) unions
+  float pre[6], suf[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) pre[q] = suf[q] = box[q];
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    float up[6], dn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) up[q] = __shfl_up(pre[q], d, 64), dn[q] = __shfl_down(suf[q], d, 64);
+    if (lane >= (uint32_t)d) box_union(pre, up);
+    if (lane + d < 64u) box_union(suf, dn);
+  }
+  // wave unions -> exclusive prefix / suffix over the waves of the block
+  if (lane == 63) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_wpre[wave * 6 + q] = pre[q];  // the wave's union
+  }
+  __syncthreads();
+  float ep[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  float es[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (tid < REFIT_CHUNK / 64) {
+    for (uint32_t w = 0; w < REFIT_CHUNK / 64; ++w) {
+      if (w < tid) box_union(ep, s_wpre + w * 6);
+      if (w > tid) box_union(es, s_wpre + w * 6);
+    }
+  }
+  __syncthreads();  // every wave union read before it is overwritten
+  if (tid < REFIT_CHUNK / 64) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) s_wpre[tid * 6 + q] = ep[q], s_wsuf[tid * 6 + q] = es[q];
+  }
+  __syncthreads();
+  box_union(pre, s_wpre + wave * 6);
+  box_union(suf, s_wsuf + wave * 6);
+  if (valid) {
+    box_store(B.pre + 6ull * k, pre);
+    box_store(B.suf + 6ull * k, suf);
+  }
+  if (tid == 0) box_store(B.cbox + 6ull * chunk, suf);  // leaf 0's suffix: the chunk's union
+  if (!valid) return;
+  if (n == 1) {
+    put_root(B.misc, box, box + 3, 0u);
+    return;
+  }
+  float* lo = box;
+  float* hi = box + 3;
+  uint32_t cur = (n - 1) + k;
+  for (int level = 0;; ++level) {
+    const int32_t pe = B.parent[cur];
+    const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
+    if (pe < 0 || node >= n - 1 || level > 64) {
+      atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
+      return;
+    }
+    put_slot(B.nodes, node, side, lo, hi);
+    const uint2 rg = B.range[node];
+    if (rg.x < chunk_lo || rg.y > chunk_hi) return;  // spanning: pass 2
+    const uint32_t old = __hip_atomic_fetch_add(&s_flag[node - chunk_lo], 1u, __ATOMIC_ACQ_REL,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0u) return;
+    union_slot(B.nodes, node, side ^ 1u, lo, hi);
+    if (node == 0u) {  // the whole tree lies in this chunk
+      float* root = reinterpret_cast<float*>(B.misc + MISC_ROOT);
+      for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];
+      return;
+    }
+    cur = node;
+  }
+}
+
+// Pass 2: the lanes of a wave each take internal node i; the wave then works
+// through its spanning nodes one at a time, all 64 lanes striding over the
+// chunks strictly inside the span (a node near the root spans them all).
+// Every lane must reach this call (the wave shuffles).
+__device__ inline void refit_span_wave(const BuildBuffers& B, uint32_t n, uint32_t i) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint2 rg = make_uint2(0u, 0u);
+  bool span = false;
+  if (i < n - 1) {
+    rg = B.range[i];
+    span = rg.x / REFIT_CHUNK != rg.y / REFIT_CHUNK;
+  }
+  uint64_t m = __ballot(span);
+  while (m) {
+    const int src = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1ull;
+    const uint32_t ca = (uint32_t)__shfl((int)(rg.x / REFIT_CHUNK), src, 64);
+    const uint32_t cb = (uint32_t)__shfl((int)(rg.y / REFIT_CHUNK), src, 64);
+    float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}, o[6];
+    for (uint32_t c = ca + 1 + lane; c < cb; c += 64) {
+      box_load(B.cbox + 6ull * c, o);
+      box_union(box, o);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+#pragma unroll
+      for (int q = 0; q < 6; ++q) o[q] = __shfl_xor(box[q], d, 64);
+      box_union(box, o);
+    }
+    if ((int)lane != src) continue;
+    box_load(B.suf + 6ull * rg.x, o);
+    box_union(box, o);
+    box_load(B.pre + 6ull * rg.y, o);
+    box_union(box, o);
+    if (i == 0u) {
+      float* root = reinterpret_cast<float*>(B.misc + MISC_ROOT);
+      for (int q = 0; q < 6; ++q) root[q] = box[q];
+      continue;
+    }
+    const int32_t pe = B.parent[i];
+    const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
+    if (pe < 0 || node >= n - 1) {
+      atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
+      continue;
+    }
+    put_slot(B.nodes, node, side, box, box + 3);
+  }
+}
+
 
 }  // namespace rq

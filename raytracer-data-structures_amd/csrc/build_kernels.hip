@@ -44,9 +44,39 @@ __global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(BuildView v) {
 
 __global__ __launch_bounds__(REFIT_TOP_BLOCK) void k_refit_top(BuildView v) { refit_top_block(v); }
 
+__global__ __launch_bounds__(REFIT_CHUNK) void k_refit_chunk(BuildView v) {
+  __shared__ uint32_t lds[REFIT_CHUNK_LDS_WORDS];
+  refit_chunk_tile(v, blockIdx.x, lds);
+}
+
+__global__ __launch_bounds__(256) void k_refit_span(BuildBuffers B, uint32_t n) {
+  refit_span_wave(B, n, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Wide nodes and the tree height: each block's maximum leaf depth goes to
+// B.flags[block] (free once the refit is done), k_height reduces them (one
+// device atomic per wave on a single word serialises across the XCDs).
 __global__ __launch_bounds__(256) void k_widen(BuildBuffers B, uint32_t n) {
+  __shared__ uint32_t s_h[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n - 1) widen_node(B, n, i);
+  uint32_t h = i < n - 1 ? widen_node(B, n, i) : 0u;
+  for (int d = 32; d > 0; d >>= 1) h = max(h, (uint32_t)__shfl_xor((int)h, d, 64));
+  if ((threadIdx.x & 63) == 0) s_h[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) B.flags[blockIdx.x] = max(max(s_h[0], s_h[1]), max(s_h[2], s_h[3]));
+}
+
+__global__ __launch_bounds__(1024) void k_height(BuildBuffers B, uint32_t nblocks) {
+  __shared__ uint32_t s_h[16];
+  uint32_t h = 0;
+  for (uint32_t b = threadIdx.x; b < nblocks; b += 1024) h = max(h, B.flags[b]);
+  for (int d = 32; d > 0; d >>= 1) h = max(h, (uint32_t)__shfl_xor((int)h, d, 64));
+  if ((threadIdx.x & 63) == 0) s_h[threadIdx.x >> 6] = h;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 16; ++w) h = max(h, s_h[w]);
+    B.misc[MISC_HEIGHT] = h;
+  }
 }
 
 }  // namespace
@@ -81,13 +111,24 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
     hipLaunchKernelGGL(k_karras, dim3(tiles_of(n - 1, BUILD_BLOCK)), dim3(BUILD_BLOCK), 0, s, v);
   }
   const uint32_t nchunks = tiles_of(n, REFIT_CHUNK);
-  {
-    ProfScope ps(lc, "refit");
-    hipLaunchKernelGGL(k_refit_local, dim3(nchunks), dim3(REFIT_CHUNK), 0, s, v);
-  }
-  if (nchunks > 1) {
-    ProfScope ps(lc, "refit_top");
-    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, v);
+  if (B.pre) {  // chunk-union refit
+    {
+      ProfScope ps(lc, "refit");
+      hipLaunchKernelGGL(k_refit_chunk, dim3(nchunks), dim3(REFIT_CHUNK), 0, s, v);
+    }
+    if (nchunks > 1) {
+      ProfScope ps(lc, "refit_top");
+      hipLaunchKernelGGL(k_refit_span, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
+    }
+  } else {  // frontier refit
+    {
+      ProfScope ps(lc, "refit");
+      hipLaunchKernelGGL(k_refit_local, dim3(nchunks), dim3(REFIT_CHUNK), 0, s, v);
+    }
+    if (nchunks > 1) {
+      ProfScope ps(lc, "refit_top");
+      hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, v);
+    }
   }
   if (B.wide) return launch_widen(lc, B, n);
   return hipGetLastError();
@@ -96,7 +137,9 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
 hipError_t launch_widen(const LaunchCtx& lc, BuildBuffers& B, uint32_t n) {
   if (n < 2 || !B.wide) return hipGetLastError();
   ProfScope ps(lc, "widen");
-  hipLaunchKernelGGL(k_widen, dim3(tiles_of(n - 1, 256)), dim3(256), 0, lc.stream, B, n);
+  const uint32_t nblocks = tiles_of(n - 1, 256);  // <= n - 1 entries of B.flags
+  hipLaunchKernelGGL(k_widen, dim3(nblocks), dim3(256), 0, lc.stream, B, n);
+  hipLaunchKernelGGL(k_height, dim3(1), dim3(1024), 0, lc.stream, B, nblocks);
   return hipGetLastError();
 }
 

@@ -129,7 +129,10 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide) {
                  48 * nn,
                  4 * nn,
                  4 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK),
-                 wide ? 128 * ni : 0};
+                 wide ? 128 * ni : 0,
+                 wide ? 24 * nn : 0,
+                 wide ? 24 * nn : 0,
+                 wide ? 24 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK) : 0};
   size_t total = 0;
   for (size_t s : sz) total += align256(s);
   void* p;
@@ -139,7 +142,8 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide) {
   void** dst[] = {(void**)&B.codes, (void**)&B.keys_a, (void**)&B.keys_b, (void**)&B.vals_a,
                   (void**)&B.vals_b, (void**)&B.status, (void**)&B.misc, (void**)&B.parent,
                   (void**)&B.flags, (void**)&B.slot_h, (void**)&B.range, (void**)&B.nodes,
-                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.fcount, (void**)&B.wide};
+                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.fcount, (void**)&B.wide,
+                  (void**)&B.pre, (void**)&B.suf, (void**)&B.cbox};
   for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); ++i) {
     *dst[i] = sz[i] ? q : nullptr;
     q += align256(sz[i]);

@@ -56,7 +56,8 @@ int ctx_alloc(rq_ctx* c, size_t bytes, void** p);
 void ctx_free(rq_ctx* c, void* p);
 LaunchCtx launch_ctx(rq_ctx* c, void* stream);
 uint32_t* next_counter_slot(rq_ctx* c);
-// Build buffers for n primitives; `wide`: also the 4-wide node array (the
-// standalone build and blob load; batch and server slots walk binary nodes).
+// Build buffers for n primitives; `wide`: also the 4-wide node array and the
+// chunk-union refit buffers (the standalone build and blob load; batch and
+// server slots walk binary nodes and refit by frontier lists).
 int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide);
 }  // namespace rq

@@ -111,6 +111,11 @@ struct BuildBuffers {
   NodeRec* nodes = nullptr;       // n-1
   TriRec* tris = nullptr;         // n
   WideNode* wide = nullptr;       // n-1 (sparse, see WideNode), or null: binary only
+  // Chunk-union refit (allocated with `wide`; null: frontier refit). Boxes
+  // are 6 floats (lo xyz, hi xyz).
+  float* pre = nullptr;           // 6n: union of the leaves from the chunk start to k
+  float* suf = nullptr;           // 6n: union of the leaves from k to the chunk end
+  float* cbox = nullptr;          // 6 per REFIT_CHUNK chunk: its leaves' union
   uint32_t ntiles = 0;            // tiles of SORT_TILE_LEAN keys
 };
 
