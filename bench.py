@@ -84,6 +84,26 @@ KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<
                  "batch": "k_batch<"}
 
 
+def tune_trace(ctx, bvh, closest_rays, any_rays, args, repeats: int):
+    """The traversal configuration (closest variant, refill, any variant,
+    refill): --trace-config fixes it (no trial launches, e.g. under a
+    profiler), --no-autotune keeps the defaults, else the autotuner picks it."""
+    if args.trace_config:
+        vc, rc, va, ra = (int(x) for x in args.trace_config.split(","))
+    elif args.no_autotune:
+        return None
+    else:
+        vc, rc, _ = ctx.autotune(bvh, closest_rays, any_hit=False, repeats=repeats)
+        va, ra, _ = ctx.autotune(bvh, any_rays, any_hit=True, repeats=repeats)
+    ctx.trace_config(vc, va)
+    ctx.trace_refill(rc, ra)
+    return vc, rc, va, ra
+
+
+def cfg_arg(t) -> str | None:
+    return ",".join(map(str, t)) if t else None
+
+
 def copy_probe(ctx) -> None:
     """A library copy kernel over a known byte count: the calibration of the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -95,7 +115,7 @@ def copy_probe(ctx) -> None:
     dst.free()
 
 
-def _pmc_run(counter: str, config: str) -> dict | None:
+def _pmc_run(counter: str, config: str, extra: list | None = None) -> dict | None:
     """One rocprofv3 --pmc child run of this script in probe mode; returns the
     counter's mean per dispatch (KiB) for each kernel name."""
     import csv
@@ -110,7 +130,7 @@ def _pmc_run(counter: str, config: str) -> dict | None:
     out = tempfile.mkdtemp(prefix="rq_pmc_", dir="/tmp")
     cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "probe", "--",
            sys.executable, str(ROOT / "bench.py"), "--pmc-probe", "--config", config,
-           "--no-cpu", "--traffic", "off", "--warmup", "1"]
+           "--no-cpu", "--traffic", "off", "--warmup", "1", *(extra or [])]
     env = dict(os.environ, TMPDIR="/tmp")
     try:
         subprocess.run(cmd, cwd="/tmp", env=env, timeout=240, check=True,
@@ -133,7 +153,7 @@ def _pmc_run(counter: str, config: str) -> dict | None:
     return {k: sums[k] / counts[k] for k in sums}
 
 
-def measure_traffic(config: str, kernel: str):
+def measure_traffic(config: str, kernel: str, trace_cfg: str | None = None):
     """HBM bytes per launch of `kernel`: FETCH_SIZE and WRITE_SIZE per dispatch,
     each scaled by known bytes / counter reading of the copy probe."""
     sym = KERNEL_SYMBOL.get(kernel)
@@ -142,7 +162,7 @@ def measure_traffic(config: str, kernel: str):
     total = 0.0
     detail = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        res = _pmc_run(counter, config)
+        res = _pmc_run(counter, config, ["--trace-config", trace_cfg] if trace_cfg else None)
         if not res:
             return None
         kv = [v for k, v in res.items() if sym in k]
@@ -178,11 +198,11 @@ def run_c3(ctx, args) -> int:
     vc, tc = counted_batch(ctx, bvh, rays, any_hit=False)
     va, ta = counted_batch(ctx, bvh, rays, any_hit=True)
     tuned = None
-    if not args.no_autotune:
-        c_v, c_r, _ = ctx.autotune(bvh, rays, any_hit=False, repeats=2)
-        a_v, a_r, _ = ctx.autotune(bvh, rays, any_hit=True, repeats=2)
-        ctx.trace_config(c_v, a_v, c_r)
-        tuned = {"closest_variant": c_v, "any_variant": a_v, "refill_idle": c_r}
+    cfg = tune_trace(ctx, bvh, rays, rays, args, repeats=2)
+    if cfg:
+        c_v, c_r, a_v, a_r = cfg
+        tuned = {"closest_variant": c_v, "any_variant": a_v,
+                 "refill_idle": {"closest": c_r, "any": a_r}}
     for _ in range(args.warmup):
         ctx.closest_hit_dev(bvh, rays, hits)
         ctx.any_hit_dev(bvh, rays, occ)
@@ -218,7 +238,7 @@ def run_c3(ctx, args) -> int:
                     unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                     bytes_per_launch=int(alg), avg_launch_ms=round(dms, 4))
     if args.traffic == "auto":
-        roofline["traffic"] = measure_traffic("c3", dom)
+        roofline["traffic"] = measure_traffic("c3", dom, cfg_arg(cfg))
     line = {
         "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
         "value": round(2 * n * args.steps / elapsed / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
@@ -336,10 +356,7 @@ def run_c4(ctx, args) -> int:
     # ---- plain per-request path on the batch's own BVH buffers
     bvhs = [batch.bvh(m) for m in range(len(meshes))]
     med = int(np.argsort([r.n for r in rays])[len(rays) // 2])
-    if not args.no_autotune:
-        vc, rc_, _ = ctx.autotune(bvhs[med], rays[med], any_hit=False, repeats=2)
-        va, _, _ = ctx.autotune(bvhs[med], rays[med], any_hit=True, repeats=2)
-        ctx.trace_config(vc, va, rc_)
+    tune_trace(ctx, bvhs[med], rays[med], rays[med], args, repeats=2)
 
     def per_request():
         for m in range(len(meshes)):
@@ -699,6 +716,9 @@ def main() -> int:
     ap.add_argument("--sweep", action="store_true",
                     help="time every traversal variant on this frame's batches (stderr)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--trace-config", default=None, metavar="VC,RC,VA,RA",
+                    help="fix the traversal variants and refill thresholds (closest, any) "
+                         "instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
@@ -745,12 +765,11 @@ def main() -> int:
     if args.sweep:
         sweep(ctx, bvh, batches)
     tuned = None
-    if not args.no_autotune:
-        vc, rc_, msc = ctx.autotune(bvh, batches["primary"], any_hit=False, repeats=3)
-        va, ra, msa = ctx.autotune(bvh, batches["shadow"], any_hit=True, repeats=3)
-        ctx.trace_config(vc, va, rc_)
+    cfg = tune_trace(ctx, bvh, batches["primary"], batches["shadow"], args, repeats=3)
+    if cfg:
+        vc, rc_, va, ra = cfg
         tuned = {"closest_variant": ctx.trace_variants()[vc], "any_variant":
-                 ctx.trace_variants()[va], "refill_idle": rc_}
+                 ctx.trace_variants()[va], "refill_idle": {"closest": rc_, "any": ra}}
         log(f"autotune: {tuned}")
         for _ in range(2):
             ctx.rebuild(bvh)
@@ -807,7 +826,7 @@ def main() -> int:
     closest_ms = per_kernel.get("trace_closest", (0, 1))[0] / args.steps
     closest_rays_n = sum(r for _, r in closest_rays)
     if roofline is not None and args.traffic == "auto":
-        roofline["traffic"] = measure_traffic(args.config, roofline["kernel"])
+        roofline["traffic"] = measure_traffic(args.config, roofline["kernel"], cfg_arg(cfg))
     cpu = None
     if not args.no_cpu:
         host = [batches[k].download() for k in batches]

@@ -164,9 +164,15 @@ int rq_trace_variant_get(int index, rq_trace_variant* out);
 int rq_trace_config_set(rq_ctx* ctx, int closest_variant, int any_variant, uint32_t refill_idle);
 int rq_trace_config_get(rq_ctx* ctx, int* closest_variant, int* any_variant,
                         uint32_t* refill_idle);
+/* Refill thresholds per query kind: a wave takes new rays once this many of
+ * its 64 lanes are idle (64: only when all are; the default). config_set sets
+ * both to its refill_idle, config_get reports the closest-hit one. */
+int rq_trace_refill_set(rq_ctx* ctx, uint32_t closest_refill, uint32_t any_refill);
+int rq_trace_refill_get(rq_ctx* ctx, uint32_t* closest_refill, uint32_t* any_refill);
 /* Times every variant (and refill threshold 16/48/56/64 idle lanes) on the given
  * device rays (default stream, `repeats` launches each after one warm-up),
- * keeps the fastest in the context and reports it. */
+ * keeps the fastest for that query kind (variant and refill) in the context
+ * and reports it. */
 int rq_autotune_trace(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, int any,
                       int repeats, int* best_variant, uint32_t* best_refill, double* best_ms);
 

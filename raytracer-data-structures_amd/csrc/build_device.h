@@ -661,14 +661,16 @@ __device__ inline void box_load(const float* src, float* b) {
   b[0] = a.x, b[1] = a.y, b[2] = c.x, b[3] = c.y, b[4] = e.x, b[5] = e.y;
 }
 
-// LDS: REFIT_CHUNK arrival counters + 2 * 16 wave unions (6 floats each).
-constexpr int REFIT_CHUNK_LDS_WORDS = REFIT_CHUNK + 2 * (REFIT_CHUNK / 64) * 6;
+// LDS: REFIT_CHUNK arrival counters, 2 * 16 wave unions (6 floats each) and
+// the child boxes of the chunk's nodes ([node - chunk start][side][6]).
+constexpr int REFIT_CHUNK_LDS_WORDS = REFIT_CHUNK + 2 * (REFIT_CHUNK / 64) * 6 + REFIT_CHUNK * 12;
 
 __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
   static_assert(REFIT_CHUNK == 1024, "one leaf per thread of a 1024-thread block");
   uint32_t* s_flag = lds;
   float* s_wpre = reinterpret_cast<float*>(lds + REFIT_CHUNK);  // [wave][6] exclusive prefix
   float* s_wsuf = s_wpre + (REFIT_CHUNK / 64) * 6;               // [wave][6] exclusive suffix
+  float* s_box = s_wsuf + (REFIT_CHUNK / 64) * 6;                // [node][side][6] child boxes
   const BuildBuffers& B = v.B;
   const uint32_t n = v.n;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -740,10 +742,13 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
     put_slot(B.nodes, node, side, lo, hi);
     const uint2 rg = B.range[node];
     if (rg.x < chunk_lo || rg.y > chunk_hi) return;  // spanning: pass 2
+    float* mine = s_box + (node - chunk_lo) * 12;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) mine[6 * side + q] = box[q];
     const uint32_t old = __hip_atomic_fetch_add(&s_flag[node - chunk_lo], 1u, __ATOMIC_ACQ_REL,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0u) return;
-    union_slot(B.nodes, node, side ^ 1u, lo, hi);
+    box_union(box, mine + 6 * (side ^ 1u));  // the sibling's box, stored before its arrival
     if (node == 0u) {  // the whole tree lies in this chunk
       float* root = reinterpret_cast<float*>(B.misc + MISC_ROOT);
       for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];

@@ -490,7 +490,7 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.counters = next_counter_slot(c);
   a.err = c->errword;
   a.grid = c->trace_grid[variant];
-  a.refill_idle = c->refill_idle;
+  a.refill_idle = any ? c->refill_any : c->refill_closest;
   a.chunk = trace_chunk(n, a.grid * (uint32_t)blk / 64u);
   a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
   hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
@@ -530,7 +530,23 @@ int rq_trace_config_set(rq_ctx* c, int closest_variant, int any_variant, uint32_
     return set_err(c, RQ_E_INVALID, "bad traversal configuration");
   c->closest_variant = closest_variant;
   c->any_variant = any_variant;
-  c->refill_idle = refill_idle;
+  c->refill_closest = c->refill_any = refill_idle;
+  return RQ_OK;
+}
+
+int rq_trace_refill_set(rq_ctx* c, uint32_t closest_refill, uint32_t any_refill) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (closest_refill < 1 || closest_refill > 64 || any_refill < 1 || any_refill > 64)
+    return set_err(c, RQ_E_INVALID, "refill threshold outside 1..64");
+  c->refill_closest = closest_refill;
+  c->refill_any = any_refill;
+  return RQ_OK;
+}
+
+int rq_trace_refill_get(rq_ctx* c, uint32_t* closest_refill, uint32_t* any_refill) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (closest_refill) *closest_refill = c->refill_closest;
+  if (any_refill) *any_refill = c->refill_any;
   return RQ_OK;
 }
 
@@ -538,7 +554,7 @@ int rq_trace_config_get(rq_ctx* c, int* closest_variant, int* any_variant, uint3
   if (check_ctx(c)) return RQ_E_INVALID;
   if (closest_variant) *closest_variant = c->closest_variant;
   if (any_variant) *any_variant = c->any_variant;
-  if (refill_idle) *refill_idle = c->refill_idle;
+  if (refill_idle) *refill_idle = c->refill_closest;
   return RQ_OK;
 }
 
@@ -554,7 +570,7 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
     return rc;
   }
   const int saved_c = c->closest_variant, saved_a = c->any_variant;
-  const uint32_t saved_r = c->refill_idle;
+  const uint32_t saved_rc = c->refill_closest, saved_ra = c->refill_any;
   const bool saved_prof = c->prof_on;
   c->prof_on = false;
   hipEvent_t e0, e1;
@@ -570,7 +586,7 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
     if (any && ts) continue;  // entry distances only help closest hit
     if (wide && !b->B.wide) continue;
     for (uint32_t rf : refills) {
-      c->closest_variant = v, c->any_variant = v, c->refill_idle = rf;
+      c->closest_variant = v, c->any_variant = v, c->refill_closest = c->refill_any = rf;
       rq_hits h{(int32_t*)out0, (float*)out1, nullptr, nullptr};
       // one untimed launch, then `repeats` timed ones
       for (int k = 0; k <= repeats && !rc; ++k) {
@@ -590,13 +606,13 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
   ctx_free(c, out0);
   ctx_free(c, out1);
   c->prof_on = saved_prof;
-  c->closest_variant = saved_c, c->any_variant = saved_a, c->refill_idle = saved_r;
+  c->closest_variant = saved_c, c->any_variant = saved_a;
+  c->refill_closest = saved_rc, c->refill_any = saved_ra;
   if (rc) return rc;
   if (any)
-    c->any_variant = bv;
+    c->any_variant = bv, c->refill_any = br;
   else
-    c->closest_variant = bv;
-  c->refill_idle = br;
+    c->closest_variant = bv, c->refill_closest = br;
   if (best_variant) *best_variant = bv;
   if (best_refill) *best_refill = br;
   if (best_ms) *best_ms = best;

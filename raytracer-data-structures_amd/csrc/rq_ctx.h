@@ -15,7 +15,8 @@ struct rq_ctx {
   bool prof_on = false;
   uint32_t trace_grid[rq::TRACE_MAX_VARIANTS] = {};  // persistent grid per kernel variant
   int closest_variant = 0, any_variant = 0;           // chosen traversal variants
-  uint32_t refill_idle = 64;  // refill a wave only once every lane is idle
+  // idle lanes before a wave refills (64: only once every lane is idle), per query kind
+  uint32_t refill_closest = 64, refill_any = 64;
   uint32_t* counters = nullptr;  // COUNTER_SLOTS x {fetch, exit}
   uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
   uint32_t counter_seq = 0;

@@ -176,6 +176,8 @@ def _load() -> C.CDLL:
         "rq_trace_variant_get": (i32, [i32, C.POINTER(TraceVariant)]),
         "rq_trace_config_set": (i32, [vp, i32, i32, u32]),
         "rq_trace_config_get": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(u32)]),
+        "rq_trace_refill_set": (i32, [vp, u32, u32]),
+        "rq_trace_refill_get": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
         "rq_autotune_trace": (i32, [vp, vp, C.POINTER(Rays), u32, i32, i32, C.POINTER(i32),
                                     C.POINTER(u32), C.POINTER(C.c_double)]),
         "rq_device_alloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
@@ -703,9 +705,22 @@ class Context:
         self._check(lib().rq_trace_config_get(self.h, C.byref(c), C.byref(a), C.byref(r)))
         nc = c.value if closest is None else closest
         na = a.value if any_hit is None else any_hit
+        rc, ra = self.trace_refill()
         nr = r.value if refill_idle is None else refill_idle
         self._check(lib().rq_trace_config_set(self.h, nc, na, nr))
+        if refill_idle is None:
+            self.trace_refill(rc, ra)  # keep the per-kind thresholds
         return nc, na, nr
+
+    def trace_refill(self, closest: int | None = None,
+                     any_hit: int | None = None) -> tuple[int, int]:
+        """Refill thresholds per query kind (rq_trace_refill_set / _get)."""
+        c, a = C.c_uint32(), C.c_uint32()
+        self._check(lib().rq_trace_refill_get(self.h, C.byref(c), C.byref(a)))
+        nc = c.value if closest is None else closest
+        na = a.value if any_hit is None else any_hit
+        self._check(lib().rq_trace_refill_set(self.h, nc, na))
+        return nc, na
 
     def autotune(self, bvh: int, rays: DeviceRays, any_hit: bool = False,
                  repeats: int = 3) -> tuple[int, int, float]:

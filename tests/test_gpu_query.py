@@ -175,8 +175,11 @@ def test_autotune_picks_a_valid_variant(ctx, rq):
         v, refill, ms = ctx.autotune(b, rays, any_hit=False, repeats=2)
         assert 0 <= v < len(ctx.trace_variants()) and refill in (16, 48, 56, 64) and ms > 0
         assert ctx.trace_config()[0] == v
-        va, _, _ = ctx.autotune(b, rays, any_hit=True, repeats=2)
+        va, ra, _ = ctx.autotune(b, rays, any_hit=True, repeats=2)
         assert not ctx.trace_variants()[va]["tstack"]
+        # each query kind keeps its own tuned refill threshold
+        assert ctx.trace_refill() == (refill, ra)
+        assert ctx.trace_config()[:2] == (v, va)
     finally:
         rays.free()
         ctx.trace_config(*saved)
