@@ -89,6 +89,15 @@ int rq_scene_upload(rq_ctx* ctx, const float* vx, const float* vy, const float* 
 int rq_scene_upload_spheres(rq_ctx* ctx, const float* cx, const float* cy, const float* cz,
                             const float* radius, const float* rgb, uint32_t n, rq_scene** out,
                             void* stream);
+/* Material per primitive for the renderer (`mat` host array of the scene's
+ * primitive count), values of the reference's MaterialType (accelerators.h:24):
+ * RQ_MAT_DIFFUSE_AND_GLOSSY, RQ_MAT_REFLECTION_AND_REFRACTION, RQ_MAT_REFLECTION.
+ * Read by a renderer created with `materials = 1`; a scene without
+ * materials is all DIFFUSE_AND_GLOSSY. Blocking; may be called again. */
+#define RQ_MAT_DIFFUSE_AND_GLOSSY 0
+#define RQ_MAT_REFLECTION_AND_REFRACTION 1
+#define RQ_MAT_REFLECTION 2
+int rq_scene_set_materials(rq_ctx* ctx, rq_scene* s, const uint8_t* mat, void* stream);
 /* Replace the vertex positions of an uploaded scene (same counts); the next
  * rq_rebuild_lbvh sees them. */
 int rq_scene_update_vertices(rq_ctx* ctx, rq_scene* s, const float* vx, const float* vy,

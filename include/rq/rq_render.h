@@ -43,7 +43,19 @@ typedef struct {
   int32_t bounce;    /* 0: primary + shadow; 1: + mirror bounce and its shadow ray */
   int32_t shadows;   /* 1: a shadow ray per hit (any hit); 0: the reference's trace_more
                         stub (main.cpp:235-245): lights never occluded, no shadow rays */
+  int32_t materials; /* 1: castRay's material switch (main.cpp:417-446) on the scene's
+                        materials (rq_scene_set_materials): a REFLECTION_AND_REFRACTION hit
+                        traces its refraction ray (refract ior 3, from p -/+ N*bias) and
+                        weights its colour by 1 - fresnel(ior 2); REFLECTION gives the grey
+                        1 - fresnel; at the second hit a refracting surface shows the sky
+                        (depth 3). Not combined with bounce. */
+  int32_t mode;      /* RQ_RENDER_SHADE, or RQ_RENDER_OCCLUSION: castRay's KDTREE case
+                        (main.cpp:362-370), black where the primary ray hits anything
+                        (any hit), else the sky */
 } rq_render_params;
+
+#define RQ_RENDER_SHADE 0
+#define RQ_RENDER_OCCLUSION 1
 
 typedef struct {
   uint64_t primary, bounce, shadow; /* rays traced in the last frame (active only) */
@@ -52,7 +64,8 @@ typedef struct {
 typedef struct rq_renderer rq_renderer;
 
 /* Fills p with the reference defaults (640x480, 1 spp, REF_JITTER, light
- * (0,3,30), emission 1, surface 0, sky (0.6,0.8,1), bias 1e-4, no bounce). */
+ * (0,3,30), emission 1, surface 0, sky (0.6,0.8,1), bias 1e-4, no bounce,
+ * shadows on, no materials, RQ_RENDER_SHADE). */
 void rq_render_defaults(rq_render_params* p);
 int rq_renderer_create(rq_ctx* ctx, const rq_render_params* p, rq_renderer** out);
 int rq_renderer_destroy(rq_ctx* ctx, rq_renderer* r);

@@ -883,6 +883,23 @@ inline V3 reflect(V3 I, V3 N) {
   float s = 2 * dot(I, N);
   return sub(I, mul(N, s));
 }
+// refract (main.cpp:223-233): eta * I + (eta * cosi - sqrt(k)) * n, or 0 on
+// total internal reflection.
+inline V3 refract(V3 I, V3 N, float ior) {
+  float cosi = clampf(-1, 1, dot(I, N));
+  float etai = 1, etat = ior;
+  V3 n = N;
+  if (cosi < 0) {
+    cosi = -cosi;
+  } else {
+    std::swap(etai, etat);
+    n = {-N.x, -N.y, -N.z};
+  }
+  float eta = etai / etat;
+  float k = 1 - eta * eta * (1 - cosi * cosi);
+  if (k < 0) return {0, 0, 0};
+  return add(mul(I, eta), mul(n, eta * cosi - std::sqrt(k)));
+}
 
 struct HitGeo {
   V3 p, n;  // hit point, normal facing against the ray
@@ -955,12 +972,57 @@ extern "C" void or_render_samples(const or_bvh* b, const float* vx, const float*
       load_ray(prim, i, &o, &d, &tmin, &tmax);
       float t, u, v;
       uint32_t vi, te;
-      int32_t p0 = trace<false>(b, prep(o, d, tmin, tmax), &t, &u, &v, &vi, &te);
-      C[0]++, C[1] += vi, C[2] += te, C[5]++;
+      int32_t p0 = -1;
+      if (sp->mode != 1) {
+        p0 = trace<false>(b, prep(o, d, tmin, tmax), &t, &u, &v, &vi, &te);
+        C[0]++, C[1] += vi, C[2] += te, C[5]++;
+      }
       int32_t p1 = -1;
       uint8_t s0 = 0, s1 = 0;
       V3 col{sp->sky[0], sp->sky[1], sp->sky[2]};
-      if (p0 >= 0) {
+      const V3 sky = col;
+      auto mat = [&](int32_t p) -> int { return sp->prim_mat ? sp->prim_mat[p] : 0; };
+      if (sp->mode == 1) {
+        // castRay's KDTREE case: black on any hit of the primary ray, else sky
+        float st, su, sv;
+        s0 = trace<true>(b, prep(o, d, tmin, tmax), &st, &su, &sv, &vi, &te) >= 0;
+        C[0]++, C[3] += vi, C[4] += te, C[6]++;
+        if (s0) col = {0, 0, 0};
+      } else if (p0 >= 0 && mat(p0) != 0) {
+        // REFLECTION_AND_REFRACTION / REFLECTION (main.cpp:418-446): the
+        // reflected colour is traced by the reference but never used, so it is
+        // not traced here; fresnel takes ior 2, refract ior 3, as written there
+        HitGeo g = hit_geo(b, leaf[p0], o, d, t);
+        const float kr = fresnel(d, g.n, 2.0f);
+        if (mat(p0) == 2) {
+          col = {1 - kr, 1 - kr, 1 - kr};
+        } else {
+          V3 rd = normalize(refract(d, g.n, 3.0f));
+          V3 ro = dot(rd, g.n) < 0 ? sub(g.p, mul(g.n, sp->bias)) : add(g.p, mul(g.n, sp->bias));
+          float t1, u1, v1;
+          p1 = trace<false>(b, prep(ro, rd, 0.0f, INFINITY), &t1, &u1, &v1, &vi, &te);
+          C[0]++, C[1] += vi, C[2] += te, C[5]++;
+          V3 rc = sky;  // depth 2 (main.cpp:311: beyond depth 2 the sky)
+          if (p1 >= 0) {
+            HitGeo g1 = hit_geo(b, leaf[p1], ro, rd, t1);
+            if (mat(p1) == 0) {
+              if (sp->shadows) {
+                float st, su, sv;
+                RayPre sr1 = shadow_ray(rd, g1, sp);
+                s1 = trace<true>(b, sr1, &st, &su, &sv, &vi, &te) >= 0;
+                C[0]++, C[3] += vi, C[4] += te, C[6]++;
+              }
+              rc = direct(rd, g1, sp, p1, s1);
+            } else {
+              const float kr1 = fresnel(rd, g1.n, 2.0f);
+              rc = mat(p1) == 2 ? V3{1 - kr1, 1 - kr1, 1 - kr1}
+                                : V3{sky.x * (1 - kr1), sky.y * (1 - kr1), sky.z * (1 - kr1)};
+            }
+          }
+          const float w = 1 - kr;
+          col = {rc.x * w, rc.y * w, rc.z * w};
+        }
+      } else if (p0 >= 0) {
         HitGeo g = hit_geo(b, leaf[p0], o, d, t);
         float st, su, sv;
         if (sp->shadows) {

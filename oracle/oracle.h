@@ -132,6 +132,10 @@ typedef struct {
   int bounce;  /* 0: primary + shadow (Lambert config); 1: + mirror bounce */
   int shadows; /* 1: shadow ray per hit; 0: trace_more stub (main.cpp:235-245), never shadowed */
   const float* prim_rgb; /* surface colour per primitive (3 floats), or NULL for `surface` */
+  const uint8_t* prim_mat; /* material per primitive (accelerators.h:24: 0 DIFFUSE_AND_GLOSSY,
+                              1 REFLECTION_AND_REFRACTION, 2 REFLECTION), or NULL: all 0 */
+  int mode; /* 0: castRay shading; 1: castRay's KDTREE case (main.cpp:362-370): per sample
+               black when the ray hits anything (any hit), else the sky */
 } or_shade_params;
 /* Full frame on the CPU: rays given, returns per-sample colour (3 floats) and the
    primary/bounce hits plus the shadow booleans. Any output pointer may be NULL. */

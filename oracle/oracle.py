@@ -31,7 +31,8 @@ class Hits(C.Structure):
 class ShadeParams(C.Structure):
     _fields_ = [("light", C.c_float * 3), ("emission", C.c_float * 3),
                 ("surface", C.c_float * 3), ("sky", C.c_float * 3), ("bias", C.c_float),
-                ("bounce", C.c_int), ("shadows", C.c_int), ("prim_rgb", C.c_void_p)]
+                ("bounce", C.c_int), ("shadows", C.c_int), ("prim_rgb", C.c_void_p),
+                ("prim_mat", C.c_void_p), ("mode", C.c_int)]
 
 
 RAY_FIELDS = ("ox", "oy", "oz", "dx", "dy", "dz", "tmin", "tmax")
@@ -110,6 +111,8 @@ def ref() -> C.CDLL | None:
             ("ref_sphere_closest", i32, [vp] * 4 + [u32] + [vp] * 6 + [u32, i32, vp, vp]),
             ("ref_random_double", None, [u32, vp]),
             ("ref_sphere_scene_colors", None, [vp] * 5 + [u32] + [vp] * 6 + [u32, vp]),
+            ("ref_sphere_scene_colors_mat", None, [vp] * 6 + [u32] + [vp] * 6 + [u32, vp]),
+            ("ref_kdtree_any", None, [vp] * 4 + [u32] + [vp] * 6 + [u32, vp]),
             ("ref_camera_dirs", None, [u32, u32, u32, vp, vp, vp, vp]),
         ]:
             fn = getattr(L, name)
@@ -310,14 +313,20 @@ def philox(ctr, key) -> np.ndarray:
 
 
 def shade_params(light=(0, 3, 30), emission=(1, 1, 1), surface=(0, 0, 0), sky=(0.6, 0.8, 1.0),
-                 bias=1e-4, bounce=0, shadows=1, prim_rgb=None) -> ShadeParams:
-    """prim_rgb: float32 array (3 per primitive) kept alive by the returned object."""
+                 bias=1e-4, bounce=0, shadows=1, prim_rgb=None, prim_mat=None,
+                 mode=0) -> ShadeParams:
+    """prim_rgb: float32 array (3 per primitive), prim_mat: uint8 material per
+    primitive (0 diffuse, 1 reflection + refraction, 2 reflection); both kept
+    alive by the returned object. mode 1: castRay's KDTREE case (black on any hit)."""
     sp = ShadeParams((C.c_float * 3)(*light), (C.c_float * 3)(*emission),
                      (C.c_float * 3)(*surface), (C.c_float * 3)(*sky), bias, bounce, shadows,
-                     None)
+                     None, None, mode)
     if prim_rgb is not None:
         sp._rgb = _f32(prim_rgb)
         sp.prim_rgb = _p(sp._rgb)
+    if prim_mat is not None:
+        sp._mat = np.ascontiguousarray(prim_mat, np.uint8)
+        sp.prim_mat = sp._mat.ctypes.data_as(C.c_void_p)
     return sp
 
 
@@ -367,10 +376,10 @@ class PortBVH:
         return dict(prim=prim, t=t)
 
 
-def ref_sphere_scene_colors(sph, rays: dict) -> np.ndarray | None:
-    """castRay of the reference itself (NONE loop, DIFFUSE_AND_GLOSSY shading,
-    trace_more stub) over a sphere set: one colour per ray (3 floats), or
-    None when the reference build is absent."""
+def ref_sphere_scene_colors(sph, rays: dict, mat=None) -> np.ndarray | None:
+    """castRay of the reference itself (NONE loop, trace_more stub) over a
+    sphere set, every sphere DIFFUSE_AND_GLOSSY or with materialType mat[i]:
+    one colour per ray (3 floats), or None when the reference build is absent."""
     L = ref()
     if L is None:
         return None
@@ -378,6 +387,21 @@ def ref_sphere_scene_colors(sph, rays: dict) -> np.ndarray | None:
     rk = [_f32(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
     n = len(rk[0])
     out = np.empty(3 * n, np.float32)
-    L.ref_sphere_scene_colors(*(_p(a) for a in keep), len(keep[0]), *(_p(a) for a in rk), n,
-                              _p(out))
+    m = None if mat is None else np.ascontiguousarray(mat, np.uint8)
+    L.ref_sphere_scene_colors_mat(*(_p(a) for a in keep), None if m is None else _p(m),
+                                  len(keep[0]), *(_p(a) for a in rk), n, _p(out))
+    return out
+
+
+def ref_kdtree_any(sph, rays: dict) -> np.ndarray | None:
+    """The reference's KDTREE path (its SAH kd-tree and kdtreeIntersect) over a
+    sphere set: 1 per ray that hits a sphere, or None without the reference build."""
+    L = ref()
+    if L is None:
+        return None
+    keep = [_f32(sph.cx), _f32(sph.cy), _f32(sph.cz), _f32(sph.r)]
+    rk = [_f32(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
+    n = len(rk[0])
+    out = np.empty(n, np.uint8)
+    L.ref_kdtree_any(*(_p(a) for a in keep), len(keep[0]), *(_p(a) for a in rk), n, _p(out))
     return out

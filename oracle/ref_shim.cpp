@@ -95,10 +95,28 @@ int ref_sphere_closest(const float* cx, const float* cy, const float* cz, const 
 // createScene_new's objects (main.cpp:677-689, per-sphere surface colour),
 // with the render loop's light (main.cpp:775), the NONE data structure and
 // depth 1 (main.cpp:558): one colour per ray.
+// castRay (NONE loop) over a sphere set, with mat[i] (or all
+// DIFFUSE_AND_GLOSSY when null) as each sphere's materialType: the switch of
+// main.cpp:417-446, including its recursion to depth 2.
+void ref_sphere_scene_colors_mat(const float* cx, const float* cy, const float* cz,
+                                 const float* rad, const float* rgb, const unsigned char* mat,
+                                 unsigned n, const float* ox, const float* oy, const float* oz,
+                                 const float* dx, const float* dy, const float* dz,
+                                 unsigned nrays, float* out);
+
 void ref_sphere_scene_colors(const float* cx, const float* cy, const float* cz, const float* rad,
                              const float* rgb, unsigned n, const float* ox, const float* oy,
                              const float* oz, const float* dx, const float* dy, const float* dz,
                              unsigned nrays, float* out) {
+  ref_sphere_scene_colors_mat(cx, cy, cz, rad, rgb, nullptr, n, ox, oy, oz, dx, dy, dz, nrays,
+                              out);
+}
+
+void ref_sphere_scene_colors_mat(const float* cx, const float* cy, const float* cz,
+                                 const float* rad, const float* rgb, const unsigned char* mat,
+                                 unsigned n, const float* ox, const float* oy, const float* oz,
+                                 const float* dx, const float* dy, const float* dz,
+                                 unsigned nrays, float* out) {
   sceneFixed.clear();
   std::vector<SceneObject> scene;
   for (unsigned i = 0; i < n; ++i) {
@@ -112,6 +130,7 @@ void ref_sphere_scene_colors(const float* cx, const float* cy, const float* cz, 
     s.boxBoundries = BoxBoundries(s.center - s.radius, s.center + s.radius);
     s.sphere = Sphere((int)i, s.center, s.radius, Vec3f(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]),
                       0, 0.0);
+    if (mat) s.sphere.materialType = (MaterialType)mat[i];
     scene.push_back(s);
     sceneFixed.push_back(s);
   }
@@ -126,6 +145,36 @@ void ref_sphere_scene_colors(const float* cx, const float* cy, const float* cz, 
                       scene, root, depth, settings);
     out[3 * r] = c.x, out[3 * r + 1] = c.y, out[3 * r + 2] = c.z;
   }
+}
+
+// The reference's KDTREE path: its own SAH kd-tree over the spheres
+// (constructKDTreeNew with main.cpp:815-819's arguments) and kdtreeIntersect
+// per ray (accelerators.h:996-1086). The tree lives in the reference's
+// globals, reset here to their initial state before the build.
+void ref_kdtree_any(const float* cx, const float* cy, const float* cz, const float* rad,
+                    unsigned n, const float* ox, const float* oy, const float* oz,
+                    const float* dx, const float* dy, const float* dz, unsigned nrays,
+                    unsigned char* out) {
+  std::vector<SceneObject> scene;
+  for (unsigned i = 0; i < n; ++i) {
+    SceneObject s;
+    s.objId = (int)i;
+    s.radius = rad[i];
+    s.center = Vec3f(cx[i], cy[i], cz[i]);
+    s.position = s.center;
+    s.shininess = 64;
+    s.isSphere = true;
+    s.boxBoundries = BoxBoundries(s.center - s.radius, s.center + s.radius);
+    s.sphere = Sphere((int)i, s.center, s.radius, Vec3f(0.8, 0.7, 0), 0, 0.0);
+    scene.push_back(s);
+  }
+  nAllocedNodes = 0, nextFreeNode = 0, nodes = nullptr, totalKdNodes = 0;
+  bounds = BoxBoundries();
+  primBounds.clear();
+  kdtreePrimitiveIndices.clear();
+  constructKDTreeNew(scene, 80, 1, 0.5f, 1, -1);
+  for (unsigned r = 0; r < nrays; ++r)
+    out[r] = kdtreeIntersect(Vec3f(ox[r], oy[r], oz[r]), Vec3f(dx[r], dy[r], dz[r])) ? 1 : 0;
 }
 
 // render()'s primary ray directions (main.cpp:542-556) through the

@@ -29,6 +29,8 @@ struct rq_renderer {
   double* jit = nullptr;
   uint32_t* counters = nullptr;
   std::vector<void*> allocs;
+  // a second ray per sample: the mirror bounce, or a refraction ray
+  bool second() const { return p.mode == RQ_RENDER_SHADE && (p.bounce || p.materials); }
 };
 
 namespace {
@@ -85,6 +87,8 @@ void rq_render_defaults(rq_render_params* p) {
   p->bias = 1e-4f;
   p->bounce = 0;
   p->shadows = 1;
+  p->materials = 0;
+  p->mode = RQ_RENDER_SHADE;
 }
 
 int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) {
@@ -94,6 +98,10 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
   if (n == 0 || 2 * n > 0xFFFFFFFFull) return set_err(c, RQ_E_INVALID, "bad image size");
   if (p->camera != RQ_CAMERA_REF_JITTER && p->camera != RQ_CAMERA_PHILOX)
     return set_err(c, RQ_E_INVALID, "unknown camera %d", p->camera);
+  if (p->mode != RQ_RENDER_SHADE && p->mode != RQ_RENDER_OCCLUSION)
+    return set_err(c, RQ_E_INVALID, "unknown render mode %d", p->mode);
+  if (p->materials && p->bounce)
+    return set_err(c, RQ_E_INVALID, "materials and the mirror bounce are exclusive");
   hipSetDevice(c->device);
   auto* r = new rq_renderer();
   r->p = *p;
@@ -107,14 +115,16 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
   r->sh.bias = p->bias;
   r->sh.bounce = p->bounce;
   r->sh.shadows = p->shadows;
-  const size_t ns = p->bounce ? 2 * n : n;
+  r->sh.materials = p->materials;
+  const bool second = r->second();
+  const size_t ns = second ? 2 * n : n;
   int rc = alloc_rays(c, r, n, r->prim);
-  if (!rc && p->bounce) rc = alloc_rays(c, r, n, r->bnc);
+  if (!rc && second) rc = alloc_rays(c, r, n, r->bnc);
   if (!rc) rc = alloc_rays(c, r, ns, r->shad);
   if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->h0);
   if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->t0);
-  if (!rc && p->bounce) rc = ralloc(c, r, n * 4, (void**)&r->h1);
-  if (!rc && p->bounce) rc = ralloc(c, r, n * 4, (void**)&r->t1);
+  if (!rc && second) rc = ralloc(c, r, n * 4, (void**)&r->h1);
+  if (!rc && second) rc = ralloc(c, r, n * 4, (void**)&r->t1);
   if (!rc) rc = ralloc(c, r, ns, (void**)&r->occ);
   if (!rc) rc = ralloc(c, r, n * 12, (void**)&r->srgb);
   if (!rc) rc = ralloc(c, r, (size_t)r->npix * 12, (void**)&r->img);
@@ -155,16 +165,22 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   if (!c || !r || !b) return RQ_E_INVALID;
   LaunchCtx lc = launch_ctx(c, stream);
   const rq_scene* s = b->scene;
-  SceneView sv{s->vx, s->vy, s->vz, s->idx, s->r, s->rgb};
+  SceneView sv{s->vx, s->vy, s->vz, s->idx, s->r, s->rgb, r->p.materials ? s->mat : nullptr};
   hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, r->prim);
   if (e != hipSuccess) return hip_err(c, e, "raygen");
   rq_rays pr = as_rays(r->prim);
+  if (r->p.mode == RQ_RENDER_OCCLUSION) {
+    int rc = rq_any_hit(c, b, &pr, r->n, r->occ, stream);
+    if (rc) return rc;
+    e = launch_shade_occlusion(lc, r->sh, r->npix, r->p.spp, r->occ, r->h0, r->srgb, r->img);
+    return e == hipSuccess ? RQ_OK : hip_err(c, e, "shade occlusion");
+  }
   rq_hits h0{r->h0, r->t0, nullptr, nullptr};
   int rc = rq_closest_hit(c, b, &pr, r->n, &h0, stream);
   if (rc) return rc;
   e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc);
   if (e != hipSuccess) return hip_err(c, e, "shade primary");
-  if (r->p.bounce) {
+  if (r->second()) {
     rq_rays br = as_rays(r->bnc);
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
     rc = rq_closest_hit(c, b, &br, r->n, &h1, stream);
@@ -174,7 +190,7 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   }
   if (r->p.shadows) {
     rq_rays sr = as_rays(r->shad);
-    rc = rq_any_hit(c, b, &sr, r->p.bounce ? 2 * r->n : r->n, r->occ, stream);
+    rc = rq_any_hit(c, b, &sr, r->second() ? 2 * r->n : r->n, r->occ, stream);
     if (rc) return rc;
   }
   e = launch_shade_final(lc, sv, r->sh, r->npix, r->p.spp, r->prim, r->bnc, r->h0, r->t0, r->h1,
@@ -188,14 +204,16 @@ int rq_render_counts(rq_ctx* c, rq_renderer* r, rq_frame_counts* out) {
   uint32_t k[4];
   hipError_t e = hipDeviceSynchronize();
   if (e == hipSuccess) e = hipMemset(r->counters, 0, 16);
-  if (e == hipSuccess)
-    e = launch_count_hits(LaunchCtx{nullptr, nullptr}, r->h0, r->p.bounce ? r->h1 : nullptr, r->n,
-                          r->counters);
+  const LaunchCtx lc{nullptr, nullptr};
+  const bool shade = r->p.mode == RQ_RENDER_SHADE;
+  if (e == hipSuccess && r->second()) e = launch_count_active(lc, r->bnc, r->n, r->counters);
+  if (e == hipSuccess && shade && r->p.shadows)
+    e = launch_count_active(lc, r->shad, r->second() ? 2ull * r->n : r->n, r->counters + 1);
   if (e == hipSuccess) e = hipMemcpy(k, r->counters, 16, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_err(c, e, "counts");
   out->primary = r->n;
-  out->bounce = r->p.bounce ? k[0] : 0;
-  out->shadow = r->p.shadows ? (uint64_t)k[0] + k[1] : 0;
+  out->bounce = r->second() ? k[0] : 0;
+  out->shadow = shade && r->p.shadows ? k[1] : 0;
   return RQ_OK;
 }
 
@@ -219,19 +237,19 @@ int rq_render_read_samples(rq_ctx* c, rq_renderer* r, int32_t* hit0, int32_t* hi
   hipError_t e = hipDeviceSynchronize();
   if (e == hipSuccess && hit0) e = hipMemcpy(hit0, r->h0, n * 4, hipMemcpyDeviceToHost);
   if (e == hipSuccess && hit1) {
-    if (r->p.bounce)
+    if (r->second())
       e = hipMemcpy(hit1, r->h1, n * 4, hipMemcpyDeviceToHost);
     else
       for (size_t i = 0; i < n; ++i) hit1[i] = -1;
   }
   if (e == hipSuccess && sh0) {
-    if (r->p.shadows)
+    if (r->p.shadows || r->p.mode == RQ_RENDER_OCCLUSION)
       e = hipMemcpy(sh0, r->occ, n, hipMemcpyDeviceToHost);
     else
       std::memset(sh0, 0, n);
   }
   if (e == hipSuccess && sh1) {
-    if (r->p.bounce && r->p.shadows)
+    if (r->second() && r->p.shadows)
       e = hipMemcpy(sh1, r->occ + n, n, hipMemcpyDeviceToHost);
     else
       std::memset(sh1, 0, n);
@@ -254,12 +272,12 @@ int rq_render_rays(rq_ctx* c, rq_renderer* r, int which, rq_rays* out, uint32_t*
   if (which == 0) {
     *out = as_rays(r->prim);
     if (n) *n = r->n;
-  } else if (which == 1 && r->p.bounce) {
+  } else if (which == 1 && r->second()) {
     *out = as_rays(r->bnc);
     if (n) *n = r->n;
   } else if (which == 2) {
     *out = as_rays(r->shad);
-    if (n) *n = r->p.bounce ? 2 * r->n : r->n;
+    if (n) *n = r->second() ? 2 * r->n : r->n;
   } else {
     return set_err(c, RQ_E_INVALID, "no ray batch %d", which);
   }

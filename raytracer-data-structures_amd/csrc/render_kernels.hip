@@ -160,6 +160,7 @@ __device__ inline V3 surface_of(const SceneView& s, const ShadeParams& sp, int32
   return {sp.surface[0], sp.surface[1], sp.surface[2]};
 }
 __device__ inline V3 reflect(V3 I, V3 N) { return sub(I, mul(N, 2.0f * dot(I, N))); }
+__device__ inline int mat_of(const SceneView& s, int32_t prim) { return s.mat ? s.mat[prim] : 0; }
 // main.cpp:85-103
 __device__ inline float fresnel(V3 I, V3 N, float ior) {
   float c = dot(I, N);
@@ -212,32 +213,50 @@ __device__ inline void shadow_ray(const RaySoA& R, uint64_t i, V3 d, const Geo& 
   ld = normalize(ld);
   put_ray(R, i, so, ld, 0.0f, sqrtf(dist2));
 }
+// refract (main.cpp:223-233): eta * I + (eta * cosi - sqrt(k)) * n, or 0 on
+// total internal reflection; clamp as main.cpp:76-79.
+__device__ inline V3 refract(V3 I, V3 N, float ior) {
+  float c = dot(I, N);
+  c = (c < 1.0f) ? c : 1.0f;
+  c = (-1.0f < c) ? c : -1.0f;
+  float etai = 1.0f, etat = ior;
+  V3 n = N;
+  if (c < 0.0f) {
+    c = -c;
+  } else {
+    const float tmp = etai;
+    etai = etat, etat = tmp;
+    n = {-N.x, -N.y, -N.z};
+  }
+  const float eta = etai / etat;
+  const float k = 1.0f - eta * eta * (1.0f - c * c);
+  if (k < 0.0f) return {0.0f, 0.0f, 0.0f};
+  return add(mul(I, eta), mul(n, eta * c - sqrtf(k)));
+}
 __device__ inline V3 get_o(const RaySoA& R, uint64_t i) { return {R.ox[i], R.oy[i], R.oz[i]}; }
 __device__ inline V3 get_d(const RaySoA& R, uint64_t i) { return {R.dx[i], R.dy[i], R.dz[i]}; }
 
-// Frame ray counts on demand: primary = n, bounce = primary hits (when
-// bouncing), shadow = primary hits + bounce hits. One atomic per block.
-__global__ __launch_bounds__(256) void k_count_hits(const int32_t* __restrict__ h0,
-                                                     const int32_t* __restrict__ h1, uint32_t n,
-                                                     uint32_t* counters) {
-  __shared__ uint32_t s[2];
-  if (threadIdx.x < 2) s[threadIdx.x] = 0u;
+// Frame ray counts on demand: the active rays (tmin < tmax) of one batch.
+// One atomic per block.
+__global__ __launch_bounds__(256) void k_count_active(const float* __restrict__ tmin,
+                                                       const float* __restrict__ tmax,
+                                                       uint64_t n, uint32_t* counter) {
+  __shared__ uint32_t s_c;
+  if (threadIdx.x == 0) s_c = 0u;
   __syncthreads();
-  uint32_t c0 = 0, c1 = 0;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    c0 += h0[i] >= 0;
-    if (h1) c1 += h1[i] >= 0;
-  }
-  atomicAdd(&s[0], c0);
-  atomicAdd(&s[1], c1);
+  uint32_t c = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    c += tmin[i] < tmax[i];
+  atomicAdd(&s_c, c);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    atomicAdd(&counters[0], s[0]);
-    atomicAdd(&counters[1], s[1]);
-  }
+  if (threadIdx.x == 0) atomicAdd(counter, s_c);
 }
 
-// Stage 1: from primary hits spawn the shadow ray (slot i) and the bounce ray.
+// Stage 1: from primary hits spawn the shadow ray (slot i) and the second
+// ray (slot i): the mirror bounce, or a REFLECTION_AND_REFRACTION hit's
+// refraction ray (main.cpp:420-429; its reflection ray is traced there but
+// its colour never used, so it is not traced here).
 __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams sp, uint32_t n,
                                                         RaySoA prim, const int32_t* __restrict__ h0,
                                                         const float* __restrict__ t0, RaySoA shad,
@@ -245,15 +264,30 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
   if (!in) return;
+  const bool second = sp.bounce || sp.materials;
   const int32_t p = h0[i];
   if (p < 0) {
     put_inactive(shad, i);
-    if (sp.bounce) put_inactive(bnc, i);
+    if (second) put_inactive(bnc, i);
     return;
   }
   const V3 o = get_o(prim, i), d = get_d(prim, i);
   const Geo g = hit_geo(s, p, o, d, t0[i]);
+  const int m = mat_of(s, p);
+  if (m != 0) {
+    put_inactive(shad, i);
+    if (m == 1) {
+      const V3 rd = normalize(refract(d, g.n, 3.0f));
+      const V3 ro =
+          dot(rd, g.n) < 0.0f ? sub(g.p, mul(g.n, sp.bias)) : add(g.p, mul(g.n, sp.bias));
+      put_ray(bnc, i, ro, rd, 0.0f, INFINITY);
+    } else {
+      put_inactive(bnc, i);
+    }
+    return;
+  }
   if (sp.shadows) shadow_ray(shad, i, d, g, sp);
+  if (sp.materials) put_inactive(bnc, i);
   if (sp.bounce) {
     const V3 rd = normalize(reflect(d, g.n));
     const V3 ro =
@@ -270,7 +304,7 @@ __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams s
   const bool in = i < n;
   if (!in) return;
   const int32_t p = h1[i];
-  if (p < 0) {
+  if (p < 0 || mat_of(s, p) != 0) {  // a second refracting / reflecting hit casts no shadow ray
     put_inactive(shad, (uint64_t)n + i);
     return;
   }
@@ -299,7 +333,33 @@ __global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp
     const uint32_t i = px * spp + k;
     const int32_t p = h0[i];
     V3 col = sky;
-    if (p >= 0) {
+    const int m = p >= 0 ? mat_of(s, p) : 0;
+    if (p >= 0 && m != 0) {
+      // main.cpp:418-446: refraction colour * (1 - kr), or the grey 1 - kr
+      const V3 o = get_o(prim, i), d = get_d(prim, i);
+      const Geo g = hit_geo(s, p, o, d, t0[i]);
+      const float kr = fresnel(d, g.n, 2.0f);
+      if (m == 2) {
+        col = {1.0f - kr, 1.0f - kr, 1.0f - kr};
+      } else {
+        V3 rc = sky;  // depth 2; a refracting surface there shows the sky (depth 3)
+        const int32_t q = h1[i];
+        if (q >= 0) {
+          const V3 o1 = get_o(bnc, i), d1 = get_d(bnc, i);
+          const Geo g1 = hit_geo(s, q, o1, d1, t1[i]);
+          const int m1 = mat_of(s, q);
+          if (m1 == 0) {
+            rc = direct(d1, g1, sp, surface_of(s, sp, q), sp.shadows && occ[n + i] != 0);
+          } else {
+            const float kr1 = fresnel(d1, g1.n, 2.0f);
+            rc = m1 == 2 ? V3{1.0f - kr1, 1.0f - kr1, 1.0f - kr1}
+                         : V3{sky.x * (1.0f - kr1), sky.y * (1.0f - kr1), sky.z * (1.0f - kr1)};
+          }
+        }
+        const float w = 1.0f - kr;
+        col = {rc.x * w, rc.y * w, rc.z * w};
+      }
+    } else if (p >= 0) {
       const V3 o = get_o(prim, i), d = get_d(prim, i);
       const Geo g = hit_geo(s, p, o, d, t0[i]);
       col = direct(d, g, sp, surface_of(s, sp, p), sp.shadows && occ[i] != 0);
@@ -315,6 +375,28 @@ __global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp
         col = {col.x + kr * bc.x, col.y + kr * bc.y, col.z + kr * bc.z};
       }
     }
+    if (srgb) srgb[3 * (size_t)i] = col.x, srgb[3 * (size_t)i + 1] = col.y,
+                  srgb[3 * (size_t)i + 2] = col.z;
+    acc = {acc.x + col.x, acc.y + col.y, acc.z + col.z};
+  }
+  img[3 * (size_t)px] = acc.x, img[3 * (size_t)px + 1] = acc.y, img[3 * (size_t)px + 2] = acc.z;
+}
+
+// castRay's KDTREE case (main.cpp:362-370): black where the primary ray hits
+// anything, the sky elsewhere; per pixel the samples summed in order.
+__global__ __launch_bounds__(256) void k_shade_occlusion(ShadeParams sp, uint32_t npix,
+                                                          uint32_t spp,
+                                                          const uint8_t* __restrict__ occ,
+                                                          int32_t* __restrict__ h0,
+                                                          float* __restrict__ srgb,
+                                                          float* __restrict__ img) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= npix) return;
+  V3 acc{0.0f, 0.0f, 0.0f};
+  for (uint32_t k = 0; k < spp; ++k) {
+    const uint32_t i = px * spp + k;
+    const V3 col = occ[i] ? V3{0.0f, 0.0f, 0.0f} : V3{sp.sky[0], sp.sky[1], sp.sky[2]};
+    h0[i] = -1;
     if (srgb) srgb[3 * (size_t)i] = col.x, srgb[3 * (size_t)i + 1] = col.y,
                   srgb[3 * (size_t)i + 2] = col.z;
     acc = {acc.x + col.x, acc.y + col.y, acc.z + col.z};
@@ -390,10 +472,20 @@ hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const Sh
   return hipGetLastError();
 }
 
-hipError_t launch_count_hits(const LaunchCtx& lc, const int32_t* h0, const int32_t* h1, uint32_t n,
-                             uint32_t* counters) {
-  hipLaunchKernelGGL(k_count_hits, dim3(std::min<uint32_t>(cdiv(n, 256), 1024u)), dim3(256), 0,
-                     lc.stream, h0, h1, n, counters);
+hipError_t launch_count_active(const LaunchCtx& lc, const RaySoA& R, uint64_t n,
+                               uint32_t* counter) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_count_active, dim3(std::min<uint32_t>(cdiv(n, 256), 1024u)), dim3(256), 0,
+                     lc.stream, R.tmin, R.tmax, n, counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_shade_occlusion(const LaunchCtx& lc, const ShadeParams& sp, uint32_t npix,
+                                  uint32_t spp, const uint8_t* occ, int32_t* h0, float* srgb,
+                                  float* img) {
+  ProfScope ps(lc, "shade_occlusion");
+  hipLaunchKernelGGL(k_shade_occlusion, dim3(cdiv(npix, 256)), dim3(256), 0, lc.stream, sp, npix,
+                     spp, occ, h0, srgb, img);
   return hipGetLastError();
 }
 

@@ -313,6 +313,23 @@ int rq_scene_upload_spheres(rq_ctx* c, const float* cx, const float* cy, const f
   return RQ_OK;
 }
 
+int rq_scene_set_materials(rq_ctx* c, rq_scene* s, const uint8_t* mat, void* stream) {
+  if (check_ctx(c) || !s || s->magic != SCENE_MAGIC || !mat)
+    return set_err(c, RQ_E_INVALID, "bad argument");
+  for (uint32_t i = 0; i < s->ntri; ++i)
+    if (mat[i] > RQ_MAT_REFLECTION) return set_err(c, RQ_E_INVALID, "material %u at %u", mat[i], i);
+  hipSetDevice(c->device);
+  if (!s->mat) {
+    int rc = ctx_alloc(c, std::max<size_t>(s->ntri, 1), (void**)&s->mat);
+    if (rc) return rc;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = s->ntri ? hipMemcpyAsync(s->mat, mat, s->ntri, hipMemcpyHostToDevice, st)
+                         : hipSuccess;
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "material upload");
+}
+
 int rq_scene_update_vertices(rq_ctx* c, rq_scene* s, const float* vx, const float* vy,
                              const float* vz, void* stream) {
   if (check_ctx(c) || !s || s->magic != SCENE_MAGIC) return set_err(c, RQ_E_INVALID, "bad scene");
@@ -712,6 +729,7 @@ int rq_release(rq_ctx* c, void* h) {
   if (magic == SCENE_MAGIC) {
     auto* s = (rq_scene*)h;
     s->magic = 0;
+    if (s->mat) ctx_free(c, s->mat);
     ctx_free(c, s->base);
     delete s;
     return RQ_OK;

@@ -35,6 +35,7 @@ struct rq_scene {
   uint32_t* idx = nullptr;                            // triangles only
   float* r = nullptr;                                 // sphere radii
   float* rgb = nullptr;                               // optional surface colour per primitive
+  uint8_t* mat = nullptr;  // optional material per primitive (own allocation)
   void* base = nullptr;
   size_t bytes = 0;
 };

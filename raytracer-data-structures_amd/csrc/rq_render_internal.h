@@ -23,6 +23,7 @@ struct ShadeParams {
   float bias;
   int32_t bounce;
   int32_t shadows;
+  int32_t materials;  // castRay's material switch on SceneView::mat
 };
 
 // Triangles: vertices + indices. Spheres: centres in vx/vy/vz, radii in r.
@@ -32,6 +33,7 @@ struct SceneView {
   const uint32_t* idx;
   const float* r;
   const float* rgb;
+  const uint8_t* mat;  // material per primitive, or null: all DIFFUSE_AND_GLOSSY
 };
 
 struct RaySoA {
@@ -49,8 +51,13 @@ hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const S
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
                                const RaySoA& shad);
-hipError_t launch_count_hits(const LaunchCtx& lc, const int32_t* h0, const int32_t* h1, uint32_t n,
-                             uint32_t* counters);
+// Adds the number of active rays (tmin < tmax) of R[0, n) to *counter.
+hipError_t launch_count_active(const LaunchCtx& lc, const RaySoA& R, uint64_t n,
+                               uint32_t* counter);
+// castRay's KDTREE case: per sample black when occ, else the sky; hit0 = -1.
+hipError_t launch_shade_occlusion(const LaunchCtx& lc, const ShadeParams& sp, uint32_t npix,
+                                  uint32_t spp, const uint8_t* occ, int32_t* h0, float* srgb,
+                                  float* img);
 hipError_t launch_shade_final(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                               uint32_t npix, uint32_t spp, const RaySoA& prim, const RaySoA& bnc,
                               const int32_t* h0, const float* t0, const int32_t* h1,

@@ -40,6 +40,11 @@ MORTON_REF_AFFINE = 0
 MORTON_SCENE_AABB = 1
 CAMERA_REF_JITTER = 0
 CAMERA_PHILOX = 1
+RENDER_SHADE = 0
+RENDER_OCCLUSION = 1  # castRay's KDTREE case: black on any hit, else sky
+MAT_DIFFUSE_AND_GLOSSY = 0
+MAT_REFLECTION_AND_REFRACTION = 1
+MAT_REFLECTION = 2
 
 RQ_OK = 0
 ERRORS = {-1: "RQ_E_INVALID", -2: "RQ_E_NOMEM", -3: "RQ_E_HIP", -4: "RQ_E_IO",
@@ -87,7 +92,7 @@ class RenderParams(C.Structure):
                 ("camera", C.c_int32), ("seed", C.c_uint64), ("light", C.c_float * 3),
                 ("emission", C.c_float * 3), ("surface", C.c_float * 3),
                 ("sky", C.c_float * 3), ("bias", C.c_float), ("bounce", C.c_int32),
-                ("shadows", C.c_int32)]
+                ("shadows", C.c_int32), ("materials", C.c_int32), ("mode", C.c_int32)]
 
 
 class Spheres(C.Structure):
@@ -219,6 +224,7 @@ def _load() -> C.CDLL:
                                     vp]),
         "rq_write_ppm": (i32, [C.c_char_p, u32, u32, vp]),
         "rq_scene_upload_spheres": (i32, [vp, vp, vp, vp, vp, vp, u32, C.POINTER(vp), vp]),
+        "rq_scene_set_materials": (i32, [vp, vp, vp, vp]),
         "rq_bvh_blob_size": (i32, [vp, vp, C.POINTER(C.c_size_t)]),
         "rq_bvh_save": (i32, [vp, vp, vp, C.c_size_t, C.POINTER(C.c_size_t)]),
         "rq_bvh_load": (i32, [vp, vp, vp, C.c_size_t, C.POINTER(vp), vp]),
@@ -510,6 +516,11 @@ class Context:
                                                   _ptr(a[3]), _ptr(a[4]) if colors else None,
                                                   a[0].size, C.byref(s), stream))
         return s.value
+
+    def set_materials(self, scene: int, mat, stream=None) -> None:
+        """Material per primitive (MAT_*), read by renderers with materials = 1."""
+        m = np.ascontiguousarray(mat, np.uint8)
+        self._check(lib().rq_scene_set_materials(self.h, scene, _ptr(m), stream))
 
     def build_lbvh(self, scene: int, morton_mode: int = MORTON_SCENE_AABB, stream=None) -> int:
         b = C.c_void_p()

@@ -151,9 +151,22 @@ def ref_sphere_scene() -> dict:
     rk = [np.ascontiguousarray(rays[k]) for k in ("ox", "oy", "oz", "dx", "dy", "dz")]
     ref.ref_sphere_closest(*(O._p(a) for a in cols), sph.n, *(O._p(a) for a in rk), n, 0,
                            O._p(hit), O._p(t))
+    # materials (accelerators.h:24) set per sphere, castRay's switch (main.cpp:417-446):
+    # bunny spheres cycle REFLECTION_AND_REFRACTION, REFLECTION, DIFFUSE_AND_GLOSSY;
+    # variant a keeps the ground diffuse, variant b makes it refract (rays inside it)
+    k = np.arange(sph.n)
+    mat_a = np.where(k % 3 == 0, 1, np.where(k % 3 == 1, 2, 0)).astype(np.uint8)
+    mat_a[-1] = 0
+    mat_b = mat_a.copy()
+    mat_b[-1] = 1
+    colors_mat_a = O.ref_sphere_scene_colors(sph, rays, mat_a)
+    colors_mat_b = O.ref_sphere_scene_colors(sph, rays, mat_b)
+    # the KDTREE path: the reference's own kd-tree build and kdtreeIntersect
+    kd_any = O.ref_kdtree_any(sph, rays)
     return dict(cx=sph.cx, cy=sph.cy, cz=sph.cz, r=sph.r, rgb=sph.rgb, width=np.int32(w),
                 height=np.int32(h), dx=dirs[0], dy=dirs[1], dz=dirs[2], colors=colors, hit=hit,
-                t=t)
+                t=t, mat_a=mat_a, colors_mat_a=colors_mat_a, mat_b=mat_b,
+                colors_mat_b=colors_mat_b, kd_any=kd_any)
 
 
 GENERATORS = {"ref_vectors": ref_vectors, "oracle_small": oracle_small,

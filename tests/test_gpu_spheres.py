@@ -129,3 +129,88 @@ def test_batch_refuses_sphere_scenes(ctx, rq):
             rq.Batch(ctx, [dict(scene=s, rays=None, hits=None, occ=None)])
     finally:
         ctx.release(s)
+
+
+@pytest.mark.parametrize("variant", ["a", "b"])
+def test_reference_scene_materials_render(ctx, O, rq, variant):
+    """castRay's material switch on the reference's bunny scene (fixture from its
+    compiled castRay with the materials set): refraction rays traced in the
+    second batch, hit ids identical to the oracle's at both depths, colours
+    within 2e-5 of the reference's (powf of the specular term is not correctly
+    rounded on either side), bit-exact elsewhere up to the few box-filtered
+    silhouette samples."""
+    z = np.load(GOLD / "ref_sphere_scene.npz")
+    sph = rq.SphereArrays(z["cx"], z["cy"], z["cz"], z["r"], z["rgb"])
+    mat = z[f"mat_{variant}"]
+    w, h = int(z["width"]), int(z["height"])
+    log(f"reference bunny scene materials {variant}")
+    s = ctx.upload_spheres(sph)
+    ctx.set_materials(s, mat)
+    b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
+    p = rq.render_defaults()
+    p.width, p.height, p.spp = w, h, 1
+    p.camera, p.shadows, p.bounce, p.materials = rq.CAMERA_REF_JITTER, 0, 0, 1
+    r = rq.Renderer(ctx, p)
+    try:
+        r.frame(b)
+        ctx.sync()
+        assert ctx.device_errors() == 0
+        g = r.samples()
+        rays = O.camera_rays_ref(w, h, 1, O.mt19937_jitter(2 * w * h))
+        o = O.render_samples(O.OracleBVH(sph, 1), sph, rays,
+                             O.shade_params(shadows=0, prim_rgb=sph.rgb, prim_mat=mat))
+        assert np.array_equal(g["hit0"], o["hit0"])
+        assert np.array_equal(g["hit1"], o["hit1"])
+        assert (g["hit1"] >= 0).sum() > 100
+        col, ref = g["rgb"].reshape(-1, 3), z[f"colors_mat_{variant}"].reshape(-1, 3)
+        same = np.all(_bits(o["rgb"].reshape(-1, 3)) == _bits(ref), axis=1)
+        assert (~same).sum() <= 16
+        assert np.max(np.abs(col[same] - ref[same])) < 2e-5
+        cnt = r.counts()
+        assert cnt.primary == w * h and cnt.shadow == 0
+        assert cnt.bounce == int((mat[o["hit0"][o["hit0"] >= 0]] == 1).sum())
+    finally:
+        r.close()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_reference_scene_kd_occlusion_render(ctx, rq):
+    """castRay's KDTREE case: the GPU any hit of the primary rays against the
+    reference's own kd-tree (kdtreeIntersect, fixture): identical per ray."""
+    z = np.load(GOLD / "ref_sphere_scene.npz")
+    sph = rq.SphereArrays(z["cx"], z["cy"], z["cz"], z["r"], z["rgb"])
+    w, h = int(z["width"]), int(z["height"])
+    s = ctx.upload_spheres(sph)
+    b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
+    p = rq.render_defaults()
+    p.width, p.height, p.spp = w, h, 1
+    p.camera, p.shadows, p.mode = rq.CAMERA_REF_JITTER, 0, rq.RENDER_OCCLUSION
+    r = rq.Renderer(ctx, p)
+    try:
+        r.frame(b)
+        ctx.sync()
+        assert ctx.device_errors() == 0
+        g = r.samples()
+        assert np.array_equal(g["shadow0"], z["kd_any"])
+        assert np.all(g["hit0"] == -1)
+        col = g["rgb"].reshape(-1, 3)
+        assert np.all(col[z["kd_any"] == 1] == 0)
+        assert np.all(col[z["kd_any"] == 0] == np.float32([0.6, 0.8, 1.0]))
+        cnt = r.counts()
+        assert cnt.primary == w * h and cnt.bounce == 0 and cnt.shadow == 0
+    finally:
+        r.close()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_render_params_refused(ctx, rq):
+    p = rq.render_defaults()
+    p.materials, p.bounce = 1, 1
+    with pytest.raises(rq.RQError):
+        rq.Renderer(ctx, p)
+    p = rq.render_defaults()
+    p.mode = 7
+    with pytest.raises(rq.RQError):
+        rq.Renderer(ctx, p)
