@@ -33,6 +33,10 @@ sys.path.insert(0, str(ROOT))
 import rq  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# LBVH build bytes per triangle of the reference byte model: Morton 56 (index +
+# vertex gather in, code + index out), sort 68 (4 passes x 16 + histogram), Karras
+# 28, refit 124
+BUILD_MODEL_BYTES_PER_TRI = 276
 
 
 def log(msg: str) -> None:
@@ -115,9 +119,9 @@ def copy_probe(ctx) -> None:
     dst.free()
 
 
-def _pmc_run(counter: str, config: str, extra: list | None = None) -> dict | None:
-    """One rocprofv3 --pmc child run of this script in probe mode; returns the
-    counter's mean per dispatch (KiB) for each kernel name."""
+def _pmc_run(counters: str, config: str, extra: list | None = None) -> dict | None:
+    """One rocprofv3 --pmc child run of this script in probe mode: for each
+    counter, {kernel name: (mean value per dispatch, dispatches)}."""
     import csv
     import glob
     import shutil
@@ -128,15 +132,15 @@ def _pmc_run(counter: str, config: str, extra: list | None = None) -> dict | Non
     if not Path(prof).exists():
         return None
     out = tempfile.mkdtemp(prefix="rq_pmc_", dir="/tmp")
-    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "probe", "--",
-           sys.executable, str(ROOT / "bench.py"), "--pmc-probe", "--config", config,
+    cmd = [prof, "--pmc", *counters.split(), "--output-format", "csv", "-d", out, "-o", "probe",
+           "--", sys.executable, str(ROOT / "bench.py"), "--pmc-probe", "--config", config,
            "--no-cpu", "--traffic", "off", "--warmup", "1", *(extra or [])]
     env = dict(os.environ, TMPDIR="/tmp")
     try:
         subprocess.run(cmd, cwd="/tmp", env=env, timeout=240, check=True,
                        stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     except (subprocess.SubprocessError, OSError) as e:
-        log(f"pmc {counter}: {e}")
+        log(f"pmc {counters}: {e}")
         return None
     files = glob.glob(f"{out}/**/*counter_collection.csv", recursive=True)
     if not files:
@@ -144,13 +148,20 @@ def _pmc_run(counter: str, config: str, extra: list | None = None) -> dict | Non
     sums, counts = {}, {}
     with open(files[0]) as f:
         for row in csv.DictReader(f):
-            if row.get("Counter_Name") != counter:
-                continue
-            k = row["Kernel_Name"]
-            sums[k] = sums.get(k, 0.0) + float(row["Counter_Value"])
-            counts[k] = counts.get(k, 0) + 1
+            key = (row.get("Counter_Name"), row["Kernel_Name"])
+            sums[key] = sums.get(key, 0.0) + float(row["Counter_Value"])
+            counts[key] = counts.get(key, 0) + 1
     shutil.rmtree(out, ignore_errors=True)
-    return {k: sums[k] / counts[k] for k in sums}
+    res = {}
+    for (c, k), v in sums.items():
+        res.setdefault(c, {})[k] = (v / counts[(c, k)], counts[(c, k)])
+    return res
+
+
+def _pick(per_kernel: dict, sym: str):
+    """The mean of the most-dispatched kernel whose name contains sym."""
+    hits = [(n, v) for k, (v, n) in per_kernel.items() if sym in k]
+    return max(hits)[1] if hits else None
 
 
 def measure_traffic(config: str, kernel: str, trace_cfg: str | None = None):
@@ -163,19 +174,48 @@ def measure_traffic(config: str, kernel: str, trace_cfg: str | None = None):
     detail = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         res = _pmc_run(counter, config, ["--trace-config", trace_cfg] if trace_cfg else None)
-        if not res:
+        if not res or counter not in res:
             return None
-        kv = [v for k, v in res.items() if sym in k]
-        cv = [v for k, v in res.items() if "k_copy16" in k]
-        if not kv or not cv or cv[0] <= 0:
+        kv, cv = _pick(res[counter], sym), _pick(res[counter], "k_copy16")
+        if kv is None or not cv:
             return None
-        scale = (COPY_PROBE_BYTES / 1024.0) / cv[0]
-        nbytes = kv[0] * 1024.0 * scale
-        detail[counter] = {"kib_per_dispatch": round(kv[0], 1), "copy_scale": round(scale, 4),
+        scale = (COPY_PROBE_BYTES / 1024.0) / cv
+        nbytes = kv * 1024.0 * scale
+        detail[counter] = {"kib_per_dispatch": round(kv, 1), "copy_scale": round(scale, 4),
                            "bytes": int(nbytes)}
         total += nbytes
     log(f"traffic {kernel}: {detail}")
     return int(total)
+
+
+# One pass: L2 hits and misses, VALU issue and lane cycles, GPU-active cycles.
+COUNTER_SET = "TCC_HIT_sum TCC_MISS_sum SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU GRBM_GUI_ACTIVE"
+MI355X_CUS, MI355X_XCDS = 256, 8
+
+
+def measure_counters(config: str, kernel: str, trace_cfg: str | None = None):
+    """L2 hit rate, VALU busy and VALU lane utilisation of `kernel` from one
+    rocprofv3 --pmc pass: hits / (hits + misses); SQ_ACTIVE_INST_VALU per CU over
+    one XCD's GPU-active cycles (GRBM_GUI_ACTIVE reads the sum over the eight
+    XCDs; one wave64 VALU instruction per CU per cycle is the issue peak); lanes
+    active per VALU instruction, SQ_THREAD_CYCLES_VALU / (64 * SQ_ACTIVE_INST_VALU)."""
+    sym = KERNEL_SYMBOL.get(kernel)
+    if sym is None:
+        return None
+    res = _pmc_run(COUNTER_SET, config, ["--trace-config", trace_cfg] if trace_cfg else None)
+    names = COUNTER_SET.split()
+    if not res or any(c not in res for c in names):
+        return None
+    v = {c: _pick(res[c], sym) for c in names}
+    if any(x is None for x in v.values()) or not v["SQ_ACTIVE_INST_VALU"]:
+        return None
+    hit, miss = v["TCC_HIT_sum"], v["TCC_MISS_sum"]
+    cycles = v["GRBM_GUI_ACTIVE"] / MI355X_XCDS
+    out = {"l2_hit": round(hit / max(hit + miss, 1.0), 4),
+           "valu_busy": round(v["SQ_ACTIVE_INST_VALU"] / MI355X_CUS / max(cycles, 1.0), 4),
+           "valu_lane_util": round(v["SQ_THREAD_CYCLES_VALU"] / (64 * v["SQ_ACTIVE_INST_VALU"]), 4)}
+    log(f"counters {kernel}: {out}")
+    return out
 
 
 def run_c3(ctx, args) -> int:
@@ -239,6 +279,7 @@ def run_c3(ctx, args) -> int:
                     bytes_per_launch=int(alg), avg_launch_ms=round(dms, 4))
     if args.traffic == "auto":
         roofline["traffic"] = measure_traffic("c3", dom, cfg_arg(cfg))
+        roofline["counters"] = measure_counters("c3", dom, cfg_arg(cfg))
     line = {
         "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
         "value": round(2 * n * args.steps / elapsed / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
@@ -748,6 +789,10 @@ def main() -> int:
     scene = ctx.upload_scene(mesh)
     bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
     ren = rq.Renderer(ctx, params)
+    if args.trace_config:  # fixed configuration: every launch, warm-up included, uses it
+        vc0, rc0, va0, ra0 = (int(x) for x in args.trace_config.split(","))
+        ctx.trace_config(vc0, va0)
+        ctx.trace_refill(rc0, ra0)
 
     for _ in range(args.warmup):
         ctx.rebuild(bvh)
@@ -821,12 +866,13 @@ def main() -> int:
                         traffic=None, bytes_per_launch=int(alg),
                         avg_launch_ms=round(avg_ms, 4))
     build_kernels = ("build_init", "centroid_bounds", "morton_hist", "radix_onesweep", "karras",
-                     "refit", "refit_top")
+                     "refit", "refit_top", "widen")
     build_ms = sum(per_kernel[k][0] for k in build_kernels if k in per_kernel) / args.steps
     closest_ms = per_kernel.get("trace_closest", (0, 1))[0] / args.steps
     closest_rays_n = sum(r for _, r in closest_rays)
     if roofline is not None and args.traffic == "auto":
         roofline["traffic"] = measure_traffic(args.config, roofline["kernel"], cfg_arg(cfg))
+        roofline["counters"] = measure_counters(args.config, roofline["kernel"], cfg_arg(cfg))
     cpu = None
     if not args.no_cpu:
         host = [batches[k].download() for k in batches]
@@ -859,6 +905,13 @@ def main() -> int:
             "bvh_height": int(info.height),
         },
         "lbvh_build_ms": round(build_ms, 4),
+        # the build's byte model: 276 B per triangle (Morton 56, sort 68, Karras 28,
+        # refit 124) against 8 TB/s
+        "lbvh_build_roofline": {"bytes_per_tri": BUILD_MODEL_BYTES_PER_TRI,
+                                "achieved_gbs": round(mesh.ntri * BUILD_MODEL_BYTES_PER_TRI
+                                                      / (build_ms * 1e-3) / 1e9, 1),
+                                "frac": round(mesh.ntri * BUILD_MODEL_BYTES_PER_TRI
+                                              / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "closest_hit_mrays_s": round(closest_rays_n / (closest_ms * 1e-3) / 1e6, 1)
         if closest_ms else None,
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in per_kernel.items()},
