@@ -119,8 +119,8 @@ int rq_bvh_export(rq_ctx* ctx, const rq_bvh* b, uint32_t* codes, uint32_t* sorte
                   int32_t* left, int32_t* right, float* boxes);
 /* Download the 4-wide nodes (see rq_trace_variant): for binary internal node
  * i, boxes[i*24 + slot*6 + (lo xyz, hi xyz)] and links[i*4 + slot] (>= 0 wide
- * node, < 0 leaf ~k, 0 unused slot). Only the root and nodes at even depth
- * are defined; the rest of each array is left as the device holds it.
+ * node, < 0 leaf ~k, 0 unused slot). Every internal node has one; the wide
+ * walk reaches the root and the nodes at even depth.
  * RQ_E_UNSUPPORTED for a BVH without wide nodes. */
 int rq_bvh_export_wide(rq_ctx* ctx, const rq_bvh* b, float* boxes, int32_t* links);
 

@@ -121,7 +121,7 @@ __device__ inline void build_init_part(const BuildView& v, uint32_t first, uint3
   const uint32_t nstatus = SORT_PASSES * tiles_of(v.n, SORT_TILE_LEAN) * 256;  // both tilings
   for (uint32_t k = first; k < nstatus; k += stride) v.B.status[k] = 0u;
   for (uint32_t k = first; k < (uint32_t)MISC_BOUNDS; k += stride) misc[k] = 0u;  // hist, tickets
-  if (first == 0) misc[MISC_ERR] = 0u, misc[MISC_HEIGHT] = 0u;
+  if (first == 0) misc[MISC_ERR] = 0u, misc[MISC_HEIGHT] = HEIGHT_UNKNOWN;
   if (first < 3) {
     misc[MISC_BOUNDS + first] = 0xFFFFFFFFu;  // lo (ordered min)
     misc[MISC_BOUNDS + 3 + first] = 0u;        // hi (ordered max)
@@ -588,12 +588,60 @@ __device__ inline void refit_top_block(const BuildView& v) {
 // child c of i: c's box if c is a leaf (one slot, the next one unused), else
 // the boxes of c's two children (two slots). Boxes are copied exactly, so a
 // slab test against a wide slot is the test the binary traversal makes
-// against the same subtree. Only nodes at even depth are written (the root,
-// and every internal slot of a written node, is at even depth); the depth
-// comes from the parent links, at most 64 steps. Returns d + 1 for a node
-// with a leaf child (the depth of that leaf), else 0: the tree height is the
-// maximum over all nodes.
-__device__ inline uint32_t widen_node(const BuildBuffers& B, uint32_t n, uint32_t i) {
+// against the same subtree. Every internal node gets its wide node (only the
+// root and even-depth nodes are reached from the root).
+struct WideSlots {
+  float lo[4][3], hi[4][3];
+  int32_t link[4];
+};
+__device__ inline void wide_empty(WideSlots& w, int k) {
+  w.lo[k][0] = w.lo[k][1] = w.lo[k][2] = INFINITY;
+  w.hi[k][0] = w.hi[k][1] = w.hi[k][2] = -INFINITY;
+  w.link[k] = 0;
+}
+__device__ inline void wide_store(WideNode* W, const WideSlots& w) {
+  W->lox = make_float4(w.lo[0][0], w.lo[1][0], w.lo[2][0], w.lo[3][0]);
+  W->hix = make_float4(w.hi[0][0], w.hi[1][0], w.hi[2][0], w.hi[3][0]);
+  W->loy = make_float4(w.lo[0][1], w.lo[1][1], w.lo[2][1], w.lo[3][1]);
+  W->hiy = make_float4(w.hi[0][1], w.hi[1][1], w.hi[2][1], w.hi[3][1]);
+  W->loz = make_float4(w.lo[0][2], w.lo[1][2], w.lo[2][2], w.lo[3][2]);
+  W->hiz = make_float4(w.hi[0][2], w.hi[1][2], w.hi[2][2], w.hi[3][2]);
+  W->link = make_int4(w.link[0], w.link[1], w.link[2], w.link[3]);
+  W->pad = make_int4(0, 0, 0, 0);
+}
+// From the finished binary node records in global memory.
+__device__ inline void widen_global(const BuildBuffers& B, uint32_t i) {
+  const NodeRec N = B.nodes[i];
+  WideSlots w;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const int32_t c = __float_as_int(side ? N.hi0.w : N.lo0.w);
+    const float4 clo = side ? N.lo1 : N.lo0, chi = side ? N.hi1 : N.hi0;
+    if (c < 0) {
+      w.lo[2 * side][0] = clo.x, w.lo[2 * side][1] = clo.y, w.lo[2 * side][2] = clo.z;
+      w.hi[2 * side][0] = chi.x, w.hi[2 * side][1] = chi.y, w.hi[2 * side][2] = chi.z;
+      w.link[2 * side] = c;
+      wide_empty(w, 2 * side + 1);
+    } else {
+      const NodeRec C = B.nodes[c];
+      w.lo[2 * side][0] = C.lo0.x, w.lo[2 * side][1] = C.lo0.y, w.lo[2 * side][2] = C.lo0.z;
+      w.hi[2 * side][0] = C.hi0.x, w.hi[2 * side][1] = C.hi0.y, w.hi[2 * side][2] = C.hi0.z;
+      w.lo[2 * side + 1][0] = C.lo1.x, w.lo[2 * side + 1][1] = C.lo1.y;
+      w.lo[2 * side + 1][2] = C.lo1.z;
+      w.hi[2 * side + 1][0] = C.hi1.x, w.hi[2 * side + 1][1] = C.hi1.y;
+      w.hi[2 * side + 1][2] = C.hi1.z;
+      w.link[2 * side] = __float_as_int(C.lo0.w), w.link[2 * side + 1] = __float_as_int(C.hi0.w);
+    }
+  }
+  wide_store(B.wide + i, w);
+}
+
+// Tree height on demand (the chunk-union refit leaves it open): the depth of
+// node i's leaf children, d + 1 for a node at depth d with a leaf child (walk
+// up the parent links, at most 64 steps), else 0; the height is the maximum.
+__device__ inline uint32_t leaf_depth(const BuildBuffers& B, uint32_t n, uint32_t i) {
+  const NodeRec N = B.nodes[i];
+  if (__float_as_int(N.lo0.w) >= 0 && __float_as_int(N.hi0.w) >= 0) return 0u;
   uint32_t d = 0, cur = i;
   while (cur != 0u) {
     const int32_t pe = B.parent[cur];
@@ -603,37 +651,7 @@ __device__ inline uint32_t widen_node(const BuildBuffers& B, uint32_t n, uint32_
       return 0u;
     }
   }
-  const NodeRec N = B.nodes[i];
-  const uint32_t h = (__float_as_int(N.lo0.w) < 0 || __float_as_int(N.hi0.w) < 0) ? d + 1u : 0u;
-  if (d & 1u) return h;
-  const float4 empty_lo = make_float4(INFINITY, INFINITY, INFINITY, 0.0f);
-  const float4 empty_hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
-  float4 lo[4], hi[4];
-  int32_t link[4];
-#pragma unroll
-  for (int side = 0; side < 2; ++side) {
-    const int32_t c = __float_as_int(side ? N.hi0.w : N.lo0.w);
-    if (c < 0) {
-      lo[2 * side] = side ? N.lo1 : N.lo0, hi[2 * side] = side ? N.hi1 : N.hi0;
-      link[2 * side] = c;
-      lo[2 * side + 1] = empty_lo, hi[2 * side + 1] = empty_hi, link[2 * side + 1] = 0;
-    } else {
-      const NodeRec C = B.nodes[c];
-      lo[2 * side] = C.lo0, hi[2 * side] = C.hi0, link[2 * side] = __float_as_int(C.lo0.w);
-      lo[2 * side + 1] = C.lo1, hi[2 * side + 1] = C.hi1;
-      link[2 * side + 1] = __float_as_int(C.hi0.w);
-    }
-  }
-  WideNode& W = B.wide[i];
-  W.lox = make_float4(lo[0].x, lo[1].x, lo[2].x, lo[3].x);
-  W.hix = make_float4(hi[0].x, hi[1].x, hi[2].x, hi[3].x);
-  W.loy = make_float4(lo[0].y, lo[1].y, lo[2].y, lo[3].y);
-  W.hiy = make_float4(hi[0].y, hi[1].y, hi[2].y, hi[3].y);
-  W.loz = make_float4(lo[0].z, lo[1].z, lo[2].z, lo[3].z);
-  W.hiz = make_float4(hi[0].z, hi[1].z, hi[2].z, hi[3].z);
-  W.link = make_int4(link[0], link[1], link[2], link[3]);
-  W.pad = make_int4(0, 0, 0, 0);
-  return h;
+  return d + 1u;
 }
 
 // ---- chunk-union refit (standalone build) ----------------------------------------
@@ -661,9 +679,12 @@ __device__ inline void box_load(const float* src, float* b) {
   b[0] = a.x, b[1] = a.y, b[2] = c.x, b[3] = c.y, b[4] = e.x, b[5] = e.y;
 }
 
-// LDS: REFIT_CHUNK arrival counters, 2 * 16 wave unions (6 floats each) and
-// the child boxes of the chunk's nodes ([node - chunk start][side][6]).
-constexpr int REFIT_CHUNK_LDS_WORDS = REFIT_CHUNK + 2 * (REFIT_CHUNK / 64) * 6 + REFIT_CHUNK * 12;
+// LDS: REFIT_CHUNK arrival counters, 2 * 16 wave unions (6 floats each), the
+// child boxes of the chunk's nodes ([node - chunk start][side][6]), and the
+// parent links of the chunk's leaves and internal nodes with the internal
+// nodes' leaf ranges (loaded once, so the climb reads no global memory).
+constexpr int REFIT_CHUNK_LDS_WORDS =
+    REFIT_CHUNK + 2 * (REFIT_CHUNK / 64) * 6 + REFIT_CHUNK * 12 + REFIT_CHUNK * 6;
 
 __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
   static_assert(REFIT_CHUNK == 1024, "one leaf per thread of a 1024-thread block");
@@ -671,6 +692,10 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
   float* s_wpre = reinterpret_cast<float*>(lds + REFIT_CHUNK);  // [wave][6] exclusive prefix
   float* s_wsuf = s_wpre + (REFIT_CHUNK / 64) * 6;               // [wave][6] exclusive suffix
   float* s_box = s_wsuf + (REFIT_CHUNK / 64) * 6;                // [node][side][6] child boxes
+  int32_t* s_pleaf = reinterpret_cast<int32_t*>(s_box + REFIT_CHUNK * 12);  // parent of leaf
+  int32_t* s_pint = s_pleaf + REFIT_CHUNK;                                   // parent of node
+  uint2* s_rng = reinterpret_cast<uint2*>(s_pint + REFIT_CHUNK);             // range of node
+  int2* s_kid = reinterpret_cast<int2*>(s_rng + REFIT_CHUNK);                // children of node
   const BuildBuffers& B = v.B;
   const uint32_t n = v.n;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
@@ -679,6 +704,13 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
   const uint32_t k = chunk_lo + tid;
   const bool valid = k <= chunk_hi;
   s_flag[tid] = 0u;
+  // the chunk's parent links and ranges (internal node ids chunk_lo + tid)
+  if (valid && n > 1) s_pleaf[tid] = B.parent[(n - 1) + k];
+  if (k < n - 1) {
+    s_pint[tid] = k == 0u ? -1 : B.parent[k];  // the root has no parent
+    s_rng[tid] = B.range[k];
+    s_kid[tid] = make_int2(__float_as_int(B.nodes[k].lo0.w), __float_as_int(B.nodes[k].hi0.w));
+  }
   float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   if (valid) {
     TriRec tr;
@@ -724,38 +756,77 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
     box_store(B.suf + 6ull * k, suf);
   }
   if (tid == 0) box_store(B.cbox + 6ull * chunk, suf);  // leaf 0's suffix: the chunk's union
-  if (!valid) return;
-  if (n == 1) {
-    put_root(B.misc, box, box + 3, 0u);
-    return;
-  }
+  if (valid && n == 1) put_root(B.misc, box, box + 3, 0u);
   float* lo = box;
   float* hi = box + 3;
-  uint32_t cur = (n - 1) + k;
-  for (int level = 0;; ++level) {
-    const int32_t pe = B.parent[cur];
+  // cur: this thread's leaf (level 0), then the internal node it completed,
+  // which lies in the chunk (its index is in its own range)
+  uint32_t cur = k;
+  for (int level = 0; valid && n > 1; ++level) {
+    const int32_t pe = level == 0 ? s_pleaf[tid] : s_pint[cur - chunk_lo];
     const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
     if (pe < 0 || node >= n - 1 || level > 64) {
       atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
-      return;
+      break;
     }
     put_slot(B.nodes, node, side, lo, hi);
-    const uint2 rg = B.range[node];
-    if (rg.x < chunk_lo || rg.y > chunk_hi) return;  // spanning: pass 2
+    // a node outside the chunk's index span cannot have its range inside it
+    if (node < chunk_lo || node > chunk_hi) break;  // spanning: pass 2
+    const uint2 rg = s_rng[node - chunk_lo];
+    if (rg.x < chunk_lo || rg.y > chunk_hi) break;  // spanning: pass 2
     float* mine = s_box + (node - chunk_lo) * 12;
 #pragma unroll
     for (int q = 0; q < 6; ++q) mine[6 * side + q] = box[q];
-    const uint32_t old = __hip_atomic_fetch_add(&s_flag[node - chunk_lo], 1u, __ATOMIC_ACQ_REL,
+    // Everything another thread of the climb reads is in LDS (the sibling's
+    // box), so the arrival counter orders LDS only: the node-record stores
+    // to global memory are read by later kernels and need not complete
+    // before the count (an all-memory release would wait for every
+    // outstanding store at each level).
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    const uint32_t old = __hip_atomic_fetch_add(&s_flag[node - chunk_lo], 1u, __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0u) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (old == 0u) break;
     box_union(box, mine + 6 * (side ^ 1u));  // the sibling's box, stored before its arrival
     if (node == 0u) {  // the whole tree lies in this chunk
       float* root = reinterpret_cast<float*>(B.misc + MISC_ROOT);
       for (int q = 0; q < 3; ++q) root[q] = lo[q], root[3 + q] = hi[q];
-      return;
+      break;
     }
     cur = node;
   }
+  if (!B.wide) return;
+  // Wide records of the chunk's completed nodes (both children arrived, so
+  // every box below them is final in LDS): thread t writes node chunk start +
+  // t, so the 128-byte records go out in consecutive lines. An internal child
+  // of such a node lies in the chunk too.
+  __syncthreads();
+  if (k >= n - 1 || s_flag[tid] != 2u) return;
+  const float* mine = s_box + tid * 12;
+  WideSlots ws;
+  const int2 kid = s_kid[tid];
+#pragma unroll
+  for (int sd = 0; sd < 2; ++sd) {
+    const int32_t c = sd ? kid.y : kid.x;
+    if (c < 0) {
+      const float* cb = mine + 6 * sd;
+      ws.lo[2 * sd][0] = cb[0], ws.lo[2 * sd][1] = cb[1], ws.lo[2 * sd][2] = cb[2];
+      ws.hi[2 * sd][0] = cb[3], ws.hi[2 * sd][1] = cb[4], ws.hi[2 * sd][2] = cb[5];
+      ws.link[2 * sd] = c;
+      wide_empty(ws, 2 * sd + 1);
+    } else {
+      const float* gb = s_box + ((uint32_t)c - chunk_lo) * 12;
+      const int2 gk = s_kid[(uint32_t)c - chunk_lo];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        ws.lo[2 * sd + g][0] = gb[6 * g], ws.lo[2 * sd + g][1] = gb[6 * g + 1];
+        ws.lo[2 * sd + g][2] = gb[6 * g + 2], ws.hi[2 * sd + g][0] = gb[6 * g + 3];
+        ws.hi[2 * sd + g][1] = gb[6 * g + 4], ws.hi[2 * sd + g][2] = gb[6 * g + 5];
+      }
+      ws.link[2 * sd] = gk.x, ws.link[2 * sd + 1] = gk.y;
+    }
+  }
+  wide_store(B.wide + k, ws);
 }
 
 // Pass 2: the lanes of a wave each take internal node i; the wave then works

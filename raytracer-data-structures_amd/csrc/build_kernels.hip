@@ -53,13 +53,27 @@ __global__ __launch_bounds__(256) void k_refit_span(BuildBuffers B, uint32_t n) 
   refit_span_wave(B, n, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Wide nodes and the tree height: each block's maximum leaf depth goes to
-// B.flags[block] (free once the refit is done), k_height reduces them (one
-// device atomic per wave on a single word serialises across the XCDs).
-__global__ __launch_bounds__(256) void k_widen(BuildBuffers B, uint32_t n) {
+// Wide nodes from the finished binary records: every internal node (after a
+// blob load), or the nodes spanning refit chunks (the chunk refit writes the
+// others).
+__global__ __launch_bounds__(256) void k_widen_all(BuildBuffers B, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n - 1) widen_global(B, i);
+}
+__global__ __launch_bounds__(256) void k_widen_span(BuildBuffers B, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const uint2 rg = B.range[i];
+  if (rg.x / REFIT_CHUNK != rg.y / REFIT_CHUNK) widen_global(B, i);
+}
+
+// Tree height: each block's deepest leaf to B.flags[block] (free once the
+// refit is done), k_height reduces them (a device atomic per wave on one word
+// serialises across the XCDs).
+__global__ __launch_bounds__(256) void k_height_walk(BuildBuffers B, uint32_t n) {
   __shared__ uint32_t s_h[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  uint32_t h = i < n - 1 ? widen_node(B, n, i) : 0u;
+  uint32_t h = i < n - 1 ? leaf_depth(B, n, i) : 0u;
   for (int d = 32; d > 0; d >>= 1) h = max(h, (uint32_t)__shfl_xor((int)h, d, 64));
   if ((threadIdx.x & 63) == 0) s_h[threadIdx.x >> 6] = h;
   __syncthreads();
@@ -120,6 +134,10 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
       ProfScope ps(lc, "refit_top");
       hipLaunchKernelGGL(k_refit_span, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
     }
+    if (nchunks > 1 && B.wide) {
+      ProfScope ps(lc, "widen");
+      hipLaunchKernelGGL(k_widen_span, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
+    }
   } else {  // frontier refit
     {
       ProfScope ps(lc, "refit");
@@ -130,15 +148,20 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
       hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, v);
     }
   }
-  if (B.wide) return launch_widen(lc, B, n);
   return hipGetLastError();
 }
 
 hipError_t launch_widen(const LaunchCtx& lc, BuildBuffers& B, uint32_t n) {
   if (n < 2 || !B.wide) return hipGetLastError();
   ProfScope ps(lc, "widen");
+  hipLaunchKernelGGL(k_widen_all, dim3(tiles_of(n - 1, 256)), dim3(256), 0, lc.stream, B, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_height(const LaunchCtx& lc, BuildBuffers& B, uint32_t n) {
+  if (n < 2) return hipGetLastError();
   const uint32_t nblocks = tiles_of(n - 1, 256);  // <= n - 1 entries of B.flags
-  hipLaunchKernelGGL(k_widen, dim3(nblocks), dim3(256), 0, lc.stream, B, n);
+  hipLaunchKernelGGL(k_height_walk, dim3(nblocks), dim3(256), 0, lc.stream, B, n);
   hipLaunchKernelGGL(k_height, dim3(1), dim3(1024), 0, lc.stream, B, nblocks);
   return hipGetLastError();
 }

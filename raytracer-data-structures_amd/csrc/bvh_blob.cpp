@@ -91,6 +91,8 @@ int rq_bvh_save(rq_ctx* c, const rq_bvh* b, void* blob, size_t cap, size_t* writ
   const uint32_t n = b->n;
   const size_t pb = payload_bytes(n), total = sizeof(Header) + pb;
   if (cap < total) return set_err(c, RQ_E_INVALID, "blob buffer too small (%zu < %zu)", cap, total);
+  int hrc = ensure_height(c, b);
+  if (hrc) return hrc;
   hipError_t e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_err(c, e, "save sync");
   Header h{};

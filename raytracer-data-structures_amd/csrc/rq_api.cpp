@@ -109,6 +109,19 @@ using namespace rq;
 
 namespace rq {
 
+int ensure_height(rq_ctx* c, const rq_bvh* b) {
+  if (b->n < 2) return RQ_OK;
+  uint32_t h = 0;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(&h, b->B.misc + MISC_HEIGHT, 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && h == HEIGHT_UNKNOWN) {
+    BuildBuffers B = b->B;
+    e = launch_height(LaunchCtx{nullptr, nullptr}, B, b->n);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+  }
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "tree height");
+}
+
 int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide) {
   BuildBuffers& B = b->B;
   B.ntiles = (n + SORT_TILE_LEAN - 1) / SORT_TILE_LEAN;
@@ -387,6 +400,8 @@ int rq_bvh_info_get(rq_ctx* c, const rq_bvh* b, rq_bvh_info* info) {
   info->morton_mode = b->mode;
   info->device_bytes = b->bytes;
   if (b->n == 0) return RQ_OK;
+  int rc = ensure_height(c, b);
+  if (rc) return rc;
   uint32_t misc[MISC_WORDS];
   hipError_t e = hipMemcpy(misc, b->B.misc, sizeof misc, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_err(c, e, "bvh info");

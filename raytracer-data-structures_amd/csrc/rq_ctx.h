@@ -62,4 +62,7 @@ uint32_t* next_counter_slot(rq_ctx* c);
 // chunk-union refit buffers (the standalone build and blob load; batch and
 // server slots walk binary nodes and refit by frontier lists).
 int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide);
+// Makes the tree height in b's misc words known (blocking; the chunk-union
+// refit leaves it to this call).
+int ensure_height(rq_ctx* c, const rq_bvh* b);
 }  // namespace rq

@@ -33,9 +33,9 @@ struct alignas(16) TriRec {
 static_assert(sizeof(TriRec) == 48, "triangle record is 48 bytes");
 
 // 4-wide node: the binary LBVH collapsed two levels at a time. The wide node
-// of a binary internal node i (kept at index i, so the array is sparse: only
-// the root and nodes at even depth are ever reached) holds the boxes of i's
-// grandchildren, or of a child that is a leaf, as four child slots in SoA
+// of a binary internal node i (kept at index i; every internal node has one,
+// the walk reaches the root and the nodes at even depth) holds the boxes of
+// i's grandchildren, or of a child that is a leaf, as four child slots in SoA
 // order, one 128-byte record per visit. Links follow the binary convention
 // (>= 0 wide node = binary index, < 0 leaf ~k); an unused slot has link 0
 // (the root is never a child) and an empty box (lo +inf, hi -inf).
@@ -110,7 +110,7 @@ struct BuildBuffers {
   uint32_t* fcount = nullptr;     // cdiv(n, REFIT_CHUNK) entries per chunk
   NodeRec* nodes = nullptr;       // n-1
   TriRec* tris = nullptr;         // n
-  WideNode* wide = nullptr;       // n-1 (sparse, see WideNode), or null: binary only
+  WideNode* wide = nullptr;       // n-1 (see WideNode), or null: binary only
   // Chunk-union refit (allocated with `wide`; null: frontier refit). Boxes
   // are 6 floats (lo xyz, hi xyz).
   float* pre = nullptr;           // 6n: union of the leaves from the chunk start to k
@@ -124,9 +124,10 @@ constexpr int MISC_HIST = 0;                          // SORT_PASSES * 256
 constexpr int MISC_TILECTR = MISC_HIST + SORT_PASSES * 256;  // SORT_PASSES
 constexpr int MISC_BOUNDS = MISC_TILECTR + SORT_PASSES;      // 6 ordered-uint (lo xyz, hi xyz)
 constexpr int MISC_ROOT = MISC_BOUNDS + 6;                   // 6 floats root box
-constexpr int MISC_HEIGHT = MISC_ROOT + 6;                   // 1
+constexpr int MISC_HEIGHT = MISC_ROOT + 6;                   // 1 (HEIGHT_UNKNOWN: not computed)
 constexpr int MISC_ERR = MISC_HEIGHT + 1;                    // 1: build error bits
 constexpr int MISC_WORDS = MISC_ERR + 1;
+constexpr uint32_t HEIGHT_UNKNOWN = 0xFFFFFFFFu;
 
 // Build error bits (never expected; they turn a logic error into a reported
 // failure instead of an out-of-range access or an endless wait).
@@ -150,9 +151,11 @@ struct LaunchCtx {
 hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, const float* vz,
                         const uint32_t* idx, uint32_t n, int morton_mode, BuildBuffers& B,
                         const float* vr = nullptr);
-// Writes B.wide from the binary nodes of a built LBVH (part of launch_build
-// when B.wide is set; called by itself after a blob load).
+// Writes B.wide from the binary nodes of a built LBVH (after a blob load; the
+// standalone build writes it during its refit).
 hipError_t launch_widen(const LaunchCtx& lc, BuildBuffers& B, uint32_t n);
+// Computes the tree height into misc (the chunk-union refit leaves it open).
+hipError_t launch_height(const LaunchCtx& lc, BuildBuffers& B, uint32_t n);
 
 struct TraceArgs {
   const NodeRec* nodes;

@@ -23,31 +23,39 @@ def _wide_variants(ctx):
     return [v for v in ctx.trace_variants() if v["wide"]]
 
 
-def _expected_collapse(e, n):
-    """Wide node of every even-depth internal node, from the binary export."""
+def _expected_collapse(e, n, reachable_only=False):
+    """Wide node of every internal node (or only of those the wide walk reaches
+    from the root: the root and the even-depth nodes), from the binary export."""
     left, right, boxes = e["left"], e["right"], e["boxes"]
     empty = np.float32([np.inf, np.inf, np.inf, -np.inf, -np.inf, -np.inf])
 
     def box(link):
         return boxes[link] if link >= 0 else boxes[(n - 1) + ~link]
 
-    out = {}
-    stack = [0]
-    while stack:
-        i = stack.pop()
+    def collapse(i):
         slots = []
         for c in (left[i], right[i]):
             if c < 0:
                 slots += [(box(c), c), (empty, 0)]
             else:
                 slots += [(box(left[c]), left[c]), (box(right[c]), right[c])]
-        out[i] = (np.stack([b for b, _ in slots]), np.int32([l for _, l in slots]))
-        stack += [int(l) for _, l in slots if l > 0]
+        return np.stack([b for b, _ in slots]), np.int32([l for _, l in slots])
+
+    if not reachable_only:
+        return {i: collapse(i) for i in range(n - 1)}
+    out = {}
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        out[i] = collapse(i)
+        stack += [int(l) for l in out[i][1] if l > 0]
     return out
 
 
 @pytest.mark.parametrize("name", ["tri2", "dup64", "soup100", "ico2", "soup4097", "ico5"])
 def test_wide_nodes_are_the_binary_collapse(ctx, rq, name):
+    """Records written inside the refit chunks (from LDS) and for the nodes
+    spanning chunks (from the finished binary records) alike."""
     mesh = mesh_configs()[name]()
     n = mesh.ntri
     log(f"wide collapse {name} n={n}")
@@ -55,10 +63,11 @@ def test_wide_nodes_are_the_binary_collapse(ctx, rq, name):
     b = ctx.build_lbvh(s, 1)
     try:
         e, w = ctx.export(b, n), ctx.export_wide(b, n)
-        exp = _expected_collapse(e, n)
-        # the wide tree reaches every leaf exactly once
-        leaves = np.concatenate([l[l < 0] for _, l in exp.values()])
+        # the wide tree reached from the root holds every leaf exactly once
+        reach = _expected_collapse(e, n, reachable_only=True)
+        leaves = np.concatenate([l[l < 0] for _, l in reach.values()])
         assert np.array_equal(np.sort(~leaves), np.arange(n))
+        exp = _expected_collapse(e, n)  # every internal node has its wide record
         for i, (bx, ln) in exp.items():
             assert np.array_equal(w["links"][i], ln), i
             assert np.array_equal(_bits(w["boxes"][i]), _bits(bx)), i
