@@ -87,16 +87,18 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   float* s_tn = reinterpret_cast<float*>(lds + DEPTH * BUILD_BLOCK);
   const uint32_t gtid = blockIdx.x * BUILD_BLOCK + threadIdx.x;
   const uint32_t gstride = gridDim.x * BUILD_BLOCK;
+  RayStream stream;
+  stream.ctr32 = s_ctr;
   if (any) {
     t.occ = m.occ + r0;
-    trace_loop<true, false, BUILD_BLOCK, DEPTH, false, false>(t, s_ctr, s_node, s_tn, gtid,
+    trace_loop<true, false, BUILD_BLOCK, DEPTH, false, false>(t, stream, s_node, s_tn, gtid,
                                                               gstride);
   } else {
     t.prim = m.prim + r0;
     t.t = m.t ? m.t + r0 : nullptr;
     t.u = m.u ? m.u + r0 : nullptr;
     t.v = m.v_ ? m.v_ + r0 : nullptr;
-    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false>(t, s_ctr, s_node, s_tn, gtid,
+    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false>(t, stream, s_node, s_tn, gtid,
                                                                 gstride);
   }
   __syncthreads();

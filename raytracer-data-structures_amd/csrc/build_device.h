@@ -296,25 +296,41 @@ __device__ inline void onesweep_tile(const BuildView& v, uint32_t tile, uint32_t
   const uint32_t blk_excl = pre_t + inc_t - total;
   const uint32_t glb_excl = pre_h + inc_h - hcount;
 
-  // decoupled look-back over earlier tiles for this digit
+  // decoupled look-back over earlier tiles for this digit, LB_WINDOW status
+  // words per round: the loads of a window are in flight together, and they
+  // are consumed in order up to the first inclusive prefix (done) or the first
+  // word not yet published (polled again from there). Tile 0 publishes an
+  // inclusive prefix, so the walk ends at tile 0 at the latest.
+  constexpr int LB_WINDOW = 8;
   uint32_t prefix = 0;
   if (tile > 0) {
     int64_t j = (int64_t)tile - 1;
     uint32_t spins = 0;
     for (;;) {
-      const uint32_t s = ld_agent(status + (size_t)j * 256 + d);
-      const uint32_t f = s & ~LB_MASK;
-      if (f == 0u) {
+      uint32_t w[LB_WINDOW];
+#pragma unroll
+      for (int q = 0; q < LB_WINDOW; ++q)
+        w[q] = j - q >= 0 ? ld_agent(status + (size_t)(j - q) * 256 + d) : 0u;
+      int used = 0;
+      bool done = false;
+#pragma unroll
+      for (int q = 0; q < LB_WINDOW; ++q) {
+        if (done || used < q || j - q < 0) break;
+        const uint32_t f = w[q] & ~LB_MASK;
+        if (f == 0u) break;
+        prefix += w[q] & LB_MASK;
+        ++used;
+        done = f == LB_INC;
+      }
+      if (done) break;
+      j -= used;
+      if (used == 0) {
         if (++spins > LOOKBACK_SPIN_LIMIT) {  // bounded wait: report, never hang
           atomicOr(&misc[MISC_ERR], ERR_LOOKBACK_SPIN);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
-        continue;
       }
-      prefix += s & LB_MASK;
-      if (f == LB_INC) break;
-      --j;
     }
     st_agent(st, LB_INC | (prefix + total));
   }

@@ -98,10 +98,7 @@ LaunchCtx launch_ctx(rq_ctx* c, void* stream) {
   return LaunchCtx{(hipStream_t)stream, c->prof_on ? c->prof : nullptr};
 }
 
-uint32_t* next_counter_slot(rq_ctx* c) {
-  uint32_t s = c->counter_seq++ % COUNTER_SLOTS;
-  return c->counters + 2 * s;
-}
+uint32_t next_counter_slot(rq_ctx* c) { return c->counter_seq++ % COUNTER_SLOTS; }
 
 }  // namespace rq
 
@@ -519,14 +516,19 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.wide = b->B.wide;
   a.gstack = c->gstack;
   a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
-  a.counters = next_counter_slot(c);
+  const uint32_t slot = next_counter_slot(c);
+  a.counter = reinterpret_cast<unsigned long long*>(c->counters) + slot;
+  a.ctr_base = c->counter_base[slot];
   a.err = c->errword;
   a.grid = c->trace_grid[variant];
   a.refill_idle = any ? c->refill_any : c->refill_closest;
   a.chunk = trace_chunk(n, a.grid * (uint32_t)blk / 64u);
   a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
   hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
-  return e == hipSuccess ? RQ_OK : hip_err(c, e, "trace launch");
+  if (e != hipSuccess) return hip_err(c, e, "trace launch");
+  // the launch moves its slot's counter by ceil(n / chunk) * chunk (RayStream)
+  c->counter_base[slot] += (uint64_t)((n + a.chunk - 1) / a.chunk) * a.chunk;
+  return RQ_OK;
 }
 
 int rq_closest_hit(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,

@@ -17,7 +17,10 @@ struct rq_ctx {
   int closest_variant = 0, any_variant = 0;           // chosen traversal variants
   // idle lanes before a wave refills (64: only once every lane is idle), per query kind
   uint32_t refill_closest = 64, refill_any = 64;
-  uint32_t* counters = nullptr;  // COUNTER_SLOTS x {fetch, exit}
+  // COUNTER_SLOTS 64-bit ray-stream counters (never reset), their values
+  // after the launches enqueued so far, and one word of traversal error bits
+  uint32_t* counters = nullptr;
+  uint64_t counter_base[rq::COUNTER_SLOTS] = {};
   uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
   uint32_t counter_seq = 0;
   int32_t* gstack = nullptr;  // traversal stack overflow ({node, distance} pairs)
@@ -57,7 +60,10 @@ int hip_err(rq_ctx* c, hipError_t e, const char* what);
 int ctx_alloc(rq_ctx* c, size_t bytes, void** p);
 void ctx_free(rq_ctx* c, void* p);
 LaunchCtx launch_ctx(rq_ctx* c, void* stream);
-uint32_t* next_counter_slot(rq_ctx* c);
+// The next ray-stream counter slot; *base receives its value at the start of
+// a launch that uses it, and a launch of n rays in slices of `chunk` must
+// report back through advance_counter_slot.
+uint32_t next_counter_slot(rq_ctx* c);
 // Build buffers for n primitives; `wide`: also the 4-wide node array and the
 // chunk-union refit buffers (the standalone build and blob load; batch and
 // server slots walk binary nodes and refit by frontier lists).

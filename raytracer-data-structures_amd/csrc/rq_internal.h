@@ -171,7 +171,8 @@ struct TraceArgs {
   uint32_t* tests;
   int32_t* gstack;
   uint32_t gdepth;      // overflow entries per thread
-  uint32_t* counters;   // 2 words: fetch, exit (reset by the last wave)
+  unsigned long long* counter;  // ray-stream counter of this launch's slot (never reset)
+  uint64_t ctr_base;    // its value when this launch starts
   uint32_t* err;        // set to 1 if a ray needed a deeper stack than provided
   uint32_t grid;        // blocks (persistent grid of the chosen variant)
   uint32_t refill_idle; // idle lanes in a wave before it refills from its slice

@@ -13,6 +13,21 @@
 
 namespace rq {
 
+// Where a wave takes its slices of the ray stream from. The standalone kernels
+// give every wave a first slice by its index (no atomic) and take the rest
+// from a 64-bit counter that is never reset: slice = counter add - base +
+// waves * chunk. Each wave with a valid first slice makes exactly one failing
+// add and every later slice is one add, so a launch of n rays in slices of c
+// moves the counter by exactly ceil(n / c) * c, which the host adds to the
+// base of the slot's next launch. The batch kernel counts from 0 in LDS.
+struct RayStream {
+  uint32_t* ctr32 = nullptr;            // counter from 0 (batch, LDS), or
+  unsigned long long* ctr64 = nullptr;  // the slot's monotonic counter
+  uint64_t base = 0;                    // ctr64 at launch
+  uint32_t first = 0;                   // this wave's first slice (ctr64 mode)
+  uint32_t dyn0 = 0;                    // first counted slice: waves * chunk
+};
+
 struct RayState {
   float ox, oy, oz, dx, dy, dz;
   float ix, iy, iz;  // 1/d
@@ -124,7 +139,7 @@ __device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) 
 // the result (the lexicographic minimum) is the same; visit order differs.
 template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true,
           bool WIDE = false>
-__device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_node,
+__device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32_t* s_node,
                                   float* s_tn, uint32_t gtid, uint32_t gstride) {
   static_assert(!(WIDE && COUNT), "counted traversal walks the binary nodes");
   const int tid = threadIdx.x, lane = tid & 63;
@@ -138,6 +153,10 @@ __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_
   // a time from the global counter (one atomic per slice, not per refill)
   uint32_t cnext = 0, cend = 0;
   bool wave_out = false;  // the global stream is exhausted for this wave
+  if (rs.ctr64) {
+    cnext = rs.first, cend = min(rs.first + a.chunk, a.nrays);
+    wave_out = rs.first >= a.nrays;
+  }
   uint32_t ray = 0;
   int32_t node = 0, best = -1;
   int sp = 0;
@@ -195,15 +214,17 @@ __device__ inline void trace_loop(const TraceArgs& a, uint32_t* ctr, int32_t* s_
     uint64_t m = __ballot(!active);
     for (int round = 0; round < 2 && m && !wave_out; ++round) {
       if (cnext >= cend) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(ctr, a.chunk);
+        uint64_t base = 0;
+        if (lane == 0)
+          base = rs.ctr64 ? atomicAdd(rs.ctr64, (unsigned long long)a.chunk) - rs.base + rs.dyn0
+                          : atomicAdd(rs.ctr32, a.chunk);
         base = __shfl(base, 0, 64);
         if (base >= a.nrays) {
           wave_out = true;
           break;
         }
-        cnext = base;
-        cend = min(base + a.chunk, a.nrays);
+        cnext = (uint32_t)base;
+        cend = min(cnext + a.chunk, a.nrays);
       }
       const uint32_t take = min((uint32_t)__popcll(m), cend - cnext);
       const uint32_t rank = (uint32_t)__popcll(m & lt);

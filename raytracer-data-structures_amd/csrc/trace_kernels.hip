@@ -19,18 +19,14 @@ template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool WIDE = f
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
   __shared__ int32_t s_node[DEPTH * BLOCK];
   __shared__ float s_tn[TSTACK ? DEPTH * BLOCK : 1];
-  trace_loop<ANY, COUNT, BLOCK, DEPTH, TSTACK, true, WIDE>(a, &a.counters[0], s_node, s_tn,
+  RayStream rs;
+  rs.ctr64 = a.counter;
+  rs.base = a.ctr_base;
+  rs.first = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) * a.chunk;
+  rs.dyn0 = gridDim.x * (BLOCK / 64) * a.chunk;
+  trace_loop<ANY, COUNT, BLOCK, DEPTH, TSTACK, true, WIDE>(a, rs, s_node, s_tn,
                                                            blockIdx.x * BLOCK + threadIdx.x,
                                                            gridDim.x * BLOCK);
-  // last wave out resets the counters for the next launch on this slot
-  if ((threadIdx.x & 63) == 0) {
-    const uint32_t waves = gridDim.x * (BLOCK / 64);
-    const uint32_t old = atomicAdd(&a.counters[1], 1u);
-    if (old == waves - 1) {
-      atomicExch(&a.counters[0], 0u);
-      atomicExch(&a.counters[1], 0u);
-    }
-  }
 }
 
 typedef void (*TraceFn)(TraceArgs);
