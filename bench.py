@@ -88,20 +88,45 @@ KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<
                  "batch": "k_batch<"}
 
 
+def parse_trace_config(text: str) -> tuple[int, ...]:
+    """--trace-config: closest variant, refill, any variant, refill and
+    optionally the largest ray-stream slices (closest, any; default 1024)."""
+    t = tuple(int(x) for x in text.split(","))
+    if len(t) == 4:
+        t += (1024, 1024)
+    if len(t) != 6:
+        raise SystemExit("--trace-config takes 4 or 6 comma-separated integers")
+    return t
+
+
+def apply_trace_config(ctx, cfg) -> None:
+    vc, rc, va, ra, sc, sa = cfg
+    ctx.trace_config(vc, va)
+    ctx.trace_refill(rc, ra)
+    ctx.trace_slice(sc, sa)
+
+
 def tune_trace(ctx, bvh, closest_rays, any_rays, args, repeats: int):
     """The traversal configuration (closest variant, refill, any variant,
-    refill): --trace-config fixes it (no trial launches, e.g. under a
-    profiler), --no-autotune keeps the defaults, else the autotuner picks it."""
+    refill, closest slice, any slice): --trace-config fixes it (no trial
+    launches, e.g. under a profiler), --no-autotune keeps the defaults, else
+    the autotuner picks it."""
     if args.trace_config:
-        vc, rc, va, ra = (int(x) for x in args.trace_config.split(","))
+        cfg = parse_trace_config(args.trace_config)
     elif args.no_autotune:
         return None
     else:
         vc, rc, _ = ctx.autotune(bvh, closest_rays, any_hit=False, repeats=repeats)
         va, ra, _ = ctx.autotune(bvh, any_rays, any_hit=True, repeats=repeats)
-    ctx.trace_config(vc, va)
-    ctx.trace_refill(rc, ra)
-    return vc, rc, va, ra
+        cfg = (vc, rc, va, ra) + ctx.trace_slice()
+    apply_trace_config(ctx, cfg)
+    return cfg
+
+
+def tuned_dict(ctx, cfg) -> dict:
+    vc, rc, va, ra, sc, sa = cfg
+    return {"closest_variant": ctx.trace_variants()[vc], "any_variant": ctx.trace_variants()[va],
+            "refill_idle": {"closest": rc, "any": ra}, "slice": {"closest": sc, "any": sa}}
 
 
 def cfg_arg(t) -> str | None:
@@ -240,9 +265,7 @@ def run_c3(ctx, args) -> int:
     tuned = None
     cfg = tune_trace(ctx, bvh, rays, rays, args, repeats=2)
     if cfg:
-        c_v, c_r, a_v, a_r = cfg
-        tuned = {"closest_variant": c_v, "any_variant": a_v,
-                 "refill_idle": {"closest": c_r, "any": a_r}}
+        tuned = tuned_dict(ctx, cfg)
     for _ in range(args.warmup):
         ctx.closest_hit_dev(bvh, rays, hits)
         ctx.any_hit_dev(bvh, rays, occ)
@@ -790,9 +813,7 @@ def main() -> int:
     bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
     ren = rq.Renderer(ctx, params)
     if args.trace_config:  # fixed configuration: every launch, warm-up included, uses it
-        vc0, rc0, va0, ra0 = (int(x) for x in args.trace_config.split(","))
-        ctx.trace_config(vc0, va0)
-        ctx.trace_refill(rc0, ra0)
+        apply_trace_config(ctx, parse_trace_config(args.trace_config))
 
     for _ in range(args.warmup):
         ctx.rebuild(bvh)
@@ -812,9 +833,7 @@ def main() -> int:
     tuned = None
     cfg = tune_trace(ctx, bvh, batches["primary"], batches["shadow"], args, repeats=3)
     if cfg:
-        vc, rc_, va, ra = cfg
-        tuned = {"closest_variant": ctx.trace_variants()[vc], "any_variant":
-                 ctx.trace_variants()[va], "refill_idle": {"closest": rc_, "any": ra}}
+        tuned = tuned_dict(ctx, cfg)
         log(f"autotune: {tuned}")
         for _ in range(2):
             ctx.rebuild(bvh)

@@ -178,10 +178,18 @@ int rq_trace_config_get(rq_ctx* ctx, int* closest_variant, int* any_variant,
  * both to its refill_idle, config_get reports the closest-hit one. */
 int rq_trace_refill_set(rq_ctx* ctx, uint32_t closest_refill, uint32_t any_refill);
 int rq_trace_refill_get(rq_ctx* ctx, uint32_t* closest_refill, uint32_t* any_refill);
+/* Largest wave slice per query kind: the persistent waves take rays from the
+ * launch's ray stream in slices of this many (a power of two, 64..4096;
+ * default 1024), halved for launches too small to give every wave two, and a
+ * quarter as many over the stream's last quarter. Larger slices cost fewer
+ * atomics, smaller ones keep the rays in flight closer together. */
+int rq_trace_slice_set(rq_ctx* ctx, uint32_t closest_slice, uint32_t any_slice);
+int rq_trace_slice_get(rq_ctx* ctx, uint32_t* closest_slice, uint32_t* any_slice);
 /* Times every variant (and refill threshold 16/48/56/64 idle lanes) on the given
- * device rays (default stream, `repeats` launches each after one warm-up),
- * keeps the fastest for that query kind (variant and refill) in the context
- * and reports it. */
+ * device rays (default stream, `repeats` launches each after one warm-up), then
+ * slice sizes 256/512/2048 against the default 1024 for the best of them, keeps
+ * the fastest for that query kind (variant, refill, slice) in the context and
+ * reports it (the slice through rq_trace_slice_get). */
 int rq_autotune_trace(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, int any,
                       int repeats, int* best_variant, uint32_t* best_refill, double* best_ms);
 

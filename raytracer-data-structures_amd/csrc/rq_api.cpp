@@ -520,14 +520,20 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.counter = reinterpret_cast<unsigned long long*>(c->counters) + slot;
   a.ctr_base = c->counter_base[slot];
   a.err = c->errword;
-  a.grid = c->trace_grid[variant];
+  // the launch grid (launch_trace takes the same), its waves and slice size
+  a.grid = std::max(1u, std::min(c->trace_grid[variant], (n + (uint32_t)blk - 1) / (uint32_t)blk));
   a.refill_idle = any ? c->refill_any : c->refill_closest;
-  a.chunk = trace_chunk(n, a.grid * (uint32_t)blk / 64u);
+  const uint32_t waves = a.grid * (uint32_t)blk / 64u;
+  const SliceSchedule sched =
+      slice_schedule(n, trace_chunk(n, c->trace_grid[variant] * (uint32_t)blk / 64u,
+                                    any ? c->slice_any : c->slice_closest));
+  a.chunk = sched.chunk, a.chunk_small = sched.small, a.big_slices = sched.big;
   a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
   hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
   if (e != hipSuccess) return hip_err(c, e, "trace launch");
-  // the launch moves its slot's counter by ceil(n / chunk) * chunk (RayStream)
-  c->counter_base[slot] += (uint64_t)((n + a.chunk - 1) / a.chunk) * a.chunk;
+  // the launch moves its slot's counter by its number of slices, or not at
+  // all when the waves' first slices cover it (RayStream)
+  if (sched.total > waves) c->counter_base[slot] += sched.total;
   return RQ_OK;
 }
 
@@ -584,6 +590,26 @@ int rq_trace_refill_get(rq_ctx* c, uint32_t* closest_refill, uint32_t* any_refil
   return RQ_OK;
 }
 
+static bool slice_ok(uint32_t s) {
+  return s >= 64 && s <= TRACE_RAY_CHUNK_MAX && (s & (s - 1)) == 0;
+}
+
+int rq_trace_slice_set(rq_ctx* c, uint32_t closest_slice, uint32_t any_slice) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (!slice_ok(closest_slice) || !slice_ok(any_slice))
+    return set_err(c, RQ_E_INVALID, "slice size not a power of two in 64..%u", TRACE_RAY_CHUNK_MAX);
+  c->slice_closest = closest_slice;
+  c->slice_any = any_slice;
+  return RQ_OK;
+}
+
+int rq_trace_slice_get(rq_ctx* c, uint32_t* closest_slice, uint32_t* any_slice) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (closest_slice) *closest_slice = c->slice_closest;
+  if (any_slice) *any_slice = c->slice_any;
+  return RQ_OK;
+}
+
 int rq_trace_config_get(rq_ctx* c, int* closest_variant, int* any_variant, uint32_t* refill_idle) {
   if (check_ctx(c)) return RQ_E_INVALID;
   if (closest_variant) *closest_variant = c->closest_variant;
@@ -605,14 +631,29 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
   }
   const int saved_c = c->closest_variant, saved_a = c->any_variant;
   const uint32_t saved_rc = c->refill_closest, saved_ra = c->refill_any;
+  const uint32_t saved_sc = c->slice_closest, saved_sa = c->slice_any;
   const bool saved_prof = c->prof_on;
   c->prof_on = false;
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
+  // mean time of `repeats` launches after an untimed one, in the current configuration
+  auto time_it = [&](double* ms_out) {
+    rq_hits h{(int32_t*)out0, (float*)out1, nullptr, nullptr};
+    for (int k = 0; k <= repeats && !rc; ++k) {
+      if (k == 1) (void)hipEventRecord(e0, nullptr);
+      rc = any ? rq_any_hit(c, b, rays, n, (uint8_t*)out0, nullptr)
+               : rq_closest_hit(c, b, rays, n, &h, nullptr);
+    }
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    *ms_out = ms / repeats;
+  };
   double best = 1e30;
   int bv = 0;
-  uint32_t br = 16;
+  uint32_t br = 16, bs = TRACE_RAY_CHUNK;
   const uint32_t refills[] = {16, 48, 56, 64};
   for (int v = 0; v < trace_num_variants() && !rc; ++v) {
     int blk, depth, ts, wide;
@@ -621,19 +662,21 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
     if (wide && !b->B.wide) continue;
     for (uint32_t rf : refills) {
       c->closest_variant = v, c->any_variant = v, c->refill_closest = c->refill_any = rf;
-      rq_hits h{(int32_t*)out0, (float*)out1, nullptr, nullptr};
-      // one untimed launch, then `repeats` timed ones
-      for (int k = 0; k <= repeats && !rc; ++k) {
-        if (k == 1) (void)hipEventRecord(e0, nullptr);
-        rc = any ? rq_any_hit(c, b, rays, n, (uint8_t*)out0, nullptr)
-                 : rq_closest_hit(c, b, rays, n, &h, nullptr);
-      }
-      (void)hipEventRecord(e1, nullptr);
-      (void)hipEventSynchronize(e1);
-      float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (!rc && ms / repeats < best) best = ms / repeats, bv = v, br = rf;
+      c->slice_closest = c->slice_any = bs;
+      double ms = 0;
+      time_it(&ms);
+      if (!rc && ms < best) best = ms, bv = v, br = rf;
     }
+  }
+  // then the slice size for the best variant and refill
+  const uint32_t slices[] = {256, 512, 2048};
+  for (uint32_t sl : slices) {
+    if (rc) break;
+    c->closest_variant = c->any_variant = bv, c->refill_closest = c->refill_any = br;
+    c->slice_closest = c->slice_any = sl;
+    double ms = 0;
+    time_it(&ms);
+    if (!rc && ms < best) best = ms, bs = sl;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -642,11 +685,12 @@ int rq_autotune_trace(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t 
   c->prof_on = saved_prof;
   c->closest_variant = saved_c, c->any_variant = saved_a;
   c->refill_closest = saved_rc, c->refill_any = saved_ra;
+  c->slice_closest = saved_sc, c->slice_any = saved_sa;
   if (rc) return rc;
   if (any)
-    c->any_variant = bv, c->refill_any = br;
+    c->any_variant = bv, c->refill_any = br, c->slice_any = bs;
   else
-    c->closest_variant = bv, c->refill_closest = br;
+    c->closest_variant = bv, c->refill_closest = br, c->slice_closest = bs;
   if (best_variant) *best_variant = bv;
   if (best_refill) *best_refill = br;
   if (best_ms) *best_ms = best;

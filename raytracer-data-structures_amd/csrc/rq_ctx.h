@@ -17,6 +17,8 @@ struct rq_ctx {
   int closest_variant = 0, any_variant = 0;           // chosen traversal variants
   // idle lanes before a wave refills (64: only once every lane is idle), per query kind
   uint32_t refill_closest = 64, refill_any = 64;
+  // largest wave slice of the ray stream, per query kind (trace_chunk)
+  uint32_t slice_closest = rq::TRACE_RAY_CHUNK, slice_any = rq::TRACE_RAY_CHUNK;
   // COUNTER_SLOTS 64-bit ray-stream counters (never reset), their values
   // after the launches enqueued so far, and one word of traversal error bits
   uint32_t* counters = nullptr;

@@ -81,16 +81,33 @@ constexpr int TRACE_MAX_STACK_WIDE = 96;
 constexpr int TRACE_MIN_LDS_DEPTH = 12;
 constexpr int TRACE_MAX_VARIANTS = 16;
 constexpr int COUNTER_SLOTS = 64;
-constexpr uint32_t TRACE_RAY_CHUNK = 256;  // rays per wave slice of the ray stream (largest)
+constexpr uint32_t TRACE_RAY_CHUNK = 1024;      // default largest wave slice of the ray stream
+constexpr uint32_t TRACE_RAY_CHUNK_MAX = 4096;  // largest that may be configured
 
 // Rays per wave slice for a launch of n rays over `waves` persistent waves:
-// 256 when every wave gets at least two full slices, halved down to one ray
-// per lane (64) for smaller launches, so that a small batch still spreads
-// over as many waves as it can fill.
-inline uint32_t trace_chunk(uint32_t n, uint32_t waves) {
-  uint32_t c = TRACE_RAY_CHUNK;
+// `largest` (a power of two, 64..4096) when every wave gets at least two full
+// slices, halved down to one ray per lane (64) for smaller launches, so that
+// a small batch still spreads over as many waves as it can fill (each slice
+// after a wave's first costs one device atomic on the launch's counter).
+inline uint32_t trace_chunk(uint32_t n, uint32_t waves, uint32_t largest = TRACE_RAY_CHUNK) {
+  uint32_t c = largest;
   while (c > 64 && (uint64_t)n < 2ull * c * waves) c >>= 1;
   return c;
+}
+// Slice schedule of a standalone launch: slices of `chunk` rays over the
+// first three quarters of the stream, then slices of chunk / 4 (at least 64)
+// so that the waves finish together; slice s starts at slice_begin(s).
+struct SliceSchedule {
+  uint32_t chunk, small, big, total;  // sizes, number of large slices, number of slices
+};
+inline SliceSchedule slice_schedule(uint32_t n, uint32_t chunk) {
+  SliceSchedule q;
+  q.chunk = chunk;
+  q.small = chunk / 4 < 64 ? 64 : chunk / 4;
+  q.big = (uint32_t)((3ull * n / 4) / chunk);
+  const uint64_t rest = n - (uint64_t)q.big * chunk;
+  q.total = q.big + (uint32_t)((rest + q.small - 1) / q.small);
+  return q;
 }
 
 // Build workspace (device), all sized for n primitives.
@@ -176,7 +193,9 @@ struct TraceArgs {
   uint32_t* err;        // set to 1 if a ray needed a deeper stack than provided
   uint32_t grid;        // blocks (persistent grid of the chosen variant)
   uint32_t refill_idle; // idle lanes in a wave before it refills from its slice
-  uint32_t chunk;       // rays per slice taken from the global counter
+  uint32_t chunk;       // rays per slice taken from the global counter (the large slices)
+  uint32_t chunk_small; // rays per slice after the first big_slices (standalone kernels)
+  uint32_t big_slices;  // slices of `chunk` rays before the small ones
   int32_t sphere;       // leaves are spheres (centre, r, r*r) instead of triangles
 };
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,

@@ -14,19 +14,26 @@
 namespace rq {
 
 // Where a wave takes its slices of the ray stream from. The standalone kernels
-// give every wave a first slice by its index (no atomic) and take the rest
-// from a 64-bit counter that is never reset: slice = counter add - base +
-// waves * chunk. Each wave with a valid first slice makes exactly one failing
-// add and every later slice is one add, so a launch of n rays in slices of c
-// moves the counter by exactly ceil(n / c) * c, which the host adds to the
-// base of the slot's next launch. The batch kernel counts from 0 in LDS.
+// number the slices (a.chunk rays each for the first a.big_slices, then
+// a.chunk_small): wave w starts on slice w (no atomic), and takes slice
+// waves + k from the k-th add to a 64-bit counter that is never reset
+// (k = counter add - base). When the waves' first slices cover the launch no
+// wave touches the counter; otherwise each wave makes exactly one failing add
+// and every later slice is one add, so the launch moves the counter by
+// exactly its number of slices, which the host adds to the base of the
+// slot's next launch. The batch kernel counts rays from 0 in LDS.
 struct RayStream {
-  uint32_t* ctr32 = nullptr;            // counter from 0 (batch, LDS), or
-  unsigned long long* ctr64 = nullptr;  // the slot's monotonic counter
+  uint32_t* ctr32 = nullptr;            // counter of rays from 0 (batch, LDS), or
+  unsigned long long* ctr64 = nullptr;  // the slot's monotonic slice counter
   uint64_t base = 0;                    // ctr64 at launch
-  uint32_t first = 0;                   // this wave's first slice (ctr64 mode)
-  uint32_t dyn0 = 0;                    // first counted slice: waves * chunk
+  uint32_t wave = 0, waves = 0;         // this wave's index, waves in the grid
 };
+__device__ inline uint32_t slice_begin(const TraceArgs& a, uint64_t s) {
+  return s <= a.big_slices ? (uint32_t)min(s * a.chunk, (uint64_t)a.nrays)
+                           : (uint32_t)min((uint64_t)a.big_slices * a.chunk +
+                                               (s - a.big_slices) * a.chunk_small,
+                                           (uint64_t)a.nrays);
+}
 
 struct RayState {
   float ox, oy, oz, dx, dy, dz;
@@ -154,8 +161,8 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
   uint32_t cnext = 0, cend = 0;
   bool wave_out = false;  // the global stream is exhausted for this wave
   if (rs.ctr64) {
-    cnext = rs.first, cend = min(rs.first + a.chunk, a.nrays);
-    wave_out = rs.first >= a.nrays;
+    cnext = slice_begin(a, rs.wave), cend = slice_begin(a, rs.wave + 1ull);
+    wave_out = cnext >= a.nrays;
   }
   uint32_t ray = 0;
   int32_t node = 0, best = -1;
@@ -214,17 +221,22 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     uint64_t m = __ballot(!active);
     for (int round = 0; round < 2 && m && !wave_out; ++round) {
       if (cnext >= cend) {
-        uint64_t base = 0;
-        if (lane == 0)
-          base = rs.ctr64 ? atomicAdd(rs.ctr64, (unsigned long long)a.chunk) - rs.base + rs.dyn0
-                          : atomicAdd(rs.ctr32, a.chunk);
-        base = __shfl(base, 0, 64);
-        if (base >= a.nrays) {
+        if (rs.ctr64 && slice_begin(a, rs.waves) >= a.nrays) {  // first slices cover all
           wave_out = true;
           break;
         }
-        cnext = (uint32_t)base;
-        cend = min(cnext + a.chunk, a.nrays);
+        uint64_t s = 0;
+        if (lane == 0)
+          s = rs.ctr64 ? atomicAdd(rs.ctr64, 1ull) - rs.base + rs.waves
+                       : atomicAdd(rs.ctr32, a.chunk);
+        s = __shfl(s, 0, 64);
+        const uint32_t b = rs.ctr64 ? slice_begin(a, s) : (uint32_t)min(s, (uint64_t)a.nrays);
+        if (b >= a.nrays) {
+          wave_out = true;
+          break;
+        }
+        cnext = b;
+        cend = rs.ctr64 ? slice_begin(a, s + 1) : min(b + a.chunk, a.nrays);
       }
       const uint32_t take = min((uint32_t)__popcll(m), cend - cnext);
       const uint32_t rank = (uint32_t)__popcll(m & lt);

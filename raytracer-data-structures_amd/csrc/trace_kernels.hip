@@ -22,8 +22,8 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
   RayStream rs;
   rs.ctr64 = a.counter;
   rs.base = a.ctr_base;
-  rs.first = ((blockIdx.x * BLOCK + threadIdx.x) >> 6) * a.chunk;
-  rs.dyn0 = gridDim.x * (BLOCK / 64) * a.chunk;
+  rs.wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  rs.waves = gridDim.x * (BLOCK / 64);
   trace_loop<ANY, COUNT, BLOCK, DEPTH, TSTACK, true, WIDE>(a, rs, s_node, s_tn,
                                                            blockIdx.x * BLOCK + threadIdx.x,
                                                            gridDim.x * BLOCK);

@@ -183,6 +183,8 @@ def _load() -> C.CDLL:
         "rq_trace_config_get": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(u32)]),
         "rq_trace_refill_set": (i32, [vp, u32, u32]),
         "rq_trace_refill_get": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
+        "rq_trace_slice_set": (i32, [vp, u32, u32]),
+        "rq_trace_slice_get": (i32, [vp, C.POINTER(u32), C.POINTER(u32)]),
         "rq_autotune_trace": (i32, [vp, vp, C.POINTER(Rays), u32, i32, i32, C.POINTER(i32),
                                     C.POINTER(u32), C.POINTER(C.c_double)]),
         "rq_device_alloc": (i32, [vp, C.c_size_t, C.POINTER(vp)]),
@@ -731,6 +733,16 @@ class Context:
         nc = c.value if closest is None else closest
         na = a.value if any_hit is None else any_hit
         self._check(lib().rq_trace_refill_set(self.h, nc, na))
+        return nc, na
+
+    def trace_slice(self, closest: int | None = None,
+                    any_hit: int | None = None) -> tuple[int, int]:
+        """Largest ray-stream slices per query kind (rq_trace_slice_set / _get)."""
+        c, a = C.c_uint32(), C.c_uint32()
+        self._check(lib().rq_trace_slice_get(self.h, C.byref(c), C.byref(a)))
+        nc = c.value if closest is None else closest
+        na = a.value if any_hit is None else any_hit
+        self._check(lib().rq_trace_slice_set(self.h, nc, na))
         return nc, na
 
     def autotune(self, bvh: int, rays: DeviceRays, any_hit: bool = False,

@@ -164,6 +164,66 @@ def test_all_trace_variants_identical(ctx, O, rq):
         ctx.release(s)
 
 
+def test_ray_stream_slices_and_counter_reuse(ctx, O, rq):
+    """Every ray of a launch is traced exactly once whatever the slice size
+    (large slices, then quarter ones, from one counter per launch slot that is
+    never reset): launches large enough that the waves take slices from the
+    counter, small ones covered by the waves' first slices, and more launches
+    than there are counter slots, each against the oracle."""
+    mesh = rq.icosphere(4, 1, center=(0, 0, -5))
+    n_big = 2_500_000
+    rays = rays_toward(n_big, 51)
+    ob = O.OracleBVH(mesh, 1)
+    o, oa = ob.closest(rays), ob.any(rays)
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    drays = rq.DeviceRays.from_host(ctx, rays)
+    hits = {k: rq.DeviceArray(ctx, 4 * n_big) for k in ("prim", "t")}
+    occ = rq.DeviceArray(ctx, n_big)
+    saved, saved_slice = ctx.trace_config(), ctx.trace_slice()
+
+    def check(n, any_hit):
+        drays.n = n
+        if any_hit:
+            occ.fill(0x7F)
+            ctx.any_hit_dev(b, drays, occ)
+            assert np.array_equal(occ.download(np.uint8, n), oa["occ"][:n]), n
+        else:
+            hits["prim"].fill(0x7F)
+            ctx.closest_hit_dev(b, drays, hits)
+            assert np.array_equal(hits["prim"].download(np.int32, n), o["prim"][:n]), n
+            assert np.array_equal(_bits(hits["t"].download(np.float32, n)), _bits(o["t"][:n])), n
+
+    try:
+        for sl in (64, 256, 4096):
+            log(f"ray stream slice {sl}")
+            ctx.trace_slice(sl, sl)
+            for n in (n_big, 300_000, 1, 100_000, n_big - 12345):
+                check(n, False)
+                check(n, True)
+        ctx.trace_slice(1024, 1024)
+        log("ray stream counter reuse")
+        for k in range(80):
+            check(300_000 + 4099 * k, k % 2 == 1)
+        assert ctx.device_errors() == 0
+    finally:
+        drays.n = n_big
+        ctx.trace_config(*saved)
+        ctx.trace_slice(*saved_slice)
+        drays.free()
+        for a in hits.values():
+            a.free()
+        occ.free()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_trace_slice_refused(ctx, rq):
+    for bad in ((0, 1024), (32, 1024), (1024, 8192), (1000, 1024)):
+        with pytest.raises(rq.RQError):
+            ctx.trace_slice(*bad)
+
+
 def test_autotune_picks_a_valid_variant(ctx, rq):
     mesh = rq.icosphere(5, 4, center=(0, 0, -5))
     s = ctx.upload_scene(mesh)
@@ -180,8 +240,10 @@ def test_autotune_picks_a_valid_variant(ctx, rq):
         # each query kind keeps its own tuned refill threshold
         assert ctx.trace_refill() == (refill, ra)
         assert ctx.trace_config()[:2] == (v, va)
+        assert all(x in (256, 512, 1024, 2048) for x in ctx.trace_slice())
     finally:
         rays.free()
         ctx.trace_config(*saved)
+        ctx.trace_slice(1024, 1024)
         ctx.release(b)
         ctx.release(s)
