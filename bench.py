@@ -129,6 +129,17 @@ def tuned_dict(ctx, cfg) -> dict:
             "refill_idle": {"closest": rc, "any": ra}, "slice": {"closest": sc, "any": sa}}
 
 
+# options every rocprofv3 child run of this script repeats (set by main)
+CHILD_ARGS: list = []
+
+
+def rebuild(ctx, bvh, args) -> None:
+    """One LBVH rebuild, then the treelet rounds of --treelets (if any)."""
+    ctx.rebuild(bvh)
+    if args.treelets:
+        ctx.treelets(bvh, args.treelets)
+
+
 def cfg_arg(t) -> str | None:
     return ",".join(map(str, t)) if t else None
 
@@ -159,7 +170,7 @@ def _pmc_run(counters: str, config: str, extra: list | None = None) -> dict | No
     out = tempfile.mkdtemp(prefix="rq_pmc_", dir="/tmp")
     cmd = [prof, "--pmc", *counters.split(), "--output-format", "csv", "-d", out, "-o", "probe",
            "--", sys.executable, str(ROOT / "bench.py"), "--pmc-probe", "--config", config,
-           "--no-cpu", "--traffic", "off", "--warmup", "1", *(extra or [])]
+           "--no-cpu", "--traffic", "off", "--warmup", "1", *CHILD_ARGS, *(extra or [])]
     env = dict(os.environ, TMPDIR="/tmp")
     try:
         subprocess.run(cmd, cwd="/tmp", env=env, timeout=240, check=True,
@@ -257,6 +268,19 @@ def run_c3(ctx, args) -> int:
     n = 1 << 24
     scene = ctx.upload_scene(mesh)
     bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
+    treelets = None
+    if args.treelets:  # the BVH is built once: time the restructuring once, on its own
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.treelets(bvh, args.treelets)
+        ctx.sync()
+        ctx.profile(False)
+        tp = ctx.profile_read()
+        treelets = {"rounds": args.treelets,
+                    "ms": round(sum(v[0] for k, v in tp.items() if k.startswith("treelet")), 3),
+                    "height": ctx.bvh_info(bvh).height}
+        log(f"treelets: {treelets}")
     rays = ctx.raygen_uniform(n, 3, lo, hi)
     hits = {k: rq.DeviceArray(ctx, 4 * n) for k in ("prim", "t")}
     occ = rq.DeviceArray(ctx, n)
@@ -319,6 +343,8 @@ def run_c3(ctx, args) -> int:
                              "any": {"node_visits": va, "tri_tests": ta}},
         "autotune": tuned, "roofline": roofline, "cpu_baseline": None,
     }
+    if treelets:
+        line["lbvh_treelets"] = treelets
     if not args.no_cpu:
         host = rays.download()
         threads = max(1, min(16, os.cpu_count() or 1))
@@ -780,12 +806,16 @@ def main() -> int:
     ap.add_argument("--sweep", action="store_true",
                     help="time every traversal variant on this frame's batches (stderr)")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--treelets", type=int, default=0, metavar="R",
+                    help="treelet restructuring rounds after every LBVH build (0: plain LBVH)")
     ap.add_argument("--trace-config", default=None, metavar="VC,RC,VA,RA",
                     help="fix the traversal variants and refill thresholds (closest, any) "
                          "instead of autotuning")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
+    if args.treelets:
+        CHILD_ARGS.extend(["--treelets", str(args.treelets)])
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the path on a single MI355X "
               "(rays split across GPUs would need no exchange; see DESIGN.md). "
@@ -811,12 +841,14 @@ def main() -> int:
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)
     bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
+    if args.treelets:
+        ctx.treelets(bvh, args.treelets)
     ren = rq.Renderer(ctx, params)
     if args.trace_config:  # fixed configuration: every launch, warm-up included, uses it
         apply_trace_config(ctx, parse_trace_config(args.trace_config))
 
     for _ in range(args.warmup):
-        ctx.rebuild(bvh)
+        rebuild(ctx, bvh, args)
         ren.frame(bvh)
     ctx.sync()
     cnt = ren.counts()
@@ -836,13 +868,13 @@ def main() -> int:
         tuned = tuned_dict(ctx, cfg)
         log(f"autotune: {tuned}")
         for _ in range(2):
-            ctx.rebuild(bvh)
+            rebuild(ctx, bvh, args)
             ren.frame(bvh)
         ctx.sync()
 
     if args.pmc_probe:
         for _ in range(2):
-            ctx.rebuild(bvh)
+            rebuild(ctx, bvh, args)
             ren.frame(bvh)
         ctx.sync()
         copy_probe(ctx)
@@ -854,7 +886,7 @@ def main() -> int:
     ctx.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        ctx.rebuild(bvh)
+        rebuild(ctx, bvh, args)
         ren.frame(bvh)
     ctx.sync()
     t1 = time.perf_counter()

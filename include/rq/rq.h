@@ -110,6 +110,16 @@ int rq_scene_update_vertices(rq_ctx* ctx, rq_scene* s, const float* vx, const fl
 int rq_build_lbvh(rq_ctx* ctx, const rq_scene* s, int morton_mode, rq_bvh** out, void* stream);
 /* Rebuild in place from the scene's current vertices (no allocation). */
 int rq_rebuild_lbvh(rq_ctx* ctx, rq_bvh* b, void* stream);
+/* Optional tree optimisation, no counterpart in the reference (it never
+ * optimises a tree; SAH-quality trees are what the LBVH trades for build
+ * speed): `rounds` (1..8) rounds of treelet restructuring (Karras & Aila,
+ * HPG 2013) of a built BVH in place. Each round reshapes every 7-leaf treelet
+ * whose optimal arrangement has a strictly lower sum of node surface areas
+ * and no greater height, so queries return the same answers with fewer node
+ * visits; the tree height never grows. Waits for the device (it reads the
+ * tree height), then runs on `stream`; rq_rebuild_lbvh returns the BVH to
+ * the plain LBVH. */
+int rq_bvh_treelets(rq_ctx* ctx, rq_bvh* b, int rounds, void* stream);
 int rq_bvh_info_get(rq_ctx* ctx, const rq_bvh* b, rq_bvh_info* info);
 /* Download the build for parity checks (host arrays, any may be NULL):
  * codes[n] in primitive order, sorted_idx[n], left/right[n-1] (child >= 0 is an

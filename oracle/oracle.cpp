@@ -546,6 +546,205 @@ extern "C" void or_bvh_arrays(const or_bvh* b, uint32_t* codes, uint32_t* sorted
 
 extern "C" uint32_t or_bvh_height(const or_bvh* b) { return b->height; }
 
+// ---------------------------------------------------------------------------
+// Treelet restructuring of a built LBVH: one round of the method of Karras &
+// Aila, "Fast Parallel Construction of High-Quality Bounding Volume
+// Hierarchies" (HPG 2013, section 4), with one primitive per leaf. Not part
+// of the reference (its trees are never optimised); the GPU's optional
+// rq_bvh_treelets restates this function and must match it node for node.
+//
+// Cost of a tree = sum of its internal nodes' half areas (dx*dy + dy*dz) +
+// dz*dx (with one primitive per leaf the leaves' cost is the same for every
+// arrangement). Every internal node with at least 7 leaves below it, children
+// before parents, roots a treelet: starting from its two children, the
+// treelet leaf with the largest half area (the first in slot order on ties)
+// that is an internal node is replaced by its two children (the second one
+// appended as the last slot) until there are 7. The optimal tree over the 7
+// leaves comes from dynamic programming over subsets (treelet_dp); it
+// replaces the treelet when its cost is strictly lower than the current
+// one's and its height is not greater (so the tree height never grows and
+// the traversal stack bound holds). The internal nodes are reused: the root
+// keeps its index, the others are handed out in formation order to the new
+// internal nodes in pre-order (a node before its first, then its second
+// child), the child holding the lowest leaf slot first. Each node's outcome
+// depends only on its own subtree, so any children-first schedule gives this
+// tree.
+namespace {
+inline float half_area(const float* b) {
+  const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
+  return (dx * dy + dy * dz) + dz * dx;
+}
+struct TreeletDP {
+  float A[128], C[128];
+  uint8_t P[128], H[128];  // split (the part with the lowest slot), height
+};
+// Subsets in numeric order (every proper subset of s is smaller than s). For
+// s with lowest slot `low` the splits are p = low | sub over the proper
+// subsets sub of s - low, in decreasing sub; the first minimum of
+// C[p] + C[s - p] wins (costs are >= +0, so this is the minimum of the pair
+// (cost bits, 127 - sub)).
+void treelet_dp(int m, const float (*box)[6], const uint8_t* hleaf, TreeletDP& d) {
+  const int S = 1 << m;
+  for (int s = 1; s < S; ++s) {
+    float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < m; ++k)
+      if (s >> k & 1)
+        for (int a = 0; a < 3; ++a) {
+          b[a] = std::min(b[a], box[k][a]);
+          b[3 + a] = std::max(b[3 + a], box[k][3 + a]);
+        }
+    d.A[s] = half_area(b);
+    if ((s & (s - 1)) == 0) {
+      const int k = __builtin_ctz(s);
+      d.C[s] = 0.0f, d.P[s] = 0, d.H[s] = hleaf[k];
+      continue;
+    }
+    const int low = s & -s, rest = s ^ low;
+    float best = 0.0f;
+    int bp = 0;
+    for (int sub = (rest - 1) & rest;; sub = (sub - 1) & rest) {
+      const int p = low | sub;
+      const float c = d.C[p] + d.C[s ^ p];
+      if (bp == 0 || c < best) best = c, bp = p;
+      if (sub == 0) break;
+    }
+    d.C[s] = d.A[s] + best;
+    d.P[s] = (uint8_t)bp;
+    d.H[s] = (uint8_t)(1 + std::max(d.H[bp], d.H[s ^ bp]));
+  }
+}
+}  // namespace
+
+extern "C" float or_treelet_dp(const float* boxes, int m, uint8_t* split, uint8_t* height) {
+  TreeletDP d;
+  uint8_t h0[7] = {};
+  treelet_dp(m, reinterpret_cast<const float(*)[6]>(boxes), h0, d);
+  for (int s = 1; s < (1 << m); ++s) {
+    if (split) split[s] = d.P[s];
+    if (height) height[s] = d.H[s];
+  }
+  return d.C[(1 << m) - 1];
+}
+
+extern "C" uint32_t or_bvh_treelets(or_bvh* b) {
+  const uint32_t n = b->n;
+  if (n < 7) return 0;
+  int32_t* left = b->left.data();
+  int32_t* right = b->right.data();
+  auto box = [&](int32_t c) -> float* {
+    return &b->boxes[6 * (size_t)(c >= 0 ? c : (int32_t)(n - 1) + ~c)];
+  };
+  std::vector<uint32_t> cnt(n - 1), ht(n - 1);
+  auto count = [&](int32_t c) { return c >= 0 ? cnt[c] : 1u; };
+  auto height = [&](int32_t c) { return c >= 0 ? ht[c] : 0u; };
+  // children-first order of the tree as built
+  std::vector<int32_t> order;
+  order.reserve(n - 1);
+  std::vector<std::pair<int32_t, int>> st{{0, 0}};
+  while (!st.empty()) {
+    auto& t = st.back();
+    const int32_t i = t.first;
+    if (t.second == 0) {
+      t.second = 1;
+      if (left[i] >= 0) st.push_back({left[i], 0});
+    } else if (t.second == 1) {
+      t.second = 2;
+      if (right[i] >= 0) st.push_back({right[i], 0});
+    } else {
+      order.push_back(i);
+      st.pop_back();
+    }
+  }
+  uint32_t rebuilt = 0;
+  for (const int32_t i : order) {
+    cnt[i] = count(left[i]) + count(right[i]);
+    ht[i] = 1 + std::max(height(left[i]), height(right[i]));
+    if (cnt[i] < 7) continue;
+    int32_t L[7] = {left[i], right[i]}, I[6] = {i};
+    int m = 2, ni = 1;
+    while (m < 7) {
+      int best = -1;
+      float ba = 0.0f;
+      for (int k = 0; k < m; ++k)
+        if (L[k] >= 0) {
+          const float a = half_area(box(L[k]));
+          if (best < 0 || a > ba) best = k, ba = a;
+        }
+      const int32_t j = L[best];  // a node with >= 7 leaves always has one to open
+      I[ni++] = j;
+      L[best] = left[j];
+      L[m++] = right[j];
+    }
+    float lb[7][6];
+    uint8_t hl[7];
+    for (int k = 0; k < 7; ++k) {
+      std::copy(box(L[k]), box(L[k]) + 6, lb[k]);
+      hl[k] = (uint8_t)height(L[k]);
+    }
+    TreeletDP d;
+    treelet_dp(7, lb, hl, d);
+    // cost of the treelet as it stands (the internal nodes in reverse
+    // formation order: a node's internal children were opened after it)
+    int mask[6];
+    float cost[6];
+    auto part = [&](int32_t c, int* mk, float* cc) {
+      for (int k = 0; k < 7; ++k)
+        if (L[k] == c) {
+          *mk = 1 << k, *cc = 0.0f;
+          return;
+        }
+      for (int j = 1; j < 6; ++j)
+        if (I[j] == c) {
+          *mk = mask[j], *cc = cost[j];
+          return;
+        }
+    };
+    for (int j = 5; j >= 0; --j) {
+      int ml = 0, mr = 0;
+      float cl = 0.0f, cr = 0.0f;
+      part(left[I[j]], &ml, &cl);
+      part(right[I[j]], &mr, &cr);
+      mask[j] = ml | mr;
+      cost[j] = d.A[mask[j]] + (cl + cr);
+    }
+    if (!(d.C[127] < cost[0]) || d.H[127] > ht[i]) continue;
+    ++rebuilt;
+    // rebuild in pre-order
+    int next = 0;
+    std::vector<int> stack{127};
+    std::vector<int32_t> at(128, -1);
+    while (!stack.empty()) {
+      const int s = stack.back();
+      stack.pop_back();
+      at[s] = I[next++];
+      const int p = d.P[s], q = s ^ p;
+      if (q & (q - 1)) stack.push_back(q);
+      if (p & (p - 1)) stack.push_back(p);
+    }
+    // links and boxes, children before parents (higher pre-order first)
+    std::vector<int> subsets;
+    for (int s = 1; s < 128; ++s)
+      if (at[s] >= 0) subsets.push_back(s);
+    std::sort(subsets.begin(), subsets.end(),
+              [](int x, int y) { return __builtin_popcount(x) < __builtin_popcount(y); });
+    auto link = [&](int s) { return (s & (s - 1)) ? at[s] : L[__builtin_ctz(s)]; };
+    for (const int s : subsets) {
+      const int32_t idx = at[s], l = link(d.P[s]), r = link(s ^ d.P[s]);
+      left[idx] = l, right[idx] = r;
+      float* bo = box(idx);
+      const float *x = box(l), *y = box(r);
+      for (int a = 0; a < 3; ++a) {
+        bo[a] = std::min(x[a], y[a]);
+        bo[3 + a] = std::max(x[3 + a], y[3 + a]);
+      }
+      cnt[idx] = count(l) + count(r);
+      ht[idx] = d.H[s];
+    }
+  }
+  b->height = or_tree_height(left, right, n);
+  return rebuilt;
+}
+
 namespace {
 // Slab test shared with the GPU kernels (same operations, same order); hit
 // when near <= far * SLAB_SLACK (inclusive, so equal-t ties are never culled).

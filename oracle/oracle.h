@@ -27,6 +27,9 @@
  *   or_mt19937_jitter              C++ standard known answer + numpy MT19937
  *   or_philox4x32_10               Random123 published known-answer vectors
  *   or_shade_*                     closed-form single-light cases
+ *   or_treelet_dp                  brute force over every binary tree of 2..7 leaves
+ *   or_bvh_treelets                invariants (leaf set, box unions, cost and height
+ *                                  never grow), queries vs brute force
  */
 #ifndef RQ_ORACLE_H
 #define RQ_ORACLE_H
@@ -102,6 +105,13 @@ void or_bvh_free(or_bvh* b);
 void or_bvh_arrays(const or_bvh* b, uint32_t* codes, uint32_t* sorted_idx, int32_t* left,
                    int32_t* right, float* boxes);
 uint32_t or_bvh_height(const or_bvh* b);
+/* Treelet restructuring of the tree in place (one round of 7-leaf treelets,
+ * Karras & Aila 2013); returns the number of treelets replaced. */
+uint32_t or_bvh_treelets(or_bvh* b);
+/* The optimal tree over m <= 7 boxes (cost = sum of internal half areas):
+ * its cost, and per subset s the split (the part with the lowest leaf) and
+ * the height (leaves at height 0). */
+float or_treelet_dp(const float* boxes, int m, uint8_t* split, uint8_t* height);
 void or_closest_bvh(const or_bvh* b, const or_rays* rays, uint32_t nrays, or_hits* hits,
                     uint32_t* visits, uint32_t* tests, int nthreads);
 void or_any_bvh(const or_bvh* b, const or_rays* rays, uint32_t nrays, uint8_t* occ,

@@ -73,6 +73,8 @@ def lib() -> C.CDLL:
             "or_bvh_free": (None, [vp]),
             "or_bvh_arrays": (None, [vp, vp, vp, vp, vp, vp]),
             "or_bvh_height": (u32, [vp]),
+            "or_bvh_treelets": (u32, [vp]),
+            "or_treelet_dp": (f, [vp, i32, vp, vp]),
             "or_closest_bvh": (None, [vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp, vp, i32]),
             "or_any_bvh": (None, [vp, C.POINTER(Rays), u32, vp, vp, vp, i32]),
             "or_mt19937_jitter": (None, [u32, vp]),
@@ -183,6 +185,16 @@ def sphere_ptrs(sph) -> tuple[list, list]:
     return [_p(a) for a in keep], keep
 
 
+def treelet_dp(boxes: np.ndarray) -> tuple[float, np.ndarray, np.ndarray]:
+    """Optimal tree over m <= 7 boxes (m x 6): cost, split and height per subset."""
+    b = np.ascontiguousarray(boxes, np.float32)
+    m = len(b)
+    split = np.zeros(1 << m, np.uint8)
+    height = np.zeros(1 << m, np.uint8)
+    c = lib().or_treelet_dp(_p(b), m, _p(split), _p(height))
+    return float(np.float32(c)), split, height
+
+
 class OracleBVH:
     """The oracle's LBVH: Morton -> stable sort -> Karras -> refit. `mesh` is a
     triangle mesh (vx, vy, vz, idx) or a sphere set (cx, cy, cz, r)."""
@@ -216,6 +228,10 @@ class OracleBVH:
     @property
     def height(self) -> int:
         return int(lib().or_bvh_height(self.h))
+
+    def treelets(self) -> int:
+        """Treelet restructuring in place (or_bvh_treelets); treelets replaced."""
+        return int(lib().or_bvh_treelets(self.h))
 
     def closest(self, rays: dict, threads: int = 8) -> dict:
         rs, keep = rays_struct(rays)

@@ -22,7 +22,8 @@ INCLUDE = PKG.parent / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "librq.so"
 
-HIP_SOURCES = ["build_kernels.hip", "trace_kernels.hip", "render_kernels.hip", "batch_kernels.hip"]
+HIP_SOURCES = ["build_kernels.hip", "trace_kernels.hip", "render_kernels.hip", "batch_kernels.hip",
+               "treelet_kernels.hip"]
 HIP_HOST_SOURCES = ["rq_api.cpp", "render.cpp", "batch.cpp", "serve.cpp", "bvh_blob.cpp"]
 CXX_SOURCES = ["host/scene_gen.cpp"]
 

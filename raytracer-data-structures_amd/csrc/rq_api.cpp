@@ -388,6 +388,25 @@ int rq_rebuild_lbvh(rq_ctx* c, rq_bvh* b, void* stream) {
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "lbvh build launch");
 }
 
+int rq_bvh_treelets(rq_ctx* c, rq_bvh* b, int rounds, void* stream) {
+  if (check_ctx(c) || !b || b->magic != BVH_MAGIC) return set_err(c, RQ_E_INVALID, "bad bvh");
+  if (rounds < 1 || rounds > 8) return set_err(c, RQ_E_INVALID, "rounds outside 1..8");
+  if (b->n < 7) return RQ_OK;
+  int rc = ensure_height(c, b);
+  if (rc) return rc;
+  uint32_t misc[MISC_WORDS];
+  hipError_t e = hipMemcpy(misc, b->B.misc, sizeof misc, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(c, e, "treelets");
+  if (misc[MISC_ERR]) return set_err(c, RQ_E_INVALID, "the BVH's build reported errors");
+  if (misc[MISC_HEIGHT] > (uint32_t)TRACE_MAX_STACK)
+    return set_err(c, RQ_E_INVALID, "tree height %u out of range", misc[MISC_HEIGHT]);
+  const LaunchCtx lc = launch_ctx(c, stream);
+  e = launch_treelets(lc, b->B, b->n, misc[MISC_HEIGHT], rounds);
+  if (e == hipSuccess && b->B.wide) e = launch_widen(lc, b->B, b->n);
+  if (e == hipSuccess) e = launch_height(lc, b->B, b->n);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "treelets launch");
+}
+
 int rq_bvh_info_get(rq_ctx* c, const rq_bvh* b, rq_bvh_info* info) {
   if (check_ctx(c) || !b || b->magic != BVH_MAGIC || !info)
     return set_err(c, RQ_E_INVALID, "bad argument");

@@ -173,6 +173,11 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
 hipError_t launch_widen(const LaunchCtx& lc, BuildBuffers& B, uint32_t n);
 // Computes the tree height into misc (the chunk-union refit leaves it open).
 hipError_t launch_height(const LaunchCtx& lc, BuildBuffers& B, uint32_t n);
+// Treelet restructuring of a built LBVH in place (`rounds` rounds of 7-leaf
+// treelets); `height` is the tree's height before. Leaves the wide nodes and
+// the height to launch_widen / launch_height.
+hipError_t launch_treelets(const LaunchCtx& lc, BuildBuffers& B, uint32_t n, uint32_t height,
+                           int rounds);
 
 struct TraceArgs {
   const NodeRec* nodes;

@@ -169,6 +169,7 @@ def _load() -> C.CDLL:
         "rq_scene_update_vertices": (i32, [vp, vp, vp, vp, vp, vp]),
         "rq_build_lbvh": (i32, [vp, vp, i32, C.POINTER(vp), vp]),
         "rq_rebuild_lbvh": (i32, [vp, vp, vp]),
+        "rq_bvh_treelets": (i32, [vp, vp, i32, vp]),
         "rq_bvh_info_get": (i32, [vp, vp, C.POINTER(BvhInfo)]),
         "rq_bvh_export": (i32, [vp, vp, vp, vp, vp, vp, vp]),
         "rq_bvh_export_wide": (i32, [vp, vp, vp, vp]),
@@ -531,6 +532,10 @@ class Context:
 
     def rebuild(self, bvh: int, stream=None) -> None:
         self._check(lib().rq_rebuild_lbvh(self.h, bvh, stream))
+
+    def treelets(self, bvh: int, rounds: int = 1, stream=None) -> None:
+        """Treelet restructuring of a built BVH in place (rq_bvh_treelets)."""
+        self._check(lib().rq_bvh_treelets(self.h, bvh, rounds, stream))
 
     def bvh_info(self, bvh: int) -> BvhInfo:
         info = BvhInfo()
