@@ -19,13 +19,15 @@
 // Schedule: a node's outcome depends only on its own subtree, and the
 // treelets rooted at one depth have disjoint subtrees, so one launch per
 // depth, deepest first, gives the same tree as any children-first order (the
-// oracle's post-order). Each launch scans every internal node; a node at that
-// depth gets its height and leaf count from its children (written by the
-// previous launch), and a wave takes its lanes' treelets one at a time, all 64
-// lanes on one treelet. Node depths come from a walk up the parent links.
+// oracle's post-order). Per depth, a scan gives every node at that depth its
+// height and leaf count from its children (written by the level below) and
+// lists the treelet roots; then persistent waves take the roots in turn, all
+// 64 lanes of a wave on one treelet. Node depths come from a walk up the
+// parent links.
 //
 // Per-node scratch (free once the LBVH is built): B.flags = depth, B.slot_h =
-// height (first n - 1 words) and leaf count (next n - 1 words).
+// height (first n - 1 words) and leaf count (next n - 1 words), B.frontier =
+// the treelet roots of the current depth, B.status[d] = their number.
 #include <algorithm>
 
 #include "rq_internal.h"
@@ -36,6 +38,8 @@ namespace {
 constexpr int TL_LEAVES = 7;
 constexpr int TL_SUBSETS = 1 << TL_LEAVES;  // 128 (index 0 unused)
 constexpr uint32_t ERR_TREELET_DEPTH = 32u;
+constexpr int TL_MAX_DEPTHS = 64;      // internal-node depths (the tree is at most 64 deep)
+constexpr uint32_t TL_PROC_WAVES = 8192;
 
 // (subset, sub) pairs of the subset DP by subset size (2..7): the split of
 // s is p = lowest(s) | sub, sub a proper subset of s - lowest(s). 966 pairs.
@@ -71,6 +75,7 @@ constexpr DpTables make_tables() {
 __constant__ DpTables kTables = make_tables();
 static_assert(make_tables().pair_off[8] == 966 && make_tables().subset_off[8] == 120,
               "subset DP tables");
+static_assert(sizeof(DpTables) % 4 == 0, "DP tables copied in words");
 
 __device__ inline float half_area(const float* b) {
   const float dx = b[3] - b[0], dy = b[4] - b[1], dz = b[5] - b[2];
@@ -107,9 +112,11 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup", "local");
 }
 
-// Depth of every internal node (walk up the parent links, at most 64 steps).
+// Depth of every internal node (walk up the parent links, at most 64 steps);
+// clears the per-depth root counters.
 __global__ __launch_bounds__(256) void k_tl_depth(BuildBuffers B, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (uint32_t)TL_MAX_DEPTHS) B.status[i] = 0u;
   if (i >= n - 1) return;
   uint32_t d = 0, cur = i;
   while (cur != 0u) {
@@ -126,62 +133,37 @@ __global__ __launch_bounds__(256) void k_tl_depth(BuildBuffers B, uint32_t n) {
 
 // One treelet rooted at `root` (height hroot, >= 7 leaves), all 64 lanes.
 __device__ void treelet(BuildBuffers& B, uint32_t n, uint32_t root, uint32_t hroot,
-                        uint32_t croot, TreeletLds& S, uint32_t lane) {
+                        const DpTables& T, TreeletLds& S, uint32_t lane) {
   uint32_t* height = B.slot_h;
   uint32_t* count = B.slot_h + (n - 1);
-  // formation (every lane computes the same; lane 0 writes)
+  // formation: lane k < 7 holds treelet slot k (its link and box)
   {
-    int32_t L[TL_LEAVES], I[TL_LEAVES - 1], IL[TL_LEAVES - 1], IR[TL_LEAVES - 1];
-    float bx[TL_LEAVES][6];
     const NodeRec r0 = B.nodes[root];
-    L[0] = link_lo(r0), L[1] = link_hi(r0);
-    child_box(r0, 0, bx[0]);
-    child_box(r0, 1, bx[1]);
-    I[0] = (int32_t)root, IL[0] = L[0], IR[0] = L[1];
-#pragma unroll
-    for (int m = 2; m < TL_LEAVES; ++m) {
-      int best = -1;
-      float ba = 0.0f;
-#pragma unroll
-      for (int k = 0; k < TL_LEAVES; ++k)
-        if (k < m && L[k] >= 0) {
-          const float a = half_area(bx[k]);
-          if (best < 0 || a > ba) best = k, ba = a;
-        }
-      if (best < 0) best = 0;  // unreachable: a node with >= 7 leaves has one to open
-      int32_t j = 0;
-#pragma unroll
-      for (int k = 0; k < TL_LEAVES; ++k)
-        if (k == best) j = L[k];
-      const NodeRec r = B.nodes[j < 0 ? 0 : j];
-      const int32_t a0 = link_lo(r), a1 = link_hi(r);
-      float b0[6], b1[6];
-      child_box(r, 0, b0);
-      child_box(r, 1, b1);
-#pragma unroll
-      for (int k = 0; k < TL_LEAVES; ++k) {
-        if (k == best) {
-          L[k] = a0;
-#pragma unroll
-          for (int q = 0; q < 6; ++q) bx[k][q] = b0[q];
-        }
-        if (k == m) {
-          L[k] = a1;
-#pragma unroll
-          for (int q = 0; q < 6; ++q) bx[k][q] = b1[q];
-        }
-      }
-      I[m - 1] = j, IL[m - 1] = a0, IR[m - 1] = a1;
+    int32_t my_link = 0;
+    float my_box[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (lane < 2) {
+      my_link = lane ? link_hi(r0) : link_lo(r0);
+      child_box(r0, (int)lane, my_box);
     }
-    if (lane == 0) {
+    if (lane == 0) S.I[0] = (int32_t)root, S.IL[0] = link_lo(r0), S.IR[0] = link_hi(r0);
+    for (uint32_t m = 2; m < (uint32_t)TL_LEAVES; ++m) {
+      // the largest half area among the slots holding an internal node, the
+      // first slot on ties (areas are >= +0, so their bits order them)
+      const bool open = lane < m && my_link >= 0;
+      uint32_t ab = open ? __float_as_uint(half_area(my_box)) : 0u, mx = ab;
+      for (int off = 32; off > 0; off >>= 1) mx = max(mx, (uint32_t)__shfl_xor((int)mx, off, 64));
+      const unsigned long long cand = __ballot(open && ab == mx);
+      const int best = cand ? __ffsll((long long)cand) - 1 : 0;  // >= 7 leaves: never empty
+      const int32_t j = __shfl(my_link, best, 64);
+      const NodeRec r = B.nodes[j < 0 ? 0 : j];
+      if ((int)lane == best) my_link = link_lo(r), child_box(r, 0, my_box);
+      if (lane == m) my_link = link_hi(r), child_box(r, 1, my_box);
+      if (lane == 0) S.I[m - 1] = j, S.IL[m - 1] = link_lo(r), S.IR[m - 1] = link_hi(r);
+    }
+    if (lane < (uint32_t)TL_LEAVES) {
+      S.L[lane] = my_link;
 #pragma unroll
-      for (int k = 0; k < TL_LEAVES; ++k) {
-        S.L[k] = L[k];
-#pragma unroll
-        for (int q = 0; q < 6; ++q) S.box[k][q] = bx[k][q];
-      }
-#pragma unroll
-      for (int j = 0; j < TL_LEAVES - 1; ++j) S.I[j] = I[j], S.IL[j] = IL[j], S.IR[j] = IR[j];
+      for (int q = 0; q < 6; ++q) S.box[lane][q] = my_box[q];
     }
   }
   wave_sync();
@@ -209,8 +191,8 @@ __device__ void treelet(BuildBuffers& B, uint32_t n, uint32_t root, uint32_t hro
   }
   wave_sync();
   for (int k = 2; k <= TL_LEAVES; ++k) {
-    for (uint32_t t = kTables.pair_off[k] + lane; t < kTables.pair_off[k + 1]; t += 64) {
-      const uint32_t e = kTables.pair[t], s = e & 127u, sub = e >> 7;
+    for (uint32_t t = T.pair_off[k] + lane; t < T.pair_off[k + 1]; t += 64) {
+      const uint32_t e = T.pair[t], s = e & 127u, sub = e >> 7;
       const uint32_t p = (s & (0u - s)) | sub;
       const float c = S.C[p] + S.C[s ^ p];
       const unsigned long long key =
@@ -218,8 +200,8 @@ __device__ void treelet(BuildBuffers& B, uint32_t n, uint32_t root, uint32_t hro
       atomicMin(&S.best[s], key);
     }
     wave_sync();
-    for (uint32_t t = kTables.subset_off[k] + lane; t < kTables.subset_off[k + 1]; t += 64) {
-      const uint32_t s = kTables.subset[t];
+    for (uint32_t t = T.subset_off[k] + lane; t < T.subset_off[k + 1]; t += 64) {
+      const uint32_t s = T.subset[t];
       const unsigned long long key = S.best[s];
       const uint32_t p = (s & (0u - s)) | (127u - (uint32_t)(key & 0xFFFFFFFFull));
       S.C[s] = S.A[s] + __uint_as_float((uint32_t)(key >> 32));
@@ -251,10 +233,7 @@ __device__ void treelet(BuildBuffers& B, uint32_t n, uint32_t root, uint32_t hro
     cost[j] = S.A[mask[j]] + (cc[0] + cc[1]);
   }
   const bool rebuild = S.C[127] < cost[0] && (uint32_t)S.H[127] <= hroot;
-  if (!rebuild) {
-    if (lane == 0) height[root] = hroot, count[root] = croot;
-    return;
-  }
+  if (!rebuild) return;  // height and leaf count of the root are already written
   // new internal nodes in pre-order (lane 0)
   if (lane == 0) {
     uint8_t stack[TL_LEAVES];
@@ -318,35 +297,58 @@ __device__ void treelet(BuildBuffers& B, uint32_t n, uint32_t root, uint32_t hro
   }
 }
 
-// All internal nodes at depth d: height and leaf count from the children,
-// and the treelet of each node with at least 7 leaves.
-__global__ __launch_bounds__(256) void k_tl_level(BuildBuffers B, uint32_t n, uint32_t d) {
-  __shared__ TreeletLds s_tl[256 / 64];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t lane = threadIdx.x & 63u;
-  TreeletLds& S = s_tl[threadIdx.x >> 6];
+// All internal nodes at depth d: height and leaf count from the children
+// (written by the level below), and the list of treelet roots (>= 7 leaves)
+// for k_tl_proc. A block takes TL_SCAN_NODES nodes and gathers its roots in
+// LDS, so the per-depth counter (cleared by k_tl_depth) gets one device
+// atomic per block: atomics on one address serialise across the XCDs.
+constexpr int TL_SCAN_BLOCK = 256;
+constexpr uint32_t TL_SCAN_NODES = 4096;
+__global__ __launch_bounds__(TL_SCAN_BLOCK) void k_tl_scan(BuildBuffers B, uint32_t n, uint32_t d) {
+  __shared__ uint32_t s_roots[TL_SCAN_NODES];
+  __shared__ uint32_t s_count, s_base;
   uint32_t* height = B.slot_h;
   uint32_t* count = B.slot_h + (n - 1);
-  bool mine = i < n - 1 && B.flags[i] == d;
-  uint32_t h = 0, c = 0;
-  if (mine) {
+  if (threadIdx.x == 0) s_count = 0u;
+  __syncthreads();
+  const uint32_t first = blockIdx.x * TL_SCAN_NODES;
+  for (uint32_t k = threadIdx.x; k < TL_SCAN_NODES; k += TL_SCAN_BLOCK) {
+    const uint32_t i = first + k;
+    if (i >= n - 1 || B.flags[i] != d) continue;
     const NodeRec r = B.nodes[i];
     const int32_t a = link_lo(r), b = link_hi(r);
-    h = 1u + max(a >= 0 ? height[a] : 0u, b >= 0 ? height[b] : 0u);
-    c = (a >= 0 ? count[a] : 1u) + (b >= 0 ? count[b] : 1u);
-    if (c < (uint32_t)TL_LEAVES) {
-      height[i] = h, count[i] = c;
-      mine = false;
-    }
+    const uint32_t c = (a >= 0 ? count[a] : 1u) + (b >= 0 ? count[b] : 1u);
+    height[i] = 1u + max(a >= 0 ? height[a] : 0u, b >= 0 ? height[b] : 0u);
+    count[i] = c;
+    if (c >= (uint32_t)TL_LEAVES) s_roots[atomicAdd(&s_count, 1u)] = i;
   }
-  unsigned long long todo = __ballot(mine);
-  while (todo) {
-    const int src = __ffsll((long long)todo) - 1;
-    todo &= todo - 1ull;
-    const uint32_t root = (uint32_t)__shfl((int)i, src, 64);
-    const uint32_t hr = (uint32_t)__shfl((int)h, src, 64);
-    const uint32_t cr = (uint32_t)__shfl((int)c, src, 64);
-    treelet(B, n, root, hr, cr, S, lane);
+  __syncthreads();
+  const uint32_t total = s_count;
+  if (total == 0u) return;
+  if (threadIdx.x == 0) s_base = atomicAdd(&B.status[d], total);
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < total; k += TL_SCAN_BLOCK) B.frontier[s_base + k] = s_roots[k];
+}
+
+// The treelets of depth d's list, one wave per treelet, the waves striding
+// over the list. The DP tables are copied to LDS once per block (read from
+// global memory they would put a load latency on every DP step).
+constexpr int TL_PROC_BLOCK = 256;
+__global__ __launch_bounds__(TL_PROC_BLOCK) void k_tl_proc(BuildBuffers B, uint32_t n, uint32_t d) {
+  __shared__ TreeletLds s_tl[TL_PROC_BLOCK / 64];
+  __shared__ DpTables s_tab;
+  {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(&kTables);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(&s_tab);
+    for (uint32_t k = threadIdx.x; k < sizeof(DpTables) / 4; k += TL_PROC_BLOCK) dst[k] = src[k];
+  }
+  __syncthreads();
+  const uint32_t total = B.status[d], lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (TL_PROC_BLOCK / 64);
+  TreeletLds& S = s_tl[threadIdx.x >> 6];
+  for (uint32_t e = blockIdx.x * (TL_PROC_BLOCK / 64) + (threadIdx.x >> 6); e < total; e += waves) {
+    const uint32_t root = B.frontier[e];
+    treelet(B, n, root, B.slot_h[root], s_tab, S, lane);
     wave_sync();
   }
 }
@@ -356,7 +358,12 @@ __global__ __launch_bounds__(256) void k_tl_level(BuildBuffers B, uint32_t n, ui
 hipError_t launch_treelets(const LaunchCtx& lc, BuildBuffers& B, uint32_t n, uint32_t height,
                            int rounds) {
   if (n < (uint32_t)TL_LEAVES || height == 0) return hipSuccess;
+  if (height > (uint32_t)TL_MAX_DEPTHS) return hipErrorInvalidValue;
   const dim3 grid((n - 1 + 255) / 256), block(256);
+  const dim3 sgrid((n - 1 + TL_SCAN_NODES - 1) / TL_SCAN_NODES), sblock(TL_SCAN_BLOCK);
+  // persistent treelet waves: enough to fill the device, at most one per 7 nodes
+  const uint32_t pwaves = std::min<uint32_t>(TL_PROC_WAVES, (n - 1) / TL_LEAVES + 1);
+  const dim3 pgrid((pwaves + TL_PROC_BLOCK / 64 - 1) / (TL_PROC_BLOCK / 64)), pblock(TL_PROC_BLOCK);
   for (int r = 0; r < rounds; ++r) {
     {
       ProfScope ps(lc, "treelet_depth");
@@ -364,8 +371,12 @@ hipError_t launch_treelets(const LaunchCtx& lc, BuildBuffers& B, uint32_t n, uin
     }
     // internal nodes lie at depths 0 .. height - 1; the height never grows
     for (uint32_t d = height; d-- > 0;) {
-      ProfScope ps(lc, "treelet_level");
-      hipLaunchKernelGGL(k_tl_level, grid, block, 0, lc.stream, B, n, d);
+      {
+        ProfScope ps(lc, "treelet_scan");
+        hipLaunchKernelGGL(k_tl_scan, sgrid, sblock, 0, lc.stream, B, n, d);
+      }
+      ProfScope ps(lc, "treelet_proc");
+      hipLaunchKernelGGL(k_tl_proc, pgrid, pblock, 0, lc.stream, B, n, d);
     }
   }
   return hipGetLastError();
