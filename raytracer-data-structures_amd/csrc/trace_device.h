@@ -53,6 +53,10 @@ constexpr float SLAB_SLACK = 1.000030517578125f;
 // hit with a 2^-9 relative slack keeps every box whose sphere's computed t
 // can still beat the best hit.
 constexpr float SPHERE_SLACK = 1.001953125f;
+// Wide walk, wave-uniform octant: the rows are loaded at byte offsets
+// node * 128 + row held in 32 bits (a scalar base plus one vector offset), so
+// the node count must stay below 2^25.
+constexpr uint32_t WIDE_UNIFORM_MAX_NODES = 1u << 25;
 
 // Slab test (same operations and order as the oracle): near/far planes by
 // the sign of 1/d, t = (plane - o) * (1/d). A zero direction component gives
@@ -258,17 +262,49 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     }
     if (__ballot(active) == 0ull && wave_out) break;
 
+    // Wide walk: when every active lane's direction has the same signs (an
+    // octant), the near / far rows are picked by scalar byte offsets instead
+    // of a select per value. Lanes that finish keep it so; a refill
+    // recomputes it.
+    uint32_t uni_rows = 0u, s_oct = 0u;
+    if (WIDE) {
+      const uint32_t oct = (uint32_t)r.sx | ((uint32_t)r.sy << 1) | ((uint32_t)r.sz << 2);
+      const uint64_t am = __ballot(active);
+      const int first = am ? __ffsll((long long)am) - 1 : 0;
+      const uint32_t oref = (uint32_t)__shfl((int)oct, first, 64);
+      const bool uni = __ballot(active && oct != oref) == 0ull &&
+                       a.nprims <= WIDE_UNIFORM_MAX_NODES;
+      uni_rows = __builtin_amdgcn_readfirstlane((uint32_t)uni);
+      s_oct = __builtin_amdgcn_readfirstlane(oref);
+    }
+
     // ---- traverse until enough lanes are idle to be worth a refill -------
     for (;;) {
       if (WIDE && active && node >= 0) {
+        float4 nx, fx, ny, fy, nz, fz;
         const WideNode* w = a.wide + node;
-        const float4 lx = w->lox, hx = w->hix, ly = w->loy, hy = w->hiy, lz = w->loz,
-                     hz = w->hiz;
         const int4 ln = w->link;
-        // near / far rows by the direction's sign (hi is near when 1/d < 0)
-        const float4 nx = r.sx ? hx : lx, fx = r.sx ? lx : hx;
-        const float4 ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
-        const float4 nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
+        if (uni_rows) {
+          // rows at byte offsets lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80;
+          // the near row of an axis is hi when its sign bit is set
+          const char* wb = reinterpret_cast<const char*>(a.wide);
+          const uint32_t ox = (s_oct & 1u) << 4, oy = 32u + ((s_oct & 2u) << 3),
+                         oz = 64u + ((s_oct & 4u) << 2);
+          const uint64_t off = (uint64_t)((uint32_t)node << 7);
+          nx = *reinterpret_cast<const float4*>(wb + ox + off);
+          fx = *reinterpret_cast<const float4*>(wb + (ox ^ 16u) + off);
+          ny = *reinterpret_cast<const float4*>(wb + oy + off);
+          fy = *reinterpret_cast<const float4*>(wb + (oy ^ 16u) + off);
+          nz = *reinterpret_cast<const float4*>(wb + oz + off);
+          fz = *reinterpret_cast<const float4*>(wb + (oz ^ 16u) + off);
+        } else {
+          const float4 lx = w->lox, hx = w->hix, ly = w->loy, hy = w->hiy, lz = w->loz,
+                       hz = w->hiz;
+          // near / far rows by the direction's sign (hi is near when 1/d < 0)
+          nx = r.sx ? hx : lx, fx = r.sx ? lx : hx;
+          ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
+          nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
+        }
         float t0, t1, t2, t3;
         const bool h0 = slab_w(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, r, bt, slack, t0) & (ln.x != 0);
         const bool h1 = slab_w(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, r, bt, slack, t1) & (ln.y != 0);
