@@ -24,6 +24,8 @@ struct rq_renderer {
   int32_t *h0 = nullptr, *h1 = nullptr;
   float *t0 = nullptr, *t1 = nullptr;
   uint8_t* occ = nullptr;
+  float4* lit0 = nullptr;  // per sample: a diffuse primary hit's L.N, Phong term, bounce weight
+  float2* lit1 = nullptr;  // per sample: a diffuse second hit's L.N and Phong term
   float* srgb = nullptr;
   float* img = nullptr;
   double* jit = nullptr;
@@ -126,6 +128,8 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
   if (!rc && second) rc = ralloc(c, r, n * 4, (void**)&r->h1);
   if (!rc && second) rc = ralloc(c, r, n * 4, (void**)&r->t1);
   if (!rc) rc = ralloc(c, r, ns, (void**)&r->occ);
+  if (!rc && p->mode == RQ_RENDER_SHADE) rc = ralloc(c, r, n * 16, (void**)&r->lit0);
+  if (!rc && second) rc = ralloc(c, r, n * 8, (void**)&r->lit1);
   if (!rc) rc = ralloc(c, r, n * 12, (void**)&r->srgb);
   if (!rc) rc = ralloc(c, r, (size_t)r->npix * 12, (void**)&r->img);
   if (!rc) rc = ralloc(c, r, 16, (void**)&r->counters);
@@ -178,14 +182,14 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   rq_hits h0{r->h0, r->t0, nullptr, nullptr};
   int rc = rq_closest_hit(c, b, &pr, r->n, &h0, stream);
   if (rc) return rc;
-  e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc);
+  e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc, r->lit0);
   if (e != hipSuccess) return hip_err(c, e, "shade primary");
   if (r->second()) {
     rq_rays br = as_rays(r->bnc);
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
     rc = rq_closest_hit(c, b, &br, r->n, &h1, stream);
     if (rc) return rc;
-    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad);
+    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad, r->lit1);
     if (e != hipSuccess) return hip_err(c, e, "shade bounce");
   }
   if (r->p.shadows) {
@@ -194,7 +198,7 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     if (rc) return rc;
   }
   e = launch_shade_final(lc, sv, r->sh, r->npix, r->p.spp, r->prim, r->bnc, r->h0, r->t0, r->h1,
-                         r->t1, r->occ, r->srgb, r->img);
+                         r->t1, r->occ, r->lit0, r->lit1, r->srgb, r->img);
   if (e != hipSuccess) return hip_err(c, e, "shade final");
   return RQ_OK;
 }

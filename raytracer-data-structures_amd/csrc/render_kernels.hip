@@ -179,22 +179,35 @@ __device__ inline float fresnel(V3 I, V3 N, float ior) {
   const float Rp = ((etai * c) - (etat * cost)) / ((etai * c) + (etat * cost));
   return (Rs * Rs + Rp * Rp) / 2.0f;
 }
-// main.cpp:463-490 for one light; diffuse = getDiffuseColor(0.2) * 0.8.
-__device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, V3 surf, bool shadow) {
+// main.cpp:463-490 for one light; diffuse = getDiffuseColor(0.2) * 0.8. The
+// terms that do not depend on the shadow test (max(0, L.N) and the Phong
+// term) are computed where the shadow ray is spawned and kept per sample;
+// the colour is finished once the shadow flag is known.
+struct LitTerms {
+  float LdotN, s25;
+};
+__device__ inline LitTerms lit_terms(V3 d, const Geo& g, const ShadeParams& sp) {
   const V3 L{sp.light[0], sp.light[1], sp.light[2]};
   const V3 ld = normalize(sub(L, g.p));
-  const float LdotN = max0(dot(ld, g.n));
-  const float vis = shadow ? 0.0f : 1.0f;
-  const V3 amt{(sp.emission[0] * vis) * LdotN, (sp.emission[1] * vis) * LdotN,
-               (sp.emission[2] * vis) * LdotN};
+  LitTerms t;
+  t.LdotN = max0(dot(ld, g.n));
   const V3 R = reflect({-ld.x, -ld.y, -ld.z}, g.n);
-  const float s25 = powf(max0(-dot(R, d)), 25.0f);
+  t.s25 = powf(max0(-dot(R, d)), 25.0f);
+  return t;
+}
+__device__ inline V3 lit_color(LitTerms t, const ShadeParams& sp, V3 surf, bool shadow) {
+  const float vis = shadow ? 0.0f : 1.0f;
+  const V3 amt{(sp.emission[0] * vis) * t.LdotN, (sp.emission[1] * vis) * t.LdotN,
+               (sp.emission[2] * vis) * t.LdotN};
   const float k0 = 0.815f * 0.8f, k1 = 0.235f * 0.8f, k2 = 0.031f * 0.8f;
   V3 c;
-  c.x = ((amt.x * k0) / 2.0f + (s25 * sp.emission[0]) * 0.5f) + surf.x;
-  c.y = ((amt.y * k1) / 2.0f + (s25 * sp.emission[1]) * 0.5f) + surf.y;
-  c.z = ((amt.z * k2) / 2.0f + (s25 * sp.emission[2]) * 0.5f) + surf.z;
+  c.x = ((amt.x * k0) / 2.0f + (t.s25 * sp.emission[0]) * 0.5f) + surf.x;
+  c.y = ((amt.y * k1) / 2.0f + (t.s25 * sp.emission[1]) * 0.5f) + surf.y;
+  c.z = ((amt.z * k2) / 2.0f + (t.s25 * sp.emission[2]) * 0.5f) + surf.z;
   return c;
+}
+__device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, V3 surf, bool shadow) {
+  return lit_color(lit_terms(d, g, sp), sp, surf, shadow);
 }
 __device__ inline void put_ray(const RaySoA& R, uint64_t i, V3 o, V3 d, float tmin, float tmax) {
   R.ox[i] = o.x, R.oy[i] = o.y, R.oz[i] = o.z;
@@ -260,7 +273,7 @@ __global__ __launch_bounds__(256) void k_count_active(const float* __restrict__ 
 __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams sp, uint32_t n,
                                                         RaySoA prim, const int32_t* __restrict__ h0,
                                                         const float* __restrict__ t0, RaySoA shad,
-                                                        RaySoA bnc) {
+                                                        RaySoA bnc, float4* __restrict__ lit0) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
   if (!in) return;
@@ -287,6 +300,9 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
     return;
   }
   if (sp.shadows) shadow_ray(shad, i, d, g, sp);
+  // the final colour's shadow-independent terms, and the bounce weight
+  const LitTerms lt = lit_terms(d, g, sp);
+  lit0[i] = make_float4(lt.LdotN, lt.s25, sp.bounce ? fresnel(d, g.n, 2.0f) : 0.0f, 0.0f);
   if (sp.materials) put_inactive(bnc, i);
   if (sp.bounce) {
     const V3 rd = normalize(reflect(d, g.n));
@@ -296,10 +312,12 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
   }
 }
 
-// Stage 2: from bounce hits spawn their shadow ray (slot n + i).
+// Stage 2: from bounce hits spawn their shadow ray (slot n + i) and keep
+// their shadow-independent colour terms.
 __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams sp, uint32_t n,
                                                        RaySoA bnc, const int32_t* __restrict__ h1,
-                                                       const float* __restrict__ t1, RaySoA shad) {
+                                                       const float* __restrict__ t1, RaySoA shad,
+                                                       float2* __restrict__ lit1) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
   if (!in) return;
@@ -311,75 +329,113 @@ __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams s
   const V3 o = get_o(bnc, i), d = get_d(bnc, i);
   const Geo g = hit_geo(s, p, o, d, t1[i]);
   if (sp.shadows) shadow_ray(shad, (uint64_t)n + i, d, g, sp);
+  const LitTerms lt = lit_terms(d, g, sp);
+  lit1[i] = make_float2(lt.LdotN, lt.s25);
 }
 
-// Final: per pixel, colour every sample and sum them in sample order
-// (main.cpp:551-560).
-__global__ __launch_bounds__(256) void k_shade_final(SceneView s, ShadeParams sp, uint32_t npix,
-                                                      uint32_t spp, RaySoA prim, RaySoA bnc,
-                                                      const int32_t* __restrict__ h0,
-                                                      const float* __restrict__ t0,
-                                                      const int32_t* __restrict__ h1,
-                                                      const float* __restrict__ t1,
-                                                      const uint8_t* __restrict__ occ,
-                                                      float* __restrict__ srgb,
-                                                      float* __restrict__ img) {
-  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
-  if (px >= npix) return;
-  const uint32_t n = npix * spp;
+// Colour of sample i (main.cpp:418-490): sky on a miss; the material switch
+// (refraction colour * (1 - kr), or the grey 1 - kr); or the diffuse colour
+// plus kr times the bounce's, from the terms the primary and bounce shading
+// kept (the same operations on the same hit geometry as computing them here).
+struct FinalArgs {
+  SceneView s;
+  ShadeParams sp;
+  uint32_t npix, spp;
+  RaySoA prim, bnc;
+  const int32_t *h0, *h1;
+  const float *t0, *t1;
+  const uint8_t* occ;
+  const float4* lit0;
+  const float2* lit1;
+  float *srgb, *img;
+};
+__device__ inline V3 sample_color(const FinalArgs& f, uint32_t i) {
+  const SceneView& s = f.s;
+  const ShadeParams& sp = f.sp;
+  const uint32_t n = f.npix * f.spp;
   const V3 sky{sp.sky[0], sp.sky[1], sp.sky[2]};
-  V3 acc{0.0f, 0.0f, 0.0f};
-  for (uint32_t k = 0; k < spp; ++k) {
-    const uint32_t i = px * spp + k;
-    const int32_t p = h0[i];
-    V3 col = sky;
-    const int m = p >= 0 ? mat_of(s, p) : 0;
-    if (p >= 0 && m != 0) {
-      // main.cpp:418-446: refraction colour * (1 - kr), or the grey 1 - kr
-      const V3 o = get_o(prim, i), d = get_d(prim, i);
-      const Geo g = hit_geo(s, p, o, d, t0[i]);
-      const float kr = fresnel(d, g.n, 2.0f);
-      if (m == 2) {
-        col = {1.0f - kr, 1.0f - kr, 1.0f - kr};
+  const int32_t p = f.h0[i];
+  if (p < 0) return sky;
+  const int m = mat_of(s, p);
+  if (m != 0) {
+    const V3 o = get_o(f.prim, i), d = get_d(f.prim, i);
+    const Geo g = hit_geo(s, p, o, d, f.t0[i]);
+    const float kr = fresnel(d, g.n, 2.0f);
+    if (m == 2) return {1.0f - kr, 1.0f - kr, 1.0f - kr};
+    V3 rc = sky;  // depth 2; a refracting surface there shows the sky (depth 3)
+    const int32_t q = f.h1[i];
+    if (q >= 0) {
+      if (mat_of(s, q) == 0) {
+        const float2 l1 = f.lit1[i];
+        rc = lit_color({l1.x, l1.y}, sp, surface_of(s, sp, q), sp.shadows && f.occ[n + i] != 0);
       } else {
-        V3 rc = sky;  // depth 2; a refracting surface there shows the sky (depth 3)
-        const int32_t q = h1[i];
-        if (q >= 0) {
-          const V3 o1 = get_o(bnc, i), d1 = get_d(bnc, i);
-          const Geo g1 = hit_geo(s, q, o1, d1, t1[i]);
-          const int m1 = mat_of(s, q);
-          if (m1 == 0) {
-            rc = direct(d1, g1, sp, surface_of(s, sp, q), sp.shadows && occ[n + i] != 0);
-          } else {
-            const float kr1 = fresnel(d1, g1.n, 2.0f);
-            rc = m1 == 2 ? V3{1.0f - kr1, 1.0f - kr1, 1.0f - kr1}
-                         : V3{sky.x * (1.0f - kr1), sky.y * (1.0f - kr1), sky.z * (1.0f - kr1)};
-          }
-        }
-        const float w = 1.0f - kr;
-        col = {rc.x * w, rc.y * w, rc.z * w};
-      }
-    } else if (p >= 0) {
-      const V3 o = get_o(prim, i), d = get_d(prim, i);
-      const Geo g = hit_geo(s, p, o, d, t0[i]);
-      col = direct(d, g, sp, surface_of(s, sp, p), sp.shadows && occ[i] != 0);
-      if (sp.bounce) {
-        const float kr = fresnel(d, g.n, 2.0f);
-        V3 bc = sky;
-        const int32_t q = h1[i];
-        if (q >= 0) {
-          const V3 o1 = get_o(bnc, i), d1 = get_d(bnc, i);
-          const Geo g1 = hit_geo(s, q, o1, d1, t1[i]);
-          bc = direct(d1, g1, sp, surface_of(s, sp, q), sp.shadows && occ[n + i] != 0);
-        }
-        col = {col.x + kr * bc.x, col.y + kr * bc.y, col.z + kr * bc.z};
+        const V3 o1 = get_o(f.bnc, i), d1 = get_d(f.bnc, i);
+        const Geo g1 = hit_geo(s, q, o1, d1, f.t1[i]);
+        const float kr1 = fresnel(d1, g1.n, 2.0f);
+        rc = mat_of(s, q) == 2
+                 ? V3{1.0f - kr1, 1.0f - kr1, 1.0f - kr1}
+                 : V3{sky.x * (1.0f - kr1), sky.y * (1.0f - kr1), sky.z * (1.0f - kr1)};
       }
     }
-    if (srgb) srgb[3 * (size_t)i] = col.x, srgb[3 * (size_t)i + 1] = col.y,
-                  srgb[3 * (size_t)i + 2] = col.z;
+    const float w = 1.0f - kr;
+    return {rc.x * w, rc.y * w, rc.z * w};
+  }
+  const float4 l0 = f.lit0[i];
+  V3 col = lit_color({l0.x, l0.y}, sp, surface_of(s, sp, p), sp.shadows && f.occ[i] != 0);
+  if (sp.bounce) {
+    const float kr = l0.z;
+    V3 bc = sky;
+    const int32_t q = f.h1[i];
+    if (q >= 0) {
+      const float2 l1 = f.lit1[i];
+      bc = lit_color({l1.x, l1.y}, sp, surface_of(s, sp, q), sp.shadows && f.occ[n + i] != 0);
+    }
+    col = {col.x + kr * bc.x, col.y + kr * bc.y, col.z + kr * bc.z};
+  }
+  return col;
+}
+__device__ inline void put_srgb(const FinalArgs& f, uint32_t i, V3 c) {
+  if (f.srgb) f.srgb[3 * (size_t)i] = c.x, f.srgb[3 * (size_t)i + 1] = c.y, f.srgb[3 * (size_t)i + 2] = c.z;
+}
+
+// Final, one thread per pixel: colour its samples and sum them in sample
+// order (main.cpp:551-560). Any samples per pixel.
+__global__ __launch_bounds__(256) void k_shade_final(FinalArgs f) {
+  const uint32_t px = blockIdx.x * blockDim.x + threadIdx.x;
+  if (px >= f.npix) return;
+  V3 acc{0.0f, 0.0f, 0.0f};
+  for (uint32_t k = 0; k < f.spp; ++k) {
+    const uint32_t i = px * f.spp + k;
+    const V3 col = sample_color(f, i);
+    put_srgb(f, i, col);
     acc = {acc.x + col.x, acc.y + col.y, acc.z + col.z};
   }
-  img[3 * (size_t)px] = acc.x, img[3 * (size_t)px + 1] = acc.y, img[3 * (size_t)px + 2] = acc.z;
+  f.img[3 * (size_t)px] = acc.x, f.img[3 * (size_t)px + 1] = acc.y, f.img[3 * (size_t)px + 2] = acc.z;
+}
+
+// Final, one thread per sample (samples per pixel a power of two up to 64):
+// coalesced loads and stores; a pixel's spp consecutive lanes are summed in
+// sample order by its first lane, as the per-pixel loop sums them.
+__global__ __launch_bounds__(256) void k_shade_final_samples(FinalArgs f) {
+  const uint32_t n = f.npix * f.spp;
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = (int)(threadIdx.x & 63u);
+  V3 col{0.0f, 0.0f, 0.0f};
+  if (i < n) {
+    col = sample_color(f, i);
+    put_srgb(f, i, col);
+  }
+  V3 acc{0.0f, 0.0f, 0.0f};
+  for (uint32_t j = 0; j < f.spp; ++j) {
+    const int src = lane + (int)j;  // the first lane of a pixel reads its group
+    acc.x = acc.x + __shfl(col.x, src, 64);
+    acc.y = acc.y + __shfl(col.y, src, 64);
+    acc.z = acc.z + __shfl(col.z, src, 64);
+  }
+  if (i < n && (i & (f.spp - 1u)) == 0u) {
+    const size_t px = i / f.spp;
+    f.img[3 * px] = acc.x, f.img[3 * px + 1] = acc.y, f.img[3 * px + 2] = acc.z;
+  }
 }
 
 // castRay's KDTREE case (main.cpp:362-370): black where the primary ray hits
@@ -456,19 +512,19 @@ hipError_t launch_raygen_pinhole(const LaunchCtx& lc, const PinholeParams& p, co
 
 hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                 uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
-                                const RaySoA& shad, const RaySoA& bnc) {
+                                const RaySoA& shad, const RaySoA& bnc, float4* lit0) {
   ProfScope ps(lc, "shade_primary");
   hipLaunchKernelGGL(k_shade_primary, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, prim,
-                     h0, t0, shad, bnc);
+                     h0, t0, shad, bnc, lit0);
   return hipGetLastError();
 }
 
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
-                               const RaySoA& shad) {
+                               const RaySoA& shad, float2* lit1) {
   ProfScope ps(lc, "shade_bounce");
   hipLaunchKernelGGL(k_shade_bounce, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, bnc,
-                     h1, t1, shad);
+                     h1, t1, shad, lit1);
   return hipGetLastError();
 }
 
@@ -492,10 +548,17 @@ hipError_t launch_shade_occlusion(const LaunchCtx& lc, const ShadeParams& sp, ui
 hipError_t launch_shade_final(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                               uint32_t npix, uint32_t spp, const RaySoA& prim, const RaySoA& bnc,
                               const int32_t* h0, const float* t0, const int32_t* h1,
-                              const float* t1, const uint8_t* occ, float* srgb, float* img) {
+                              const float* t1, const uint8_t* occ, const float4* lit0,
+                              const float2* lit1, float* srgb, float* img) {
   ProfScope ps(lc, "shade_final");
-  hipLaunchKernelGGL(k_shade_final, dim3(cdiv(npix, 256)), dim3(256), 0, lc.stream, s, sp, npix,
-                     spp, prim, bnc, h0, t0, h1, t1, occ, srgb, img);
+  const FinalArgs f{s, sp, npix, spp, prim, bnc, h0, h1, t0, t1, occ, lit0, lit1, srgb, img};
+  if (spp >= 1 && spp <= 64 && (spp & (spp - 1)) == 0) {
+    const uint64_t n = (uint64_t)npix * spp;
+    hipLaunchKernelGGL(k_shade_final_samples, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0,
+                       lc.stream, f);
+  } else {
+    hipLaunchKernelGGL(k_shade_final, dim3(cdiv(npix, 256)), dim3(256), 0, lc.stream, f);
+  }
   return hipGetLastError();
 }
 

@@ -47,10 +47,10 @@ hipError_t launch_raygen_uniform(const LaunchCtx& lc, uint32_t n, uint64_t seed,
 hipError_t launch_raygen_pinhole(const LaunchCtx& lc, const PinholeParams& p, const RaySoA& R);
 hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                 uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
-                                const RaySoA& shad, const RaySoA& bnc);
+                                const RaySoA& shad, const RaySoA& bnc, float4* lit0);
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
-                               const RaySoA& shad);
+                               const RaySoA& shad, float2* lit1);
 // Adds the number of active rays (tmin < tmax) of R[0, n) to *counter.
 hipError_t launch_count_active(const LaunchCtx& lc, const RaySoA& R, uint64_t n,
                                uint32_t* counter);
@@ -61,6 +61,7 @@ hipError_t launch_shade_occlusion(const LaunchCtx& lc, const ShadeParams& sp, ui
 hipError_t launch_shade_final(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                               uint32_t npix, uint32_t spp, const RaySoA& prim, const RaySoA& bnc,
                               const int32_t* h0, const float* t0, const int32_t* h1,
-                              const float* t1, const uint8_t* occ, float* srgb, float* img);
+                              const float* t1, const uint8_t* occ, const float4* lit0,
+                              const float2* lit1, float* srgb, float* img);
 
 }  // namespace rq
