@@ -15,7 +15,9 @@
  *    exceeding it returns RQ_E_NOMEM (admission failure) without allocating.
  *  - Ray and hit buffers passed to queries are DEVICE pointers (SoA).
  *  - `stream` is a hipStream_t passed as void* (NULL = default stream); calls
- *    are asynchronous on it and rq_sync() waits.
+ *    are asynchronous on it and rq_sync() waits. Query launches take one of
+ *    the context's 64 ray-stream counters in turn: at most 64 query launches
+ *    of one context may be in flight at once, whatever their streams.
  *  - One rq_ctx is used by one host thread at a time.
  */
 #ifndef RQ_RQ_H

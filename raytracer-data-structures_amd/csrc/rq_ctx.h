@@ -62,9 +62,10 @@ int hip_err(rq_ctx* c, hipError_t e, const char* what);
 int ctx_alloc(rq_ctx* c, size_t bytes, void** p);
 void ctx_free(rq_ctx* c, void* p);
 LaunchCtx launch_ctx(rq_ctx* c, void* stream);
-// The next ray-stream counter slot; *base receives its value at the start of
-// a launch that uses it, and a launch of n rays in slices of `chunk` must
-// report back through advance_counter_slot.
+// The next ray-stream counter slot (round robin over COUNTER_SLOTS). A launch
+// passes counter_base[slot] to its kernel and then adds the number of slices
+// its waves take from the counter (RayStream), so the slot's next launch
+// starts from the counter's value after this one.
 uint32_t next_counter_slot(rq_ctx* c);
 // Build buffers for n primitives; `wide`: also the 4-wide node array and the
 // chunk-union refit buffers (the standalone build and blob load; batch and
