@@ -158,7 +158,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
   const int max_sp = DEPTH + (int)a.gdepth;
   const float slack = (SPHERES && a.sphere) ? SPHERE_SLACK : SLAB_SLACK;
 
-  RayState r;
+  RayState r{};  // lanes that never take a ray still read their (unused) octant
   bool active = false;
   // wave-private slice of the ray stream [cnext, cend), refilled CHUNK rays at
   // a time from the global counter (one atomic per slice, not per refill)
