@@ -443,8 +443,10 @@ def run_c4(ctx, args) -> int:
     ref = {m: (hits[m]["prim"].download(np.int32, rays[m].n), occ[m].download(np.uint8, rays[m].n))
            for m in check}
 
-    # ---- plain per-request path on the batch's own BVH buffers
-    bvhs = [batch.bvh(m) for m in range(len(meshes))]
+    # ---- plain per-request path: what a caller without the batch API does,
+    # per mesh a standalone LBVH (allocated here, rebuilt in the timed loop)
+    bvhs = [ctx.build_lbvh(scenes[m], rq.MORTON_SCENE_AABB) for m in range(len(meshes))]
+    ctx.sync()
     med = int(np.argsort([r.n for r in rays])[len(rays) // 2])
     tune_trace(ctx, bvhs[med], rays[med], rays[med], args, repeats=2)
 

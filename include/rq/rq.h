@@ -174,9 +174,10 @@ int rq_any_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32
  * beyond the best hit when popped), and the node format walked: binary
  * (64-byte nodes, two children) or wide (the binary tree collapsed into
  * 128-byte 4-child nodes, written by every rq_build_lbvh / rq_rebuild_lbvh /
- * rq_bvh_load). Results are identical for all. A BVH without wide nodes (one
- * owned by a batch) is walked by variant 0 when a wide variant is
- * configured. */
+ * rq_bvh_load). Results are identical for all. A new context uses wide
+ * variants (9 for closest hit, 14 for any hit, refill 64, slices of 1024); a
+ * BVH without wide nodes (one owned by a batch) is walked by variant 0 when
+ * a wide variant is configured. */
 typedef struct {
   int32_t block, lds_depth, tstack, wide;
 } rq_trace_variant;

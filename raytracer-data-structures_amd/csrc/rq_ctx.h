@@ -14,7 +14,8 @@ struct rq_ctx {
   rq::Profiler* prof = nullptr;
   bool prof_on = false;
   uint32_t trace_grid[rq::TRACE_MAX_VARIANTS] = {};  // persistent grid per kernel variant
-  int closest_variant = 0, any_variant = 0;           // chosen traversal variants
+  // chosen traversal variants (the 4-wide walks until tuned or set)
+  int closest_variant = rq::TRACE_DEFAULT_CLOSEST, any_variant = rq::TRACE_DEFAULT_ANY;
   // idle lanes before a wave refills (64: only once every lane is idle), per query kind
   uint32_t refill_closest = 64, refill_any = 64;
   // largest wave slice of the ray stream, per query kind (trace_chunk)

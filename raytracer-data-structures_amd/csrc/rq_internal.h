@@ -80,6 +80,11 @@ constexpr int TRACE_MAX_STACK = 64;
 constexpr int TRACE_MAX_STACK_WIDE = 96;
 constexpr int TRACE_MIN_LDS_DEPTH = 12;
 constexpr int TRACE_MAX_VARIANTS = 16;
+// Variants a new context uses: the 4-wide walk, 128 threads and 24 LDS entries
+// for closest hit, 256 threads and 16 entries for any hit (the fastest or
+// within a few per cent on camera, shadow, bounce and uniform random rays).
+constexpr int TRACE_DEFAULT_CLOSEST = 9;
+constexpr int TRACE_DEFAULT_ANY = 14;
 constexpr int COUNTER_SLOTS = 64;
 constexpr uint32_t TRACE_RAY_CHUNK = 1024;      // default largest wave slice of the ray stream
 constexpr uint32_t TRACE_RAY_CHUNK_MAX = 4096;  // largest that may be configured

@@ -52,6 +52,8 @@ const Variant kVariants[] = {
 };
 #undef RQ_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+static_assert(TRACE_DEFAULT_CLOSEST < kNumVariants && TRACE_DEFAULT_ANY < kNumVariants,
+              "default variants in the table");
 
 }  // namespace
 

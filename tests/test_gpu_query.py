@@ -218,6 +218,20 @@ def test_ray_stream_slices_and_counter_reuse(ctx, O, rq):
         ctx.release(s)
 
 
+def test_fresh_context_defaults(rq):
+    """A new context walks the 4-wide nodes (variant 9 closest, 14 any hit),
+    refills once every lane is idle and takes 1024-ray slices."""
+    c = rq.Context(0)
+    try:
+        assert c.trace_config() == (9, 14, 64)
+        assert c.trace_refill() == (64, 64)
+        assert c.trace_slice() == (1024, 1024)
+        v = c.trace_variants()
+        assert v[9]["wide"] and v[14]["wide"] and not v[14]["tstack"]
+    finally:
+        c.close()
+
+
 def test_trace_slice_refused(ctx, rq):
     for bad in ((0, 1024), (32, 1024), (1024, 8192), (1000, 1024)):
         with pytest.raises(rq.RQError):
