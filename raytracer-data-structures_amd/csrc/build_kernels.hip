@@ -31,9 +31,23 @@ __global__ __launch_bounds__(BUILD_BLOCK) void k_onesweep(BuildView v) {
   __syncthreads();
   onesweep_tile<PASS>(v, s_tile, lds);
 }
+// Small builds sort 8 keys per thread (tiles of SORT_TILE_LEAN keys): with
+// few tiles a pass costs one tile's latency, and a tile of half the keys
+// ranks in half the steps.
+template <int PASS>
+__global__ __launch_bounds__(BUILD_BLOCK) void k_onesweep_lean(BuildView v) {
+  __shared__ uint32_t lds[BUILD_LDS_WORDS];
+  __shared__ uint32_t s_tile;
+  if (threadIdx.x == 0) s_tile = atomicAdd(&v.B.misc[MISC_TILECTR + PASS], 1u);
+  __syncthreads();
+  onesweep_tile<PASS, SORT_ITEMS_LEAN>(v, s_tile, lds);
+}
 typedef void (*OnesweepFn)(BuildView);
 const OnesweepFn kOnesweep[SORT_PASSES] = {k_onesweep<0>, k_onesweep<1>, k_onesweep<2>,
                                            k_onesweep<3>};
+const OnesweepFn kOnesweepLean[SORT_PASSES] = {k_onesweep_lean<0>, k_onesweep_lean<1>,
+                                               k_onesweep_lean<2>, k_onesweep_lean<3>};
+constexpr uint32_t SORT_LEAN_MAX_KEYS = 1u << 20;
 
 __global__ __launch_bounds__(BUILD_BLOCK) void k_karras(BuildView v) { karras_tile<BUILD_BLOCK>(v, blockIdx.x); }
 
@@ -100,7 +114,8 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
   if (n == 0) return hipSuccess;
   hipStream_t s = lc.stream;
   const BuildView v{vx, vy, vz, idx, n, mode, B, vr, vr != nullptr};
-  const uint32_t ntiles = tiles_of(n, SORT_TILE);
+  const bool lean = n < SORT_LEAN_MAX_KEYS;
+  const uint32_t ntiles = tiles_of(n, lean ? SORT_TILE_LEAN : SORT_TILE);
   const uint32_t nstatus = SORT_PASSES * ntiles * 256;
   {
     ProfScope ps(lc, "build_init");
@@ -118,7 +133,8 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
   }
   for (int pass = 0; pass < SORT_PASSES; ++pass) {
     ProfScope ps(lc, "radix_onesweep");
-    hipLaunchKernelGGL(kOnesweep[pass], dim3(ntiles), dim3(BUILD_BLOCK), 0, s, v);
+    hipLaunchKernelGGL((lean ? kOnesweepLean : kOnesweep)[pass], dim3(ntiles), dim3(BUILD_BLOCK), 0,
+                       s, v);
   }
   if (n >= 2) {
     ProfScope ps(lc, "karras");
