@@ -926,7 +926,10 @@ def main() -> int:
     dom = max(per_kernel, key=lambda k: per_kernel[k][0])
     ms_tot, launches = per_kernel[dom]
     avg_ms = ms_tot / launches
-    alg = {"trace_closest": closest_bytes, "trace_any": any_bytes}.get(dom)
+    # the shadow rays may go out in several launches per frame (primary hits'
+    # and second hits' concurrently): bytes per launch = frame bytes / launches
+    any_launches = max(1, per_kernel.get("trace_any", (0, args.steps))[1] // args.steps)
+    alg = {"trace_closest": closest_bytes, "trace_any": any_bytes / any_launches}.get(dom)
     roofline = None
     if alg is not None:
         achieved = alg / (avg_ms * 1e-3) / 1e9

@@ -30,6 +30,10 @@ struct rq_renderer {
   float* img = nullptr;
   double* jit = nullptr;
   uint32_t* counters = nullptr;
+  // the primary hits' shadow rays are traced on `side` while the bounce rays
+  // are traced on the frame's stream (default priority, ordered by events)
+  hipStream_t side = nullptr;
+  hipEvent_t ev_primary = nullptr, ev_shadow = nullptr;
   std::vector<void*> allocs;
   // a second ray per sample: the mirror bounce, or a refraction ray
   bool second() const { return p.mode == RQ_RENDER_SHADE && (p.bounce || p.materials); }
@@ -54,6 +58,9 @@ int alloc_rays(rq_ctx* c, rq_renderer* r, size_t n, RaySoA& R) {
 
 void free_renderer(rq_ctx* c, rq_renderer* r) {
   for (void* p : r->allocs) ctx_free(c, p);
+  if (r->ev_primary) (void)hipEventDestroy(r->ev_primary);
+  if (r->ev_shadow) (void)hipEventDestroy(r->ev_shadow);
+  if (r->side) (void)hipStreamDestroy(r->side);
   delete r;
 }
 
@@ -150,6 +157,12 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
     hipError_t e = hipMemset(r->counters, 0, 16);
     if (e != hipSuccess) rc = hip_err(c, e, "counter init");
   }
+  if (!rc && r->second() && p->shadows) {
+    hipError_t e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_primary, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_shadow, hipEventDisableTiming);
+    if (e != hipSuccess) rc = hip_err(c, e, "renderer streams");
+  }
   if (rc) {
     free_renderer(c, r);
     return rc;
@@ -184,6 +197,19 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   if (rc) return rc;
   e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc, r->lit0);
   if (e != hipSuccess) return hip_err(c, e, "shade primary");
+  if (r->side) {
+    // the primary shadow rays on the side stream, concurrently with the
+    // second rays and their shadow rays here: each launch's last waves
+    // overlap the other's
+    e = hipEventRecord(r->ev_primary, (hipStream_t)stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(r->side, r->ev_primary, 0);
+    if (e != hipSuccess) return hip_err(c, e, "frame streams");
+    rq_rays sr = as_rays(r->shad);
+    rc = rq_any_hit(c, b, &sr, r->n, r->occ, r->side);
+    if (rc) return rc;
+    e = hipEventRecord(r->ev_shadow, r->side);
+    if (e != hipSuccess) return hip_err(c, e, "frame streams");
+  }
   if (r->second()) {
     rq_rays br = as_rays(r->bnc);
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
@@ -192,7 +218,13 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad, r->lit1);
     if (e != hipSuccess) return hip_err(c, e, "shade bounce");
   }
-  if (r->p.shadows) {
+  if (r->side) {
+    rq_rays sr = as_rays(r->shad, r->n);
+    rc = rq_any_hit(c, b, &sr, r->n, r->occ + r->n, stream);
+    if (rc) return rc;
+    e = hipStreamWaitEvent((hipStream_t)stream, r->ev_shadow, 0);
+    if (e != hipSuccess) return hip_err(c, e, "frame streams");
+  } else if (r->p.shadows) {
     rq_rays sr = as_rays(r->shad);
     rc = rq_any_hit(c, b, &sr, r->second() ? 2 * r->n : r->n, r->occ, stream);
     if (rc) return rc;
