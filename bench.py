@@ -835,8 +835,8 @@ def main() -> int:
     if args.treelets:
         CHILD_ARGS.extend(["--treelets", str(args.treelets)])
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the path on a single MI355X "
-              "(rays split across GPUs would need no exchange; see DESIGN.md). "
+        print("bench.py measures one GPU: the ray-query path runs on a single MI355X (rays "
+              "would split across GPUs with no exchange but a gather of the hits). "
               f"--gpus {args.gpus} is not supported.", file=sys.stderr)
         return 2
 
