@@ -593,7 +593,7 @@ def run_c5(ctx, args) -> int:
     target = rng.integers(0, len(slots), n_req)
     upload = rng.random(n_req) < 0.05
     max_n = int(sizes.max())
-    ring = 80  # page-locked result buffers (at most 80 requests in flight from this driver)
+    ring = 80  # page-locked result buffers (at most 80 requests in flight from this load generator)
     bufs = [(rq.HostArray(max_n, np.int32), rq.HostArray(max_n, np.float32)) for _ in range(ring)]
     log(f"c5: {n_req} requests over {warm + meas:.0f} s, {int(upload.sum())} uploads, "
         f"offered {sizes.sum() / (warm + meas) / 1e6:.0f} Mrays/s")

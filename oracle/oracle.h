@@ -1,7 +1,7 @@
 /*
  * CPU ORACLE — test infrastructure only.
  *
- * This library is the checker for the GPU ray-query engine. It is a plain
+ * This library is the oracle the GPU ray-query engine is tested against. It is a plain
  * CPU restatement of the reference algorithms on the acceleration-structure
  * path of Alhajras/Raytracer-Data-structures (project/raytracer/...). Only the
  * test suite, the smoke check and the CPU-baseline leg of the benchmark may

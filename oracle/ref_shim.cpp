@@ -1,4 +1,4 @@
-// Checker-side wrapper around the reference's own sources (test
+// Oracle-side wrapper around the reference's own sources (test
 // infrastructure only). It is compiled by oracle/Makefile from the reference
 // checkout where it lies (-I <reference>/project/raytracer) into
 // oracle/_ref/libref.so, and exports C entry points that call the reference's
