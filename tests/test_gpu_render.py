@@ -96,3 +96,37 @@ def test_render_surface_colour(ctx, O, rq):
     log("render surface colour")
     g, ppm, cnt = _render_parity(ctx, O, rq, mesh, p)
     assert g["shadow0"].sum() > 0
+
+
+def test_render_frames_back_to_back_on_a_stream(ctx, rq):
+    """Several bounce + shadow frames queued on a caller stream without waits
+    in between (the primary shadow rays run on the renderer's second stream
+    and are joined back by events): every frame equals a single frame."""
+    mesh = rq.icosphere(5, 3, center=(0.0, 0.0, -2.5))
+    p = rq.render_defaults()
+    p.width, p.height, p.spp = 160, 96, 2
+    p.camera, p.seed, p.bounce = rq.CAMERA_PHILOX, 5, 1
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
+    r = rq.Renderer(ctx, p)
+    st = ctx.stream_create()
+    try:
+        r.frame(b)
+        ctx.sync()
+        ref = r.samples()
+        ref_rgb, ref_ppm = r.image()
+        for _ in range(3):
+            r.frame(b, stream=st)
+        ctx.sync(st)
+        g = r.samples()
+        for k in ("hit0", "hit1", "shadow0", "shadow1"):
+            assert np.array_equal(g[k], ref[k]), k
+        assert np.array_equal(g["rgb"].view(np.uint32), ref["rgb"].view(np.uint32))
+        rgb, ppm = r.image()
+        assert np.array_equal(ppm, ref_ppm)
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.stream_destroy(st)
+        r.close()
+        ctx.release(b)
+        ctx.release(s)
