@@ -131,6 +131,10 @@ def tuned_dict(ctx, cfg) -> dict:
 
 # options every rocprofv3 child run of this script repeats (set by main)
 CHILD_ARGS: list = []
+# kernel names timed inside a frame-loop timed region, and the untimed frames
+# that give the per-kernel breakdown
+TRACE_KERNELS = ["trace_closest", "trace_any"]
+BREAKDOWN_FRAMES = 3
 
 
 def rebuild(ctx, bvh, args) -> None:
@@ -898,7 +902,10 @@ def main() -> int:
         copy_probe(ctx)
         return 0
 
-    # ---- timed region: K frames, per-kernel HIP events on the launch stream
+    # ---- timed region: K frames, HIP events on the launch streams around the
+    # traversal launches only (the dominant-kernel candidates): events around
+    # every kernel cost ~4 % of a C2 frame
+    ctx.profile_only(TRACE_KERNELS)
     ctx.profile(True)
     ctx.profile_reset()
     ctx.sync()
@@ -914,6 +921,18 @@ def main() -> int:
     elapsed = t1 - t0
     ms_per_step = elapsed / args.steps * 1e3
     value = rays_per_frame * args.steps / elapsed / 1e6
+
+    # ---- per-kernel breakdown (build and shading included) from more frames
+    # after the timed region, every kernel bracketed by events
+    ctx.profile_only(None)
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(BREAKDOWN_FRAMES):
+        rebuild(ctx, bvh, args)
+        ren.frame(bvh)
+    ctx.sync()
+    ctx.profile(False)
+    breakdown = ctx.profile_read()
 
     # ---- roofline of the dominant kernel (algorithmic bytes / event time)
     n = ren.n
@@ -939,7 +958,7 @@ def main() -> int:
                         avg_launch_ms=round(avg_ms, 4))
     build_kernels = ("build_init", "centroid_bounds", "morton_hist", "radix_onesweep", "karras",
                      "refit", "refit_top", "widen")
-    build_ms = sum(per_kernel[k][0] for k in build_kernels if k in per_kernel) / args.steps
+    build_ms = sum(breakdown[k][0] for k in build_kernels if k in breakdown) / BREAKDOWN_FRAMES
     closest_ms = per_kernel.get("trace_closest", (0, 1))[0] / args.steps
     closest_rays_n = sum(r for _, r in closest_rays)
     if roofline is not None and args.traffic == "auto":
@@ -986,7 +1005,9 @@ def main() -> int:
                                               / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "closest_hit_mrays_s": round(closest_rays_n / (closest_ms * 1e-3) / 1e6, 1)
         if closest_ms else None,
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in per_kernel.items()},
+        # per frame, from BREAKDOWN_FRAMES frames after the timed region with
+        # every kernel bracketed by events
+        "kernel_ms_per_step": {k: round(v[0] / BREAKDOWN_FRAMES, 4) for k, v in breakdown.items()},
         "traversal_counts": {k: {"node_visits": v[0], "tri_tests": v[1]} for k, v in
                              counts.items()},
         "autotune": tuned,

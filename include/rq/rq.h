@@ -225,6 +225,9 @@ int rq_stream_destroy(rq_ctx* ctx, void* stream);
  * When enabled, every kernel the library launches is bracketed by events on
  * its own stream; rq_profile_read sums the durations per kernel name. */
 int rq_profile_enable(rq_ctx* ctx, int on);
+/* Times only the kernels named in a comma-separated list (NULL or "": all);
+ * the others run without events around them. */
+int rq_profile_only(rq_ctx* ctx, const char* names);
 int rq_profile_reset(rq_ctx* ctx);
 /* Returns the number of kernel names; fills up to cap entries. */
 int rq_profile_read(rq_ctx* ctx, char names[][48], double* total_ms, uint64_t* launches, int cap);

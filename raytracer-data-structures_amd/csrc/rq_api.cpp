@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -24,6 +25,7 @@ struct Profiler {
   };
   std::vector<Pending> pending;
   std::map<std::string, std::pair<double, uint64_t>> totals;
+  std::set<std::string> only;  // kernel names to time (empty: every kernel)
   void resolve() {
     for (auto& p : pending) {
       float ms = 0.0f;
@@ -40,11 +42,15 @@ struct Profiler {
 };
 
 void prof_begin(Profiler* p, hipStream_t s, const char* name, hipEvent_t* ev) {
-  (void)p, (void)name;
+  if (!p->only.empty() && !p->only.count(name)) {
+    *ev = nullptr;  // not timed: no events on the stream
+    return;
+  }
   hipEventCreate(ev);
   hipEventRecord(*ev, s);
 }
 void prof_end(Profiler* p, hipStream_t s, const char* name, hipEvent_t ev) {
+  if (!ev) return;
   hipEvent_t b;
   hipEventCreate(&b);
   hipEventRecord(b, s);
@@ -779,6 +785,22 @@ int rq_device_errors(rq_ctx* c, uint32_t* bits) {
 int rq_profile_enable(rq_ctx* c, int on) {
   if (check_ctx(c)) return RQ_E_INVALID;
   c->prof_on = on != 0;
+  return RQ_OK;
+}
+int rq_profile_only(rq_ctx* c, const char* names) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  c->prof->only.clear();
+  if (!names) return RQ_OK;
+  std::string cur;
+  for (const char* q = names;; ++q) {
+    if (*q == ',' || *q == '\0') {
+      if (!cur.empty()) c->prof->only.insert(cur);
+      cur.clear();
+      if (*q == '\0') break;
+    } else {
+      cur.push_back(*q);
+    }
+  }
   return RQ_OK;
 }
 int rq_profile_reset(rq_ctx* c) {

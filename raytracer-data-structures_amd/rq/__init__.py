@@ -200,6 +200,7 @@ def _load() -> C.CDLL:
         "rq_stream_destroy": (i32, [vp, vp]),
         "rq_profile_enable": (i32, [vp, i32]),
         "rq_profile_reset": (i32, [vp]),
+        "rq_profile_only": (i32, [vp, C.c_char_p]),
         "rq_profile_read": (i32, [vp, vp, vp, vp, i32]),
         "rq_release": (i32, [vp, vp]),
         "rq_gen_icosphere": (i32, [u32, u32, C.c_float, C.c_float, f32p, C.c_float,
@@ -764,6 +765,11 @@ class Context:
 
     def profile_reset(self) -> None:
         self._check(lib().rq_profile_reset(self.h))
+
+    def profile_only(self, names=None) -> None:
+        """Time only these kernel names (None: every kernel)."""
+        arg = ",".join(names).encode() if names else None
+        self._check(lib().rq_profile_only(self.h, arg))
 
     def profile_read(self) -> dict:
         cap = 64
