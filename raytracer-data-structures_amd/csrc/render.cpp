@@ -60,7 +60,7 @@ void free_renderer(rq_ctx* c, rq_renderer* r) {
   for (void* p : r->allocs) ctx_free(c, p);
   if (r->ev_primary) (void)hipEventDestroy(r->ev_primary);
   if (r->ev_shadow) (void)hipEventDestroy(r->ev_shadow);
-  if (r->side) (void)hipStreamDestroy(r->side);
+  if (r->side) (void)rq_stream_destroy(c, r->side);  // and the stream's stack overflow area
   delete r;
 }
 

@@ -106,6 +106,20 @@ LaunchCtx launch_ctx(rq_ctx* c, void* stream) {
 
 uint32_t next_counter_slot(rq_ctx* c) { return c->counter_seq++ % COUNTER_SLOTS; }
 
+int32_t* stream_gstack(rq_ctx* c, void* stream) {
+  if (!stream) return c->gstack;
+  for (auto& kv : c->gstack_streams)
+    if (kv.first == stream) return kv.second;
+  if (c->gstack_streams.size() >= MAX_TRACE_STREAMS) {
+    set_err(c, RQ_E_NOMEM, "traversal launches on more than %d streams", MAX_TRACE_STREAMS);
+    return nullptr;
+  }
+  void* p = nullptr;
+  if (ctx_alloc(c, c->gstack_bytes, &p) != RQ_OK) return nullptr;
+  c->gstack_streams.push_back({stream, static_cast<int32_t*>(p)});
+  return static_cast<int32_t*>(p);
+}
+
 }  // namespace rq
 
 using namespace rq;
@@ -206,7 +220,8 @@ int rq_create(rq_ctx** out, int device, size_t budget) {
   if (rc == RQ_OK) {
     c->errword = c->counters + COUNTER_SLOTS * 2;
     hipMemset(c->counters, 0, cbytes);
-    rc = ctx_alloc(c, gmax * 2 * sizeof(int32_t), (void**)&c->gstack);
+    c->gstack_bytes = gmax * 2 * sizeof(int32_t);
+    rc = ctx_alloc(c, c->gstack_bytes, (void**)&c->gstack);
   }
   if (rc == RQ_OK && hipDeviceSynchronize() != hipSuccess) rc = RQ_E_HIP;
   if (rc) {
@@ -539,7 +554,8 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
     trace_variant_info(variant, &blk, &depth, &ts, &wide);
   }
   a.wide = b->B.wide;
-  a.gstack = c->gstack;
+  a.gstack = stream_gstack(c, stream);
+  if (!a.gstack) return RQ_E_NOMEM;
   a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
   const uint32_t slot = next_counter_slot(c);
   a.counter = reinterpret_cast<unsigned long long*>(c->counters) + slot;
@@ -770,7 +786,16 @@ int rq_stream_create(rq_ctx* c, void** stream) {
 }
 int rq_stream_destroy(rq_ctx* c, void* stream) {
   if (check_ctx(c)) return RQ_E_INVALID;
-  hipError_t e = hipStreamDestroy((hipStream_t)stream);
+  // its overflow area goes with it, once its work is done (a later stream
+  // may get the same handle)
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  for (size_t i = 0; i < c->gstack_streams.size(); ++i)
+    if (c->gstack_streams[i].first == stream) {
+      ctx_free(c, c->gstack_streams[i].second);
+      c->gstack_streams.erase(c->gstack_streams.begin() + (long)i);
+      break;
+    }
+  if (e == hipSuccess) e = hipStreamDestroy((hipStream_t)stream);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream destroy");
 }
 

@@ -1,6 +1,8 @@
 // Host-side handle types behind the opaque ABI pointers.
 #pragma once
 #include <string>
+#include <utility>
+#include <vector>
 #include <unordered_map>
 
 #include "rq_internal.h"
@@ -26,7 +28,12 @@ struct rq_ctx {
   uint64_t counter_base[rq::COUNTER_SLOTS] = {};
   uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
   uint32_t counter_seq = 0;
-  int32_t* gstack = nullptr;  // traversal stack overflow ({node, distance} pairs)
+  // Traversal stack overflow ({node, distance} pairs), indexed by global
+  // thread id: launches that may run at once need separate areas, so each
+  // stream gets its own (the default stream's is allocated with the context).
+  int32_t* gstack = nullptr;
+  size_t gstack_bytes = 0;
+  std::vector<std::pair<void*, int32_t*>> gstack_streams;
 };
 
 constexpr uint32_t SCENE_MAGIC = 0x53435131u;  // "SCQ1"
@@ -68,6 +75,10 @@ LaunchCtx launch_ctx(rq_ctx* c, void* stream);
 // its waves take from the counter (RayStream), so the slot's next launch
 // starts from the counter's value after this one.
 uint32_t next_counter_slot(rq_ctx* c);
+// The stack overflow area of launches on `stream` (allocated on a stream's
+// first traversal launch; null with the context's error set when the budget
+// or the limit of MAX_TRACE_STREAMS streams is exceeded).
+int32_t* stream_gstack(rq_ctx* c, void* stream);
 // Build buffers for n primitives; `wide`: also the 4-wide node array and the
 // chunk-union refit buffers (the standalone build and blob load; batch and
 // server slots walk binary nodes and refit by frontier lists).

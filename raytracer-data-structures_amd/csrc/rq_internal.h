@@ -86,6 +86,7 @@ constexpr int TRACE_MAX_VARIANTS = 16;
 constexpr int TRACE_DEFAULT_CLOSEST = 9;
 constexpr int TRACE_DEFAULT_ANY = 14;
 constexpr int COUNTER_SLOTS = 64;
+constexpr int MAX_TRACE_STREAMS = 16;  // streams besides the default one with traversal launches
 constexpr uint32_t TRACE_RAY_CHUNK = 1024;      // default largest wave slice of the ray stream
 constexpr uint32_t TRACE_RAY_CHUNK_MAX = 4096;  // largest that may be configured
 

@@ -261,3 +261,71 @@ def test_autotune_picks_a_valid_variant(ctx, rq):
         ctx.trace_slice(1024, 1024)
         ctx.release(b)
         ctx.release(s)
+
+
+def test_concurrent_streams_deep_stacks(ctx, O, rq):
+    """Queries on two streams at once, in a variant with 12 LDS stack entries
+    over a scene of large overlapping triangles, so that rays spill their
+    stacks to the global overflow area in both launches: each stream has an
+    area of its own, every answer equals the oracle's."""
+    mesh = random_soup(6000, 27, spread=1.0, size=0.8)
+    ra = concat_rays(rays_toward(30000, 81), random_rays(20000, 82))
+    rb = concat_rays(rays_toward(30000, 83), random_rays(20000, 84))
+    ob = O.OracleBVH(mesh, 1)
+    oa, obb = ob.closest(ra), ob.closest(rb)
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    da, db = rq.DeviceRays.from_host(ctx, ra), rq.DeviceRays.from_host(ctx, rb)
+    ha = {k: rq.DeviceArray(ctx, 4 * da.n) for k in ("prim", "t")}
+    hb = {k: rq.DeviceArray(ctx, 4 * db.n) for k in ("prim", "t")}
+    st = [ctx.stream_create(), ctx.stream_create()]
+    saved = ctx.trace_config()
+    v12 = [v["index"] for v in ctx.trace_variants() if v["lds_depth"] == 12][0]
+    try:
+        ctx.trace_config(v12, 0, 64)
+        for rep in range(4):
+            for h in (ha, hb):
+                h["prim"].fill(0x7F)
+            ctx.closest_hit_dev(b, da, ha, stream=st[0])
+            ctx.closest_hit_dev(b, db, hb, stream=st[1])
+            ctx.sync(st[0])
+            ctx.sync(st[1])
+            for h, o, n in ((ha, oa, da.n), (hb, obb, db.n)):
+                assert np.array_equal(h["prim"].download(np.int32, n), o["prim"]), rep
+                assert np.array_equal(_bits(h["t"].download(np.float32, n)), _bits(o["t"])), rep
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.trace_config(*saved)
+        for x in st:
+            ctx.stream_destroy(x)
+        for x in (da, db):
+            x.free()
+        for h in (ha, hb):
+            for a in h.values():
+                a.free()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_stream_overflow_areas_are_released(ctx, rq):
+    """A stream's stack overflow area is freed with the stream: many streams
+    created, used and destroyed in turn never reach the per-context limit."""
+    mesh = random_soup(500, 29, spread=1.5)
+    rays = rays_toward(4000, 85)
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    d = rq.DeviceRays.from_host(ctx, rays)
+    h = {k: rq.DeviceArray(ctx, 4 * d.n) for k in ("prim", "t")}
+    used = ctx.bytes_in_use
+    try:
+        for _ in range(40):
+            x = ctx.stream_create()
+            ctx.closest_hit_dev(b, d, h, stream=x)
+            ctx.stream_destroy(x)
+        assert ctx.bytes_in_use == used
+    finally:
+        d.free()
+        for a in h.values():
+            a.free()
+        ctx.release(b)
+        ctx.release(s)
