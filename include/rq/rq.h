@@ -218,6 +218,12 @@ int rq_sync(rq_ctx* ctx, void* stream);
 /* Reads and clears the device error bits of past queries (synchronises the
  * device): bit 0 = a ray needed a deeper traversal stack than provided. */
 int rq_device_errors(rq_ctx* ctx, uint32_t* bits);
+/* Streams for concurrent queries. Queries on different streams may run at
+ * once: each non-null stream gets its own traversal-stack overflow area on
+ * its first query (at most 16 such streams per context; RQ_E_NOMEM beyond).
+ * rq_stream_destroy waits for the stream, frees its area and destroys it.
+ * A caller's own stream keeps its area until rq_destroy. Queries on one
+ * stream run in order and share its area. */
 int rq_stream_create(rq_ctx* ctx, void** stream);
 int rq_stream_destroy(rq_ctx* ctx, void* stream);
 
