@@ -53,9 +53,10 @@ constexpr float SLAB_SLACK = 1.000030517578125f;
 // hit with a 2^-9 relative slack keeps every box whose sphere's computed t
 // can still beat the best hit.
 constexpr float SPHERE_SLACK = 1.001953125f;
-// Wide walk, wave-uniform octant: the rows are loaded at byte offsets
-// node * 128 + row held in 32 bits (a scalar base plus one vector offset), so
-// the node count must stay below 2^25.
+// Wide walk: the near / far rows are loaded at byte offsets node * 128 + row
+// held in 32 bits (a scalar base plus one vector offset; the row is scalar
+// when the wave's octant is uniform, per lane otherwise), so the node count
+// must stay below 2^25; larger trees select the rows per value.
 constexpr uint32_t WIDE_UNIFORM_MAX_NODES = 1u << 25;
 
 // Slab test (same operations and order as the oracle): near/far planes by
@@ -267,6 +268,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     // of a select per value. Lanes that finish keep it so; a refill
     // recomputes it.
     uint32_t uni_rows = 0u, s_oct = 0u;
+    uint64_t mx = 0, my = 0, mz = 0;  // lanes whose 1/d has that sign bit set
     if (WIDE) {
       const uint32_t oct = (uint32_t)r.sx | ((uint32_t)r.sy << 1) | ((uint32_t)r.sz << 2);
       const uint64_t am = __ballot(active);
@@ -276,6 +278,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
                        a.nprims <= WIDE_UNIFORM_MAX_NODES;
       uni_rows = __builtin_amdgcn_readfirstlane((uint32_t)uni);
       s_oct = __builtin_amdgcn_readfirstlane(oref);
+      if (!ANY) mx = __ballot(r.sx), my = __ballot(r.sy), mz = __ballot(r.sz);
     }
 
     // ---- traverse until enough lanes are idle to be worth a refill -------
@@ -283,14 +286,37 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
       if (WIDE && active && node >= 0) {
         float4 nx, fx, ny, fy, nz, fz;
         const WideNode* w = a.wide + node;
-        const int4 ln = w->link;
-        if (uni_rows) {
+        int4 ln;
+        if (ANY) ln = w->link;  // the any-hit walk measures faster with it first
+        // the node's row offset, shared by both offset paths (computed here, a
+        // path that only issues loads needs no wait behind the other's loads)
+        const uint32_t base = (uint32_t)node << 7;
+        if (!ANY && !uni_rows && a.nprims <= WIDE_UNIFORM_MAX_NODES) {
+          // closest hit, mixed octants: the same rows at per-lane byte
+          // offsets (an OR into the node's 128-byte row, cheaper than a select
+          // per value). The offsets are formed from the wave's sign masks at
+          // each visit (volatile: not hoisted out of the loop, where they
+          // would hold three registers across it and, at 91 registers, leave
+          // less room beside a concurrent any-hit launch). The any-hit walk
+          // keeps the selects: shadow rays toward one light share an octant.
+          const char* wb = reinterpret_cast<const char*>(a.wide);
+          uint32_t qx, qy, qz;
+          asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qx) : "s"(mx));
+          asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qy) : "s"(my));
+          asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qz) : "s"(mz));
+          nx = *reinterpret_cast<const float4*>(wb + (base | qx));
+          fx = *reinterpret_cast<const float4*>(wb + (base | (qx ^ 16u)));
+          ny = *reinterpret_cast<const float4*>(wb + 32 + (base | qy));
+          fy = *reinterpret_cast<const float4*>(wb + 32 + (base | (qy ^ 16u)));
+          nz = *reinterpret_cast<const float4*>(wb + 64 + (base | qz));
+          fz = *reinterpret_cast<const float4*>(wb + 64 + (base | (qz ^ 16u)));
+        } else if (uni_rows) {
           // rows at byte offsets lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80;
           // the near row of an axis is hi when its sign bit is set
           const char* wb = reinterpret_cast<const char*>(a.wide);
           const uint32_t ox = (s_oct & 1u) << 4, oy = 32u + ((s_oct & 2u) << 3),
                          oz = 64u + ((s_oct & 4u) << 2);
-          const uint64_t off = (uint64_t)((uint32_t)node << 7);
+          const uint64_t off = base;
           nx = *reinterpret_cast<const float4*>(wb + ox + off);
           fx = *reinterpret_cast<const float4*>(wb + (ox ^ 16u) + off);
           ny = *reinterpret_cast<const float4*>(wb + oy + off);
@@ -305,6 +331,10 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
           ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
           nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
         }
+        // closest hit: after the rows, so that the uniform path, which
+        // statically follows the per-lane path's loads, waits on nothing in
+        // flight when it starts
+        if (!ANY) ln = w->link;
         float t0, t1, t2, t3;
         const bool h0 = slab_w(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, r, bt, slack, t0) & (ln.x != 0);
         const bool h1 = slab_w(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, r, bt, slack, t1) & (ln.y != 0);
