@@ -278,7 +278,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
                        a.nprims <= WIDE_UNIFORM_MAX_NODES;
       uni_rows = __builtin_amdgcn_readfirstlane((uint32_t)uni);
       s_oct = __builtin_amdgcn_readfirstlane(oref);
-      if (!ANY) mx = __ballot(r.sx), my = __ballot(r.sy), mz = __ballot(r.sz);
+      mx = __ballot(r.sx), my = __ballot(r.sy), mz = __ballot(r.sz);
     }
 
     // ---- traverse until enough lanes are idle to be worth a refill -------
@@ -291,14 +291,13 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
         // the node's row offset, shared by both offset paths (computed here, a
         // path that only issues loads needs no wait behind the other's loads)
         const uint32_t base = (uint32_t)node << 7;
-        if (!ANY && !uni_rows && a.nprims <= WIDE_UNIFORM_MAX_NODES) {
-          // closest hit, mixed octants: the same rows at per-lane byte
-          // offsets (an OR into the node's 128-byte row, cheaper than a select
-          // per value). The offsets are formed from the wave's sign masks at
-          // each visit (volatile: not hoisted out of the loop, where they
-          // would hold three registers across it and, at 91 registers, leave
-          // less room beside a concurrent any-hit launch). The any-hit walk
-          // keeps the selects: shadow rays toward one light share an octant.
+        if (!uni_rows && a.nprims <= WIDE_UNIFORM_MAX_NODES) {
+          // mixed octants: the same rows at per-lane byte offsets (an OR into
+          // the node's 128-byte row, cheaper than a select per value). The
+          // offsets are formed from the wave's sign masks at each visit
+          // (volatile: not hoisted out of the loop, where they would hold
+          // three registers across it; the closest-hit kernel would reach 91
+          // and leave less room beside a concurrent any-hit launch).
           const char* wb = reinterpret_cast<const char*>(a.wide);
           uint32_t qx, qy, qz;
           asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qx) : "s"(mx));
