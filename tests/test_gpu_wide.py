@@ -217,3 +217,45 @@ def test_wide_variant_on_batch_bvh_walks_binary(ctx, O, rq):
         for a in hits.values():
             a.free()
         ctx.release(s)
+
+
+def test_wide_walk_on_a_tree_past_the_offset_limit(ctx, rq):
+    """A tree of 2^25 wide nodes or more: the row offsets no longer fit 32
+    bits, so the wide walks select the near / far rows per value. Their
+    answers equal the binary walk's (which the oracle pins on every other
+    scene) on rays through a dense soup of 2^25 + 4096 triangles."""
+    n = (1 << 25) + 4096
+    rng = np.random.default_rng(31)
+    c = rng.random((n, 1, 3), dtype=np.float32) * np.float32(8) - np.float32(4)
+    v = c + (rng.random((n, 3, 3), dtype=np.float32) - np.float32(0.5)) * np.float32(0.02)
+    del c
+    mesh = rq.MeshArrays(np.ascontiguousarray(v[:, :, 0].reshape(-1)),
+                         np.ascontiguousarray(v[:, :, 1].reshape(-1)),
+                         np.ascontiguousarray(v[:, :, 2].reshape(-1)),
+                         np.arange(3 * n, dtype=np.uint32))
+    del v
+    rays = concat_rays(random_rays(60000, 91, lo=(-4, -4, -4), hi=(4, 4, 4)),
+                       rays_toward(20000, 92, target=(0.0, 0.0, 0.0), spread=3.0,
+                                   origin=(0.0, 0.0, -12.0)))
+    log(f"wide walk past the offset limit: {n} triangles")
+    s = ctx.upload_scene(mesh)
+    del mesh
+    b = ctx.build_lbvh(s, 1)
+    saved = ctx.trace_config()
+    try:
+        assert ctx.bvh_info(b).n_internal >= 1 << 25
+        ctx.trace_config(0, 0, 64)
+        o, oa = ctx.closest_hit(b, rays), ctx.any_hit(b, rays)
+        assert (o["prim"] >= 0).mean() > 0.3
+        for wv in _wide_variants(ctx):
+            for refill in (16, 64):
+                ctx.trace_config(wv["index"], wv["index"] if not wv["tstack"] else 0, refill)
+                g = ctx.closest_hit(b, rays)
+                assert np.array_equal(g["prim"], o["prim"]), (wv, refill)
+                assert np.array_equal(_bits(g["t"]), _bits(o["t"])), (wv, refill)
+                assert np.array_equal(ctx.any_hit(b, rays)["occ"], oa["occ"]), (wv, refill)
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.trace_config(*saved)
+        ctx.release(b)
+        ctx.release(s)
