@@ -287,7 +287,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
         float4 nx, fx, ny, fy, nz, fz;
         const WideNode* w = a.wide + node;
         int4 ln;
-        if (ANY) ln = w->link;  // the any-hit walk measures faster with it first
+        if (ANY) ln = w->link;  // the any-hit walk's order, as measured
         // the node's row offset, shared by both offset paths (computed here, a
         // path that only issues loads needs no wait behind the other's loads)
         const uint32_t base = (uint32_t)node << 7;
