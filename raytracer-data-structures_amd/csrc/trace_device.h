@@ -286,8 +286,6 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
       if (WIDE && active && node >= 0) {
         float4 nx, fx, ny, fy, nz, fz;
         const WideNode* w = a.wide + node;
-        int4 ln;
-        if (ANY) ln = w->link;  // the any-hit walk's order, as measured
         // the node's row offset, shared by both offset paths (computed here, a
         // path that only issues loads needs no wait behind the other's loads)
         const uint32_t base = (uint32_t)node << 7;
@@ -330,10 +328,10 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
           ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
           nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
         }
-        // closest hit: after the rows, so that the uniform path, which
-        // statically follows the per-lane path's loads, waits on nothing in
-        // flight when it starts
-        if (!ANY) ln = w->link;
+        // the link row after the rows: the uniform path, which statically
+        // follows the per-lane path's loads, then starts with nothing in
+        // flight to wait on
+        const int4 ln = w->link;
         float t0, t1, t2, t3;
         const bool h0 = slab_w(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, r, bt, slack, t0) & (ln.x != 0);
         const bool h1 = slab_w(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, r, bt, slack, t1) & (ln.y != 0);
