@@ -31,6 +31,7 @@ sys.path.insert(0, str(ROOT / "raytracer-data-structures_amd"))
 sys.path.insert(0, str(ROOT))
 
 import rq  # noqa: E402
+from rq import workloads as W  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # LBVH build bytes per triangle of the reference byte model: Morton 56 (index +
@@ -44,20 +45,6 @@ def log(msg: str) -> None:
 
 
 # ---------------------------------------------------------------- workloads
-def c2_setup(ctx):
-    mesh = rq.icosphere(8, 2, center=(0.0, 0.0, -2.0))
-    p = rq.render_defaults()
-    p.width, p.height, p.spp = 1920, 1080, 4
-    p.camera, p.seed, p.bounce = rq.CAMERA_PHILOX, 2, 1
-    return mesh, p, "c2_incoherent_large_mesh"
-
-
-def c1_setup(ctx):
-    mesh = rq.icosphere(6, 1, center=(0.0, 0.0, -5.0))
-    p = rq.render_defaults()
-    return mesh, p, "c1_plain_640x480"
-
-
 def counted_batch(ctx, bvh, rays, any_hit: bool):
     """Traversal counts of the canonical traversal for one ray batch."""
     n = rays.n
@@ -263,13 +250,8 @@ def run_c3(ctx, args) -> int:
     with Poisson-disk centres in [0,100]^3; 2^24 rays with origins uniform in
     the scene box inflated by 10% per side and directions uniform on the
     sphere; one step = closest hit + any hit over the same rays."""
-    mesh = rq.icosphere_field(256, 5, 3, 3, 100.0, 2.5)
+    mesh, (lo, hi), n, ray_seed = W.c3()
     log(f"c3_raw_ray_batch: {mesh.ntri} triangles")
-    lo, hi = mesh.bounds()
-    ext = hi - lo
-    lo = (lo - ext * np.float32(0.1)).astype(np.float32)
-    hi = (hi + ext * np.float32(0.1)).astype(np.float32)
-    n = 1 << 24
     scene = ctx.upload_scene(mesh)
     bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
     treelets = None
@@ -285,7 +267,7 @@ def run_c3(ctx, args) -> int:
                     "ms": round(sum(v[0] for k, v in tp.items() if k.startswith("treelet")), 3),
                     "height": ctx.bvh_info(bvh).height}
         log(f"treelets: {treelets}")
-    rays = ctx.raygen_uniform(n, 3, lo, hi)
+    rays = ctx.raygen_uniform(n, ray_seed, lo, hi)
     hits = {k: rq.DeviceArray(ctx, 4 * n) for k in ("prim", "t")}
     occ = rq.DeviceArray(ctx, n)
     vc, tc = counted_batch(ctx, bvh, rays, any_hit=False)
@@ -369,34 +351,15 @@ BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
 
 
 def c4_workload(ctx, count: int = 512):
-    """Batched build and traverse: `count` independent meshes with
-    F_m = round(exp(U[ln 1e3, ln 5e5])) triangles (numpy seed 4); even m a
-    displaced icosphere at the largest level with 20*4^L <= F_m, odd m a
-    random soup of F_m triangles (centroids uniform in the unit cube, edge
-    length ~N(0.02, 0.005) clipped at 0.002). R_m = round(exp(U[ln 2^16,
-    ln 2^20])) rays per mesh (numpy seed 5), origins uniform in the mesh box
-    inflated by 10% per side, directions uniform on the sphere."""
-    rng = np.random.default_rng(4)
-    tris = np.rint(np.exp(rng.uniform(np.log(1e3), np.log(5e5), count))).astype(np.int64)
-    nrays = np.rint(np.exp(np.random.default_rng(5).uniform(np.log(65536.0), np.log(1048576.0),
-                                                             count))).astype(np.int64)
+    """The batch's meshes (uploaded) and device rays: uniform in each mesh's
+    box inflated by 10% per side, directions uniform on the sphere."""
+    tris, nrays = W.c4_sizes(count)
     meshes, scenes, rays = [], [], []
     for m in range(count):
-        if m % 2 == 0:
-            level = int(np.floor(np.log(tris[m] / 20.0) / np.log(4.0)))
-            while 20 * 4 ** (level + 1) <= tris[m]:
-                level += 1
-            while level > 0 and 20 * 4 ** level > tris[m]:
-                level -= 1
-            mesh = rq.icosphere(level, 1000 + m)
-        else:
-            mesh = rq.soup(int(tris[m]), 2000 + m)
-        lo, hi = mesh.bounds()
-        ext = hi - lo
-        lo = (lo - ext * np.float32(0.1)).astype(np.float32)
-        hi = (hi + ext * np.float32(0.1)).astype(np.float32)
+        mesh = W.c4_mesh(m, tris[m])
+        lo, hi = W.inflated_box(mesh)
         scenes.append(ctx.upload_scene(mesh))
-        rays.append(ctx.raygen_uniform(int(nrays[m]), 3000 + m, lo, hi))
+        rays.append(ctx.raygen_uniform(int(nrays[m]), W.c4_ray_seed(m), lo, hi))
         meshes.append(mesh)
     return meshes, scenes, rays
 
@@ -529,44 +492,18 @@ def run_c4(ctx, args) -> int:
     return 0
 
 
-def c5_meshes(count: int, seed: int, lo: float = 1e4, hi: float = 2e6):
-    """F ~ round(exp(U[ln lo, ln hi])) (numpy seed); even m a displaced
-    icosphere at the largest level with 20*4^L <= F, odd m a random soup."""
-    rng = np.random.default_rng(seed)
-    tris = np.rint(np.exp(rng.uniform(np.log(lo), np.log(hi), count))).astype(np.int64)
-    out = []
-    for m in range(count):
-        if m % 2 == 0:
-            level = 0
-            while 20 * 4 ** (level + 1) <= tris[m]:
-                level += 1
-            out.append(rq.icosphere(level, 10 * seed + m))
-        else:
-            out.append(rq.soup(int(tris[m]), 1000 * seed + m))
-    return out
-
-
 _BOUNDS: dict = {}
 
 
 def _c5_rays_spec(mesh, kind: int, seed: int) -> dict:
     if id(mesh) not in _BOUNDS:
         _BOUNDS[id(mesh)] = mesh.bounds()
-    lo, hi = _BOUNDS[id(mesh)]
-    c = (lo + hi) * 0.5
-    r = float(np.linalg.norm(hi - lo)) * 0.5
-    if kind == 0:
-        return dict(pinhole=dict(eye=(float(c[0]), float(c[1]), float(c[2] + 2.5 * r)),
-                                 look_at=tuple(float(x) for x in c), up=(0.0, 1.0, 0.0),
-                                 fov_deg=40.0, width=1024), seed=seed)
-    ext = hi - lo
-    return dict(box=(tuple(float(x) for x in lo - 0.1 * ext), tuple(float(x) for x in hi + 0.1 * ext)),
-                seed=seed)
+    return W.c5_rays_spec(_BOUNDS[id(mesh)], kind, seed)
 
 
 def run_c5(ctx, args) -> int:
     """Serving: 64 resident meshes (F ~ log-U[1e4, 2e6], seed 5) in a server
-    with 64 slots of 2^21 triangles and a 200 GB HBM budget. Open loop:
+    with 72 slots of 2^21 triangles (64 resident, 8 for uploads) and a 200 GB HBM budget. Open loop:
     Poisson arrivals at 2000 requests/s for 1 s of warm-up plus
     `--serve-seconds` measured; request sizes log-U[4096, 4194304] rays (seed
     7), alternately pinhole rays aimed at the mesh and uniform random rays
@@ -580,11 +517,12 @@ def run_c5(ctx, args) -> int:
     import ctypes as C
 
     t_setup = time.perf_counter()
-    resident = c5_meshes(64, 5)
-    uploads = c5_meshes(8, 6)
+    resident = W.c5_meshes(64, 5)
+    uploads = W.c5_meshes(8, 6)
     log(f"c5: {sum(m.ntri for m in resident)} resident triangles, "
         f"{sum(m.ntri for m in uploads)} in the upload pool ({time.perf_counter() - t_setup:.1f} s)")
-    srv = rq.Server(0)
+    # the resident meshes keep their slots; uploads recycle slots of their own
+    srv = rq.Server(0, mesh_slots=len(resident) + len(uploads))
     slots = [srv.load_mesh(m) for m in resident]
     slot_mesh = {s: m for s, m in zip(slots, resident)}
     lam, warm, meas = 2000.0, 1.0, float(args.serve_seconds)
@@ -859,7 +797,7 @@ def main() -> int:
         return run_c4(ctx, args)
     if args.config == "c5":
         return run_c5(ctx, args)
-    mesh, params, workload = (c2_setup if args.config == "c2" else c1_setup)(ctx)
+    mesh, params, workload = (W.c2 if args.config == "c2" else W.c1)()
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)
     bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)

@@ -36,6 +36,7 @@ extern "C" {
 #define RQ_E_NODEVICE (-5)    /* no usable gfx950 device                     */
 #define RQ_E_UNSUPPORTED (-6) /* feature not available for this input       */
 #define RQ_E_DEVICE (-7)      /* a kernel reported an error (stack, barrier)  */
+#define RQ_E_CANCELED (-8)    /* a served request was cancelled              */
 
 /* Morton normalisation (accelerators.h:577 vs Karras 2012). */
 #define RQ_MORTON_REF_AFFINE 0 /* (centroid + 30) / 1000, bit-exact with the reference */
@@ -143,13 +144,27 @@ int rq_bvh_export_wide(rq_ctx* ctx, const rq_bvh* b, float* boxes, int32_t* link
  * checksum, then codes, sorted indices, parent links, nodes, leaf records and
  * the root box) and loaded into a new BVH of the same scene without a build.
  * Load errors: RQ_E_IO for a damaged or foreign blob, RQ_E_INVALID when the
- * scene does not match the fingerprint. */
+ * scene does not match the fingerprint. A checksum detects accidents only,
+ * so before anything reaches the device the payload is checked against the
+ * scene's primitives, in O(n) on the host: every node and leaf reached once
+ * from the root through in-range links, parent links and height consistent
+ * (the height within the traversal stack), the sorted order a permutation,
+ * every leaf record the build's record of its primitive, every child box the
+ * exact union below it; any failure is RQ_E_IO. */
 int rq_bvh_blob_size(rq_ctx* ctx, const rq_bvh* b, size_t* bytes);
 int rq_bvh_save(rq_ctx* ctx, const rq_bvh* b, void* blob, size_t capacity, size_t* written);
 int rq_bvh_load(rq_ctx* ctx, const rq_scene* s, const void* blob, size_t bytes, rq_bvh** out,
                 void* stream);
 int rq_bvh_save_file(rq_ctx* ctx, const rq_bvh* b, const char* path);
 int rq_bvh_load_file(rq_ctx* ctx, const rq_scene* s, const char* path, rq_bvh** out);
+/* The check rq_bvh_load makes, on host arrays and without a device: a blob
+ * against triangles (vx/vy/vz with nv vertices, idx 3 per primitive) or
+ * spheres (centres vx/vy/vz, radius; nv == nprim). RQ_OK, RQ_E_IO (damaged,
+ * or not a well-formed LBVH of these primitives) or RQ_E_INVALID (a blob of
+ * other primitives, or bad arguments); the reason goes to msg (may be NULL). */
+int rq_bvh_blob_validate(const void* blob, size_t bytes, int prim_kind, const float* vx,
+                         const float* vy, const float* vz, uint32_t nv, const uint32_t* idx,
+                         const float* radius, uint32_t nprim, char* msg, size_t msg_cap);
 
 /* ---- queries -----------------------------------------------------------
  * Closest hit replaces boxIntersect (accelerators.h:668-690) plus the
@@ -220,12 +235,24 @@ int rq_sync(rq_ctx* ctx, void* stream);
 int rq_device_errors(rq_ctx* ctx, uint32_t* bits);
 /* Streams for concurrent queries. Queries on different streams may run at
  * once: each non-null stream gets its own traversal-stack overflow area on
- * its first query (at most 16 such streams per context; RQ_E_NOMEM beyond).
- * rq_stream_destroy waits for the stream, frees its area and destroys it.
- * A caller's own stream keeps its area until rq_destroy. Queries on one
- * stream run in order and share its area. */
+ * its first query. Areas are kept for at most 16 streams per context; the
+ * 17th stream takes over the area of a stream whose work has all completed
+ * (RQ_E_NOMEM only when 16 streams are busy at once). rq_stream_destroy
+ * waits for the stream, frees its area and destroys it. A caller's own stream
+ * keeps its area until rq_stream_forget or rq_destroy. Queries on one stream
+ * run in order and share its area.
+ * Graph capture: a query enqueued on a stream under capture zeroes a counter
+ * of its own inside the graph, so the graph can be replayed any number of
+ * times (at most 64 captured queries of one context may run at once). The
+ * stream must have queried once before the capture (its overflow area is
+ * allocated then, and allocation is not allowed under capture:
+ * RQ_E_UNSUPPORTED otherwise); a graph keeps that area, so it must not run
+ * at the same time as other queries on the stream it was captured on. */
 int rq_stream_create(rq_ctx* ctx, void** stream);
 int rq_stream_destroy(rq_ctx* ctx, void* stream);
+/* The context forgets a caller-owned stream: waits for it and frees its
+ * overflow area (call before destroying a stream that has queried). */
+int rq_stream_forget(rq_ctx* ctx, void* stream);
 
 /* ---- per-kernel timing (HIP events on the launch stream) ---------------
  * When enabled, every kernel the library launches is bracketed by events on

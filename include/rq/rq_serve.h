@@ -54,9 +54,12 @@ typedef struct {
 } rq_pinhole;
 
 typedef struct {
-  uint32_t mesh;           /* resident mesh slot; ignored when new_mesh is set       */
-  const rq_mesh* new_mesh; /* non-NULL: upload this host mesh into a recycled slot,
-                              build its LBVH, then trace (host arrays valid until done) */
+  uint32_t mesh;           /* resident mesh slot (rq_server_load_mesh); ignored when
+                              new_mesh is set                                        */
+  const rq_mesh* new_mesh; /* non-NULL: upload this host mesh into a slot that holds
+                              no resident mesh, build its LBVH, then trace (host
+                              arrays valid until done); the mesh serves this request
+                              only, and its slot is recycled by later uploads        */
   uint32_t nrays;
   int32_t source;          /* RQ_RAYS_*                                              */
   rq_rays host_rays;       /* RQ_RAYS_HOST: host SoA arrays, valid until done        */
@@ -108,12 +111,23 @@ void rq_server_defaults(rq_server_config* cfg);
 int rq_server_create(int device, const rq_server_config* cfg, rq_server** out);
 /* Stops the loop after the requests in flight and frees everything. */
 int rq_server_destroy(rq_server* s);
-/* Blocking: uploads and builds a mesh into a free slot (LRU when all are
- * used and idle); returns the slot. */
+/* Blocking: uploads and builds a resident mesh into a slot that holds none
+ * (a free one, else the least recently used idle upload slot); returns the
+ * slot. A resident mesh keeps its slot until the server is destroyed, so the
+ * slot id always names it; RQ_E_NOMEM when every slot holds a resident mesh.
+ * Uploads (requests with new_mesh) need a slot beyond the resident ones. */
 int rq_server_load_mesh(rq_server* s, const rq_mesh* m, uint32_t* slot);
 /* Non-blocking, any thread: queues a request. RQ_E_NOMEM if it can never fit
- * (buffers beyond the arena, mesh beyond the slot capacity). */
+ * (buffers beyond the arena, mesh beyond the slot capacity, an upload when
+ * every slot holds a resident mesh); RQ_E_INVALID if `mesh` is not a
+ * resident slot. */
 int rq_server_submit(rq_server* s, const rq_request* r, uint64_t* ticket);
+/* Any thread: asks a queued or running request to stop. It finishes with
+ * state RQ_E_CANCELED (its host outputs are left unspecified): at once if it
+ * was not admitted yet, else once the trace launches that hold its rays have
+ * retired, without its result copy; the launches of other requests are not
+ * affected. RQ_E_INVALID for an unknown ticket or a finished request. */
+int rq_server_cancel(rq_server* s, uint64_t ticket);
 int rq_server_poll(rq_server* s, uint64_t ticket, rq_request_status* st);
 /* Blocks until the request is done or failed. */
 int rq_server_wait(rq_server* s, uint64_t ticket, rq_request_status* st);

@@ -830,7 +830,7 @@ namespace {
 // any-hit stops at the first valid t.
 template <bool ANY>
 int32_t trace(const or_bvh* b, const RayPre& r, float* bt, float* bu, float* bv, uint32_t* vis,
-              uint32_t* tst) {
+              uint32_t* tst, uint32_t* max_stack = nullptr) {
   uint32_t n = b->n;
   int32_t best = -1;
   float t_best = r.tmax, u_best = 0, v_best = 0;
@@ -840,7 +840,7 @@ int32_t trace(const or_bvh* b, const RayPre& r, float* bt, float* bu, float* bv,
     return -1;
   }
   int32_t stack[128];
-  int sp = 0;
+  int sp = 0, sp_max = 0;
   int32_t node = n == 1 ? ~0 : 0;
   for (;;) {
     if (node >= 0) {
@@ -859,6 +859,7 @@ int32_t trace(const or_bvh* b, const RayPre& r, float* bt, float* bu, float* bv,
           stack[sp++] = c1;
           node = c0;
         }
+        sp_max = std::max(sp_max, sp);
         continue;
       } else if (h0) {
         node = c0;
@@ -895,9 +896,24 @@ int32_t trace(const or_bvh* b, const RayPre& r, float* bt, float* bu, float* bv,
   *bu = u_best, *bv = v_best;
   if (vis) *vis = visits;
   if (tst) *tst = tests;
+  if (max_stack) *max_stack = (uint32_t)sp_max;
   return best;
 }
 }  // namespace
+
+// The deepest stack of each ray's canonical closest-hit walk (entries held at
+// once: the far child is pushed when both children's boxes are hit).
+extern "C" void or_closest_bvh_stack(const or_bvh* b, const or_rays* rays, uint32_t nrays,
+                                     uint32_t* max_stack, int nthreads) {
+  parallel_for(nrays, nthreads, [&](uint32_t i) {
+    V3 o, d;
+    float tmin, tmax, t, u, v;
+    load_ray(rays, i, &o, &d, &tmin, &tmax);
+    max_stack[i] = 0;
+    if (tmin < tmax)
+      trace<false>(b, prep(o, d, tmin, tmax), &t, &u, &v, nullptr, nullptr, &max_stack[i]);
+  });
+}
 
 extern "C" void or_closest_bvh(const or_bvh* b, const or_rays* rays, uint32_t nrays, or_hits* hits,
                                uint32_t* visits, uint32_t* tests, int nthreads) {
@@ -1031,31 +1047,94 @@ extern "C" void or_camera_rays_philox(uint32_t w, uint32_t h, uint32_t spp, uint
       }
 }
 
+namespace {
+// Ray i of the uniform generator: origin uniform in [lo, hi) by Philox words
+// (seed, counter (i, 0, 0, 'ORIG')); direction uniform on S^2 by rejection
+// from the cube [-1,1)^3, at most 64 attempts (counter word 1 = attempt,
+// word 3 'DIR!'), then +z. r2 summed as ((x*x + y*y) + z*z).
+inline void uniform_ray(uint32_t i, uint64_t seed, const float lo[3], const float ext[3], V3* o3,
+                        V3* d3) {
+  uint32_t o[4];
+  philox(seed, i, 0u, 0u, 0x4F524947u, o);
+  *o3 = {lo[0] + ext[0] * u24(o[0]), lo[1] + ext[1] * u24(o[1]), lo[2] + ext[2] * u24(o[2])};
+  V3 d{0.0f, 0.0f, 1.0f};
+  for (uint32_t a = 0; a < 64; ++a) {
+    philox(seed, i, a, 1u, 0x44495221u, o);
+    float x = u24(o[0]) * 2.0f - 1.0f, y = u24(o[1]) * 2.0f - 1.0f, z = u24(o[2]) * 2.0f - 1.0f;
+    float r2 = x * x + y * y + z * z;
+    if (r2 > 1e-6f && r2 <= 1.0f) {
+      float f = 1 / std::sqrt(r2);
+      d = {x * f, y * f, z * f};
+      break;
+    }
+  }
+  *d3 = d;
+}
+}  // namespace
+
 extern "C" void or_uniform_rays(uint32_t n, uint64_t seed, const float lo[3], const float hi[3],
                                 float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
                                 float* tmin, float* tmax) {
-  float ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
-  for (uint32_t i = 0; i < n; ++i) {
+  or_uniform_rays_at(nullptr, n, seed, lo, hi, ox, oy, oz, dx, dy, dz, tmin, tmax);
+}
+
+extern "C" void or_uniform_rays_at(const uint32_t* ids, uint32_t n, uint64_t seed,
+                                   const float lo[3], const float hi[3], float* ox, float* oy,
+                                   float* oz, float* dx, float* dy, float* dz, float* tmin,
+                                   float* tmax) {
+  const float ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  for (uint32_t k = 0; k < n; ++k) {
+    V3 o, d;
+    uniform_ray(ids ? ids[k] : k, seed, lo, ext, &o, &d);
+    ox[k] = o.x, oy[k] = o.y, oz[k] = o.z;
+    dx[k] = d.x, dy[k] = d.y, dz[k] = d.z;
+    tmin[k] = 0.0f, tmax[k] = INFINITY;
+  }
+}
+
+// The serving form's pinhole camera (no reference counterpart: the reference
+// renders one fixed camera per process, main.cpp:541-558). Basis on the host:
+// f = normalize(look_at - eye), r = normalize(f x up), u = r x f, each
+// normalised by dividing by the float sqrt of ((x*x + y*y) + z*z). Image
+// w = max(1, width) by h = ceil(n / w); ray i goes through pixel
+// (i % w, i / w) jittered by the first two words of Philox(seed,
+// (i, 0, 0, 'PINH')) as 24-bit fractions:
+//   sx = ((2 * ((x + j0) * (1/w)) - 1) * tan_half) * (w/h),
+//   sy = (1 - 2 * ((y + j1) * (1/h))) * tan_half,
+//   d  = normalize((f + sx*r) + sy*u)  (per component, double-sqrt normalize),
+// tan_half = tanf(0.5f * fov_deg * 3.14159265358979f / 180.0f) (float, left to right).
+extern "C" void or_pinhole_rays_at(const uint32_t* ids, uint32_t count, uint32_t n,
+                                   uint64_t seed, const float eye[3], const float look_at[3],
+                                   const float up_in[3], float fov_deg, uint32_t width,
+                                   float* ox, float* oy, float* oz, float* dx, float* dy,
+                                   float* dz, float* tmin, float* tmax) {
+  auto nrm = [](float* v) {
+    const float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (l > 0.0f) v[0] /= l, v[1] /= l, v[2] /= l;
+  };
+  float f[3] = {look_at[0] - eye[0], look_at[1] - eye[1], look_at[2] - eye[2]};
+  nrm(f);
+  float r[3] = {f[1] * up_in[2] - f[2] * up_in[1], f[2] * up_in[0] - f[0] * up_in[2],
+                f[0] * up_in[1] - f[1] * up_in[0]};
+  nrm(r);
+  const float u[3] = {r[1] * f[2] - r[2] * f[1], r[2] * f[0] - r[0] * f[2],
+                      r[0] * f[1] - r[1] * f[0]};
+  const uint32_t w = std::max<uint32_t>(1, width);
+  const uint32_t h = std::max<uint32_t>(1, (uint32_t)(((uint64_t)n + w - 1) / w));
+  const float tan_half = std::tan(0.5f * fov_deg * 3.14159265358979f / 180.0f);
+  const float aspect = (float)w / (float)h, inv_w = 1.0f / (float)w, inv_h = 1.0f / (float)h;
+  for (uint32_t k = 0; k < count; ++k) {
+    const uint32_t i = ids ? ids[k] : k;
+    const uint32_t x = i % w, y = i / w;
     uint32_t o[4];
-    philox(seed, i, 0u, 0u, 0x4F524947u, o);
-    ox[i] = lo[0] + ex * u24(o[0]);
-    oy[i] = lo[1] + ey * u24(o[1]);
-    oz[i] = lo[2] + ez * u24(o[2]);
-    // Uniform direction on S^2 by rejection from the cube [-1,1)^3, at most
-    // 64 attempts (counter word 1 = attempt), then +z.
-    V3 d{0.0f, 0.0f, 1.0f};
-    for (uint32_t a = 0; a < 64; ++a) {
-      philox(seed, i, a, 1u, 0x44495221u, o);
-      float x = u24(o[0]) * 2.0f - 1.0f, y = u24(o[1]) * 2.0f - 1.0f, z = u24(o[2]) * 2.0f - 1.0f;
-      float r2 = x * x + y * y + z * z;
-      if (r2 > 1e-6f && r2 <= 1.0f) {
-        float f = 1 / std::sqrt(r2);
-        d = {x * f, y * f, z * f};
-        break;
-      }
-    }
-    dx[i] = d.x, dy[i] = d.y, dz[i] = d.z;
-    tmin[i] = 0.0f, tmax[i] = INFINITY;
+    philox(seed, i, 0u, 0u, 0x50494E48u, o);
+    const float sx = (2.0f * (((float)x + u24(o[0])) * inv_w) - 1.0f) * tan_half * aspect;
+    const float sy = (1.0f - 2.0f * (((float)y + u24(o[1])) * inv_h)) * tan_half;
+    const V3 d = normalize({(f[0] + sx * r[0]) + sy * u[0], (f[1] + sx * r[1]) + sy * u[1],
+                            (f[2] + sx * r[2]) + sy * u[2]});
+    ox[k] = eye[0], oy[k] = eye[1], oz[k] = eye[2];
+    dx[k] = d.x, dy[k] = d.y, dz[k] = d.z;
+    tmin[k] = 0.0f, tmax[k] = INFINITY;
   }
 }
 

@@ -114,6 +114,9 @@ uint32_t or_bvh_treelets(or_bvh* b);
 float or_treelet_dp(const float* boxes, int m, uint8_t* split, uint8_t* height);
 void or_closest_bvh(const or_bvh* b, const or_rays* rays, uint32_t nrays, or_hits* hits,
                     uint32_t* visits, uint32_t* tests, int nthreads);
+/* Deepest stack of each ray's canonical closest-hit walk. */
+void or_closest_bvh_stack(const or_bvh* b, const or_rays* rays, uint32_t nrays,
+                          uint32_t* max_stack, int nthreads);
 void or_any_bvh(const or_bvh* b, const or_rays* rays, uint32_t nrays, uint8_t* occ,
                 uint32_t* visits, uint32_t* tests, int nthreads);
 
@@ -131,6 +134,16 @@ void or_camera_rays_philox(uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
 void or_uniform_rays(uint32_t n, uint64_t seed, const float lo[3], const float hi[3],
                      float* ox, float* oy, float* oz, float* dx, float* dy, float* dz,
                      float* tmin, float* tmax);
+/* Rays ids[k] (k < n; ids NULL: rays 0..n-1) of or_uniform_rays, written at k. */
+void or_uniform_rays_at(const uint32_t* ids, uint32_t n, uint64_t seed, const float lo[3],
+                        const float hi[3], float* ox, float* oy, float* oz, float* dx, float* dy,
+                        float* dz, float* tmin, float* tmax);
+/* Rays ids[k] (k < count; ids NULL: rays 0..count-1) of the serving pinhole
+   camera for a request of n rays, written at k. */
+void or_pinhole_rays_at(const uint32_t* ids, uint32_t count, uint32_t n, uint64_t seed,
+                        const float eye[3], const float look_at[3], const float up[3],
+                        float fov_deg, uint32_t width, float* ox, float* oy, float* oz, float* dx,
+                        float* dy, float* dz, float* tmin, float* tmax);
 
 /* ---- shading (main.cpp:394-497, fresnel :85-103, reflect :218-221) ---- */
 typedef struct {

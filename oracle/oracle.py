@@ -77,12 +77,15 @@ def lib() -> C.CDLL:
             "or_treelet_dp": (f, [vp, i32, vp, vp]),
             "or_closest_bvh": (None, [vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp, vp, i32]),
             "or_any_bvh": (None, [vp, C.POINTER(Rays), u32, vp, vp, vp, i32]),
+            "or_closest_bvh_stack": (None, [vp, C.POINTER(Rays), u32, vp, i32]),
             "or_mt19937_jitter": (None, [u32, vp]),
             "or_mt19937_raw": (None, [u32, vp]),
             "or_philox4x32_10": (None, [vp, vp, vp]),
             "or_camera_rays_ref": (None, [u32, u32, u32, vp] + [vp] * 8),
             "or_camera_rays_philox": (None, [u32, u32, u32, u64] + [vp] * 8),
             "or_uniform_rays": (None, [u32, u64, vp, vp] + [vp] * 8),
+            "or_uniform_rays_at": (None, [vp, u32, u64, vp, vp] + [vp] * 8),
+            "or_pinhole_rays_at": (None, [vp, u32, u32, u64, vp, vp, vp, f, u32] + [vp] * 8),
             "or_render_samples": (None, [vp, vp, vp, vp, vp, C.POINTER(Rays), u32,
                                          C.POINTER(ShadeParams), vp, vp, vp, vp, vp, vp, i32]),
             "or_resolve_ppm": (None, [vp, u32, u32, vp]),
@@ -244,6 +247,14 @@ class OracleBVH:
                              _p(out["tests"]), threads)
         return out
 
+    def max_stack(self, rays: dict, threads: int = 8) -> np.ndarray:
+        """Deepest stack of each ray's canonical closest-hit walk."""
+        rs, keep = rays_struct(rays)
+        n = len(keep[0])
+        out = np.empty(n, np.uint32)
+        lib().or_closest_bvh_stack(self.h, C.byref(rs), n, _p(out), threads)
+        return out
+
     def any(self, rays: dict, threads: int = 8) -> dict:
         rs, keep = rays_struct(rays)
         n = len(keep[0])
@@ -305,6 +316,32 @@ def uniform_rays(n: int, seed: int, lo, hi) -> dict:
     out = _ray_out(n)
     lo3, hi3 = _f32(lo), _f32(hi)
     lib().or_uniform_rays(n, seed, _p(lo3), _p(hi3), *(_p(out[k]) for k in RAY_FIELDS))
+    return out
+
+
+def uniform_rays_at(ids, seed: int, lo, hi) -> dict:
+    """Rays ids[k] of uniform_rays(., seed, lo, hi) (any n above max(ids))."""
+    ids = np.ascontiguousarray(ids, np.uint32)
+    out = _ray_out(ids.size)
+    lo3, hi3 = _f32(lo), _f32(hi)
+    lib().or_uniform_rays_at(_p(ids), ids.size, seed, _p(lo3), _p(hi3),
+                             *(_p(out[k]) for k in RAY_FIELDS))
+    return out
+
+
+def pinhole_rays_at(ids, n: int, seed: int, eye, look_at, up=(0.0, 1.0, 0.0),
+                    fov_deg: float = 40.0, width: int = 1024) -> dict:
+    """Rays ids[k] of the serving pinhole camera for a request of n rays
+    (ids None: all n)."""
+    if ids is None:
+        ids_p, count = None, n
+    else:
+        ids = np.ascontiguousarray(ids, np.uint32)
+        ids_p, count = _p(ids), ids.size
+    out = _ray_out(count)
+    e, la, u = _f32(eye), _f32(look_at), _f32(up)
+    lib().or_pinhole_rays_at(ids_p, count, n, seed, _p(e), _p(la), _p(u), fov_deg, width,
+                             *(_p(out[k]) for k in RAY_FIELDS))
     return out
 
 

@@ -106,12 +106,26 @@ LaunchCtx launch_ctx(rq_ctx* c, void* stream) {
 
 uint32_t next_counter_slot(rq_ctx* c) { return c->counter_seq++ % COUNTER_SLOTS; }
 
-int32_t* stream_gstack(rq_ctx* c, void* stream) {
+int32_t* stream_gstack(rq_ctx* c, void* stream, bool may_alloc) {
   if (!stream) return c->gstack;
   for (auto& kv : c->gstack_streams)
     if (kv.first == stream) return kv.second;
+  if (!may_alloc) {
+    set_err(c, RQ_E_UNSUPPORTED,
+            "first traversal on a stream under graph capture: run one query on the stream "
+            "before capturing (its stack overflow area is allocated then)");
+    return nullptr;
+  }
   if (c->gstack_streams.size() >= MAX_TRACE_STREAMS) {
-    set_err(c, RQ_E_NOMEM, "traversal launches on more than %d streams", MAX_TRACE_STREAMS);
+    // take over the area of a stream whose work has all completed (a stream
+    // destroyed without rq_stream_forget reports an error: also free)
+    for (auto& kv : c->gstack_streams)
+      if (hipStreamQuery((hipStream_t)kv.first) != hipErrorNotReady) {
+        kv.first = stream;
+        return kv.second;
+      }
+    set_err(c, RQ_E_NOMEM, "traversal launches on more than %d busy streams",
+            MAX_TRACE_STREAMS);
     return nullptr;
   }
   void* p = nullptr;
@@ -214,11 +228,11 @@ int rq_create(rq_ctx** out, int device, size_t budget) {
     trace_variant_info(v, &blk, &depth, &ts);
     gmax = std::max(gmax, (size_t)c->trace_grid[v] * blk * (trace_max_stack(v) - depth));
   }
-  // counter slots + one trailing word of traversal error bits
-  const size_t cbytes = (COUNTER_SLOTS * 2 + 1) * sizeof(uint32_t);
+  // counter slots (eager, then capture) + one trailing word of traversal error bits
+  const size_t cbytes = ((COUNTER_SLOTS + CAPTURE_SLOTS) * 2 + 1) * sizeof(uint32_t);
   int rc = ctx_alloc(c, cbytes, (void**)&c->counters);
   if (rc == RQ_OK) {
-    c->errword = c->counters + COUNTER_SLOTS * 2;
+    c->errword = c->counters + (COUNTER_SLOTS + CAPTURE_SLOTS) * 2;
     hipMemset(c->counters, 0, cbytes);
     c->gstack_bytes = gmax * 2 * sizeof(int32_t);
     rc = ctx_alloc(c, c->gstack_bytes, (void**)&c->gstack);
@@ -554,12 +568,24 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
     trace_variant_info(variant, &blk, &depth, &ts, &wide);
   }
   a.wide = b->B.wide;
-  a.gstack = stream_gstack(c, stream);
-  if (!a.gstack) return RQ_E_NOMEM;
+  // An error left by an earlier call must not read as this launch's (the
+  // slot's base below advances only for a launch that was enqueued).
+  (void)hipGetLastError();
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing((hipStream_t)stream, &cap);
+  if (e != hipSuccess) return hip_err(c, e, "stream capture status");
+  const bool captured = cap == hipStreamCaptureStatusActive;
+  a.gstack = stream_gstack(c, stream, !captured);
+  if (!a.gstack) return captured ? RQ_E_UNSUPPORTED : RQ_E_NOMEM;
   a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
-  const uint32_t slot = next_counter_slot(c);
+  // Eager launches share a slot's monotonic counter and start from its value
+  // after the launches enqueued before them (the base). A base baked into a
+  // captured graph would be stale at the next replay, so a captured query
+  // takes a capture slot of its own, zeroed in the graph before the trace.
+  const uint32_t slot = captured ? COUNTER_SLOTS + (c->capture_seq++ % CAPTURE_SLOTS)
+                                 : next_counter_slot(c);
   a.counter = reinterpret_cast<unsigned long long*>(c->counters) + slot;
-  a.ctr_base = c->counter_base[slot];
+  a.ctr_base = captured ? 0ull : c->counter_base[slot];
   a.err = c->errword;
   // the launch grid (launch_trace takes the same), its waves and slice size
   a.grid = std::max(1u, std::min(c->trace_grid[variant], (n + (uint32_t)blk - 1) / (uint32_t)blk));
@@ -570,11 +596,15 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
                                     any ? c->slice_any : c->slice_closest));
   a.chunk = sched.chunk, a.chunk_small = sched.small, a.big_slices = sched.big;
   a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
-  hipError_t e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
+  if (captured) {
+    e = launch_counter_reset(launch_ctx(c, stream), a.counter);
+    if (e != hipSuccess) return hip_err(c, e, "counter reset");
+  }
+  e = launch_trace(launch_ctx(c, stream), a, any, counted, variant);
   if (e != hipSuccess) return hip_err(c, e, "trace launch");
   // the launch moves its slot's counter by its number of slices, or not at
   // all when the waves' first slices cover it (RayStream)
-  if (sched.total > waves) c->counter_base[slot] += sched.total;
+  if (!captured && sched.total > waves) c->counter_base[slot] += sched.total;
   return RQ_OK;
 }
 
@@ -784,10 +814,10 @@ int rq_stream_create(rq_ctx* c, void** stream) {
   *stream = (void*)s;
   return RQ_OK;
 }
-int rq_stream_destroy(rq_ctx* c, void* stream) {
-  if (check_ctx(c)) return RQ_E_INVALID;
-  // its overflow area goes with it, once its work is done (a later stream
-  // may get the same handle)
+int rq_stream_forget(rq_ctx* c, void* stream) {
+  if (check_ctx(c) || !stream) return RQ_E_INVALID;
+  // its overflow area is freed once its work is done (a later stream may get
+  // the same handle)
   hipError_t e = hipStreamSynchronize((hipStream_t)stream);
   for (size_t i = 0; i < c->gstack_streams.size(); ++i)
     if (c->gstack_streams[i].first == stream) {
@@ -795,7 +825,14 @@ int rq_stream_destroy(rq_ctx* c, void* stream) {
       c->gstack_streams.erase(c->gstack_streams.begin() + (long)i);
       break;
     }
-  if (e == hipSuccess) e = hipStreamDestroy((hipStream_t)stream);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream forget");
+}
+
+int rq_stream_destroy(rq_ctx* c, void* stream) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  const int rc = stream ? rq_stream_forget(c, stream) : RQ_OK;
+  if (rc) return rc;
+  const hipError_t e = hipStreamDestroy((hipStream_t)stream);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream destroy");
 }
 

@@ -28,6 +28,7 @@ struct rq_ctx {
   uint64_t counter_base[rq::COUNTER_SLOTS] = {};
   uint32_t* errword = nullptr;   // traversal error bits (stack deeper than provided)
   uint32_t counter_seq = 0;
+  uint32_t capture_seq = 0;      // next capture slot (after the COUNTER_SLOTS eager ones)
   // Traversal stack overflow ({node, distance} pairs), indexed by global
   // thread id: launches that may run at once need separate areas, so each
   // stream gets its own (the default stream's is allocated with the context).
@@ -75,10 +76,12 @@ LaunchCtx launch_ctx(rq_ctx* c, void* stream);
 // its waves take from the counter (RayStream), so the slot's next launch
 // starts from the counter's value after this one.
 uint32_t next_counter_slot(rq_ctx* c);
-// The stack overflow area of launches on `stream` (allocated on a stream's
-// first traversal launch; null with the context's error set when the budget
-// or the limit of MAX_TRACE_STREAMS streams is exceeded).
-int32_t* stream_gstack(rq_ctx* c, void* stream);
+// The stack overflow area of launches on `stream`, allocated on a stream's
+// first traversal launch (not under graph capture: `may_alloc` false then);
+// null with the context's error set when the budget or the limit of
+// MAX_TRACE_STREAMS streams is exceeded, or when allocation is not allowed.
+// At the limit, the area of a stream with no work left is taken over.
+int32_t* stream_gstack(rq_ctx* c, void* stream, bool may_alloc = true);
 // Build buffers for n primitives; `wide`: also the 4-wide node array and the
 // chunk-union refit buffers (the standalone build and blob load; batch and
 // server slots walk binary nodes and refit by frontier lists).

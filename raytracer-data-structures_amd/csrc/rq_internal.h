@@ -86,6 +86,11 @@ constexpr int TRACE_MAX_VARIANTS = 16;
 constexpr int TRACE_DEFAULT_CLOSEST = 9;
 constexpr int TRACE_DEFAULT_ANY = 14;
 constexpr int COUNTER_SLOTS = 64;
+// Counter slots of queries enqueued on a stream under graph capture: each
+// captured query zeroes its own slot in the graph before its trace, so every
+// replay starts from 0 (round robin: at most CAPTURE_SLOTS captured queries
+// per context may run at once).
+constexpr int CAPTURE_SLOTS = 64;
 constexpr int MAX_TRACE_STREAMS = 16;  // streams besides the default one with traversal launches
 constexpr uint32_t TRACE_RAY_CHUNK = 1024;      // default largest wave slice of the ray stream
 constexpr uint32_t TRACE_RAY_CHUNK_MAX = 4096;  // largest that may be configured
@@ -216,6 +221,9 @@ void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide = 
 int trace_max_stack(int variant);  // total stack entries per lane (LDS + overflow)
 uint32_t trace_grid_blocks(int device, int variant);
 hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64_t bytes);
+// Zeroes one ray-stream counter with a one-thread kernel (a memset node would
+// not be safe to replay in a captured graph).
+hipError_t launch_counter_reset(const LaunchCtx& lc, unsigned long long* counter);
 
 // Profiler hooks (rq_api.cpp)
 void prof_begin(Profiler* p, hipStream_t s, const char* name, hipEvent_t* ev);

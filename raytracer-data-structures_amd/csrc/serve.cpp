@@ -90,7 +90,8 @@ struct Slot {
   rq_bvh bvh;
   uint32_t vcap = 0, tcap = 0, n = 0;
   bool loaded = false;
-  uint32_t pins = 0;  // queued or admitted requests that trace this slot
+  bool resident = false;  // loaded by rq_server_load_mesh: never recycled
+  uint32_t pins = 0;      // queued or admitted requests that trace this slot
   uint64_t last_use = 0;
 };
 
@@ -109,6 +110,8 @@ struct Req {
   uint8_t* occ = nullptr;
   hipEvent_t prep = nullptr, out = nullptr;
   bool pinned = false;        // holds a pin on `slot`
+  bool resident = false;      // a mesh load of rq_server_load_mesh (a slot of its own)
+  std::atomic<bool> cancel{false};  // rq_server_cancel: finish as RQ_E_CANCELED
   bool ready = false;         // rays (and a new mesh) are on the device
   uint32_t assigned = 0;      // rays handed to steps
   uint32_t pending = 0;       // chunks in steps not yet retired
@@ -168,6 +171,7 @@ struct rq_server {
   std::deque<uint64_t> finished;                          // tickets not yet taken by take_done
   std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;  // every known request (owned)
   uint64_t next_ticket = 1;
+  uint32_t resident_slots = 0;  // slots taken (or claimed by queued loads) by load_mesh
   bool stop = false;
   std::thread th;
   rq_server_stats stats{};
@@ -196,23 +200,31 @@ void publish(rq_server* s, Req* q, int32_t state) {
       ++s->stats.failed;
     else
       ++s->stats.completed;
+    const bool had_slot = q->pinned;
     if (q->pinned) {
       --s->slots[q->slot].pins;
       s->slots[q->slot].last_use = ++s->use_clock;
       q->pinned = false;
+    }
+    if (q->resident && state < 0) {
+      --s->resident_slots;  // a load that did not happen gives its claim back
+      if (had_slot) s->slots[q->slot].resident = false, s->slots[q->slot].loaded = false;
     }
     s->finished.push_back(q->ticket);
     s->cv_done.notify_all();
   }
 }
 
-// A slot for a new mesh: an empty one, else the least recently used idle one.
+// A slot for a new mesh: an empty one, else the least recently used idle
+// slot that holds a per-request upload. A resident mesh (rq_server_load_mesh)
+// keeps its slot for the server's life, so a slot id a caller holds always
+// names the mesh it loaded.
 int pick_slot(rq_server* s) {
   std::lock_guard<std::mutex> g(s->mu);
   int best = -1;
   for (size_t i = 0; i < s->slots.size(); ++i) {
     const Slot& sl = s->slots[i];
-    if (sl.pins) continue;
+    if (sl.pins || sl.resident) continue;
     if (!sl.loaded) return (int)i;
     if (best < 0 || sl.last_use < s->slots[best].last_use) best = (int)i;
   }
@@ -263,6 +275,12 @@ void admit(rq_server* s) {
        wi != s->waiting.end() && s->admitted.size() < s->cfg.max_inflight;) {
     Req* q = *wi;
     const rq_request& r = q->r;
+    if (q->cancel.load()) {
+      wi = s->waiting.erase(wi);
+      q->t_done = now_s();
+      publish(s, q, RQ_E_CANCELED);
+      continue;
+    }
     size_t off = 0;
     if (q->bytes && !s->arena.alloc(q->bytes, &off)) break;
     if (r.new_mesh) {
@@ -301,6 +319,7 @@ void admit(rq_server* s) {
       {
         std::lock_guard<std::mutex> g(s->mu);
         sl.loaded = true;
+        sl.resident = q->resident;
       }
       sl.n = m.ntri;
       e = hipMemcpyAsync(sl.vx, m.vx, 4ull * m.nv, hipMemcpyHostToDevice, s->s_prep);
@@ -359,7 +378,7 @@ bool launch_step(rq_server* s) {
   st->chunks.clear();
   uint64_t total = 0;
   for (Req* q : s->admitted) {
-    if (!q->ready || q->err || q->assigned >= q->r.nrays) continue;
+    if (!q->ready || q->err || q->cancel.load() || q->assigned >= q->r.nrays) continue;
     if (st->chunks.size() >= s->cfg.max_inflight || total >= s->cfg.step_rays) break;
     const uint32_t c = std::min<uint32_t>(q->r.nrays - q->assigned, s->cfg.chunk_rays);
     st->chunks.push_back(Chunk{q, q->assigned, c});
@@ -492,7 +511,8 @@ void retire_steps(rq_server* s) {
       --q->pending;
       if (bad) q->err = e != hipSuccess ? RQ_E_HIP : RQ_E_DEVICE;
       rays += (uint64_t)c.count * ((q->prim ? 1 : 0) + (q->occ ? 1 : 0));
-      if (q->pending == 0 && q->assigned >= q->r.nrays && !q->out_enqueued && !q->err)
+      if (q->pending == 0 && q->assigned >= q->r.nrays && !q->out_enqueued && !q->err &&
+          !q->cancel.load())
         enqueue_out(s, q, st->end);
     }
     {
@@ -514,6 +534,8 @@ void complete(rq_server* s) {
     if (q->ready && q->pending == 0) {
       if (q->err || q->r.nrays == 0) {
         done = true;  // failed, or a mesh load without rays
+      } else if (q->cancel.load() && !q->out_enqueued) {
+        q->err = RQ_E_CANCELED, done = true;  // no step holds it and no copy was queued
       } else if (q->out_enqueued) {
         const hipError_t e = hipEventQuery(q->out);
         if (e == hipSuccess)
@@ -740,7 +762,13 @@ int rq_server_destroy(rq_server* s) {
   return RQ_OK;
 }
 
-int rq_server_submit(rq_server* s, const rq_request* r, uint64_t* ticket) {
+}  // extern "C"
+
+namespace {
+
+// resident: a mesh load of rq_server_load_mesh, which takes a slot for the
+// server's life; other new-mesh requests upload into recycled slots.
+int submit(rq_server* s, const rq_request* r, uint64_t* ticket, bool resident) {
   if (!s || !r || !ticket) return RQ_E_INVALID;
   const bool closest = r->query & RQ_Q_CLOSEST, any = r->query & RQ_Q_ANY;
   if ((r->query & ~(RQ_Q_CLOSEST | RQ_Q_ANY)) || (r->nrays && !closest && !any) ||
@@ -778,23 +806,55 @@ int rq_server_submit(rq_server* s, const rq_request* r, uint64_t* ticket) {
       return srv_err(s, RQ_E_HIP, "request events");
   }
   std::lock_guard<std::mutex> g(s->mu);
-  if (s->stop) return RQ_E_INVALID;
+  auto refuse = [&](int code, const char* why) {
+    (void)hipEventDestroy(q->prep);
+    (void)hipEventDestroy(q->out);
+    s->err = why;
+    return code;
+  };
+  if (s->stop) return refuse(RQ_E_INVALID, "server stopping");
   if (!r->new_mesh) {
-    if (r->mesh >= s->slots.size() || !s->slots[r->mesh].loaded) {
-      (void)hipEventDestroy(q->prep);
-      (void)hipEventDestroy(q->out);
-      s->err = "request: mesh slot holds no mesh";
-      return RQ_E_INVALID;
-    }
+    if (r->mesh >= s->slots.size() || !s->slots[r->mesh].loaded || !s->slots[r->mesh].resident)
+      return refuse(RQ_E_INVALID,
+                    "request: mesh slot holds no resident mesh (an uploaded mesh serves only "
+                    "the request that brought it)");
     q->slot = r->mesh;
     q->pinned = true;
     ++s->slots[r->mesh].pins;
+  } else if (s->resident_slots >= s->slots.size()) {
+    return refuse(RQ_E_NOMEM, resident ? "every mesh slot holds a resident mesh"
+                                       : "no mesh slot for uploads: every slot holds a "
+                                         "resident mesh");
+  } else if (resident) {
+    ++s->resident_slots;
+    q->resident = true;
   }
   q->ticket = s->next_ticket++;
   *ticket = q->ticket;
   s->incoming.push_back(q.get());
   s->reqs.emplace(q->ticket, std::move(q));
   ++s->stats.submitted;
+  s->cv_loop.notify_one();
+  return RQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rq_server_submit(rq_server* s, const rq_request* r, uint64_t* ticket) {
+  return submit(s, r, ticket, false);
+}
+
+int rq_server_cancel(rq_server* s, uint64_t ticket) {
+  if (!s) return RQ_E_INVALID;
+  std::lock_guard<std::mutex> g(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return RQ_E_INVALID;
+  Req* q = it->second.get();
+  if (q->state == RQ_REQ_DONE || q->state < 0) return RQ_E_INVALID;  // already finished
+  if (q->resident) return RQ_E_INVALID;  // a mesh load of rq_server_load_mesh
+  q->cancel.store(true);
   s->cv_loop.notify_one();
   return RQ_OK;
 }
@@ -865,7 +925,7 @@ int rq_server_load_mesh(rq_server* s, const rq_mesh* m, uint32_t* slot) {
   rq_request r{};
   r.new_mesh = m;
   uint64_t t = 0;
-  int rc = rq_server_submit(s, &r, &t);
+  int rc = submit(s, &r, &t, true);
   if (rc) return rc;
   rq_request_status st{};
   rc = rq_server_wait(s, t, &st);

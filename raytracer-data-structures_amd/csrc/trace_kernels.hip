@@ -29,6 +29,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                                                            gridDim.x * BLOCK);
 }
 
+__global__ __launch_bounds__(64) void k_counter_reset(unsigned long long* ctr) {
+  if (threadIdx.x == 0) *ctr = 0ull;
+}
+
 typedef void (*TraceFn)(TraceArgs);
 struct Variant {
   int block, depth;
@@ -77,6 +81,11 @@ uint32_t trace_grid_blocks(int device, int v) {
   if (cus <= 0) cus = 256;
   if (per <= 0) per = 2;
   return (uint32_t)(cus * per);
+}
+
+hipError_t launch_counter_reset(const LaunchCtx& lc, unsigned long long* counter) {
+  hipLaunchKernelGGL(k_counter_reset, dim3(1), dim3(64), 0, lc.stream, counter);
+  return hipGetLastError();
 }
 
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,

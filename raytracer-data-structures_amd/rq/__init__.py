@@ -46,9 +46,14 @@ MAT_DIFFUSE_AND_GLOSSY = 0
 MAT_REFLECTION_AND_REFRACTION = 1
 MAT_REFLECTION = 2
 
+PRIM_TRIANGLE = 0
+PRIM_SPHERE = 1
+
 RQ_OK = 0
+RQ_E_INVALID, RQ_E_NOMEM, RQ_E_HIP, RQ_E_IO = -1, -2, -3, -4
+RQ_E_NODEVICE, RQ_E_UNSUPPORTED, RQ_E_DEVICE, RQ_E_CANCELED = -5, -6, -7, -8
 ERRORS = {-1: "RQ_E_INVALID", -2: "RQ_E_NOMEM", -3: "RQ_E_HIP", -4: "RQ_E_IO",
-          -5: "RQ_E_NODEVICE", -6: "RQ_E_UNSUPPORTED", -7: "RQ_E_DEVICE"}
+          -5: "RQ_E_NODEVICE", -6: "RQ_E_UNSUPPORTED", -7: "RQ_E_DEVICE", -8: "RQ_E_CANCELED"}
 
 
 class RQError(RuntimeError):
@@ -198,6 +203,7 @@ def _load() -> C.CDLL:
         "rq_device_errors": (i32, [vp, C.POINTER(u32)]),
         "rq_stream_create": (i32, [vp, C.POINTER(vp)]),
         "rq_stream_destroy": (i32, [vp, vp]),
+        "rq_stream_forget": (i32, [vp, vp]),
         "rq_profile_enable": (i32, [vp, i32]),
         "rq_profile_reset": (i32, [vp]),
         "rq_profile_only": (i32, [vp, C.c_char_p]),
@@ -234,6 +240,8 @@ def _load() -> C.CDLL:
         "rq_bvh_load": (i32, [vp, vp, vp, C.c_size_t, C.POINTER(vp), vp]),
         "rq_bvh_save_file": (i32, [vp, vp, C.c_char_p]),
         "rq_bvh_load_file": (i32, [vp, vp, C.c_char_p, C.POINTER(vp)]),
+        "rq_bvh_blob_validate": (i32, [vp, C.c_size_t, i32, vp, vp, vp, u32, vp, vp, u32,
+                                       C.c_char_p, C.c_size_t]),
         "rq_gen_reference_scene": (i32, [C.c_char_p, u32, u32, C.POINTER(Spheres)]),
         "rq_spheres_free": (None, [C.POINTER(Spheres)]),
         "rq_batch_create": (i32, [vp, C.POINTER(BatchItem), u32, i32, C.POINTER(vp)]),
@@ -256,6 +264,7 @@ def _load() -> C.CDLL:
         "rq_server_poll": (i32, [vp, u64, C.POINTER(RequestStatus)]),
         "rq_server_wait": (i32, [vp, u64, C.POINTER(RequestStatus)]),
         "rq_server_forget": (i32, [vp, u64]),
+        "rq_server_cancel": (i32, [vp, u64]),
         "rq_server_take_done": (i32, [vp, vp, vp, u32, C.POINTER(u32)]),
         "rq_server_stats_get": (i32, [vp, C.POINTER(ServerStats)]),
         "rq_server_last_error": (C.c_char_p, [vp]),
@@ -707,6 +716,10 @@ class Context:
     def stream_destroy(self, s) -> None:
         self._check(lib().rq_stream_destroy(self.h, s))
 
+    def stream_forget(self, s) -> None:
+        """rq_stream_forget: free the overflow area of a caller-owned stream."""
+        self._check(lib().rq_stream_forget(self.h, s))
+
     # traversal variants + autotuner
     @staticmethod
     def trace_variants() -> list[dict]:
@@ -999,6 +1012,10 @@ class Server:
         self._check(lib().rq_server_forget(self.h, ticket))
         self._keep.pop(ticket, None)
 
+    def cancel(self, ticket: int) -> None:
+        """rq_server_cancel: the request finishes with state RQ_E_CANCELED."""
+        self._check(lib().rq_server_cancel(self.h, ticket))
+
     def stats(self) -> dict:
         st = ServerStats()
         self._check(lib().rq_server_stats_get(self.h, C.byref(st)))
@@ -1069,3 +1086,22 @@ def write_ppm(path: str | Path, w: int, h: int, data: np.ndarray) -> None:
     rc = lib().rq_write_ppm(str(path).encode(), w, h, _ptr(data))
     if rc != RQ_OK:
         raise RQError(rc, f"write_ppm {path}")
+
+
+def bvh_blob_validate(blob: bytes, prims) -> tuple[int, str]:
+    """rq_bvh_blob_validate (host only, no device): (status, reason) of a BVH
+    blob against a triangle mesh (MeshArrays) or a sphere set (SphereArrays)."""
+    b = np.frombuffer(blob, np.uint8)
+    msg = C.create_string_buffer(256)
+    if hasattr(prims, "cx"):
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (prims.cx, prims.cy, prims.cz, prims.r)]
+        rc = lib().rq_bvh_blob_validate(_ptr(b), b.size, PRIM_SPHERE, _ptr(arrs[0]), _ptr(arrs[1]),
+                                        _ptr(arrs[2]), arrs[0].size, None, _ptr(arrs[3]),
+                                        arrs[0].size, msg, 256)
+    else:
+        arrs = [np.ascontiguousarray(a, np.float32) for a in (prims.vx, prims.vy, prims.vz)]
+        idx = np.ascontiguousarray(prims.idx, np.uint32)
+        rc = lib().rq_bvh_blob_validate(_ptr(b), b.size, PRIM_TRIANGLE, _ptr(arrs[0]),
+                                        _ptr(arrs[1]), _ptr(arrs[2]), arrs[0].size, _ptr(idx),
+                                        None, idx.size // 3, msg, 256)
+    return rc, msg.value.decode()

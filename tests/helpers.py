@@ -78,3 +78,73 @@ def mesh_configs():
         "soup4097": lambda: random_soup(4097, 3, spread=1.5, size=0.02),
         "ico5": lambda: rq.icosphere(5, 2, center=(0, 0, -5)),
     }
+
+
+# ---- BVH blob written from the oracle's tree (the rq_bvh_save format) ----
+BLOB_MAGIC = b"RQBVHBL1"
+
+
+def fnv1a64(data: bytes, h: int = 0xCBF29CE484222325) -> int:
+    for b in data:
+        h = ((h ^ b) * 0x100000001B3) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _prims_fingerprint(prims) -> int:
+    h = 0xCBF29CE484222325
+    if hasattr(prims, "cx"):
+        for a in (prims.cx, prims.cy, prims.cz, prims.r):
+            h = fnv1a64(np.ascontiguousarray(a, np.float32).tobytes(), h)
+        return h
+    for a in (prims.vx, prims.vy, prims.vz):
+        h = fnv1a64(np.ascontiguousarray(a, np.float32).tobytes(), h)
+    return fnv1a64(np.ascontiguousarray(prims.idx, np.uint32).tobytes(), h)
+
+
+def blob_payload(prims, arrays: dict, height: int) -> bytearray:
+    """The blob payload of a tree given as the oracle's arrays (codes,
+    sorted_idx, left, right, boxes[2n-1, 6]: internal nodes, then leaves)."""
+    codes, sidx = arrays["codes"], arrays["sorted_idx"]
+    left, right, boxes = arrays["left"], arrays["right"], arrays["boxes"].reshape(-1, 6)
+    n = sidx.size
+    ni = n - 1
+    parent = np.full(2 * n - 1, -1, np.int32)
+    nodes = np.zeros((ni, 16), np.float32)
+    nv = nodes.view(np.int32)
+    for i in range(ni):
+        for side, c in enumerate((int(left[i]), int(right[i]))):
+            u = c if c >= 0 else ni + ~c
+            parent[u] = 2 * i + side
+            nodes[i, 8 * side:8 * side + 3] = boxes[u, :3]
+            nodes[i, 8 * side + 4:8 * side + 7] = boxes[u, 3:]
+        nv[i, 3], nv[i, 7] = left[i], right[i]
+    tris = np.zeros((n, 12), np.float32)
+    tv = tris.view(np.int32)
+    p = sidx.astype(np.int64)
+    if hasattr(prims, "cx"):
+        r = np.asarray(prims.r, np.float32)[p]
+        tris[:, 0], tris[:, 1], tris[:, 2] = (np.asarray(a, np.float32)[p]
+                                              for a in (prims.cx, prims.cy, prims.cz))
+        tris[:, 4], tris[:, 5] = r, r * r
+    else:
+        f = np.asarray(prims.idx, np.int64).reshape(-1, 3)[p]
+        v = np.stack([np.asarray(a, np.float32) for a in (prims.vx, prims.vy, prims.vz)], 1)
+        v0, v1, v2 = v[f[:, 0]], v[f[:, 1]], v[f[:, 2]]
+        tris[:, 0:3], tris[:, 4:7], tris[:, 8:11] = v0, v1 - v0, v2 - v0
+    tv[:, 3] = sidx.astype(np.int32)
+    out = bytearray()
+    for a in (codes.astype(np.uint32), sidx.astype(np.uint32), parent, nodes, tris,
+              boxes[0].astype(np.float32), np.array([height], np.uint32)):
+        out += np.ascontiguousarray(a).tobytes()
+    return out
+
+
+def blob_from_payload(prims, payload: bytes, mode: int = 1) -> bytes:
+    n = prims.n if hasattr(prims, "cx") else prims.ntri
+    kind = 1 if hasattr(prims, "cx") else 0
+    elems = n if kind else prims.nv
+    hdr = (BLOB_MAGIC + np.array([1, kind, mode, n, elems, 0], np.uint32).tobytes() +
+           np.array([_prims_fingerprint(prims), len(payload), fnv1a64(bytes(payload)), 0],
+                    np.uint64).tobytes())
+    assert len(hdr) == 64
+    return hdr + bytes(payload)

@@ -93,3 +93,58 @@ def test_blob_refusals(ctx, rq):
     finally:
         ctx.release(b)
         ctx.release(s)
+
+
+@pytest.mark.parametrize("which", ["ico5", "spheres", "tri1"])
+def test_saved_blob_equals_the_oracle_tree(ctx, rq, O, which):
+    """rq_bvh_save writes, byte for byte, the blob of the oracle's tree
+    (helpers.blob_payload: codes, order, parent links, node records, leaf
+    records, root box, height, FNV-1a 64 checksum and scene fingerprint)."""
+    from helpers import blob_from_payload, blob_payload
+
+    log(f"blob vs oracle {which}")
+    if which == "spheres":
+        rng = np.random.default_rng(9)
+        c = rng.uniform(-2, 2, (2000, 3)).astype(np.float32)
+        prims = rq.SphereArrays(c[:, 0].copy(), c[:, 1].copy(), c[:, 2].copy(),
+                                rng.uniform(0.02, 0.2, 2000).astype(np.float32),
+                                np.zeros(6000, np.float32))
+        s = ctx.upload_spheres(prims)
+    else:
+        prims = (rq.icosphere(5, 2, center=(0, 0, -5)) if which == "ico5"
+                 else random_soup(1, 5, spread=0.2, size=1.0))
+        s = ctx.upload_scene(prims)
+    b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
+    try:
+        blob = ctx.bvh_save(b)
+        ob = O.OracleBVH(prims, rq.MORTON_SCENE_AABB)
+        mine = blob_from_payload(prims, blob_payload(prims, ob.arrays(), ob.height))
+        assert blob == mine
+        assert rq.bvh_blob_validate(blob, prims)[0] == rq.RQ_OK
+    finally:
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_crafted_blob_refused_before_upload(ctx, rq):
+    """A blob whose checksum was recomputed after an edit (a child link out
+    of range) is refused by the host check of rq_bvh_load (RQ_E_IO); the
+    device never sees it."""
+    from helpers import fnv1a64
+
+    mesh = rq.icosphere(3, 7, center=(0, 0, -5))
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
+    try:
+        blob = bytearray(ctx.bvh_save(b))
+        n = mesh.ntri
+        link0 = 64 + 4 * n + 4 * n + 4 * (2 * n - 1) + 12  # node 0, left link
+        blob[link0:link0 + 4] = np.int32(5 * n).tobytes()
+        blob[48:56] = np.uint64(fnv1a64(bytes(blob[64:]))).tobytes()
+        with pytest.raises(rq.RQError) as e:
+            ctx.bvh_load(s, bytes(blob))
+        assert e.value.code == rq.RQ_E_IO
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.release(b)
+        ctx.release(s)

@@ -263,16 +263,37 @@ def test_autotune_picks_a_valid_variant(ctx, rq):
         ctx.release(s)
 
 
+def _hip():
+    """The HIP runtime, for stream and graph calls around the library's own."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    vp = C.c_void_p
+    for name, args in (("hipStreamCreate", [C.POINTER(vp)]), ("hipStreamDestroy", [vp]),
+                       ("hipStreamBeginCapture", [vp, C.c_int]),
+                       ("hipStreamEndCapture", [vp, C.POINTER(vp)]),
+                       ("hipGraphInstantiate", [C.POINTER(vp), vp, vp, vp, C.c_size_t]),
+                       ("hipGraphLaunch", [vp, vp]), ("hipGraphExecDestroy", [vp]),
+                       ("hipGraphDestroy", [vp])):
+        getattr(hip, name).argtypes = args
+        getattr(hip, name).restype = C.c_int
+    return hip
+
+
 def test_concurrent_streams_deep_stacks(ctx, O, rq):
-    """Queries on two streams at once, in a variant with 12 LDS stack entries
-    over a scene of large overlapping triangles, so that rays spill their
-    stacks to the global overflow area in both launches: each stream has an
-    area of its own, every answer equals the oracle's."""
-    mesh = random_soup(6000, 27, spread=1.0, size=0.8)
+    """Queries on two streams at once over a scene of large overlapping
+    triangles, in binary variants with 12 (entry distances kept) and 16 LDS
+    stack entries: the oracle's walk, which pushes exactly where the binary
+    kernel does, needs more than 16 entries on tens of thousands of these
+    rays, so the launches spill to the global overflow area; each stream has
+    an area of its own, and every answer equals the oracle's."""
+    mesh = random_soup(60000, 27, spread=1.0, size=1.0)
     ra = concat_rays(rays_toward(30000, 81), random_rays(20000, 82))
     rb = concat_rays(rays_toward(30000, 83), random_rays(20000, 84))
     ob = O.OracleBVH(mesh, 1)
     oa, obb = ob.closest(ra), ob.closest(rb)
+    deep = [(ob.max_stack(r) > 16).sum() for r in (ra, rb)]
+    assert min(deep) > 1000, deep
     s = ctx.upload_scene(mesh)
     b = ctx.build_lbvh(s, 1)
     da, db = rq.DeviceRays.from_host(ctx, ra), rq.DeviceRays.from_host(ctx, rb)
@@ -280,19 +301,23 @@ def test_concurrent_streams_deep_stacks(ctx, O, rq):
     hb = {k: rq.DeviceArray(ctx, 4 * db.n) for k in ("prim", "t")}
     st = [ctx.stream_create(), ctx.stream_create()]
     saved = ctx.trace_config()
-    v12 = [v["index"] for v in ctx.trace_variants() if v["lds_depth"] == 12][0]
+    vs = ctx.trace_variants()
+    v12 = [v["index"] for v in vs if v["lds_depth"] == 12][0]
+    v16 = [v["index"] for v in vs if v["lds_depth"] == 16 and not v["tstack"] and not v["wide"]][0]
     try:
-        ctx.trace_config(v12, 0, 64)
-        for rep in range(4):
-            for h in (ha, hb):
-                h["prim"].fill(0x7F)
-            ctx.closest_hit_dev(b, da, ha, stream=st[0])
-            ctx.closest_hit_dev(b, db, hb, stream=st[1])
-            ctx.sync(st[0])
-            ctx.sync(st[1])
-            for h, o, n in ((ha, oa, da.n), (hb, obb, db.n)):
-                assert np.array_equal(h["prim"].download(np.int32, n), o["prim"]), rep
-                assert np.array_equal(_bits(h["t"].download(np.float32, n)), _bits(o["t"])), rep
+        for var in (v12, v16):
+            ctx.trace_config(var, 0, 64)
+            for rep in range(3):
+                log(f"deep stacks: variant {var} rep {rep}")
+                for h in (ha, hb):
+                    h["prim"].fill(0x7F)
+                ctx.closest_hit_dev(b, da, ha, stream=st[0])
+                ctx.closest_hit_dev(b, db, hb, stream=st[1])
+                ctx.sync(st[0])
+                ctx.sync(st[1])
+                for h, o, n in ((ha, oa, da.n), (hb, obb, db.n)):
+                    assert np.array_equal(h["prim"].download(np.int32, n), o["prim"]), rep
+                    assert np.array_equal(_bits(h["t"].download(np.float32, n)), _bits(o["t"]))
         assert ctx.device_errors() == 0
     finally:
         ctx.trace_config(*saved)
@@ -303,6 +328,133 @@ def test_concurrent_streams_deep_stacks(ctx, O, rq):
         for h in (ha, hb):
             for a in h.values():
                 a.free()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_seventeenth_caller_stream(ctx, rq):
+    """Caller-owned streams beyond the per-context limit of areas (16): a
+    query on a new stream takes over the area of a stream whose work has
+    completed instead of failing; rq_stream_forget frees an area."""
+    import ctypes as C
+
+    hip = _hip()
+    mesh = random_soup(500, 29, spread=1.5)
+    rays = rays_toward(4000, 86)
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    d = rq.DeviceRays.from_host(ctx, rays)
+    h = {k: rq.DeviceArray(ctx, 4 * d.n) for k in ("prim", "t")}
+    ctx.closest_hit_dev(b, d, h)
+    ctx.sync()
+    ref = h["prim"].download(np.int32, d.n)
+    streams = []
+    try:
+        for k in range(24):
+            x = C.c_void_p()
+            assert hip.hipStreamCreate(C.byref(x)) == 0
+            streams.append(x)
+            h["prim"].fill(0x7F)
+            ctx.closest_hit_dev(b, d, h, stream=x.value)
+            ctx.sync(x.value)
+            assert np.array_equal(h["prim"].download(np.int32, d.n), ref), k
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.sync()
+        for x in streams:
+            ctx.stream_forget(x.value)
+            hip.hipStreamDestroy(x)
+        d.free()
+        for a in h.values():
+            a.free()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_captured_query_replays(ctx, O, rq):
+    """Closest-hit and any-hit queries captured into a HIP graph on a caller
+    stream, replayed four times with eager queries on the same stream and
+    synchronisations between the replays: every replay equals the oracle
+    (a captured query zeroes its own ray-stream counter inside the graph)."""
+    import ctypes as C
+
+    hip = _hip()
+    mesh = rq.icosphere(6, 5, center=(0, 0, -5))
+    rays = concat_rays(rays_toward(300000, 87), random_rays(200000, 88))
+    ob = O.OracleBVH(mesh, 1)
+    oc, oa = ob.closest(rays), ob.any(rays)["occ"]
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    d = rq.DeviceRays.from_host(ctx, rays)
+    h = {k: rq.DeviceArray(ctx, 4 * d.n) for k in ("prim", "t")}
+    occ = rq.DeviceArray(ctx, d.n)
+    he = {k: rq.DeviceArray(ctx, 4 * d.n) for k in ("prim", "t")}
+    st = ctx.stream_create()
+    graph, exe = C.c_void_p(), C.c_void_p()
+    try:
+        ctx.closest_hit_dev(b, d, he, stream=st)  # the stream's overflow area exists
+        ctx.sync(st)
+        assert hip.hipStreamBeginCapture(C.c_void_p(st), 0) == 0  # hipStreamCaptureModeGlobal
+        ctx.closest_hit_dev(b, d, h, stream=st)
+        ctx.any_hit_dev(b, d, occ, stream=st)
+        assert hip.hipStreamEndCapture(C.c_void_p(st), C.byref(graph)) == 0
+        assert hip.hipGraphInstantiate(C.byref(exe), graph, None, None, 0) == 0
+        for rep in range(4):
+            log(f"captured query replay {rep}")
+            h["prim"].fill(0x7F)
+            occ.fill(0x7F)
+            ctx.sync()
+            assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+            ctx.closest_hit_dev(b, d, he, stream=st)  # eager work between the replays
+            ctx.sync(st)
+            assert np.array_equal(h["prim"].download(np.int32, d.n), oc["prim"]), rep
+            assert np.array_equal(_bits(h["t"].download(np.float32, d.n)), _bits(oc["t"])), rep
+            assert np.array_equal(occ.download(np.uint8, d.n), oa), rep
+            assert np.array_equal(he["prim"].download(np.int32, d.n), oc["prim"]), rep
+        assert ctx.device_errors() == 0
+    finally:
+        if exe.value:
+            hip.hipGraphExecDestroy(exe)
+        if graph.value:
+            hip.hipGraphDestroy(graph)
+        ctx.stream_destroy(st)
+        d.free()
+        occ.free()
+        for a in list(h.values()) + list(he.values()):
+            a.free()
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_capture_on_a_new_stream_is_refused(ctx, rq):
+    """The first query on a stream allocates its overflow area, which is not
+    allowed under capture: RQ_E_UNSUPPORTED, and the capture can be ended."""
+    import ctypes as C
+
+    hip = _hip()
+    mesh = random_soup(300, 30, spread=1.5)
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    d = rq.DeviceRays.from_host(ctx, rays_toward(1000, 89))
+    h = {k: rq.DeviceArray(ctx, 4 * d.n) for k in ("prim", "t")}
+    st = ctx.stream_create()
+    graph = C.c_void_p()
+    try:
+        assert hip.hipStreamBeginCapture(C.c_void_p(st), 2) == 0  # relaxed mode
+        with pytest.raises(rq.RQError) as e:
+            ctx.closest_hit_dev(b, d, h, stream=st)
+        assert e.value.code == rq.RQ_E_UNSUPPORTED
+        hip.hipStreamEndCapture(C.c_void_p(st), C.byref(graph))
+        if graph.value:
+            hip.hipGraphDestroy(graph)
+        ctx.closest_hit_dev(b, d, h, stream=st)  # eager: fine
+        ctx.sync(st)
+        assert (h["prim"].download(np.int32, d.n) >= 0).any()
+    finally:
+        ctx.stream_destroy(st)
+        d.free()
+        for a in h.values():
+            a.free()
         ctx.release(b)
         ctx.release(s)
 

@@ -194,3 +194,76 @@ def test_serve_admission_and_refusals(rq):
             srv.load_mesh(random_soup(70000, 3))
         with pytest.raises(rq.RQError):
             srv.submit(10, rq.Q_CLOSEST, prim=np.empty(10, np.int32), mesh=1, box=box)
+
+
+def test_serve_resident_slots_are_never_recycled(rq):
+    """Meshes of rq_server_load_mesh keep their slots: uploads recycle only
+    the slots beyond them, a request may not name an upload's slot, and a
+    server whose slots all hold resident meshes refuses uploads."""
+    log("serve: resident slots")
+    cfg = dict(SMALL, mesh_slots=3)
+    box = ((-2.0, -2.0, -7.0), (2.0, 2.0, -3.0))
+    with rq.Server(0, **cfg) as srv:
+        meshes = [rq.icosphere(3, 40 + k, center=(0, 0, -5)) for k in range(2)]
+        slots = [srv.load_mesh(m) for m in meshes]
+        ref = []
+        for sl in slots:
+            prim = np.empty(5000, np.int32)
+            srv.wait(srv.submit(5000, rq.Q_CLOSEST, prim=prim, mesh=sl, box=box, seed=7))
+            ref.append(prim)
+        up_slots = set()
+        for k in range(4):  # uploads through the one free slot, none evicts a resident mesh
+            prim = np.empty(3000, np.int32)
+            st = srv.wait(srv.submit(3000, rq.Q_CLOSEST, prim=prim,
+                                     new_mesh=random_soup(2000 + k, 60 + k, spread=1.5),
+                                     box=box, seed=k))
+            assert st.state == rq.REQ_DONE
+            up_slots.add(st.mesh)
+        assert len(up_slots) == 1 and not up_slots & set(slots)
+        for sl, r in zip(slots, ref):
+            prim = np.empty(5000, np.int32)
+            srv.wait(srv.submit(5000, rq.Q_CLOSEST, prim=prim, mesh=sl, box=box, seed=7))
+            assert np.array_equal(prim, r)
+        with pytest.raises(rq.RQError) as e:  # an upload's slot is not a resident mesh
+            srv.submit(10, rq.Q_CLOSEST, prim=np.empty(10, np.int32), mesh=up_slots.pop(),
+                       box=box)
+        assert e.value.code == rq.RQ_E_INVALID
+        srv.load_mesh(rq.icosphere(2, 3, center=(0, 0, -5)))  # the last slot becomes resident
+        with pytest.raises(rq.NoMemory):
+            srv.submit(10, rq.Q_CLOSEST, prim=np.empty(10, np.int32),
+                       new_mesh=random_soup(100, 1), box=box)
+        with pytest.raises(rq.NoMemory):
+            srv.load_mesh(rq.icosphere(2, 4, center=(0, 0, -5)))
+
+
+def test_serve_cancel(rq):
+    """Cancelled requests finish as RQ_E_CANCELED, queued ones at once and a
+    running one once its launches retire; the others complete with their
+    alone results, and the server keeps serving."""
+    log("serve: cancel")
+    cfg = dict(SMALL, max_inflight=2, chunk_rays=1 << 12, step_rays=1 << 13)
+    box = ((-2.0, -2.0, -7.0), (2.0, 2.0, -3.0))
+    with rq.Server(0, **cfg) as srv:
+        slot = srv.load_mesh(rq.icosphere(5, 9, center=(0, 0, -5)))
+        n = 60000
+        outs = [np.full(n, -7, np.int32) for _ in range(8)]
+        tks = [srv.submit(n, rq.Q_CLOSEST, prim=outs[k], mesh=slot, box=box, seed=200 + k)
+               for k in range(8)]
+        cancelled = {1, 5, 6}
+        for k in sorted(cancelled):
+            srv.cancel(tks[k])
+        st = [srv.wait(t) for t in tks]
+        for k in range(8):
+            if k in cancelled:
+                assert st[k].state in (rq.RQ_E_CANCELED, rq.REQ_DONE), (k, st[k].state)
+            else:
+                assert st[k].state == rq.REQ_DONE, (k, st[k].state)
+        assert st[5].state == rq.RQ_E_CANCELED and st[6].state == rq.RQ_E_CANCELED  # queued
+        with pytest.raises(rq.RQError):
+            srv.cancel(tks[0])  # finished
+        for k in (0, 2, 7):
+            prim = np.empty(n, np.int32)
+            srv.wait(srv.submit(n, rq.Q_CLOSEST, prim=prim, mesh=slot, box=box, seed=200 + k))
+            assert np.array_equal(prim, outs[k])
+        s = srv.stats()
+        assert s["failed"] >= 2

@@ -65,3 +65,77 @@ def test_uniform_rays(O):
     assert np.abs((d ** 2).mean(axis=0) - 1 / 3).max() < 0.01
     r2 = O.uniform_rays(1000, 7, lo, hi)
     assert np.array_equal(r2["dx"], r["dx"][:1000])  # counter-based: prefix stable
+
+
+def test_uniform_rays_at_equals_full_generator(O):
+    lo, hi = (-1.0, 2.0, 0.0), (3.0, 4.0, 10.0)
+    full = O.uniform_rays(50000, 9, lo, hi)
+    ids = np.random.default_rng(1).choice(50000, 3000, replace=False).astype(np.uint32)
+    at = O.uniform_rays_at(ids, 9, lo, hi)
+    for k in O.RAY_FIELDS:
+        assert np.array_equal(at[k].view(np.uint32), full[k][ids].view(np.uint32)), k
+
+
+def test_uniform_ray_origin_counter_layout(O):
+    """Origin x of ray i is lo + ext * (word0 >> 8) / 2^24 of Philox(counter
+    (i, 0, 0, 'ORIG'), key seed): recomputed in float32 from the published-
+    vector-pinned Philox."""
+    lo, hi, seed = (-1.0, 2.0, 0.0), (3.0, 4.0, 10.0), 0x1234567890
+    ids = np.array([0, 1, 77, 4095, 1 << 20], np.uint32)
+    r = O.uniform_rays_at(ids, seed, lo, hi)
+    for k, i in enumerate(ids):
+        w = O.philox([int(i), 0, 0, 0x4F524947], [seed & 0xFFFFFFFF, seed >> 32])
+        for a, f in enumerate(("ox", "oy", "oz")):
+            frac = np.float32(w[a] >> 8) * np.float32(1.0 / 16777216.0)
+            e = np.float32(lo[a]) + np.float32(np.float32(hi[a]) - np.float32(lo[a])) * frac
+            assert r[f][k] == e, (i, f)
+
+
+def _pinhole_pixels(r, eye, look, up, fov, w, h):
+    """Image coordinates (px, py) at which each ray crosses the image plane,
+    from an independent float64 camera basis."""
+    f = np.asarray(look, np.float64) - np.asarray(eye, np.float64)
+    f /= np.linalg.norm(f)
+    rt = np.cross(f, np.asarray(up, np.float64))
+    rt /= np.linalg.norm(rt)
+    u = np.cross(rt, f)
+    d = np.stack([r["dx"], r["dy"], r["dz"]], 1).astype(np.float64)
+    th = np.tan(np.radians(fov) / 2)
+    sx = (d @ rt) / (d @ f)
+    sy = (d @ u) / (d @ f)
+    px = (sx / (th * w / h) + 1) / 2 * w
+    py = (1 - sy / th) / 2 * h
+    return px, py
+
+
+def test_pinhole_rays_pixel_footprint_and_jitter(O):
+    """Ray i crosses the image plane inside pixel (i % w, i / w) of a w x
+    ceil(n/w) image (row-major, y down, right = f x up), at the jitter given
+    by the first two Philox words of counter (i, 0, 0, 'PINH')."""
+    eye, look, up, fov, w = (1.0, 2.0, 3.0), (1.5, 1.0, -4.0), (0.0, 1.0, 0.0), 40.0, 64
+    n = 64 * 48 - 5                     # ragged last row
+    h = -(-n // w)
+    seed = 77
+    r = O.pinhole_rays_at(None, n, seed, eye, look, up, fov, w)
+    d = np.stack([r["dx"], r["dy"], r["dz"]], 1).astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    assert (r["ox"] == np.float32(eye[0])).all() and (r["oz"] == np.float32(eye[2])).all()
+    assert (r["tmin"] == 0).all() and np.isinf(r["tmax"]).all()
+    px, py = _pinhole_pixels(r, eye, look, up, fov, w, h)
+    i = np.arange(n)
+    jx, jy = px - i % w, py - i // w
+    assert (jx > -1e-3).all() and (jx < 1 + 1e-3).all()
+    assert (jy > -1e-3).all() and (jy < 1 + 1e-3).all()
+    for k in (0, 1, 63, 64, 1000, n - 1):
+        wd = O.philox([k, 0, 0, 0x50494E48], [seed, 0])
+        assert abs(jx[k] - (wd[0] >> 8) / 2.0 ** 24) < 2e-3, k
+        assert abs(jy[k] - (wd[1] >> 8) / 2.0 ** 24) < 2e-3, k
+
+
+def test_pinhole_rays_at_equals_full_generator(O):
+    args = (5000, 3, (0.0, 0.0, 4.0), (0.1, -0.2, 0.0), (0.0, 1.0, 0.0), 35.0, 100)
+    full = O.pinhole_rays_at(None, *args)
+    ids = np.array([0, 99, 100, 2500, 4999], np.uint32)
+    at = O.pinhole_rays_at(ids, *args)
+    for k in O.RAY_FIELDS:
+        assert np.array_equal(at[k].view(np.uint32), full[k][ids].view(np.uint32)), k
