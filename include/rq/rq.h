@@ -176,6 +176,20 @@ int rq_closest_hit(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n
                    void* stream);
 int rq_any_hit(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, uint8_t* occluded,
                void* stream);
+/* rq_closest_hit that allocates nothing. It has no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture on a stream that never queried, tight serving loops, harnesses that
+ * forbid hipMalloc). Its scratch (the ray-stream counter and the traversal
+ * stack overflow area) is a static device array of 64 MiB per device, held
+ * from library load until the process ends, instead of the per-stream area a
+ * first query on a stream allocates; the launch grid is capped so that every
+ * thread's overflow entries fit (any ray count is accepted; large launches
+ * run on fewer waves than rq_closest_hit). The counter is zeroed by a kernel
+ * at the start of each call. Every call on one device shares the array:
+ * calls that overlap on a device (other streams, other contexts) race and
+ * must be ordered by the caller. Results equal rq_closest_hit's. */
+int rq_closest_hit_noalloc(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                           rq_hits* hits, void* stream);
 /* Instrumented variants: per-ray node-record visits and triangle tests of the
  * canonical traversal (device uint32 arrays). Slower; for measurement. */
 int rq_closest_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,

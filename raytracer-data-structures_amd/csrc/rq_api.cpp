@@ -237,6 +237,12 @@ int rq_create(rq_ctx** out, int device, size_t budget) {
     c->gstack_bytes = gmax * 2 * sizeof(int32_t);
     rc = ctx_alloc(c, c->gstack_bytes, (void**)&c->gstack);
   }
+  if (rc == RQ_OK) {
+    void* sp = nullptr;
+    const hipError_t e = noalloc_scratch_address(&sp);
+    if (e != hipSuccess || !sp) rc = hip_err(c, e, "scratch symbol address");
+    c->scratch = static_cast<unsigned char*>(sp);
+  }
   if (rc == RQ_OK && hipDeviceSynchronize() != hipSuccess) rc = RQ_E_HIP;
   if (rc) {
     rq_destroy(c);
@@ -539,9 +545,12 @@ int rq_bvh_export_wide(rq_ctx* c, const rq_bvh* b, float* boxes, int32_t* links)
   return RQ_OK;
 }
 
+// noalloc: the allocation-free entry (rq_closest_hit_noalloc): the device
+// scratch array in place of the stream's overflow area and counter slot, the
+// grid capped to fit it, no profiling events.
 static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
                  uint8_t* occ, uint32_t* visits, uint32_t* tests, bool any, bool counted,
-                 void* stream) {
+                 void* stream, bool noalloc = false) {
   if (check_ctx(c) || !b || b->magic != BVH_MAGIC || !rays)
     return set_err(c, RQ_E_INVALID, "bad argument");
   if (n == 0) return RQ_OK;
@@ -575,9 +584,29 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   hipError_t e = hipStreamIsCapturing((hipStream_t)stream, &cap);
   if (e != hipSuccess) return hip_err(c, e, "stream capture status");
   const bool captured = cap == hipStreamCaptureStatusActive;
+  a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
+  if (noalloc) {
+    const size_t per_block = (size_t)blk * a.gdepth * 2 * sizeof(int32_t);
+    const size_t blocks = (NOALLOC_SCRATCH_BYTES - NOALLOC_COUNTER_BYTES) / per_block;
+    if (blocks == 0) return set_err(c, RQ_E_UNSUPPORTED, "variant too large for the scratch");
+    a.gstack = reinterpret_cast<int32_t*>(c->scratch + NOALLOC_COUNTER_BYTES);
+    a.counter = reinterpret_cast<unsigned long long*>(c->scratch);
+    a.ctr_base = 0ull;
+    a.err = c->errword;
+    a.grid = std::max(1u, std::min({c->trace_grid[variant], (uint32_t)blocks,
+                                     (n + (uint32_t)blk - 1) / (uint32_t)blk}));
+    a.refill_idle = c->refill_closest;
+    const SliceSchedule sched = slice_schedule(
+        n, trace_chunk(n, a.grid * (uint32_t)blk / 64u, c->slice_closest));
+    a.chunk = sched.chunk, a.chunk_small = sched.small, a.big_slices = sched.big;
+    a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
+    const LaunchCtx lc{(hipStream_t)stream, nullptr};
+    e = launch_counter_reset(lc, a.counter);
+    if (e == hipSuccess) e = launch_trace(lc, a, false, false, variant);
+    return e == hipSuccess ? RQ_OK : hip_err(c, e, "trace launch (no-alloc)");
+  }
   a.gstack = stream_gstack(c, stream, !captured);
   if (!a.gstack) return captured ? RQ_E_UNSUPPORTED : RQ_E_NOMEM;
-  a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
   // Eager launches share a slot's monotonic counter and start from its value
   // after the launches enqueued before them (the base). A base baked into a
   // captured graph would be stale at the next replay, so a captured query
@@ -611,6 +640,10 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
 int rq_closest_hit(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
                    void* stream) {
   return query(c, b, rays, n, hits, nullptr, nullptr, nullptr, false, false, stream);
+}
+int rq_closest_hit_noalloc(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                           rq_hits* hits, void* stream) {
+  return query(c, b, rays, n, hits, nullptr, nullptr, nullptr, false, false, stream, true);
 }
 int rq_any_hit(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, uint8_t* occ,
                void* stream) {

@@ -35,6 +35,9 @@ struct rq_ctx {
   int32_t* gstack = nullptr;
   size_t gstack_bytes = 0;
   std::vector<std::pair<void*, int32_t*>> gstack_streams;
+  // device address of the allocation-free entry's scratch on this device
+  // (hipGetSymbolAddress at rq_create)
+  unsigned char* scratch = nullptr;
 };
 
 constexpr uint32_t SCENE_MAGIC = 0x53435131u;  // "SCQ1"

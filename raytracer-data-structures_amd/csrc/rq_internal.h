@@ -224,6 +224,14 @@ hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64
 // Zeroes one ray-stream counter with a one-thread kernel (a memset node would
 // not be safe to replay in a captured graph).
 hipError_t launch_counter_reset(const LaunchCtx& lc, unsigned long long* counter);
+// Scratch of the allocation-free closest-hit entry: NOALLOC_SCRATCH_BYTES per
+// device (a static device array), its first NOALLOC_COUNTER_BYTES the
+// counter, the rest the overflow area; the launch grid is capped so that the
+// overflow area of every thread fits.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 64ull << 20;
+constexpr size_t NOALLOC_COUNTER_BYTES = 256;
+// The device address of the scratch on the current device.
+hipError_t noalloc_scratch_address(void** ptr);
 
 // Profiler hooks (rq_api.cpp)
 void prof_begin(Profiler* p, hipStream_t s, const char* name, hipEvent_t* ev);

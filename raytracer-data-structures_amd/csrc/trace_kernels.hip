@@ -29,6 +29,15 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                                                            gridDim.x * BLOCK);
 }
 
+// Device scratch of the allocation-free closest-hit entry
+// (rq_closest_hit_noalloc): the launch's ray-stream counter in the first 256
+// bytes, then the traversal stack overflow area ([depth][thread] pairs). One
+// copy per device, held from module load until the process ends; its address
+// comes from hipGetSymbolAddress (noalloc_scratch_address), never from the
+// array's name in host code. Calls that overlap on one device share it and
+// race.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
 __global__ __launch_bounds__(64) void k_counter_reset(unsigned long long* ctr) {
   if (threadIdx.x == 0) *ctr = 0ull;
 }
@@ -81,6 +90,11 @@ uint32_t trace_grid_blocks(int device, int v) {
   if (cus <= 0) cus = 256;
   if (per <= 0) per = 2;
   return (uint32_t)(cus * per);
+}
+
+hipError_t noalloc_scratch_address(void** ptr) {
+  *ptr = nullptr;
+  return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch));
 }
 
 hipError_t launch_counter_reset(const LaunchCtx& lc, unsigned long long* counter) {
