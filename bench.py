@@ -8,18 +8,23 @@ every hit; the 30-bit Morton LBVH is rebuilt every frame. One step = one
 frame: LBVH rebuild + raygen + closest hit (primary) + shade + closest hit
 (bounce) + shade + any hit (shadow) + resolve, inputs resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3]
+    python bench.py [--steps K] [--warmup W] [--config c2|c1|c3|c4|c5]
 
-Prints one JSON line: Mrays/s over all rays traced per frame, the live
-HIP-event roofline of the dominant kernel, and the reference CPU path
-(median-split BVH + collect-then-test query, restated in oracle/port.cpp)
-timed on this host on a bounded sample of the same rays.
+Prints one JSON line: Mrays/s over all rays traced per frame; the roofline of
+the dominant kernel and of the closest-hit kernel (algorithmic bytes per
+launch / HIP-event launch time against 8 TB/s, with the measured HBM traffic
+and issue counters of rocprofv3 --pmc child runs); the LBVH build time; the
+p50 / p99 latency of a short serving run (64 resident meshes, Poisson
+requests); and the reference CPU path (restated in oracle/port.cpp) timed on
+this host at 1 thread and at this process's core share on a bounded sample of
+the same rays.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import signal
 import sys
 import time
 from pathlib import Path
@@ -33,18 +38,54 @@ sys.path.insert(0, str(ROOT))
 import rq  # noqa: E402
 from rq import workloads as W  # noqa: E402
 
+METRIC = "Mrays/s closest-hit (+% HBM roofline); LBVH build ms; serving p50/p99 latency"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+MI355X_CUS, MI355X_XCDS = 256, 8
 # LBVH build bytes per triangle of the reference byte model: Morton 56 (index +
 # vertex gather in, code + index out), sort 68 (4 passes x 16 + histogram), Karras
 # 28, refit 124
 BUILD_MODEL_BYTES_PER_TRI = 276
+# Algorithmic bytes of one LBVH build per triangle in the batch kernel (SCENE_AABB
+# Morton map): bounds 48 (3 indices + 9 vertex floats), Morton 52 (same + code),
+# sort 60 (pass 0: key in, key + index out; passes 1-3: key + index in and out),
+# Karras 28 (key, two child links, two parent words, leaf range), refit 176
+# (sorted index, 3 indices, 9 floats, 48 B triangle record, 64 B node boxes,
+# arrival and height words).
+BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
+# Traversal configuration per workload (closest variant, refill, any variant,
+# refill, closest slice, any slice): the fastest of a sweep of every variant and
+# refill threshold on MI355X (--sweep); fixed so that a profile of the same
+# command times the same kernels. --autotune times the variants again instead.
+TRACE_CONFIG = {"c1": None, "c2": "14,64,14,64,2048,1024", "c3": "14,16,9,16,2048,2048"}
+ALG_BASIS = ("algorithmic bytes of the canonical traversal (rays in, hits out, 32 B per node "
+             "box tested, 36 B per triangle tested); node and triangle reads are mostly served "
+             "from L2 and the Infinity Cache, so the fraction can exceed 1")
 
 
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-# ---------------------------------------------------------------- workloads
+def cpu_share() -> int:
+    """Host cores this process may run on, at most 16 (one GPU's share of the
+    GPU box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def counted_batch(ctx, bvh, rays, any_hit: bool):
     """Traversal counts of the canonical traversal for one ray batch."""
     n = rays.n
@@ -70,14 +111,10 @@ def counted_batch(ctx, bvh, rays, any_hit: bool):
     return int(v.sum()), int(t.sum())
 
 
-COPY_PROBE_BYTES = 1 << 30
-KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<true, false",
-                 "batch": "k_batch<"}
-
-
+# ----------------------------------------------------------- traversal config
 def parse_trace_config(text: str) -> tuple[int, ...]:
-    """--trace-config: closest variant, refill, any variant, refill and
-    optionally the largest ray-stream slices (closest, any; default 1024)."""
+    """closest variant, refill, any variant, refill and optionally the largest
+    ray-stream slices (closest, any; default 1024)."""
     t = tuple(int(x) for x in text.split(","))
     if len(t) == 4:
         t += (1024, 1024)
@@ -93,51 +130,52 @@ def apply_trace_config(ctx, cfg) -> None:
     ctx.trace_slice(sc, sa)
 
 
-def tune_trace(ctx, bvh, closest_rays, any_rays, args, repeats: int):
-    """The traversal configuration (closest variant, refill, any variant,
-    refill, closest slice, any slice): --trace-config fixes it (no trial
-    launches, e.g. under a profiler), --no-autotune keeps the defaults, else
-    the autotuner picks it."""
+def choose_trace(ctx, bvh, closest_rays, any_rays, args, repeats: int):
+    """--trace-config, else --autotune (time every variant on these rays), else
+    the workload's fixed configuration (none: the context's defaults)."""
     if args.trace_config:
         cfg = parse_trace_config(args.trace_config)
-    elif args.no_autotune:
-        return None
-    else:
+    elif args.autotune:
         vc, rc, _ = ctx.autotune(bvh, closest_rays, any_hit=False, repeats=repeats)
         va, ra, _ = ctx.autotune(bvh, any_rays, any_hit=True, repeats=repeats)
         cfg = (vc, rc, va, ra) + ctx.trace_slice()
+    elif TRACE_CONFIG.get(args.config):
+        cfg = parse_trace_config(TRACE_CONFIG[args.config])
+    else:
+        return None
     apply_trace_config(ctx, cfg)
     return cfg
 
 
-def tuned_dict(ctx, cfg) -> dict:
+def trace_dict(ctx, cfg, how: str) -> dict | None:
+    if not cfg:
+        return None
     vc, rc, va, ra, sc, sa = cfg
-    return {"closest_variant": ctx.trace_variants()[vc], "any_variant": ctx.trace_variants()[va],
-            "refill_idle": {"closest": rc, "any": ra}, "slice": {"closest": sc, "any": sa}}
-
-
-# options every rocprofv3 child run of this script repeats (set by main)
-CHILD_ARGS: list = []
-# kernel names timed inside a frame-loop timed region, and the untimed frames
-# that give the per-kernel breakdown
-TRACE_KERNELS = ["trace_closest", "trace_any"]
-BREAKDOWN_FRAMES = 3
-
-
-def rebuild(ctx, bvh, args) -> None:
-    """One LBVH rebuild, then the treelet rounds of --treelets (if any)."""
-    ctx.rebuild(bvh)
-    if args.treelets:
-        ctx.treelets(bvh, args.treelets)
+    return {"chosen_by": how, "closest_variant": ctx.trace_variants()[vc],
+            "any_variant": ctx.trace_variants()[va], "refill_idle": {"closest": rc, "any": ra},
+            "slice": {"closest": sc, "any": sa}, "as_argument": ",".join(map(str, cfg))}
 
 
 def cfg_arg(t) -> str | None:
     return ",".join(map(str, t)) if t else None
 
 
+# ------------------------------------------------------- rocprofv3 counters
+COPY_PROBE_BYTES = 1 << 30
+KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<true, false",
+                 "batch": "k_batch<"}
+COPY_SYMBOL = "k_copy16"
+# Three passes: HBM bytes read, HBM bytes written, and the issue / cache counters
+# (at most 4 TCC, 8 SQ and 2 GRBM counters per pass).
+PMC_PASSES = ("FETCH_SIZE", "WRITE_SIZE",
+              "TCC_HIT_sum TCC_MISS_sum TCC_REQ_sum SQ_INSTS_VALU SQ_ACTIVE_INST_VALU "
+              "SQ_THREAD_CYCLES_VALU SQ_WAVE_CYCLES SQ_WAIT_INST_ANY GRBM_GUI_ACTIVE")
+CHILD_ARGS: list = []  # options every child run of this script repeats (set by main)
+
+
 def copy_probe(ctx) -> None:
     """A library copy kernel over a known byte count: the calibration of the
-    FETCH_SIZE / WRITE_SIZE counters."""
+    FETCH_SIZE / WRITE_SIZE / TCC_REQ counters."""
     src = rq.DeviceArray(ctx, COPY_PROBE_BYTES)
     dst = rq.DeviceArray(ctx, COPY_PROBE_BYTES)
     ctx.copy(dst, src, COPY_PROBE_BYTES)
@@ -146,9 +184,10 @@ def copy_probe(ctx) -> None:
     dst.free()
 
 
-def _pmc_run(counters: str, config: str, extra: list | None = None) -> dict | None:
-    """One rocprofv3 --pmc child run of this script in probe mode: for each
-    counter, {kernel name: (mean value per dispatch, dispatches)}."""
+def _pmc_run(counters: str, config: str, extra: list) -> dict | None:
+    """One rocprofv3 --pmc child run of this script in probe mode (its own
+    process group, removed whole afterwards): {counter: {kernel: (mean per
+    dispatch, dispatches)}}."""
     import csv
     import glob
     import shutil
@@ -161,16 +200,24 @@ def _pmc_run(counters: str, config: str, extra: list | None = None) -> dict | No
     out = tempfile.mkdtemp(prefix="rq_pmc_", dir="/tmp")
     cmd = [prof, "--pmc", *counters.split(), "--output-format", "csv", "-d", out, "-o", "probe",
            "--", sys.executable, str(ROOT / "bench.py"), "--pmc-probe", "--config", config,
-           "--no-cpu", "--traffic", "off", "--warmup", "1", *CHILD_ARGS, *(extra or [])]
+           "--no-cpu", "--no-serve", "--traffic", "off", "--warmup", "1", *CHILD_ARGS, *extra]
     env = dict(os.environ, TMPDIR="/tmp")
+    proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                            stderr=subprocess.DEVNULL, start_new_session=True)
     try:
-        subprocess.run(cmd, cwd="/tmp", env=env, timeout=240, check=True,
-                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
-    except (subprocess.SubprocessError, OSError) as e:
-        log(f"pmc {counters}: {e}")
-        return None
+        rc = proc.wait(timeout=240)
+    except subprocess.TimeoutExpired:
+        rc = None
+    finally:
+        try:  # whatever the profiler left in its group
+            os.killpg(proc.pid, signal.SIGKILL)
+        except (ProcessLookupError, PermissionError):
+            pass
+        proc.wait()
     files = glob.glob(f"{out}/**/*counter_collection.csv", recursive=True)
     if not files:
+        log(f"pmc {counters}: no counter file (rc {rc})")
+        shutil.rmtree(out, ignore_errors=True)
         return None
     sums, counts = {}, {}
     with open(files[0]) as f:
@@ -191,58 +238,443 @@ def _pick(per_kernel: dict, sym: str):
     return max(hits)[1] if hits else None
 
 
-def measure_traffic(config: str, kernel: str, trace_cfg: str | None = None):
-    """HBM bytes per launch of `kernel`: FETCH_SIZE and WRITE_SIZE per dispatch,
-    each scaled by known bytes / counter reading of the copy probe."""
-    sym = KERNEL_SYMBOL.get(kernel)
-    if sym is None:
-        return None
-    total = 0.0
-    detail = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
-        res = _pmc_run(counter, config, ["--trace-config", trace_cfg] if trace_cfg else None)
-        if not res or counter not in res:
-            return None
-        kv, cv = _pick(res[counter], sym), _pick(res[counter], "k_copy16")
-        if kv is None or not cv:
-            return None
-        scale = (COPY_PROBE_BYTES / 1024.0) / cv
-        nbytes = kv * 1024.0 * scale
-        detail[counter] = {"kib_per_dispatch": round(kv, 1), "copy_scale": round(scale, 4),
-                           "bytes": int(nbytes)}
-        total += nbytes
-    log(f"traffic {kernel}: {detail}")
-    return int(total)
-
-
-# One pass: L2 hits and misses, VALU issue and lane cycles, GPU-active cycles.
-COUNTER_SET = "TCC_HIT_sum TCC_MISS_sum SQ_ACTIVE_INST_VALU SQ_THREAD_CYCLES_VALU GRBM_GUI_ACTIVE"
-MI355X_CUS, MI355X_XCDS = 256, 8
-
-
-def measure_counters(config: str, kernel: str, trace_cfg: str | None = None):
-    """L2 hit rate, VALU busy and VALU lane utilisation of `kernel` from one
-    rocprofv3 --pmc pass: hits / (hits + misses); SQ_ACTIVE_INST_VALU per CU over
-    one XCD's GPU-active cycles (GRBM_GUI_ACTIVE reads the sum over the eight
-    XCDs; one wave64 VALU instruction per CU per cycle is the issue peak); lanes
-    active per VALU instruction, SQ_THREAD_CYCLES_VALU / (64 * SQ_ACTIVE_INST_VALU)."""
-    sym = KERNEL_SYMBOL.get(kernel)
-    if sym is None:
-        return None
-    res = _pmc_run(COUNTER_SET, config, ["--trace-config", trace_cfg] if trace_cfg else None)
-    names = COUNTER_SET.split()
-    if not res or any(c not in res for c in names):
-        return None
-    v = {c: _pick(res[c], sym) for c in names}
-    if any(x is None for x in v.values()) or not v["SQ_ACTIVE_INST_VALU"]:
-        return None
-    hit, miss = v["TCC_HIT_sum"], v["TCC_MISS_sum"]
-    cycles = v["GRBM_GUI_ACTIVE"] / MI355X_XCDS
-    out = {"l2_hit": round(hit / max(hit + miss, 1.0), 4),
-           "valu_busy": round(v["SQ_ACTIVE_INST_VALU"] / MI355X_CUS / max(cycles, 1.0), 4),
-           "valu_lane_util": round(v["SQ_THREAD_CYCLES_VALU"] / (64 * v["SQ_ACTIVE_INST_VALU"]), 4)}
-    log(f"counters {kernel}: {out}")
+def measure_pmc(config: str, kernels, trace_cfg: str | None) -> dict:
+    """Per kernel: HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, each scaled by
+    known bytes / counter reading of the 1 GiB copy probe) and issue / cache
+    figures: L2 hit rate; VALU issue (wave64 VALU instructions per CU per
+    cycle, peak 1); lanes active per VALU instruction; the share of wave
+    cycles spent waiting on memory counters; L2 request bytes per launch
+    (TCC_REQ_sum scaled by the copy probe's bytes per request)."""
+    extra = ["--trace-config", trace_cfg] if trace_cfg else []
+    runs = [_pmc_run(p, config, extra) for p in PMC_PASSES]
+    out = {}
+    need = ("TCC_HIT_sum", "TCC_MISS_sum", "TCC_REQ_sum", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU",
+            "SQ_THREAD_CYCLES_VALU", "SQ_WAVE_CYCLES", "SQ_WAIT_INST_ANY", "GRBM_GUI_ACTIVE")
+    for k in kernels:
+        sym = KERNEL_SYMBOL[k]
+        d = {}
+        f, w, c = runs
+        if f and w and "FETCH_SIZE" in f and "WRITE_SIZE" in w:
+            kf, cf = _pick(f["FETCH_SIZE"], sym), _pick(f["FETCH_SIZE"], COPY_SYMBOL)
+            kw, cw = _pick(w["WRITE_SIZE"], sym), _pick(w["WRITE_SIZE"], COPY_SYMBOL)
+            if None not in (kf, cf, kw, cw) and cf and cw:
+                sf, sw = (COPY_PROBE_BYTES / 1024.0) / cf, (COPY_PROBE_BYTES / 1024.0) / cw
+                d["traffic"] = int(kf * 1024.0 * sf + kw * 1024.0 * sw)
+                d["traffic_detail"] = {"fetch_kib": round(kf, 1), "fetch_scale": round(sf, 4),
+                                       "write_kib": round(kw, 1), "write_scale": round(sw, 4)}
+        if c and all(n in c for n in need):
+            v = {n: _pick(c[n], sym) for n in need}
+            if all(x is not None for x in v.values()):
+                cycles = v["GRBM_GUI_ACTIVE"] / MI355X_XCDS
+                copy_req = _pick(c["TCC_REQ_sum"], COPY_SYMBOL)
+                req_bytes = (2.0 * COPY_PROBE_BYTES / copy_req) if copy_req else None
+                d["counters"] = {
+                    "l2_hit": round(v["TCC_HIT_sum"] / max(v["TCC_HIT_sum"] + v["TCC_MISS_sum"],
+                                                           1.0), 4),
+                    "valu_issue": round(v["SQ_INSTS_VALU"] / MI355X_CUS / max(cycles, 1.0), 4),
+                    "valu_lane_util": round(v["SQ_THREAD_CYCLES_VALU"] /
+                                            (64 * max(v["SQ_ACTIVE_INST_VALU"], 1.0)), 4),
+                    "wave_wait_frac": round(v["SQ_WAIT_INST_ANY"] / max(v["SQ_WAVE_CYCLES"], 1.0),
+                                            4),
+                    "valu_insts": int(v["SQ_INSTS_VALU"]),
+                    "l2_request_bytes": int(v["TCC_REQ_sum"] * req_bytes) if req_bytes else None,
+                    "gpu_cycles": int(cycles),
+                }
+        out[k] = d
+        log(f"pmc {k}: {d}")
     return out
+
+
+def roofline_entry(kernel: str, alg: float, avg_ms: float, pmc: dict | None) -> dict:
+    """The roofline object of one kernel: algorithmic bytes per launch / mean
+    launch time against the HBM peak, the measured HBM traffic of the same
+    launch and what it is as a fraction of the peak, and the issue counters
+    (the binding limit is the larger of valu_issue and hbm_traffic_frac)."""
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    r = dict(kernel=kernel, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
+             unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+             basis=ALG_BASIS, bytes_per_launch=int(alg), avg_launch_ms=round(avg_ms, 4))
+    pmc = pmc or {}
+    if pmc.get("traffic") is not None:
+        r["traffic"] = pmc["traffic"]
+        r["traffic_detail"] = pmc["traffic_detail"]
+        r["hbm_traffic_frac"] = round(pmc["traffic"] / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    if pmc.get("counters"):
+        c = pmc["counters"]
+        r["counters"] = c
+        lim = {"valu_issue": c["valu_issue"], "hbm": r.get("hbm_traffic_frac", 0.0)}
+        r["limiter"] = max(lim, key=lim.get)
+    return r
+
+
+# ------------------------------------------------------------- CPU baseline
+CPU_SAMPLE_RAYS = 1_000_000  # rays of one timed pass (about 1.5 s at 1 thread)
+
+
+def _median_time(fn, runs: int, warmup: bool = True) -> tuple[float, list]:
+    if warmup:
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), [round(t, 4) for t in ts]
+
+
+def cpu_baseline(mesh, batches, what: str, target_rays: int = CPU_SAMPLE_RAYS,
+                 build_runs: int = 3, build_warmup: bool = True, query_runs: int = 3) -> dict:
+    """The reference CPU path restated on triangles (oracle/port.cpp), timed on
+    this host with steady wall clocks: the reference's median-split BVH build
+    (constructBVHNew), its "LBVH" build (constructLBVHTree), and this build's
+    CPU LBVH (Morton, stable sort, Karras, refit) next to the GPU build; then
+    collect-then-test closest hit on every k-th active ray of each batch (k
+    chosen for about target_rays rays) at 1 thread, as the reference's render
+    loop, and on this process's core share. Each: 1 warm-up, median of 3."""
+    from oracle import oracle as O
+
+    cores = cpu_share()
+    builds = {}
+    for name, fn in (("reference_bvh_constructBVHNew", lambda: O.PortBVH(mesh)),
+                     ("reference_lbvh_constructLBVHTree", lambda: O.PortBVH(mesh, lbvh=True)),
+                     ("oracle_lbvh", lambda: O.OracleBVH(mesh, rq.MORTON_SCENE_AABB))):
+        med, ts = _median_time(fn, build_runs, build_warmup)
+        builds[name] = {"ms": round(med * 1e3, 1), "runs_s": ts}
+    active = sum(int((h["tmin"] < h["tmax"]).sum()) for h in batches)
+    every = max(1, active // target_rays)
+    sample = {k: [] for k in O.RAY_FIELDS}
+    for host in batches:
+        idx = np.nonzero(host["tmin"] < host["tmax"])[0][::every]
+        for k in O.RAY_FIELDS:
+            sample[k].append(host[k][idx])
+    sample = {k: np.concatenate(v) for k, v in sample.items()}
+    n = len(sample["ox"])
+    port = O.PortBVH(mesh)
+    t1, ts1 = _median_time(lambda: port.closest(sample, threads=1), query_runs)
+    tn, tsn = _median_time(lambda: port.closest(sample, threads=cores), query_runs)
+    timing = f"1 warm-up, median of {query_runs} (builds: " + (
+        f"1 warm-up, median of {build_runs})" if build_warmup else f"{build_runs} run)")
+    return dict(value=round(n / tn / 1e6, 4), unit="Mrays/s", cores=cores, kind="port",
+                sample=f"every {every}th active ray {what}: {n} rays per pass, closest hit",
+                rays=n, single_thread={"value": round(n / t1 / 1e6, 4), "cores": 1,
+                                       "runs_s": ts1},
+                all_cores_runs_s=tsn, builds=builds, timing=timing,
+                host=f"{cpu_model()}, {os.cpu_count()} logical CPUs visible, {cores} used")
+
+
+# ------------------------------------------------------------------ serving
+def serve_run(ctx, warm: float, meas: float, saturation: bool) -> dict:
+    """Serving: 64 resident meshes (F ~ log-U[1e4, 2e6], seed 5) in a server
+    with 72 slots of 2^21 triangles (64 resident, 8 for uploads) and a 200 GB
+    HBM budget. Open loop: Poisson arrivals at 2000 requests/s for `warm`
+    seconds of warm-up plus `meas` measured; request sizes log-U[4096,
+    4194304] rays (seed 7), alternately pinhole rays aimed at the mesh and
+    uniform random rays (generated on the device from the request's
+    parameters), closest hit with prim + t returned to page-locked host
+    arrays; 5% of requests upload a new mesh (from a pool of 8 pre-generated
+    meshes, seed 6) that is built before its rays are traced. Latency =
+    results on the host - scheduled arrival. With `saturation`, a burst of the
+    first 1000 non-upload requests measures the served throughput, and the
+    same requests go through the plain per-request path (ray generation,
+    rq_closest_hit, copy back, wait)."""
+    import ctypes as C
+
+    t_setup = time.perf_counter()
+    resident = W.c5_meshes(64, 5)
+    uploads = W.c5_meshes(8, 6)
+    bounds = {id(m): m.bounds() for m in resident + uploads}
+    log(f"serve: {sum(m.ntri for m in resident)} resident triangles, "
+        f"{sum(m.ntri for m in uploads)} in the upload pool ({time.perf_counter() - t_setup:.1f} s)")
+    srv = rq.Server(0, mesh_slots=len(resident) + len(uploads))
+    slots = [srv.load_mesh(m) for m in resident]
+    slot_mesh = dict(zip(slots, resident))
+    lam = 2000.0
+    rng = np.random.default_rng(7)
+    n_req = int(lam * (warm + meas) * 1.2) + 16
+    arrival = np.cumsum(rng.exponential(1.0 / lam, n_req))
+    n_req = int(np.searchsorted(arrival, warm + meas))
+    arrival = arrival[:n_req]
+    sizes = np.rint(np.exp(rng.uniform(np.log(4096.0), np.log(4194304.0), n_req))).astype(np.int64)
+    target = rng.integers(0, len(slots), n_req)
+    upload = rng.random(n_req) < 0.05
+    max_n = int(sizes.max())
+    ring = 80  # page-locked result buffers (at most 80 requests in flight from this load generator)
+    bufs = [(rq.HostArray(max_n, np.int32), rq.HostArray(max_n, np.float32)) for _ in range(ring)]
+    log(f"serve: {n_req} requests over {warm + meas:.1f} s, {int(upload.sum())} uploads, "
+        f"offered {sizes.sum() / (warm + meas) / 1e6:.0f} Mrays/s")
+
+    def spec(j):
+        if upload[j]:
+            m = uploads[j % len(uploads)]
+            sp = W.c5_rays_spec(bounds[id(m)], j % 2, 100000 + j)
+            sp["new_mesh"] = m
+        else:
+            m = slot_mesh[slots[target[j]]]
+            sp = W.c5_rays_spec(bounds[id(m)], j % 2, 100000 + j)
+            sp["mesh"] = slots[target[j]]
+        return sp
+
+    def drive(order, t0, times):
+        """Submits order[k] at t0 + times[k] (or as soon as a buffer is free);
+        returns per-request (t_done - scheduled arrival, steps)."""
+        free = list(range(ring))
+        inflight = {}
+        lat = np.full(len(order), np.nan)
+        steps = np.zeros(len(order), np.int64)
+        k = 0
+        while k < len(order) or inflight:
+            now = rq.server_clock()
+            while k < len(order) and free and t0 + times[k] <= now:
+                j = order[k]
+                b = free.pop()
+                tk = srv.submit(int(sizes[j]), rq.Q_CLOSEST, prim=bufs[b][0].array,
+                                t=bufs[b][1].array, **spec(j))
+                inflight[tk] = (k, b)
+                k += 1
+            for tk, st in srv.take_done():
+                kk, b = inflight.pop(tk)
+                if st.state != rq.REQ_DONE:
+                    raise RuntimeError(f"request {order[kk]} failed: {st.state}")
+                lat[kk] = st.t_done - (t0 + times[kk])
+                steps[kk] = st.steps
+                free.append(b)
+                srv.forget(tk)
+        return lat, steps
+
+    s0 = srv.stats()
+    t0 = rq.server_clock() + 0.05
+    lat, steps = drive(list(range(n_req)), t0, arrival)
+    wall = rq.server_clock() - t0
+    s1 = srv.stats()
+    sel = arrival >= warm
+    lat_ms = lat[sel] * 1e3
+    up_ms = lat[sel & upload] * 1e3
+    p50, p99 = (float(np.percentile(lat_ms, q)) for q in (50, 99))
+    log(f"serve: p50 {p50:.2f} ms, p99 {p99:.2f} ms, wall {wall:.2f} s")
+    out = {
+        "resident_meshes": len(slots), "resident_triangles": int(sum(m.ntri for m in resident)),
+        "arrival_rate_per_s": lam, "warmup_s": warm, "measured_s": meas,
+        "requests": int(sel.sum()), "uploads": int((sel & upload).sum()),
+        "hbm_budget_bytes": 200 * 10 ** 9,
+        "offered_mrays_s": round(float(sizes[sel].sum()) / meas / 1e6, 1),
+        "query": "closest (prim, t) to host", "rays": "half pinhole, half uniform",
+        "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3),
+                       "mean": round(float(np.mean(lat_ms)), 3),
+                       "max": round(float(np.max(lat_ms)), 3),
+                       "upload_p50": round(float(np.percentile(up_ms, 50)), 3) if len(up_ms) else None,
+                       "upload_p99": round(float(np.percentile(up_ms, 99)), 3) if len(up_ms) else None},
+        "trace_launches": int(s1["steps"] - s0["steps"]),
+        "mean_launches_per_request": round(float(steps[sel].mean()), 2),
+        "max_requests_per_launch": int(s1["max_step_requests"]),
+        "gpu_trace_busy_fraction": round((s1["busy_s"] - s0["busy_s"]) / wall, 3),
+    }
+    if saturation:
+        burst = [j for j in range(n_req) if not upload[j]][:1000]
+        t0 = rq.server_clock() + 0.01
+        blat, _ = drive(burst, t0, np.zeros(len(burst)))
+        served_s = float(np.nanmax(blat))
+        burst_rays = int(sizes[burst].sum())
+        out["server_stats"] = {k: (round(v, 4) if isinstance(v, float) else v)
+                               for k, v in srv.stats().items()}
+        srv.close()
+        lib = rq.lib()
+        bvhs = {}
+        for sl, m in slot_mesh.items():
+            sc = ctx.upload_scene(m)
+            bvhs[sl] = ctx.build_lbvh(sc, rq.MORTON_SCENE_AABB)
+        dr = rq.DeviceRays(ctx, max_n)
+        dprim, dt = rq.DeviceArray(ctx, 4 * max_n), rq.DeviceArray(ctx, 4 * max_n)
+        ptrs = [dr.arrays[f].ptr for f in rq.RAY_FIELDS]
+        rays_st = dr.struct()
+
+        def plain(j):
+            n = int(sizes[j])
+            sp = spec(j)
+            if "pinhole" in sp:
+                p = sp["pinhole"]
+                cam = rq.Pinhole((C.c_float * 3)(*p["eye"]), (C.c_float * 3)(*p["look_at"]),
+                                 (C.c_float * 3)(*p["up"]), p["fov_deg"], p["width"])
+                ctx._check(lib.rq_raygen_pinhole(ctx.h, C.byref(cam), n, sp["seed"], *ptrs, None))
+            else:
+                lo3 = (C.c_float * 3)(*sp["box"][0])
+                hi3 = (C.c_float * 3)(*sp["box"][1])
+                ctx._check(lib.rq_raygen_uniform(ctx.h, n, sp["seed"], lo3, hi3, *ptrs, None))
+            h = rq.Hits(dprim.ptr, dt.ptr, None, None)
+            ctx._check(lib.rq_closest_hit(ctx.h, bvhs[sp["mesh"]], C.byref(rays_st), n,
+                                          C.byref(h), None))
+            ctx._check(lib.rq_memcpy_d2h(ctx.h, bufs[0][0].ptr, dprim.ptr, 4 * n, None))
+            ctx._check(lib.rq_memcpy_d2h(ctx.h, bufs[0][1].ptr, dt.ptr, 4 * n, None))
+
+        for j in burst[:20]:
+            plain(j)
+        ctx.sync()
+        t_a = time.perf_counter()
+        for j in burst:
+            plain(j)
+        ctx.sync()
+        plain_s = time.perf_counter() - t_a
+        served_mrays, plain_mrays = burst_rays / served_s / 1e6, burst_rays / plain_s / 1e6
+        out["saturation"] = {"requests": len(burst), "rays": burst_rays,
+                             "served_mrays_s": round(served_mrays, 1),
+                             "plain_per_request_mrays_s": round(plain_mrays, 1),
+                             "speedup": round(served_mrays / plain_mrays, 3)}
+        dr.free()
+        dprim.free()
+        dt.free()
+        for b in bvhs.values():
+            ctx.release(b)
+    else:
+        srv.close()
+    for a, b in bufs:
+        a.free()
+        b.free()
+    return out
+
+
+# -------------------------------------------------------------- workloads
+def base_line(value, ms_per_step, steps, warmup, config) -> dict:
+    return {"metric": METRIC, "value": round(value, 2), "unit": "Mrays/s", "n_gpus": 1,
+            "steps": steps, "warmup": warmup, "ms_per_step": round(ms_per_step, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic", "config": config}
+
+
+def run_frames(ctx, args) -> int:
+    """C1 / C2: rebuild + one frame of the wavefront renderer per step."""
+    mesh, params, workload = (W.c2 if args.config == "c2" else W.c1)()
+    log(f"{workload}: {mesh.ntri} triangles")
+    scene = ctx.upload_scene(mesh)
+    bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
+    if args.treelets:
+        ctx.treelets(bvh, args.treelets)
+    ren = rq.Renderer(ctx, params)
+    cfg = None
+    if not args.autotune:  # fixed configuration: every launch, warm-up included, uses it
+        cfg = choose_trace(ctx, None, None, None, args, 0)
+
+    def frame():
+        ctx.rebuild(bvh)
+        if args.treelets:
+            ctx.treelets(bvh, args.treelets)
+        ren.frame(bvh)
+
+    for _ in range(args.warmup):
+        frame()
+    ctx.sync()
+    cnt = ren.counts()
+    rays_per_frame = int(cnt.primary + cnt.bounce + cnt.shadow)
+    info = ctx.bvh_info(bvh)
+    batches = {"primary": ren.rays(0), "shadow": ren.rays(2)}
+    if params.bounce:
+        batches["bounce"] = ren.rays(1)
+    counts = {k: counted_batch(ctx, bvh, r, any_hit=(k == "shadow")) for k, r in batches.items()}
+    if args.sweep:
+        sweep(ctx, bvh, batches)
+    how = "--trace-config" if args.trace_config else "fixed for this workload"
+    if args.autotune:
+        cfg = choose_trace(ctx, bvh, batches["primary"], batches["shadow"], args, 3)
+        how = "autotuned on this frame's primary and shadow rays"
+        for _ in range(2):
+            frame()
+        ctx.sync()
+    if args.pmc_probe:
+        for _ in range(2):
+            frame()
+        ctx.sync()
+        copy_probe(ctx)
+        return 0
+
+    # ---- timed region: K frames, HIP events on the launch streams around the
+    # traversal launches only (events around every kernel cost ~4 % of a C2 frame)
+    ctx.profile_only(["trace_closest", "trace_any"])
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        frame()
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.profile(False)
+    prof = ctx.profile_read()
+    assert ctx.device_errors() == 0
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = rays_per_frame * args.steps / elapsed / 1e6
+
+    # ---- per-kernel breakdown (build and shading included) from frames after
+    # the timed region, every kernel bracketed by events
+    breakdown_frames = 3
+    ctx.profile_only(None)
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(breakdown_frames):
+        frame()
+    ctx.sync()
+    ctx.profile(False)
+    breakdown = ctx.profile_read()
+
+    # ---- rooflines: algorithmic bytes per launch / event time
+    n = ren.n
+    closest_rays = [("primary", n)] + ([("bounce", n)] if params.bounce else [])
+    closest_bytes = sum(r * (32 + 8) + 32 * counts[k][0] + 36 * counts[k][1]
+                        for k, r in closest_rays) / len(closest_rays)
+    nshadow = batches["shadow"].n
+    any_bytes = nshadow * (32 + 1) + 32 * counts["shadow"][0] + 36 * counts["shadow"][1]
+    # the shadow rays may go out in several launches per frame (primary hits'
+    # and second hits' concurrently): bytes per launch = frame bytes / launches
+    any_launches = max(1, prof.get("trace_any", (0, args.steps))[1] // args.steps)
+    alg = {"trace_closest": closest_bytes, "trace_any": any_bytes / any_launches}
+    avg = {k: prof[k][0] / prof[k][1] for k in alg if k in prof}
+    dom = max(avg, key=lambda k: prof[k][0])
+    pmc = {}
+    if args.traffic == "auto":
+        pmc = measure_pmc(args.config, list(avg), cfg_arg(cfg))
+    roofline = roofline_entry(dom, alg[dom], avg[dom], pmc.get(dom))
+    roof_closest = (roofline_entry("trace_closest", alg["trace_closest"], avg["trace_closest"],
+                                   pmc.get("trace_closest")) if "trace_closest" in avg else None)
+    build_kernels = ("build_init", "centroid_bounds", "morton_hist", "radix_onesweep", "karras",
+                     "refit", "refit_top", "widen")
+    build_ms = sum(breakdown[k][0] for k in build_kernels if k in breakdown) / breakdown_frames
+    closest_ms = prof.get("trace_closest", (0, 1))[0] / args.steps
+    closest_rays_n = sum(r for _, r in closest_rays)
+
+    line = base_line(value, ms_per_step, args.steps, args.warmup, {
+        "workload": workload, "triangles": mesh.ntri,
+        "width": params.width, "height": params.height, "spp": params.spp,
+        "bounce": int(params.bounce), "rays_per_frame": rays_per_frame,
+        "rays_primary": int(cnt.primary), "rays_bounce": int(cnt.bounce),
+        "rays_shadow": int(cnt.shadow), "lbvh_rebuilt_per_frame": True,
+        "bvh_height": int(info.height)})
+    line["lbvh_build_ms"] = round(build_ms, 4)
+    line["lbvh_build_roofline"] = {
+        "bytes_per_tri": BUILD_MODEL_BYTES_PER_TRI,
+        "achieved_gbs": round(mesh.ntri * BUILD_MODEL_BYTES_PER_TRI / (build_ms * 1e-3) / 1e9, 1),
+        "frac": round(mesh.ntri * BUILD_MODEL_BYTES_PER_TRI / (build_ms * 1e-3) / 1e9 /
+                      HBM_PEAK_GBS, 4)}
+    line["closest_hit_mrays_s"] = (round(closest_rays_n / (closest_ms * 1e-3) / 1e6, 1)
+                                   if closest_ms else None)
+    line["kernel_ms_per_step"] = {k: round(v[0] / breakdown_frames, 4)
+                                  for k, v in breakdown.items()}
+    line["traversal_counts"] = {k: {"node_visits": v[0], "tri_tests": v[1]}
+                                for k, v in counts.items()}
+    line["trace_config"] = trace_dict(ctx, cfg, how)
+    line["roofline"] = roofline
+    line["roofline_closest"] = roof_closest
+    line["serving"] = None
+    line["cpu_baseline"] = None
+    if args.config == "c2" and not args.no_serve:
+        line["serving"] = serve_run(ctx, 0.5, args.serve_seconds_short, saturation=False)
+    if not args.no_cpu:
+        host = [batches[k].download() for k in batches]
+        log(f"cpu baseline: 1 and {cpu_share()} threads")
+        line["cpu_baseline"] = cpu_baseline(
+            mesh, host, "of each batch of one frame (primary, bounce, shadow as closest hit)",
+            target_rays=args.cpu_rays)
+    print(json.dumps(line), flush=True)
+    ren.close()
+    ctx.close()
+    return 0
 
 
 def run_c3(ctx, args) -> int:
@@ -272,10 +704,9 @@ def run_c3(ctx, args) -> int:
     occ = rq.DeviceArray(ctx, n)
     vc, tc = counted_batch(ctx, bvh, rays, any_hit=False)
     va, ta = counted_batch(ctx, bvh, rays, any_hit=True)
-    tuned = None
-    cfg = tune_trace(ctx, bvh, rays, rays, args, repeats=2)
-    if cfg:
-        tuned = tuned_dict(ctx, cfg)
+    cfg = choose_trace(ctx, bvh, rays, rays, args, 2)
+    how = ("--trace-config" if args.trace_config else
+           "autotuned on these rays" if args.autotune else "fixed for this workload")
     for _ in range(args.warmup):
         ctx.closest_hit_dev(bvh, rays, hits)
         ctx.any_hit_dev(bvh, rays, occ)
@@ -300,54 +731,36 @@ def run_c3(ctx, args) -> int:
     elapsed = t1 - t0
     prim = hits["prim"].download(np.int32, n)
     occ_h = occ.download(np.uint8, n)
-    cl_ms = prof["trace_closest"][0] / prof["trace_closest"][1]
-    an_ms = prof["trace_any"][0] / prof["trace_any"][1]
-    b_closest = n * (32 + 8) + 32 * vc + 36 * tc
-    b_any = n * (32 + 1) + 32 * va + 36 * ta
-    dom = "trace_closest" if cl_ms >= an_ms else "trace_any"
-    alg, dms = (b_closest, cl_ms) if dom == "trace_closest" else (b_any, an_ms)
-    achieved = alg / (dms * 1e-3) / 1e9
-    roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    bytes_per_launch=int(alg), avg_launch_ms=round(dms, 4))
-    if args.traffic == "auto":
-        roofline["traffic"] = measure_traffic("c3", dom, cfg_arg(cfg))
-        roofline["counters"] = measure_counters("c3", dom, cfg_arg(cfg))
-    line = {
-        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
-        "value": round(2 * n * args.steps / elapsed / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "c3_raw_ray_batch", "triangles": mesh.ntri, "rays": n,
-                   "queries": "closest+any", "closest_hit_rate": round(float((prim >= 0).mean()), 4),
-                   "occluded_rate": round(float(occ_h.mean()), 4)},
-        "closest_hit_mrays_s": round(n / (cl_ms * 1e-3) / 1e6, 1),
-        "any_hit_mrays_s": round(n / (an_ms * 1e-3) / 1e6, 1),
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
-        "traversal_counts": {"closest": {"node_visits": vc, "tri_tests": tc},
-                             "any": {"node_visits": va, "tri_tests": ta}},
-        "autotune": tuned, "roofline": roofline, "cpu_baseline": None,
-    }
+    avg = {k: prof[k][0] / prof[k][1] for k in ("trace_closest", "trace_any")}
+    alg = {"trace_closest": n * (32 + 8) + 32 * vc + 36 * tc,
+           "trace_any": n * (32 + 1) + 32 * va + 36 * ta}
+    dom = max(avg, key=avg.get)
+    pmc = measure_pmc("c3", list(avg), cfg_arg(cfg)) if args.traffic == "auto" else {}
+    line = base_line(2 * n * args.steps / elapsed / 1e6, elapsed / args.steps * 1e3, args.steps,
+                     args.warmup, {"workload": "c3_raw_ray_batch", "triangles": mesh.ntri,
+                                   "rays": n, "queries": "closest+any",
+                                   "closest_hit_rate": round(float((prim >= 0).mean()), 4),
+                                   "occluded_rate": round(float(occ_h.mean()), 4)})
+    line["closest_hit_mrays_s"] = round(n / (avg["trace_closest"] * 1e-3) / 1e6, 1)
+    line["any_hit_mrays_s"] = round(n / (avg["trace_any"] * 1e-3) / 1e6, 1)
+    line["kernel_ms_per_step"] = {k: round(v[0] / args.steps, 4) for k, v in prof.items()}
+    line["traversal_counts"] = {"closest": {"node_visits": vc, "tri_tests": tc},
+                                "any": {"node_visits": va, "tri_tests": ta}}
+    line["trace_config"] = trace_dict(ctx, cfg, how)
+    line["roofline"] = roofline_entry(dom, alg[dom], avg[dom], pmc.get(dom))
+    line["roofline_closest"] = roofline_entry("trace_closest", alg["trace_closest"],
+                                              avg["trace_closest"], pmc.get("trace_closest"))
+    line["cpu_baseline"] = None
     if treelets:
         line["lbvh_treelets"] = treelets
     if not args.no_cpu:
         host = rays.download()
-        threads = max(1, min(16, os.cpu_count() or 1))
-        line["cpu_baseline"] = cpu_baseline(mesh, [host], 0, threads,
-                                            what="of the 2^24 uniform rays (closest hit)")
+        line["cpu_baseline"] = cpu_baseline(mesh, [host], "of the 2^24 uniform rays",
+                                            target_rays=args.cpu_rays, build_runs=1,
+                                            build_warmup=False)
     print(json.dumps(line), flush=True)
     ctx.close()
     return 0
-
-
-# Algorithmic bytes of one LBVH build per triangle (SCENE_AABB Morton map):
-# bounds 48 (3 indices + 9 vertex floats), Morton 52 (same + code), sort 60
-# (pass 0: key in, key + index out; passes 1-3: key + index in and out),
-# Karras 28 (key, two child links, two parent words, leaf range), refit 176
-# (sorted index, 3 indices, 9 floats, 48 B triangle record, 64 B node boxes,
-# arrival and height words).
-BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
 
 
 def c4_workload(ctx, count: int = 512):
@@ -380,7 +793,7 @@ def run_c4(ctx, args) -> int:
     items = [dict(scene=s, rays=r, hits=h, occ=o) for s, r, h, o in zip(scenes, rays, hits, occ)]
     batch = rq.Batch(ctx, items, rq.MORTON_SCENE_AABB)
     tuned = None
-    if not args.no_autotune:
+    if args.autotune:
         v, tile, refill, ms = batch.autotune(1)
         tuned = {"variant": rq.Batch.configs()[v], "ray_tile": tile, "refill_idle": refill,
                  "ms": round(ms, 3)}
@@ -414,8 +827,9 @@ def run_c4(ctx, args) -> int:
     # per mesh a standalone LBVH (allocated here, rebuilt in the timed loop)
     bvhs = [ctx.build_lbvh(scenes[m], rq.MORTON_SCENE_AABB) for m in range(len(meshes))]
     ctx.sync()
-    med = int(np.argsort([r.n for r in rays])[len(rays) // 2])
-    tune_trace(ctx, bvhs[med], rays[med], rays[med], args, repeats=2)
+    if args.autotune:
+        med = int(np.argsort([r.n for r in rays])[len(rays) // 2])
+        choose_trace(ctx, bvhs[med], rays[med], rays[med], args, 2)
 
     def per_request():
         for m in range(len(meshes)):
@@ -445,238 +859,60 @@ def run_c4(ctx, args) -> int:
     alg = (BUILD_BYTES_PER_TRI * ntri + nray * (32 + 8) + 32 * vis_c + 36 * tst_c +
            nray * (32 + 1) + 32 * vis_a + 36 * tst_a)
     ms_tot, launches = prof["batch_graph"]
-    avg_ms = ms_tot / launches
-    achieved = alg / (avg_ms * 1e-3) / 1e9
-    roofline = dict(kernel="k_batch", bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS,
-                    unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    bytes_per_launch=int(alg), avg_launch_ms=round(avg_ms, 4))
-    if args.traffic == "auto":
-        roofline["traffic"] = measure_traffic("c4", "batch")
-    line = {
-        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
-        "value": round(2 * nray / batch_s / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(batch_s * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "c4_batched_build_and_trace", "meshes": len(meshes),
-                   "triangles": ntri, "rays": nray, "queries": "build+closest+any",
-                   "launch": "one persistent kernel per step, hipGraph replay"},
-        "per_request_ms": round(plain_s * 1e3, 3),
-        "per_request_mrays_s": round(2 * nray / plain_s / 1e6, 2),
-        "speedup_vs_per_request": round(plain_s / batch_s, 3),
-        "traversal_counts": {"closest": {"node_visits": vis_c, "tri_tests": tst_c},
-                             "any": {"node_visits": vis_a, "tri_tests": tst_a}},
-        "autotune": tuned, "roofline": roofline, "cpu_baseline": None,
-    }
+    pmc = measure_pmc("c4", ["batch"], None) if args.traffic == "auto" else {}
+    line = base_line(2 * nray / batch_s / 1e6, batch_s * 1e3, args.steps, args.warmup, {
+        "workload": "c4_batched_build_and_trace", "meshes": len(meshes), "triangles": ntri,
+        "rays": nray, "queries": "build+closest+any",
+        "launch": "one persistent kernel per step, hipGraph replay"})
+    line["per_request_ms"] = round(plain_s * 1e3, 3)
+    line["per_request_mrays_s"] = round(2 * nray / plain_s / 1e6, 2)
+    line["speedup_vs_per_request"] = round(plain_s / batch_s, 3)
+    line["traversal_counts"] = {"closest": {"node_visits": vis_c, "tri_tests": tst_c},
+                                "any": {"node_visits": vis_a, "tri_tests": tst_a}}
+    line["autotune"] = tuned
+    line["roofline"] = roofline_entry("k_batch", alg, ms_tot / launches, pmc.get("batch"))
+    line["roofline"]["basis"] = "build bytes (364 per triangle) + " + ALG_BASIS
+    line["cpu_baseline"] = None
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
         sample = list(range(0, len(meshes), 61))  # icospheres and soups
-        total_rays = 0
-        total_s = 0.0
-        build_ms = 0.0
-        for m in sample:
-            host = rays[m].download()
-            cb = cpu_baseline(meshes[m], [host], 1, threads, what="",
-                              min_seconds=CPU_MIN_SECONDS / len(sample))
-            total_rays += cb["rays"]
-            total_s += cb["seconds"]
-            build_ms += cb["build_ms"]
+        parts = [cpu_baseline(meshes[m], [rays[m].download()], "", target_rays=200_000,
+                              build_runs=1, build_warmup=False, query_runs=1) for m in sample]
+        rays_n = sum(p["rays"] for p in parts)
+        t_n = sum(p["rays"] / p["value"] for p in parts)
+        t_1 = sum(p["rays"] / p["single_thread"]["value"] for p in parts)
         line["cpu_baseline"] = dict(
-            value=round(total_rays / total_s / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
-            sample=f"meshes {sample}: every active ray (closest hit), {total_rays} rays over "
-                   f"repeated passes in {total_s:.1f} s; "
-                   f"median-split BVH builds {build_ms:.0f} ms in total")
+            value=round(rays_n / t_n, 4), unit="Mrays/s", cores=parts[0]["cores"], kind="port",
+            sample=f"meshes {sample}: about 200,000 rays of each (closest hit), 1 warm-up and "
+                   f"1 timed pass; builds once each",
+            single_thread={"value": round(rays_n / t_1, 4), "cores": 1},
+            builds_ms={k: round(sum(p["builds"][k]["ms"] for p in parts), 1)
+                       for k in parts[0]["builds"]},
+            host=parts[0]["host"])
     print(json.dumps(line), flush=True)
     batch.close()
     ctx.close()
     return 0
 
 
-_BOUNDS: dict = {}
-
-
-def _c5_rays_spec(mesh, kind: int, seed: int) -> dict:
-    if id(mesh) not in _BOUNDS:
-        _BOUNDS[id(mesh)] = mesh.bounds()
-    return W.c5_rays_spec(_BOUNDS[id(mesh)], kind, seed)
-
-
 def run_c5(ctx, args) -> int:
-    """Serving: 64 resident meshes (F ~ log-U[1e4, 2e6], seed 5) in a server
-    with 72 slots of 2^21 triangles (64 resident, 8 for uploads) and a 200 GB HBM budget. Open loop:
-    Poisson arrivals at 2000 requests/s for 1 s of warm-up plus
-    `--serve-seconds` measured; request sizes log-U[4096, 4194304] rays (seed
-    7), alternately pinhole rays aimed at the mesh and uniform random rays
-    (generated on the device from the request's parameters), closest hit with
-    prim + t returned to page-locked host arrays; 5% of requests upload a new
-    mesh (from a pool of 8 pre-generated meshes, seed 6) that is built before
-    its rays are traced. Latency = results on the host - scheduled arrival.
-    Then a burst of the first 1000 non-upload requests measures the served
-    throughput at saturation, and the same 1000 requests through the plain
-    per-request path (ray generation, rq_closest_hit, copy back, wait)."""
-    import ctypes as C
-
-    t_setup = time.perf_counter()
-    resident = W.c5_meshes(64, 5)
-    uploads = W.c5_meshes(8, 6)
-    log(f"c5: {sum(m.ntri for m in resident)} resident triangles, "
-        f"{sum(m.ntri for m in uploads)} in the upload pool ({time.perf_counter() - t_setup:.1f} s)")
-    # the resident meshes keep their slots; uploads recycle slots of their own
-    srv = rq.Server(0, mesh_slots=len(resident) + len(uploads))
-    slots = [srv.load_mesh(m) for m in resident]
-    slot_mesh = {s: m for s, m in zip(slots, resident)}
-    lam, warm, meas = 2000.0, 1.0, float(args.serve_seconds)
-    rng = np.random.default_rng(7)
-    n_req = int(lam * (warm + meas) * 1.2) + 16
-    arrival = np.cumsum(rng.exponential(1.0 / lam, n_req))
-    n_req = int(np.searchsorted(arrival, warm + meas))
-    arrival = arrival[:n_req]
-    sizes = np.rint(np.exp(rng.uniform(np.log(4096.0), np.log(4194304.0), n_req))).astype(np.int64)
-    target = rng.integers(0, len(slots), n_req)
-    upload = rng.random(n_req) < 0.05
-    max_n = int(sizes.max())
-    ring = 80  # page-locked result buffers (at most 80 requests in flight from this load generator)
-    bufs = [(rq.HostArray(max_n, np.int32), rq.HostArray(max_n, np.float32)) for _ in range(ring)]
-    log(f"c5: {n_req} requests over {warm + meas:.0f} s, {int(upload.sum())} uploads, "
-        f"offered {sizes.sum() / (warm + meas) / 1e6:.0f} Mrays/s")
-
-    def spec(j):
-        if upload[j]:
-            m = uploads[j % len(uploads)]
-            sp = _c5_rays_spec(m, j % 2, 100000 + j)
-            sp["new_mesh"] = m
-        else:
-            sp = _c5_rays_spec(slot_mesh[slots[target[j]]], j % 2, 100000 + j)
-            sp["mesh"] = slots[target[j]]
-        return sp
-
-    def drive(order, t0, times):
-        """Submits order[k] at t0 + times[k] (or as soon as a buffer is free);
-        returns per-request (t_done - scheduled arrival, steps)."""
-        free = list(range(ring))
-        inflight = {}
-        lat = np.full(len(order), np.nan)
-        steps = np.zeros(len(order), np.int64)
-        k = 0
-        while k < len(order) or inflight:
-            now = rq.server_clock()
-            while k < len(order) and free and t0 + times[k] <= now:
-                j = order[k]
-                b = free.pop()
-                tk = srv.submit(int(sizes[j]), rq.Q_CLOSEST, prim=bufs[b][0].array,
-                                t=bufs[b][1].array, **spec(j))
-                inflight[tk] = (k, b)
-                k += 1
-            for tk, st in srv.take_done():
-                kk, b = inflight.pop(tk)
-                if st.state != rq.REQ_DONE:
-                    raise RuntimeError(f"request {order[kk]} failed: {st.state}")
-                lat[kk] = st.t_done - (t0 + times[kk])
-                steps[kk] = st.steps
-                free.append(b)
-                srv.forget(tk)
-        return lat, steps
-
-    # ---- open loop at lambda = 2000/s
-    s0 = srv.stats()
-    t0 = rq.server_clock() + 0.05
-    lat, steps = drive(list(range(n_req)), t0, arrival)
-    wall = rq.server_clock() - t0
-    s1 = srv.stats()
-    sel = arrival >= warm
-    lat_ms = lat[sel] * 1e3
-    up_ms = lat[sel & upload] * 1e3
-    done_rays = int(sizes[sel].sum())
-    p50, p99 = (float(np.percentile(lat_ms, q)) for q in (50, 99))
-    log(f"c5 open loop: p50 {p50:.2f} ms, p99 {p99:.2f} ms, wall {wall:.2f} s")
-
-    # ---- saturation: a burst of 1000 resident-mesh requests, served vs plain
-    burst = [j for j in range(n_req) if not upload[j]][:1000]
-    t0 = rq.server_clock() + 0.01
-    blat, _ = drive(burst, t0, np.zeros(len(burst)))
-    served_s = float(np.nanmax(blat))
-    burst_rays = int(sizes[burst].sum())
-    srv_stats = srv.stats()
-    srv.close()
-
-    lib = rq.lib()
-    bvhs = {}
-    for sl, m in slot_mesh.items():
-        sc = ctx.upload_scene(m)
-        bvhs[sl] = ctx.build_lbvh(sc, rq.MORTON_SCENE_AABB)
-    dr = rq.DeviceRays(ctx, max_n)
-    dprim, dt = rq.DeviceArray(ctx, 4 * max_n), rq.DeviceArray(ctx, 4 * max_n)
-    ptrs = [dr.arrays[f].ptr for f in rq.RAY_FIELDS]
-    rays_st = dr.struct()
-
-    def plain(j):
-        n = int(sizes[j])
-        sp = spec(j)
-        if "pinhole" in sp:
-            p = sp["pinhole"]
-            cam = rq.Pinhole((C.c_float * 3)(*p["eye"]), (C.c_float * 3)(*p["look_at"]),
-                             (C.c_float * 3)(*p["up"]), p["fov_deg"], p["width"])
-            ctx._check(lib.rq_raygen_pinhole(ctx.h, C.byref(cam), n, sp["seed"], *ptrs, None))
-        else:
-            lo3 = (C.c_float * 3)(*sp["box"][0])
-            hi3 = (C.c_float * 3)(*sp["box"][1])
-            ctx._check(lib.rq_raygen_uniform(ctx.h, n, sp["seed"], lo3, hi3, *ptrs, None))
-        h = rq.Hits(dprim.ptr, dt.ptr, None, None)
-        ctx._check(lib.rq_closest_hit(ctx.h, bvhs[sp["mesh"]], C.byref(rays_st), n, C.byref(h),
-                                      None))
-        ctx._check(lib.rq_memcpy_d2h(ctx.h, bufs[0][0].ptr, dprim.ptr, 4 * n, None))
-        ctx._check(lib.rq_memcpy_d2h(ctx.h, bufs[0][1].ptr, dt.ptr, 4 * n, None))
-
-    for j in burst[:20]:
-        plain(j)
-    ctx.sync()
-    t_a = time.perf_counter()
-    for j in burst:
-        plain(j)
-    ctx.sync()
-    plain_s = time.perf_counter() - t_a
-    served_mrays = burst_rays / served_s / 1e6
-    plain_mrays = burst_rays / plain_s / 1e6
-    line = {
-        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms; serving p50/p99 latency",
-        "value": round(done_rays / meas / 1e6, 2), "unit": "Mrays/s", "n_gpus": 1,
-        "steps": int(sel.sum()), "warmup": int((~sel).sum()),
-        "ms_per_step": round(float(np.mean(lat_ms)), 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "c5_serving", "resident_meshes": len(slots),
-                   "resident_triangles": int(sum(m.ntri for m in resident)),
-                   "arrival_rate_per_s": lam, "measured_s": meas, "requests": int(sel.sum()),
-                   "uploads": int((sel & upload).sum()), "hbm_budget_bytes": 200 * 10 ** 9,
-                   "offered_mrays_s": round(float(sizes[sel].sum()) / meas / 1e6, 1),
-                   "query": "closest (prim, t) to host", "rays": "half pinhole, half uniform"},
-        "latency_ms": {"p50": round(p50, 3), "p99": round(p99, 3),
-                       "max": round(float(np.max(lat_ms)), 3),
-                       "upload_p50": round(float(np.percentile(up_ms, 50)), 3) if len(up_ms) else None,
-                       "upload_p99": round(float(np.percentile(up_ms, 99)), 3) if len(up_ms) else None},
-        "trace_launches": int(s1["steps"] - s0["steps"]),
-        "mean_launches_per_request": round(float(steps[sel].mean()), 2),
-        "max_requests_per_launch": int(s1["max_step_requests"]),
-        "gpu_trace_busy_fraction": round((s1["busy_s"] - s0["busy_s"]) / wall, 3),
-        "saturation": {"requests": len(burst), "rays": burst_rays,
-                       "served_mrays_s": round(served_mrays, 1),
-                       "plain_per_request_mrays_s": round(plain_mrays, 1),
-                       "speedup": round(served_mrays / plain_mrays, 3)},
-        "server_stats": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in srv_stats.items()},
-        "roofline": None, "cpu_baseline": None,
-    }
+    """Serving (serve_run) for --serve-seconds, with the saturation burst and
+    the plain per-request comparison."""
+    s = serve_run(ctx, 1.0, float(args.serve_seconds), saturation=True)
+    keys = ("resident_meshes", "resident_triangles", "arrival_rate_per_s", "measured_s",
+            "requests", "uploads", "hbm_budget_bytes", "offered_mrays_s", "query", "rays")
+    line = base_line(s["offered_mrays_s"], s["latency_ms"]["mean"], s["requests"], 0,
+                     {"workload": "c5_serving", **{k: s[k] for k in keys}})
+    line.update({k: v for k, v in s.items() if k not in keys})
+    line["roofline"] = None
+    line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
-    for a, b in bufs:
-        a.free()
-        b.free()
     ctx.close()
     return 0
 
 
 def sweep(ctx, bvh, batches) -> None:
     """Time each traversal variant / refill threshold on this frame's batches."""
-    import ctypes as C
-
-    saved = ctx.trace_config()
+    saved = (ctx.trace_config()[:2], ctx.trace_refill())
     for v in ctx.trace_variants():
         for refill in (16, 48, 56, 64):
             row = []
@@ -705,47 +941,8 @@ def sweep(ctx, bvh, batches) -> None:
                 tt.free()
             log(f"variant {v['index']} b{v['block']} d{v['lds_depth']} t{v['tstack']} "
                 f"refill {refill:2d}: " + " ".join(row))
-    ctx.trace_config(*saved)
-
-
-CPU_SAMPLE_RAYS = 3_000_000  # rays per timed pass of the CPU baseline (auto sampling)
-CPU_MIN_SECONDS = 10.0        # passes are repeated until this much time is measured
-
-
-def cpu_baseline(mesh, batches, sample_every: int, threads: int,
-                 what: str = "of each batch of one frame (primary, bounce, shadow as closest hit)",
-                 min_seconds: float = CPU_MIN_SECONDS):
-    """The reference CPU path restated on triangles (oracle/port.cpp), timed
-    on this host: median-split BVH build + collect-then-test closest hit on
-    every `sample_every`-th active ray of each batch (0: as many as give about
-    CPU_SAMPLE_RAYS rays), repeated until `min_seconds` have been timed."""
-    from oracle import oracle as O
-
-    t0 = time.perf_counter()
-    port = O.PortBVH(mesh)
-    build_s = time.perf_counter() - t0
-    if sample_every <= 0:
-        active = sum(int((h["tmin"] < h["tmax"]).sum()) for h in batches)
-        sample_every = max(1, active // CPU_SAMPLE_RAYS)
-    sample = {k: [] for k in O.RAY_FIELDS}
-    for host in batches:
-        active = host["tmin"] < host["tmax"]
-        idx = np.nonzero(active)[0][::sample_every]
-        for k in O.RAY_FIELDS:
-            sample[k].append(host[k][idx])
-    sample = {k: np.concatenate(v) for k, v in sample.items()}
-    n = len(sample["ox"])
-    passes, dt = 0, 0.0
-    while passes == 0 or dt < min_seconds:
-        t0 = time.perf_counter()
-        port.closest(sample, threads=threads)
-        dt += time.perf_counter() - t0
-        passes += 1
-    return dict(value=round(n * passes / dt / 1e6, 4), unit="Mrays/s", cores=threads, kind="port",
-                sample=f"{'every' if sample_every == 1 else f'every {sample_every}th'} active ray "
-                       f"{what}: {n} rays, {passes} passes in "
-                       f"{dt:.1f} s; median-split BVH built in {build_s * 1e3:.0f} ms",
-                build_ms=round(build_s * 1e3, 1), rays=n * passes, seconds=round(dt, 4))
+    ctx.trace_config(*saved[0])
+    ctx.trace_refill(*saved[1])
 
 
 def main() -> int:
@@ -756,21 +953,25 @@ def main() -> int:
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="c5: measured seconds of arrivals (after 1 s of warm-up)")
+    ap.add_argument("--serve-seconds-short", type=float, default=2.0,
+                    help="c2: measured seconds of the serving run reported beside the frame")
+    ap.add_argument("--no-serve", action="store_true", help="c2: no serving run")
     ap.add_argument("--traffic", default="auto", choices=["auto", "off"],
-                    help="measure HBM bytes per launch with two rocprofv3 --pmc child runs")
+                    help="HBM bytes and issue counters per launch from rocprofv3 --pmc child runs")
     ap.add_argument("--pmc-probe", action="store_true",
                     help="(child of --traffic) run the workload briefly, then a 1 GiB copy")
-    ap.add_argument("--cpu-sample", type=int, default=0,
-                    help="CPU baseline on 1/k of each ray batch (0: about 3 M rays)")
+    ap.add_argument("--cpu-rays", type=int, default=CPU_SAMPLE_RAYS,
+                    help="rays of one timed CPU-baseline pass")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", action="store_true",
                     help="time every traversal variant on this frame's batches (stderr)")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the traversal variants on the workload's rays instead of "
+                         "the fixed configuration")
     ap.add_argument("--treelets", type=int, default=0, metavar="R",
                     help="treelet restructuring rounds after every LBVH build (0: plain LBVH)")
-    ap.add_argument("--trace-config", default=None, metavar="VC,RC,VA,RA",
-                    help="fix the traversal variants and refill thresholds (closest, any) "
-                         "instead of autotuning")
+    ap.add_argument("--trace-config", default=None, metavar="VC,RC,VA,RA[,SC,SA]",
+                    help="fix the traversal variants, refill thresholds and slices")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
@@ -781,7 +982,6 @@ def main() -> int:
               "would split across GPUs with no exchange but a gather of the hits). "
               f"--gpus {args.gpus} is not supported.", file=sys.stderr)
         return 2
-
     ctx = rq.Context(0)
     if args.copy_calibration:
         nb = args.copy_calibration
@@ -791,171 +991,8 @@ def main() -> int:
         ctx.sync()
         print(json.dumps({"copy_bytes": nb, "launches": 3}), flush=True)
         return 0
-    if args.config == "c3":
-        return run_c3(ctx, args)
-    if args.config == "c4":
-        return run_c4(ctx, args)
-    if args.config == "c5":
-        return run_c5(ctx, args)
-    mesh, params, workload = (W.c2 if args.config == "c2" else W.c1)()
-    log(f"{workload}: {mesh.ntri} triangles")
-    scene = ctx.upload_scene(mesh)
-    bvh = ctx.build_lbvh(scene, rq.MORTON_SCENE_AABB)
-    if args.treelets:
-        ctx.treelets(bvh, args.treelets)
-    ren = rq.Renderer(ctx, params)
-    if args.trace_config:  # fixed configuration: every launch, warm-up included, uses it
-        apply_trace_config(ctx, parse_trace_config(args.trace_config))
-
-    for _ in range(args.warmup):
-        rebuild(ctx, bvh, args)
-        ren.frame(bvh)
-    ctx.sync()
-    cnt = ren.counts()
-    rays_per_frame = int(cnt.primary + cnt.bounce + cnt.shadow)
-    info = ctx.bvh_info(bvh)
-
-    # canonical traversal counts of this frame's batches (outside the timed region)
-    batches = {"primary": ren.rays(0), "shadow": ren.rays(2)}
-    if params.bounce:
-        batches["bounce"] = ren.rays(1)
-    counts = {k: counted_batch(ctx, bvh, r, any_hit=(k == "shadow")) for k, r in batches.items()}
-    if args.sweep:
-        sweep(ctx, bvh, batches)
-    tuned = None
-    cfg = tune_trace(ctx, bvh, batches["primary"], batches["shadow"], args, repeats=3)
-    if cfg:
-        tuned = tuned_dict(ctx, cfg)
-        log(f"autotune: {tuned}")
-        for _ in range(2):
-            rebuild(ctx, bvh, args)
-            ren.frame(bvh)
-        ctx.sync()
-
-    if args.pmc_probe:
-        for _ in range(2):
-            rebuild(ctx, bvh, args)
-            ren.frame(bvh)
-        ctx.sync()
-        copy_probe(ctx)
-        return 0
-
-    # ---- timed region: K frames, HIP events on the launch streams around the
-    # traversal launches only (the dominant-kernel candidates): events around
-    # every kernel cost ~4 % of a C2 frame
-    ctx.profile_only(TRACE_KERNELS)
-    ctx.profile(True)
-    ctx.profile_reset()
-    ctx.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        rebuild(ctx, bvh, args)
-        ren.frame(bvh)
-    ctx.sync()
-    t1 = time.perf_counter()
-    ctx.profile(False)
-    prof = ctx.profile_read()
-    assert ctx.device_errors() == 0
-    elapsed = t1 - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    value = rays_per_frame * args.steps / elapsed / 1e6
-
-    # ---- per-kernel breakdown (build and shading included) from more frames
-    # after the timed region, every kernel bracketed by events
-    ctx.profile_only(None)
-    ctx.profile(True)
-    ctx.profile_reset()
-    for _ in range(BREAKDOWN_FRAMES):
-        rebuild(ctx, bvh, args)
-        ren.frame(bvh)
-    ctx.sync()
-    ctx.profile(False)
-    breakdown = ctx.profile_read()
-
-    # ---- roofline of the dominant kernel (algorithmic bytes / event time)
-    n = ren.n
-    closest_rays = [("primary", n)] + ([("bounce", n)] if params.bounce else [])
-    closest_bytes = sum(r * (32 + 8) + 32 * counts[k][0] + 36 * counts[k][1]
-                        for k, r in closest_rays) / len(closest_rays)
-    nshadow = batches["shadow"].n
-    any_bytes = nshadow * (32 + 1) + 32 * counts["shadow"][0] + 36 * counts["shadow"][1]
-    per_kernel = {k: (ms, c) for k, (ms, c) in prof.items()}
-    dom = max(per_kernel, key=lambda k: per_kernel[k][0])
-    ms_tot, launches = per_kernel[dom]
-    avg_ms = ms_tot / launches
-    # the shadow rays may go out in several launches per frame (primary hits'
-    # and second hits' concurrently): bytes per launch = frame bytes / launches
-    any_launches = max(1, per_kernel.get("trace_any", (0, args.steps))[1] // args.steps)
-    alg = {"trace_closest": closest_bytes, "trace_any": any_bytes / any_launches}.get(dom)
-    roofline = None
-    if alg is not None:
-        achieved = alg / (avg_ms * 1e-3) / 1e9
-        roofline = dict(kernel=dom, bound="hbm", achieved=round(achieved, 1),
-                        peak=HBM_PEAK_GBS, unit="GB/s", frac=round(achieved / HBM_PEAK_GBS, 4),
-                        traffic=None, bytes_per_launch=int(alg),
-                        avg_launch_ms=round(avg_ms, 4))
-    build_kernels = ("build_init", "centroid_bounds", "morton_hist", "radix_onesweep", "karras",
-                     "refit", "refit_top", "widen")
-    build_ms = sum(breakdown[k][0] for k in build_kernels if k in breakdown) / BREAKDOWN_FRAMES
-    closest_ms = per_kernel.get("trace_closest", (0, 1))[0] / args.steps
-    closest_rays_n = sum(r for _, r in closest_rays)
-    if roofline is not None and args.traffic == "auto":
-        roofline["traffic"] = measure_traffic(args.config, roofline["kernel"], cfg_arg(cfg))
-        roofline["counters"] = measure_counters(args.config, roofline["kernel"], cfg_arg(cfg))
-    cpu = None
-    if not args.no_cpu:
-        host = [batches[k].download() for k in batches]
-        threads = max(1, min(16, os.cpu_count() or 1))
-        log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(mesh, host, args.cpu_sample, threads)
-
-    line = {
-        "metric": "Mrays/s closest-hit (+% HBM roofline); LBVH build ms",
-        "value": round(value, 2),
-        "unit": "Mrays/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {
-            "workload": workload,
-            "triangles": mesh.ntri,
-            "width": params.width, "height": params.height, "spp": params.spp,
-            "bounce": int(params.bounce),
-            "rays_per_frame": rays_per_frame,
-            "rays_primary": int(cnt.primary), "rays_bounce": int(cnt.bounce),
-            "rays_shadow": int(cnt.shadow),
-            "lbvh_rebuilt_per_frame": True,
-            "bvh_height": int(info.height),
-        },
-        "lbvh_build_ms": round(build_ms, 4),
-        # the build's byte model: 276 B per triangle (Morton 56, sort 68, Karras 28,
-        # refit 124) against 8 TB/s
-        "lbvh_build_roofline": {"bytes_per_tri": BUILD_MODEL_BYTES_PER_TRI,
-                                "achieved_gbs": round(mesh.ntri * BUILD_MODEL_BYTES_PER_TRI
-                                                      / (build_ms * 1e-3) / 1e9, 1),
-                                "frac": round(mesh.ntri * BUILD_MODEL_BYTES_PER_TRI
-                                              / (build_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
-        "closest_hit_mrays_s": round(closest_rays_n / (closest_ms * 1e-3) / 1e6, 1)
-        if closest_ms else None,
-        # per frame, from BREAKDOWN_FRAMES frames after the timed region with
-        # every kernel bracketed by events
-        "kernel_ms_per_step": {k: round(v[0] / BREAKDOWN_FRAMES, 4) for k, v in breakdown.items()},
-        "traversal_counts": {k: {"node_visits": v[0], "tri_tests": v[1]} for k, v in
-                             counts.items()},
-        "autotune": tuned,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    ren.close()
-    ctx.close()
-    return 0
+    run = {"c3": run_c3, "c4": run_c4, "c5": run_c5}.get(args.config, run_frames)
+    return run(ctx, args)
 
 
 if __name__ == "__main__":

@@ -90,6 +90,7 @@ def lib() -> C.CDLL:
                                          C.POINTER(ShadeParams), vp, vp, vp, vp, vp, vp, i32]),
             "or_resolve_ppm": (None, [vp, u32, u32, vp]),
             "port_bvh_build": (vp, [vp, vp, vp, vp, u32, vp]),
+            "port_lbvh_build": (vp, [vp, vp, vp, vp, u32, vp]),
             "port_bvh_free": (None, [vp]),
             "port_closest": (None, [vp] + [vp] * 8 + [u32, vp, vp, i32]),
         }
@@ -407,12 +408,14 @@ def resolve_ppm(rgb: np.ndarray, npix: int, spp: int) -> np.ndarray:
 
 
 class PortBVH:
-    """CPU baseline: the reference's median-split BVH + collect-then-test query."""
+    """CPU baseline: the reference's median-split BVH (constructBVHNew) or its
+    "LBVH" (constructLBVHTree, lbvh=True) + collect-then-test query."""
 
-    def __init__(self, mesh):
+    def __init__(self, mesh, lbvh: bool = False):
         (vx, vy, vz, idx), self._keep = mesh_ptrs(mesh)
         nn = C.c_uint32()
-        self.h = lib().port_bvh_build(vx, vy, vz, idx, mesh.ntri, C.byref(nn))
+        fn = lib().port_lbvh_build if lbvh else lib().port_bvh_build
+        self.h = fn(vx, vy, vz, idx, mesh.ntri, C.byref(nn))
         self.n_nodes = nn.value
 
     def __del__(self):

@@ -8,6 +8,8 @@
 //           slab test of accelerators.h:588-626 (divisions, no t clip), then
 //           the candidate loop of main.cpp:343-358 (strict <, so the first
 //           candidate in DFS order wins ties).
+//   build : constructLBVHTree, accelerators.h:570-586 (port_lbvh_build): the
+//           reference's "LBVH", timed as the reference LBVH build
 // Deviations, each needed for triangles: the centroid bounds start empty
 // (the reference seeds them with the node bounds, :275, which for unequal
 // primitive boxes can leave every centroid on one side and recurse without
@@ -17,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <list>
 #include <thread>
 #include <vector>
 
@@ -102,6 +105,70 @@ int build(std::vector<Obj>& o, int s, int e, PortBVH& t) {
   return me;
 }
 
+// ---- constructLBVHTree (accelerators.h:570-586) ------------------------------
+// expandBits / morton3D, accelerators.h:374-394 (multiply-mask spread, clamp to
+// [0, 1023] after scaling by 1024, x in the high bit of each triple).
+inline uint32_t expand_bits(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+inline uint32_t morton3d(float x, float y, float z) {
+  x = std::min(std::max(x * 1024.0f, 0.0f), 1023.0f);
+  y = std::min(std::max(y * 1024.0f, 0.0f), 1023.0f);
+  z = std::min(std::max(z * 1024.0f, 0.0f), 1023.0f);
+  return expand_bits((uint32_t)x) * 4 + expand_bits((uint32_t)y) * 2 + expand_bits((uint32_t)z);
+}
+// radixSort, accelerators.h:348-369: a decimal LSD sort of `max` passes into ten
+// std::list pockets (push_back per element, then erase from the front), on a
+// vector taken by value, so the sorted copy is discarded. The reference's
+// int m = pow(10, i + 1) overflows at i = 9; the pass divisors here are 64-bit
+// (same work, no undefined behaviour). Returns a checksum so the work is kept.
+uint64_t radix_sort_copy(std::vector<uint32_t> codes, int max) {
+  std::list<uint32_t> pocket[10];
+  uint64_t p = 1;
+  for (int i = 0; i < max; ++i, p *= 10) {
+    for (uint32_t c : codes) pocket[(c % (p * 10)) / p].push_back(c);
+    size_t k = 0;
+    for (int j = 0; j < 10; ++j)
+      while (!pocket[j].empty()) {
+        codes[k++] = *pocket[j].begin();
+        pocket[j].erase(pocket[j].begin());
+      }
+  }
+  return codes.empty() ? 0 : codes.front() + codes.back();
+}
+// findSplit, accelerators.h:397-449 (vector taken by value; leading zeros of
+// the low 8 bits only); constructLBVHNew discards its result (:518).
+int find_split(std::vector<uint32_t> codes, int first, int last) {
+  auto prefix8 = [](uint32_t c) {
+    int len = -1;
+    while ((++len < 8) && ((c & 0x80) == 0)) c <<= 1;
+    return len;
+  };
+  const uint32_t fc = codes[first], lc = codes[last];
+  if (fc == lc) return (first + last) >> 1;
+  const int common = prefix8(fc ^ lc);
+  int split = first, step = last - first;
+  do {
+    step = (step + 1) >> 1;
+    const int ns = split + step;
+    if (ns < last && prefix8(fc ^ codes[ns]) > common) split = ns;
+  } while (step > 1);
+  return split;
+}
+// Appends tree b (built on its own) to t, its node indices shifted.
+int append_tree(PortBVH& t, PortBVH&& b) {
+  const int off = (int)t.nodes.size();
+  for (Node& n : b.nodes) {
+    if (n.left >= 0) n.left += off, n.right += off;
+    t.nodes.push_back(std::move(n));
+  }
+  return b.root + off;
+}
+
 // accelerators.h:588-626
 inline bool slab(V3 p, V3 d, const Box& b) {
   float tmin = (b.lo.x - p.x) / d.x, tmax = (b.hi.x - p.x) / d.x;
@@ -172,6 +239,90 @@ void* port_bvh_build(const float* vx, const float* vy, const float* vz, const ui
     t->nodes.reserve(2 * (size_t)ntri);
     t->root = build(objs, 0, (int)ntri, *t);
   }
+  if (n_nodes) *n_nodes = (uint32_t)t->nodes.size();
+  return t;
+}
+
+// constructLBVHTree (accelerators.h:570-586) on the same objects: Morton codes
+// of the centres mapped by (c + 30) / 1000 (:576-580), radixSort of a copy
+// (:581, discarded), then constructLBVHNew (:452-567) at the root: bounds,
+// centroid bounds, findSplit on a copy (discarded), the median split, and the
+// two halves built by constructBVHNew, on two std::threads when the root holds
+// more than 500 objects (:543-548). Every object is kept (the reference passes
+// size - 1 as the end, :583, and drops the last one). Queries run on the
+// result like on port_bvh_build's.
+void* port_lbvh_build(const float* vx, const float* vy, const float* vz, const uint32_t* idx,
+                      uint32_t ntri, uint32_t* n_nodes) {
+  auto* t = new PortBVH();
+  std::vector<Obj> objs(ntri);
+  t->tri.resize((size_t)ntri * 9);
+  for (uint32_t p = 0; p < ntri; ++p) {
+    V3 a{vx[idx[3 * p]], vy[idx[3 * p]], vz[idx[3 * p]]};
+    V3 b{vx[idx[3 * p + 1]], vy[idx[3 * p + 1]], vz[idx[3 * p + 1]]};
+    V3 c{vx[idx[3 * p + 2]], vy[idx[3 * p + 2]], vz[idx[3 * p + 2]]};
+    Obj& o = objs[p];
+    o.id = (int)p;
+    o.center = {((a.x + b.x) + c.x) / 3.0f, ((a.y + b.y) + c.y) / 3.0f, ((a.z + b.z) + c.z) / 3.0f};
+    o.box = joinp(joinp(joinp(Box(), a), b), c);
+    V3 e1 = sub(b, a), e2 = sub(c, a);
+    float* T = &t->tri[9 * (size_t)p];
+    T[0] = a.x, T[1] = a.y, T[2] = a.z, T[3] = e1.x, T[4] = e1.y, T[5] = e1.z, T[6] = e2.x,
+    T[7] = e2.y, T[8] = e2.z;
+  }
+  const int n = (int)ntri;
+  std::vector<uint32_t> codes(ntri);
+  for (int i = 0; i < n; ++i) {
+    const V3 m = objs[i].center;
+    codes[i] = morton3d((m.x + 30) / 1000, (m.y + 30) / 1000, (m.z + 30) / 1000);
+  }
+  volatile uint64_t keep = radix_sort_copy(codes, 10);
+  (void)keep;
+  if (n == 0) {
+    if (n_nodes) *n_nodes = 0;
+    return t;
+  }
+  t->nodes.reserve(2 * (size_t)ntri);
+  t->nodes.emplace_back();
+  t->root = 0;
+  Box bounds;
+  for (int i = 0; i < n; ++i) bounds = join(bounds, objs[i].box);
+  t->nodes[0].box = bounds;
+  Box cb;
+  for (int i = 0; i < n; ++i) cb = joinp(cb, objs[i].center);
+  const int dim = max_axis(cb);
+  if (n <= 1 || cb.hi[dim] == cb.lo[dim]) {
+    for (int i = 0; i < n; ++i) t->nodes[0].objs.push_back(objs[i].id);
+    if (n_nodes) *n_nodes = 1;
+    return t;
+  }
+  volatile int split = find_split(codes, 0, n - 1);
+  (void)split;
+  const float pmid = (cb.lo[dim] + cb.hi[dim]) / 2;
+  Obj* mid = std::partition(&objs[0], &objs[n - 1] + 1,
+                            [dim, pmid](const Obj& p) { return p.center[dim] < pmid; });
+  int m = (int)(mid - &objs[0]);
+  if (m != 0 && m != n) {
+    m = n / 2;
+    std::nth_element(&objs[0], &objs[m], &objs[n - 1] + 1,
+                     [dim](const Obj& a, const Obj& b) { return a.center[dim] < b.center[dim]; });
+  }
+  if (m == 0 || m == n) m = n / 2;
+  PortBVH lt, rt;
+  lt.nodes.reserve(2 * (size_t)m);
+  rt.nodes.reserve(2 * (size_t)(n - m));
+  if (n > 500) {
+    std::thread tl([&] { lt.root = build(objs, 0, m, lt); });
+    std::thread tr([&] { rt.root = build(objs, m, n, rt); });
+    tl.join();
+    tr.join();
+  } else {
+    lt.root = build(objs, 0, m, lt);
+    rt.root = build(objs, m, n, rt);
+  }
+  const int l = append_tree(*t, std::move(lt));
+  const int r = append_tree(*t, std::move(rt));
+  t->nodes[0].left = l;
+  t->nodes[0].right = r;
   if (n_nodes) *n_nodes = (uint32_t)t->nodes.size();
   return t;
 }

@@ -124,3 +124,16 @@ def test_port_baseline_matches_brute_force(O):
     bf = O.closest_brute(mesh, rays)
     assert np.array_equal(p["prim"], bf["prim"])
     assert O.PortBVH(mesh).n_nodes == 2 * 3000 - 1
+
+
+@pytest.mark.parametrize("ntri", [300, 3000])
+def test_port_reference_lbvh_matches_brute_force(O, ntri):
+    """The restated constructLBVHTree (Morton codes, radixSort of a copy,
+    root split, halves on two threads above 500 objects) builds a tree whose
+    collect-then-test closest hit equals brute force, every object kept."""
+    mesh = random_soup(ntri, 23, spread=1.0, size=0.05)
+    rays = random_rays(3000, 24, lo=(-1.2, -1.2, -1.2), hi=(1.2, 1.2, 1.2))
+    t = O.PortBVH(mesh, lbvh=True)
+    assert t.n_nodes == 2 * ntri - 1
+    p = t.closest(rays, threads=4)
+    assert np.array_equal(p["prim"], O.closest_brute(mesh, rays)["prim"])
