@@ -243,6 +243,7 @@ int rq_memcpy_d2h(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* s
 /* Device-to-device copy by a library kernel (16-byte aligned pointers/size). */
 int rq_memcpy_d2d(rq_ctx* ctx, void* dst, const void* src, size_t bytes, void* stream);
 int rq_memset_d(rq_ctx* ctx, void* dst, int value, size_t bytes, void* stream);
+/* Waits for the work enqueued on `stream`; NULL: for all work on the device. */
 int rq_sync(rq_ctx* ctx, void* stream);
 /* Reads and clears the device error bits of past queries (synchronises the
  * device): bit 0 = a ray needed a deeper traversal stack than provided. */
@@ -267,6 +268,11 @@ int rq_stream_destroy(rq_ctx* ctx, void* stream);
 /* The context forgets a caller-owned stream: waits for it and frees its
  * overflow area (call before destroying a stream that has queried). */
 int rq_stream_forget(rq_ctx* ctx, void* stream);
+/* Orders `waiter` after the work enqueued so far on `signaler` (an event
+ * recorded on signaler that waiter waits for, no host wait): e.g. the render
+ * of a frame after its BVH's rebuild on another stream, and the next rebuild
+ * of that BVH after the render. */
+int rq_stream_order(rq_ctx* ctx, void* waiter, void* signaler);
 
 /* ---- per-kernel timing (HIP events on the launch stream) ---------------
  * When enabled, every kernel the library launches is bracketed by events on

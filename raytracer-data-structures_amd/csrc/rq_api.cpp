@@ -257,6 +257,7 @@ int rq_destroy(rq_ctx* c) {
   hipSetDevice(c->device);
   hipDeviceSynchronize();
   if (c->prof) c->prof->resolve();
+  for (hipEvent_t e : c->order_events) (void)hipEventDestroy(e);
   for (auto& kv : c->sizes) hipFree(kv.first);
   c->sizes.clear();
   delete c->prof;
@@ -835,7 +836,10 @@ int rq_memset_d(rq_ctx* c, void* dst, int value, size_t bytes, void* stream) {
 }
 int rq_sync(rq_ctx* c, void* stream) {
   if (check_ctx(c)) return RQ_E_INVALID;
-  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  // NULL: every stream of the device (the library's streams do not
+  // synchronise with the null stream)
+  (void)hipSetDevice(c->device);
+  hipError_t e = stream ? hipStreamSynchronize((hipStream_t)stream) : hipDeviceSynchronize();
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream sync");
 }
 int rq_stream_create(rq_ctx* c, void** stream) {
@@ -859,6 +863,25 @@ int rq_stream_forget(rq_ctx* c, void* stream) {
       break;
     }
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream forget");
+}
+
+int rq_stream_order(rq_ctx* c, void* waiter, void* signaler) {
+  if (check_ctx(c)) return RQ_E_INVALID;
+  if (waiter == signaler) return RQ_OK;  // one stream runs in order
+  constexpr size_t kEvents = 16;
+  if (c->order_events.empty()) {
+    (void)hipSetDevice(c->device);
+    for (size_t i = 0; i < kEvents; ++i) {
+      hipEvent_t e = nullptr;
+      const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      if (r != hipSuccess) return hip_err(c, r, "order event");
+      c->order_events.push_back(e);
+    }
+  }
+  hipEvent_t ev = c->order_events[c->order_seq++ % kEvents];
+  hipError_t e = hipEventRecord(ev, (hipStream_t)signaler);
+  if (e == hipSuccess) e = hipStreamWaitEvent((hipStream_t)waiter, ev, 0);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "stream order");
 }
 
 int rq_stream_destroy(rq_ctx* c, void* stream) {

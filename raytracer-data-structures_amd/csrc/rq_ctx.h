@@ -35,6 +35,10 @@ struct rq_ctx {
   int32_t* gstack = nullptr;
   size_t gstack_bytes = 0;
   std::vector<std::pair<void*, int32_t*>> gstack_streams;
+  // events of rq_stream_order, reused round robin (a wait enqueued on a
+  // stream refers to the record that preceded it)
+  std::vector<hipEvent_t> order_events;
+  uint32_t order_seq = 0;
   // device address of the allocation-free entry's scratch on this device
   // (hipGetSymbolAddress at rq_create)
   unsigned char* scratch = nullptr;

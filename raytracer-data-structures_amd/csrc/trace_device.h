@@ -203,15 +203,18 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     }
     sp = min(sp + 1, max_sp);
   };
-  // Pops the next entry into `node`; false when the stack is empty.
+  // Pops the next entry into `node`; false when the stack is empty. The LDS
+  // entry is read unconditionally (at a clamped depth) and replaced by the
+  // overflow entry only past the LDS depth: with one read per branch the
+  // compiler merges them into a generic (flat) load, which takes the vector
+  // memory path and waits on both counters for every pop.
   auto pop = [&]() -> bool {
     while (sp > 0) {
       --sp;
-      float tn = 0.0f;
-      if (sp < DEPTH) {
-        node = s_node[sp * BLOCK + tid];
-        if (TSTACK) tn = s_tn[sp * BLOCK + tid];
-      } else {
+      const int ls = min(sp, DEPTH - 1);
+      node = s_node[ls * BLOCK + tid];
+      float tn = TSTACK ? s_tn[ls * BLOCK + tid] : 0.0f;
+      if (sp >= DEPTH) {
         const size_t g = (size_t)(sp - DEPTH) * gstride + gtid;
         node = a.gstack[2 * g];
         if (TSTACK) tn = __int_as_float(a.gstack[2 * g + 1]);
@@ -418,6 +421,5 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
   }
 
 }
-
 
 }  // namespace rq

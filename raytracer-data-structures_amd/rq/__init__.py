@@ -203,6 +203,7 @@ def _load() -> C.CDLL:
         "rq_device_errors": (i32, [vp, C.POINTER(u32)]),
         "rq_stream_create": (i32, [vp, C.POINTER(vp)]),
         "rq_stream_destroy": (i32, [vp, vp]),
+        "rq_stream_order": (i32, [vp, vp, vp]),
         "rq_stream_forget": (i32, [vp, vp]),
         "rq_profile_enable": (i32, [vp, i32]),
         "rq_profile_reset": (i32, [vp]),
@@ -716,6 +717,10 @@ class Context:
     def stream_destroy(self, s) -> None:
         self._check(lib().rq_stream_destroy(self.h, s))
 
+    def stream_order(self, waiter, signaler) -> None:
+        """rq_stream_order: waiter runs after the work enqueued on signaler."""
+        self._check(lib().rq_stream_order(self.h, waiter, signaler))
+
     def stream_forget(self, s) -> None:
         """rq_stream_forget: free the overflow area of a caller-owned stream."""
         self._check(lib().rq_stream_forget(self.h, s))
@@ -1079,6 +1084,67 @@ class Renderer:
         if self.h:
             lib().rq_renderer_destroy(self.ctx.h, self.h)
             self.h = None
+
+
+class FramePipeline:
+    """Frames that rebuild their LBVH every frame, the rebuild for frame k+1
+    overlapping the render of frame k on the device.
+
+    Two BVHs of the scene alternate, each rebuilt on its own stream
+    (rq_rebuild_lbvh, then rq_bvh_treelets when `treelets` > 0); the renderer
+    runs on a third. rq_stream_order makes a render wait for its own BVH's
+    rebuild and a rebuild wait for the render that last read its BVH, with no
+    host wait. Not pipelined (`pipelined` False): rebuild and render one after
+    the other on the default stream, with one BVH. The scene must not change
+    while frames are in flight. frame() returns the BVH handle it rendered.
+    """
+
+    def __init__(self, ctx: Context, renderer: Renderer, scene: int, bvh: int,
+                 morton_mode: int = MORTON_SCENE_AABB, treelets: int = 0,
+                 pipelined: bool = True):
+        self.ctx, self.ren, self.treelets, self.pipelined = ctx, renderer, treelets, pipelined
+        self.bvhs = [bvh]
+        self.streams: list = []
+        self.k = 0
+        if pipelined:
+            self.bvhs.append(ctx.build_lbvh(scene, morton_mode))
+            self.streams = [ctx.stream_create() for _ in range(3)]
+
+    def _rebuild(self, b: int, stream) -> None:
+        self.ctx.rebuild(b, stream=stream)
+        if self.treelets:
+            self.ctx.treelets(b, self.treelets, stream=stream)
+
+    def frame(self) -> int:
+        if not self.pipelined:
+            self._rebuild(self.bvhs[0], None)
+            self.ren.frame(self.bvhs[0])
+            return self.bvhs[0]
+        k, s_build, s_render = self.k, self.streams[:2], self.streams[2]
+        cur, nxt = k % 2, (k + 1) % 2
+        if k == 0:
+            self._rebuild(self.bvhs[cur], s_build[cur])
+        self.ctx.stream_order(s_build[nxt], s_render)  # after the render that read bvhs[nxt]
+        self._rebuild(self.bvhs[nxt], s_build[nxt])    # frame k+1's BVH
+        self.ctx.stream_order(s_render, s_build[cur])  # after this frame's rebuild
+        self.ren.frame(self.bvhs[cur], stream=s_render)
+        self.k = k + 1
+        return self.bvhs[cur]
+
+    def render_stream(self):
+        """The stream frames render on (None when not pipelined)."""
+        return self.streams[2] if self.pipelined else None
+
+    def close(self) -> None:
+        """Waits for the frames in flight, frees the second BVH and the streams
+        (the BVH passed in stays the caller's)."""
+        self.ctx.sync()
+        for s in self.streams:
+            self.ctx.stream_destroy(s)
+        self.streams = []
+        for b in self.bvhs[1:]:
+            self.ctx.release(b)
+        self.bvhs = self.bvhs[:1]
 
 
 def write_ppm(path: str | Path, w: int, h: int, data: np.ndarray) -> None:
