@@ -190,6 +190,33 @@ int rq_any_hit(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n, ui
  * must be ordered by the caller. Results equal rq_closest_hit's. */
 int rq_closest_hit_noalloc(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
                            rq_hits* hits, void* stream);
+/* Host-pointer forms, for callers whose rays and results live in host memory
+ * (no counterpart of their own in the reference, whose queries take host
+ * values: they let its per-pixel loop call GPU_LBVH unchanged). Synchronous:
+ * the rays are staged through a pinned buffer and a device buffer the context
+ * keeps (grown on demand, at most 48 MiB each, the device one charged to the
+ * budget), in launches of at most 2^20 rays, and the call returns once the
+ * results are in the caller's arrays. `rays`/`hits`/`occluded` point to HOST
+ * arrays; hits->t/u/v may be NULL. Results equal rq_closest_hit's and
+ * rq_any_hit's. Stack-depth errors are reported by rq_device_errors as for
+ * the device queries. */
+int rq_closest_hit_host(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                        rq_hits* hits, void* stream);
+int rq_any_hit_host(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,
+                    uint8_t* occluded, void* stream);
+/* One ray from host values, the form of the reference's per-ray calls:
+ * rq_cast_ray replaces boxIntersect(rayorig, raydir, tree, boxes) plus the
+ * candidate loop of castRay (main.cpp:333-358): the closest primitive with t
+ * in (tmin, tmax) (-1 on a miss; t = +inf then), lowest index on ties; uv
+ * (2 floats, may be NULL) receives the barycentrics. rq_occluded_ray replaces
+ * kdtreeIntersect(rayorig, raydir) (main.cpp:362-366, accelerators.h:997):
+ * 1 iff some primitive has t in (tmin, tmax). Each call is a launch and a
+ * host wait (tens of microseconds): a loop over many rays is faster through
+ * rq_closest_hit_host on arrays. */
+int rq_cast_ray(rq_ctx* ctx, const rq_bvh* b, const float orig[3], const float dir[3], float tmin,
+                float tmax, int32_t* prim, float* t, float* uv);
+int rq_occluded_ray(rq_ctx* ctx, const rq_bvh* b, const float orig[3], const float dir[3],
+                    float tmin, float tmax, uint8_t* occluded);
 /* Instrumented variants: per-ray node-record visits and triangle tests of the
  * canonical traversal (device uint32 arrays). Slower; for measurement. */
 int rq_closest_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32_t n,

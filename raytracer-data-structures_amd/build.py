@@ -24,7 +24,8 @@ LIB = PKG / "librq.so"
 
 HIP_SOURCES = ["build_kernels.hip", "trace_kernels.hip", "render_kernels.hip", "batch_kernels.hip",
                "treelet_kernels.hip"]
-HIP_HOST_SOURCES = ["rq_api.cpp", "render.cpp", "batch.cpp", "serve.cpp", "bvh_blob.cpp"]
+HIP_HOST_SOURCES = ["rq_api.cpp", "render.cpp", "batch.cpp", "serve.cpp", "bvh_blob.cpp",
+                    "host_query.cpp"]
 CXX_SOURCES = ["host/scene_gen.cpp"]
 
 ARCH = os.environ.get("RQ_OFFLOAD_ARCH", "gfx950")

@@ -258,6 +258,7 @@ int rq_destroy(rq_ctx* c) {
   hipDeviceSynchronize();
   if (c->prof) c->prof->resolve();
   for (hipEvent_t e : c->order_events) (void)hipEventDestroy(e);
+  stage_release(c);
   for (auto& kv : c->sizes) hipFree(kv.first);
   c->sizes.clear();
   delete c->prof;

@@ -39,6 +39,11 @@ struct rq_ctx {
   // stream refers to the record that preceded it)
   std::vector<hipEvent_t> order_events;
   uint32_t order_seq = 0;
+  // staging of the host-pointer queries (host_query.cpp): a pinned host
+  // buffer and a device buffer (charged to the budget) of stage_bytes each
+  void* stage_host = nullptr;
+  void* stage_dev = nullptr;
+  size_t stage_bytes = 0;
   // device address of the allocation-free entry's scratch on this device
   // (hipGetSymbolAddress at rq_create)
   unsigned char* scratch = nullptr;
@@ -96,4 +101,6 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide);
 // Makes the tree height in b's misc words known (blocking; the chunk-union
 // refit leaves it to this call).
 int ensure_height(rq_ctx* c, const rq_bvh* b);
+// Frees the pinned staging buffer of the host-pointer queries (rq_destroy).
+void stage_release(rq_ctx* c);
 }  // namespace rq

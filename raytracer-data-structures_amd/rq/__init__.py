@@ -180,6 +180,13 @@ def _load() -> C.CDLL:
         "rq_bvh_export_wide": (i32, [vp, vp, vp, vp]),
         "rq_closest_hit": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp]),
         "rq_any_hit": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp]),
+        "rq_closest_hit_host": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp]),
+        "rq_any_hit_host": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp]),
+        "rq_cast_ray": (i32, [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                              C.c_float, C.POINTER(C.c_int32), C.POINTER(C.c_float),
+                              C.POINTER(C.c_float)]),
+        "rq_occluded_ray": (i32, [vp, vp, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                  C.c_float, C.POINTER(C.c_uint8)]),
         "rq_closest_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, C.POINTER(Hits), vp, vp,
                                          vp]),
         "rq_any_hit_counted": (i32, [vp, vp, C.POINTER(Rays), u32, vp, vp, vp, vp]),
@@ -661,6 +668,43 @@ class Context:
             occ.free()
             if cnt:
                 cnt[0].free(), cnt[1].free()
+
+    def closest_hit_host(self, bvh: int, rays: dict, uv: bool = True) -> dict:
+        """rq_closest_hit_host: rays and results in host memory (staged by the library)."""
+        n = len(rays["ox"])
+        cols = {k: np.ascontiguousarray(rays[k], np.float32) for k in RAY_FIELDS}
+        hr = Rays(*[_ptr(cols[k]) for k in RAY_FIELDS])
+        out = {"prim": np.empty(n, np.int32), "t": np.empty(n, np.float32)}
+        if uv:
+            out["u"], out["v"] = np.empty(n, np.float32), np.empty(n, np.float32)
+        h = Hits(_ptr(out["prim"]), _ptr(out["t"]), _ptr(out.get("u")), _ptr(out.get("v")))
+        self._check(lib().rq_closest_hit_host(self.h, bvh, C.byref(hr), n, C.byref(h), None))
+        return out
+
+    def any_hit_host(self, bvh: int, rays: dict) -> np.ndarray:
+        """rq_any_hit_host: occlusion flags of host rays."""
+        n = len(rays["ox"])
+        cols = {k: np.ascontiguousarray(rays[k], np.float32) for k in RAY_FIELDS}
+        hr = Rays(*[_ptr(cols[k]) for k in RAY_FIELDS])
+        occ = np.empty(n, np.uint8)
+        self._check(lib().rq_any_hit_host(self.h, bvh, C.byref(hr), n, _ptr(occ), None))
+        return occ
+
+    def cast_ray(self, bvh: int, orig, d, tmin: float = 0.0, tmax: float = float("inf")):
+        """rq_cast_ray: (prim, t, (u, v)) of one host ray."""
+        o3, d3 = (C.c_float * 3)(*orig), (C.c_float * 3)(*d)
+        prim, t, uv = C.c_int32(), C.c_float(), (C.c_float * 2)()
+        self._check(lib().rq_cast_ray(self.h, bvh, o3, d3, tmin, tmax, C.byref(prim),
+                                      C.byref(t), uv))
+        return prim.value, t.value, (uv[0], uv[1])
+
+    def occluded_ray(self, bvh: int, orig, d, tmin: float = 0.0,
+                     tmax: float = float("inf")) -> bool:
+        """rq_occluded_ray: whether one host ray hits anything in (tmin, tmax)."""
+        o3, d3 = (C.c_float * 3)(*orig), (C.c_float * 3)(*d)
+        occ = C.c_uint8()
+        self._check(lib().rq_occluded_ray(self.h, bvh, o3, d3, tmin, tmax, C.byref(occ)))
+        return bool(occ.value)
 
     def raygen_uniform(self, n: int, seed: int, lo, hi, stream=None) -> DeviceRays:
         r = DeviceRays(self, n)
