@@ -101,3 +101,37 @@ def test_scene_aabb_flat_axis(O):
     codes = O.morton_codes(mesh, 1)
     z_bits = sum(1 << (3 * k) for k in range(10))
     assert all((int(c) & z_bits) == 0 for c in codes)
+
+
+def _centroids(O, mesh):
+    import ctypes as C
+
+    (vx, vy, vz, idx), keep = O.mesh_ptrs(mesh)
+    out = (C.c_float * 3)()
+    O.lib().or_centroid.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p]
+    res = []
+    for t in range(mesh.ntri):
+        O.lib().or_centroid(vx, vy, vz, idx, t, C.cast(out, C.c_void_p))
+        res.append(tuple(np.float32(x) for x in out))
+    return res
+
+
+def test_centroid_closed_forms(O):
+    """Triangle centroids with distinct vertices, where the value is fixed:
+    exact sums give the exact mean; 1e8 + 1 rounds to 1e8 in float32, so the
+    order (v0 + v1) + v2 decides whether the 1 survives; the mean is a
+    correctly rounded division by 3 (5 / 3 differs from 5 * float32(1/3))."""
+    tris = [((0, 0, 0), (3, 0, 0), (0, 6, 9)),            # (1, 2, 3)
+            ((-1.5, 2, 7), (4.5, -8, 1), (0, 0, 1)),      # (1, -2, 3)
+            ((1e8, 0, 0), (1, 0, 0), (-1e8, 0, 0)),       # ((1e8 + 1) - 1e8) / 3 = 0
+            ((1e8, 0, 0), (-1e8, 0, 0), (1, 0, 0)),       # ((1e8 - 1e8) + 1) / 3 = 1/3
+            ((5, 0, 0), (0, 0, 0), (0, 0, 0))]            # 5 / 3 correctly rounded
+    v = [p for t in tris for p in t]
+    got = _centroids(O, mesh_from(v, np.arange(len(v))))
+    third = np.float32(1) / np.float32(3)
+    expect = [(1, 2, 3), (1, -2, 3), (0, 0, 0), (third, 0, 0),
+              (np.float32(5) / np.float32(3), 0, 0)]
+    for g, e in zip(got, expect):
+        assert [np.float32(x).view(np.uint32) for x in g] == \
+            [np.float32(x).view(np.uint32) for x in e], (g, e)
+    assert got[4][0] != np.float32(5) * np.float32(1 / 3)

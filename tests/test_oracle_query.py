@@ -137,3 +137,65 @@ def test_port_reference_lbvh_matches_brute_force(O, ntri):
     assert t.n_nodes == 2 * ntri - 1
     p = t.closest(rays, threads=4)
     assert np.array_equal(p["prim"], O.closest_brute(mesh, rays)["prim"])
+
+
+def _slab_hits(boxes, rays, slack=np.float32(1.000030517578125)):
+    """Boxes (m, 6) hit by each ray with tmax = +inf, by the conservative slab
+    test (near planes by the sign of 1/d, far bound times the slack)."""
+    o = np.stack([rays["ox"], rays["oy"], rays["oz"]], 1).astype(np.float32)
+    with np.errstate(divide="ignore"):
+        inv = (np.float32(1) / np.stack([rays["dx"], rays["dy"], rays["dz"]], 1)).astype(
+            np.float32)
+    lo, hi = boxes[None, :, :3], boxes[None, :, 3:]
+    neg = np.signbit(inv)[:, None, :]
+    near = np.where(neg, hi, lo)
+    far = np.where(neg, lo, hi)
+    tn = ((near - o[:, None, :]) * inv[:, None, :]).astype(np.float32)
+    tf = ((far - o[:, None, :]) * inv[:, None, :]).astype(np.float32)
+    tmin = rays["tmin"][:, None].astype(np.float32)
+    tnear = np.fmax(np.fmax(tn[..., 0], tn[..., 1]), np.fmax(tn[..., 2], tmin))
+    tfar = np.fmin(np.fmin(tf[..., 0], tf[..., 1]), tf[..., 2])
+    return tnear <= (tfar * slack).astype(np.float32)
+
+
+def test_counts_of_rays_that_hit_nothing(O):
+    """For a ray that hits no triangle nothing is culled, and child boxes lie
+    inside their parents' (exact unions, monotone slab test), so the walk
+    reaches exactly the nodes whose own box the ray hits: visits = 2 x (the
+    root + internal nodes with a hit box), tests = leaves with a hit box."""
+    mesh = random_soup(300, 31, spread=1.0, size=0.08)
+    rays = random_rays(3000, 32, lo=(-1.3, -1.3, -1.3), hi=(1.3, 1.3, 1.3))
+    ob = O.OracleBVH(mesh, 1)
+    o = ob.closest(rays)
+    a = ob.any(rays)
+    miss = np.flatnonzero(o["prim"] < 0)
+    assert len(miss) > 1000
+    sub = {k: v[miss] for k, v in rays.items()}
+    boxes = ob.arrays()["boxes"].astype(np.float32)
+    n = mesh.ntri
+    hit = _slab_hits(boxes, sub)
+    internal = hit[:, 1:n - 1].sum(1)   # rows 1..n-2: internal nodes other than the root
+    leaves = hit[:, n - 1:].sum(1)      # rows n-1..: leaves
+    assert np.array_equal(o["visits"][miss], 2 * (1 + internal))
+    assert np.array_equal(o["tests"][miss], leaves)
+    assert np.array_equal(a["visits"][miss], o["visits"][miss])
+    assert np.array_equal(a["tests"][miss], o["tests"][miss])
+    assert leaves.max() > 3
+
+
+def test_counts_when_every_box_is_hit(O, rq):
+    """n identical triangles and a ray through them all at one t: closest hit
+    visits every internal node (2 (n - 1) records) and tests every leaf; any
+    hit follows the left links (ties go left) to the first leaf and stops."""
+    for n in (2, 37, 64):
+        mesh = duplicate_mesh(n)
+        ob = O.OracleBVH(mesh, 1)
+        rays = rays_toward(1, 0, target=(0.0, 0.0, -5.0), spread=0.0)
+        o, a = ob.closest(rays), ob.any(rays)
+        assert o["prim"][0] == 0 and o["visits"][0] == 2 * (n - 1) and o["tests"][0] == n
+        left = ob.arrays()["left"]
+        depth, node = 0, 0
+        while node >= 0:
+            depth += 1
+            node = left[node]
+        assert a["occ"][0] == 1 and a["tests"][0] == 1 and a["visits"][0] == 2 * depth
