@@ -10,6 +10,15 @@
  * phases. rq_batch_run_graph captures the launch (and its counter reset) in a
  * hipGraph once and replays it. The results of every mesh are identical to
  * rq_build_lbvh + rq_closest_hit / rq_any_hit on that mesh alone.
+ *
+ * Co-residency: the grid is sized from the kernel's occupancy (blocks per CU
+ * x CUs), but the phase barriers never assume that every block is placed:
+ * they count only the blocks that have started. The first barrier waits
+ * briefly for the rest of the grid, then closes the membership; blocks the
+ * device places later (behind kernels of other streams that held CUs) leave
+ * without work. Work items come from queues, so the members complete every
+ * phase: a contended run is slower, never wrong. rq_batch_occupancy reports
+ * how many blocks took part in the last run.
  */
 #ifndef RQ_BATCH_H
 #define RQ_BATCH_H
@@ -61,10 +70,14 @@ int rq_batch_autotune(rq_ctx* ctx, rq_batch* b, int repeats, int* best_index,
 /* Items of the work queue per phase (11 phases), for inspection. */
 int rq_batch_phase_items(rq_ctx* ctx, rq_batch* b, uint32_t* counts, int cap);
 /* Synchronises and reports the error bits of the last run: 1 = a grid
- * barrier gave up waiting (grid not fully resident, e.g. another kernel held
- * part of the device; the results of that run are invalid), 2 = a ray needed a
- * deeper stack than provided, 4 = a mesh build reported an error. */
+ * barrier gave up waiting (a member block did not arrive within seconds; the
+ * results of that run are invalid), 2 = a ray needed a deeper stack than
+ * provided, 4 = a mesh build reported an error. */
 int rq_batch_errors(rq_ctx* ctx, rq_batch* b, uint32_t* bits);
+/* Synchronises and reports the blocks that took part in the last run
+ * (members) and the launch grid: members < grid when part of the grid could
+ * not be placed before the first barrier (the device was busy). */
+int rq_batch_occupancy(rq_ctx* ctx, rq_batch* b, uint32_t* members, uint32_t* grid);
 
 #ifdef __cplusplus
 }

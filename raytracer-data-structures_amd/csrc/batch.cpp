@@ -395,4 +395,16 @@ int rq_batch_errors(rq_ctx* c, rq_batch* b, uint32_t* bits) {
   return RQ_OK;
 }
 
+int rq_batch_occupancy(rq_ctx* c, rq_batch* b, uint32_t* members, uint32_t* grid) {
+  if (!c || !b || !members || !grid) return RQ_E_INVALID;
+  uint32_t m = 0;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(&m, b->ctl + BATCH_CTL_MEMBERS, sizeof m, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_err(c, e, "batch occupancy");
+  *members = m;
+  *grid = b->grid[b->variant];
+  return RQ_OK;
+}
+
 }  // extern "C"

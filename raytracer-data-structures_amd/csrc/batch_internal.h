@@ -30,7 +30,10 @@ constexpr int BATCH_CTL_BARRIER = BATCH_PHASES;            // grid barrier arriv
 constexpr int BATCH_CTL_BARRIER_ERR = BATCH_PHASES + 1;    // barrier wait gave up (this run)
 constexpr int BATCH_CTL_TRACE_ERR = BATCH_PHASES + 2;      // stack deeper than provided (this run)
 constexpr int BATCH_CTL_XTICKET = BATCH_PHASES + 3;        // BATCH_XCDS trace segment tickets
-constexpr int BATCH_CTL_WORDS = BATCH_CTL_XTICKET + BATCH_XCDS;
+constexpr int BATCH_CTL_START = BATCH_CTL_XTICKET + BATCH_XCDS;  // blocks started (+ BATCH_CLOSED)
+constexpr int BATCH_CTL_MEMBERS = BATCH_CTL_START + 1;     // blocks taking part (0 until closed)
+constexpr int BATCH_CTL_WORDS = BATCH_CTL_MEMBERS + 1;
+constexpr uint32_t BATCH_CLOSED = 0x80000000u;             // start word: membership closed
 constexpr uint32_t BATCH_ANY_BIT = 0x80000000u;            // trace item tile: any-hit query
 
 struct BatchMeshDev {

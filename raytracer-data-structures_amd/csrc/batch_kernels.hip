@@ -19,21 +19,63 @@ namespace {
 // s_getreg operand of HW_REG_XCC_ID (register 20, bits [3:0]): the XCD id.
 constexpr int XCC_ID_HWREG = 20 | (0 << 6) | ((4 - 1) << 11);
 
-// About 4M polls (a few seconds): far beyond any phase of a resident grid.
+// About 4M polls (a few seconds): far beyond any phase of the member blocks.
 constexpr uint32_t BARRIER_SPIN_LIMIT = 1u << 22;
+// Polls the first barrier waits for the rest of the grid to start before it
+// closes the membership with the blocks that have.
+constexpr uint32_t JOIN_SPIN_LIMIT = 1u << 14;
+
+// Membership: the barriers count only the blocks that have started, so that
+// they never wait for a block the device has not placed (kernels on other
+// streams can hold CUs when the launch begins). A block joins by counting
+// itself in the start word at entry; a block that finds the membership
+// closed leaves at once. The first block to reach the first barrier waits
+// (bounded) until the whole grid has started, then closes the membership
+// with an atomic OR, whose returned count is the number of members. Work is
+// pulled from queues, so any number of members completes every phase: a
+// grid that could not be placed whole runs on fewer blocks, not wrongly.
+__device__ inline bool join_grid(uint32_t* ctl, uint32_t* s_flag) {
+  if (threadIdx.x == 0) *s_flag = atomicAdd(ctl + BATCH_CTL_START, 1u);
+  __syncthreads();
+  const bool member = (*s_flag & BATCH_CLOSED) == 0u;
+  __syncthreads();
+  return member;
+}
 
 // Every thread releases its writes at agent scope, the block arrives, thread 0
-// waits for all blocks and acquires before the block goes on. The wait is
-// bounded: a grid that is not fully resident sets the sticky error word, and
-// once it is set no later barrier of the launch waits at all, so such a
-// launch ends after one time-out with the error reported.
-__device__ inline void grid_barrier(uint32_t* bar, uint32_t target, uint32_t* err) {
+// waits for all member blocks and acquires before the block goes on. The
+// members are running, so every wait ends; it is bounded all the same: a wait
+// that gives up sets the sticky error word, and once it is set no later
+// barrier of the launch waits at all.
+__device__ inline void grid_barrier(uint32_t* ctl, uint32_t phases_done) {
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) {
+    uint32_t* const bar = ctl + BATCH_CTL_BARRIER;
+    uint32_t* const err = ctl + BATCH_CTL_BARRIER_ERR;
     atomicAdd(bar, 1u);
+    uint32_t members = ld_agent(ctl + BATCH_CTL_MEMBERS);
+    if (members == 0u) {  // the first barrier of the launch: close the membership
+      uint32_t spins = 0;
+      for (;;) {
+        const uint32_t st = ld_agent(ctl + BATCH_CTL_START);
+        if ((st & BATCH_CLOSED) || st >= gridDim.x || ++spins > JOIN_SPIN_LIMIT) break;
+        __builtin_amdgcn_s_sleep(2);
+      }
+      const uint32_t old = atomicOr(ctl + BATCH_CTL_START, BATCH_CLOSED);
+      if (!(old & BATCH_CLOSED)) atomicExch(ctl + BATCH_CTL_MEMBERS, old);
+      spins = 0;
+      while ((members = ld_agent(ctl + BATCH_CTL_MEMBERS)) == 0u) {
+        if (++spins > BARRIER_SPIN_LIMIT) {
+          atomicOr(err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+    const uint32_t target = members * phases_done;
     uint32_t spins = 0;
-    while (ld_agent(bar) < target) {
+    while (members != 0u && ld_agent(bar) < target) {
       if (++spins > BARRIER_SPIN_LIMIT || ((spins & 1023u) == 0u && ld_agent(err) != 0u)) {
         atomicOr(err, 1u);
         break;
@@ -123,6 +165,8 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
   __shared__ uint32_t lds[batch_lds_words(DEPTH, TSTACK)];
   __shared__ uint32_t s_item, s_ctr;
   uint32_t* const ctl = a.ctl;
+  // blocks the device places after the membership closed take no part
+  if (!TRACE_ONLY && !join_grid(ctl, &s_item)) return;
   const BatchWork* const items = a.items;
   const BatchMeshDev* const meshes = a.meshes;
   // XCD this block runs on (hardware register; any value is safe: blocks
@@ -212,9 +256,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
       __syncthreads();
     }
    }
-    if (ph + 1 < BATCH_PHASES)
-      grid_barrier(ctl + BATCH_CTL_BARRIER, (uint32_t)(ph + 1) * gridDim.x,
-                   ctl + BATCH_CTL_BARRIER_ERR);
+    if (ph + 1 < BATCH_PHASES) grid_barrier(ctl, (uint32_t)(ph + 1));
   }
 }
 

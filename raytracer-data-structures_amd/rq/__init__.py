@@ -264,6 +264,7 @@ def _load() -> C.CDLL:
                                     C.POINTER(C.c_double)]),
         "rq_batch_phase_items": (i32, [vp, vp, vp, i32]),
         "rq_batch_errors": (i32, [vp, vp, C.POINTER(u32)]),
+        "rq_batch_occupancy": (i32, [vp, vp, C.POINTER(u32), C.POINTER(u32)]),
         "rq_server_defaults": (None, [C.POINTER(ServerConfig)]),
         "rq_server_create": (i32, [i32, C.POINTER(ServerConfig), C.POINTER(vp)]),
         "rq_server_destroy": (i32, [vp]),
@@ -911,6 +912,12 @@ class Batch:
         self.ctx._check(lib().rq_batch_autotune(self.ctx.h, self.h, repeats, C.byref(v),
                                                 C.byref(t), C.byref(r), C.byref(ms)))
         return v.value, t.value, r.value, ms.value
+
+    def occupancy(self) -> tuple[int, int]:
+        """rq_batch_occupancy: (blocks that took part in the last run, launch grid)."""
+        m, g = C.c_uint32(), C.c_uint32()
+        self.ctx._check(lib().rq_batch_occupancy(self.ctx.h, self.h, C.byref(m), C.byref(g)))
+        return m.value, g.value
 
     def phase_items(self) -> dict:
         counts = np.zeros(len(BATCH_PHASES), np.uint32)

@@ -165,6 +165,9 @@ def test_batch_matches_per_mesh(setup):
     setup.batch.run()
     got = setup.results()
     assert setup.batch.errors() == 0
+    members, grid = setup.batch.occupancy()
+    log(f"batch members {members} of {grid} blocks")
+    assert members == grid  # an idle device places the whole grid
     setup.check(got, "run")
     setup.check_trees()
 
@@ -239,6 +242,43 @@ def test_batch_autotune(setup):
     setup.batch.run_graph()
     setup.check(setup.results(), "autotuned")
     assert setup.batch.errors() == 0
+
+
+def test_batch_under_contention(setup, ctx, rq):
+    """Batch runs (plain and graph) enqueued on their own stream between large
+    queries on another stream, which can hold CUs when the batch starts: the
+    barriers count only the blocks that started, so every run equals the
+    per-mesh path with no barrier time-out, however many blocks took part."""
+    mesh = rq.icosphere(6, 4, center=(0.0, 0.0, -5.0))
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
+    n = 1 << 23
+    rays = ctx.raygen_uniform(n, 11, (-1.5, -1.5, -6.5), (1.5, 1.5, -3.5))
+    prim, t = rq.DeviceArray(ctx, 4 * n), rq.DeviceArray(ctx, 4 * n)
+    s_q, s_b = ctx.stream_create(), ctx.stream_create()
+    try:
+        for k in range(4):
+            log(f"batch under contention {k}")
+            setup.clear()
+            ctx.closest_hit_dev(b, rays, {"prim": prim, "t": t}, stream=s_q)
+            if k % 2:
+                setup.batch.run_graph(stream=s_b)
+            else:
+                setup.batch.run(stream=s_b)
+            ctx.closest_hit_dev(b, rays, {"prim": prim, "t": t}, stream=s_q)
+            ctx.sync()
+            assert setup.batch.errors() == 0, k
+            members, grid = setup.batch.occupancy()
+            log(f"contended run {k}: {members} of {grid} blocks took part")
+            assert 1 <= members <= grid
+            setup.check(setup.results(), f"contended run {k}")
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.stream_destroy(s_q)
+        ctx.stream_destroy(s_b)
+        prim.free(), t.free(), rays.free()
+        ctx.release(b)
+        ctx.release(s)
 
 
 def test_batch_ref_affine_mode(ctx):
