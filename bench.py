@@ -802,7 +802,9 @@ def run_c4(ctx, args) -> int:
         v, tile, refill, ms = batch.autotune(1)
         tuned = {"variant": rq.Batch.configs()[v], "ray_tile": tile, "refill_idle": refill,
                  "ms": round(ms, 3)}
+        tuned["build_waves"] = batch.build_waves()
         log(f"batch autotune: {tuned}")
+    batch_cfg = batch.config()
     for _ in range(max(args.warmup, 1)):
         batch.run_graph()
     assert batch.errors() == 0, "batch errors"
@@ -824,6 +826,18 @@ def run_c4(ctx, args) -> int:
     prof = ctx.profile_read()
     assert batch.errors() == 0, "batch errors"
     batch_s = (t1 - t0) / args.steps
+    # the run's two kernels (build phases + widen, then the trace), event-timed
+    # on plain runs after the timed replays
+    ctx.profile(True)
+    ctx.profile_reset()
+    for _ in range(3):
+        batch.run()
+    ctx.sync()
+    ctx.profile(False)
+    split = ctx.profile_read()
+    phase_ms = {k: round(split[f"batch_{k}"][0] / split[f"batch_{k}"][1], 3)
+                for k in ("build", "trace") if f"batch_{k}" in split}
+    log(f"batch kernels: {phase_ms}")
     check = [0, 1, len(meshes) // 2, len(meshes) - 1]
     ref = {m: (hits[m]["prim"].download(np.int32, rays[m].n), occ[m].download(np.uint8, rays[m].n))
            for m in check}
@@ -850,6 +864,15 @@ def run_c4(ctx, args) -> int:
         per_request()
     t1 = time.perf_counter()
     plain_s = (t1 - t0) / base_steps
+    # its kernel time alone (rebuild kernels + the two traversals per mesh),
+    # event-timed on one more pass
+    ctx.profile(True)
+    ctx.profile_reset()
+    per_request()
+    ctx.profile(False)
+    pr = ctx.profile_read()
+    per_request_kernels = {k: round(v[0], 3) for k, v in pr.items()}
+    log(f"per-request kernels (ms per pass): {per_request_kernels}")
     assert ctx.device_errors() == 0
     for m in check:
         assert np.array_equal(hits[m]["prim"].download(np.int32, rays[m].n), ref[m][0]), m
@@ -868,7 +891,14 @@ def run_c4(ctx, args) -> int:
     line = base_line(2 * nray / batch_s / 1e6, batch_s * 1e3, args.steps, args.warmup, {
         "workload": "c4_batched_build_and_trace", "meshes": len(meshes), "triangles": ntri,
         "rays": nray, "queries": "build+closest+any",
-        "launch": "one persistent kernel per step, hipGraph replay"})
+        "launch": "build kernel (every mesh's build phases + widen) then trace kernel "
+                  "(4-wide walk), one hipGraph replay per step",
+        "batch_config": {"trace": rq.Batch.configs()[batch_cfg[0]], "ray_tile": batch_cfg[1],
+                         "refill_idle": batch_cfg[2], "build_waves": batch.build_waves()}})
+    line["batch_kernel_ms"] = phase_ms
+    line["per_request_kernel_ms"] = round(sum(per_request_kernels.values()), 3)
+    line["per_request_trace_ms"] = round(sum(v for k, v in per_request_kernels.items()
+                                             if k.startswith("trace_")), 3)
     line["per_request_ms"] = round(plain_s * 1e3, 3)
     line["per_request_mrays_s"] = round(2 * nray / plain_s / 1e6, 2)
     line["speedup_vs_per_request"] = round(plain_s / batch_s, 3)

@@ -53,9 +53,13 @@ typedef struct rq_batch rq_batch;
 int rq_batch_create(rq_ctx* ctx, const rq_batch_item* items, uint32_t count, int morton_mode,
                     rq_batch** out);
 int rq_batch_destroy(rq_ctx* ctx, rq_batch* b);
-/* One persistent launch: build all meshes, then trace all ray batches. */
+/* One run: the build kernel (every mesh's build phases and its 4-wide nodes),
+ * then the trace kernel (every ray batch). Asynchronous on `stream`. Runs of
+ * one batch share its control words: they must not overlap (enqueue them on
+ * one stream, or order them). */
 int rq_batch_run(rq_ctx* ctx, rq_batch* b, void* stream);
-/* Same work replayed from a hipGraph captured on the first call. */
+/* Same work replayed from a hipGraph captured on the first call, launched on
+ * `stream` (NULL: the null stream, and the call waits for the run). */
 int rq_batch_run_graph(rq_ctx* ctx, rq_batch* b, void* stream);
 /* Non-owning handle on mesh i's BVH (rq_bvh_export / rq_bvh_info_get / queries). */
 int rq_batch_bvh(rq_ctx* ctx, rq_batch* b, uint32_t i, rq_bvh** out);
@@ -63,8 +67,15 @@ int rq_batch_config_count(void);
 int rq_batch_config_get(int index, rq_batch_config* out);
 int rq_batch_set_config(rq_ctx* ctx, rq_batch* b, int index, uint32_t ray_tile,
                         uint32_t refill_idle);
-/* Times the batch kernel variants, ray-tile sizes and refill thresholds on the
- * batch itself (`repeats` runs each), keeps and reports the fastest. */
+int rq_batch_get_config(rq_ctx* ctx, const rq_batch* b, int* index, uint32_t* ray_tile,
+                        uint32_t* refill_idle);
+/* The build kernel's waves per SIMD (8, 7, 6 or 5: its register cap, traded
+ * against occupancy); the autotuner chooses it after the trace configuration. */
+int rq_batch_set_build_waves(rq_ctx* ctx, rq_batch* b, int waves);
+int rq_batch_get_build_waves(rq_ctx* ctx, const rq_batch* b, int* waves);
+/* Times the trace configurations, ray-tile sizes and refill thresholds on the
+ * batch itself (`repeats` runs each), then the build kernels; keeps and
+ * reports the fastest. */
 int rq_batch_autotune(rq_ctx* ctx, rq_batch* b, int repeats, int* best_index,
                       uint32_t* best_ray_tile, uint32_t* best_refill, double* best_ms);
 /* Items of the work queue per phase (11 phases), for inspection. */

@@ -14,7 +14,7 @@
 using namespace rq;
 
 namespace {
-constexpr uint32_t kRayTiles[] = {2048, 4096, 8192, 32768};
+constexpr uint32_t kRayTiles[] = {1024, 2048, 4096, 8192, 32768};
 constexpr uint32_t kRefills[] = {16, 32, 64};
 constexpr uint32_t kMinRayTile = 1024;
 }  // namespace
@@ -29,16 +29,21 @@ struct rq_batch {
   uint32_t* ctl = nullptr;
   int32_t* gstack = nullptr;
   uint32_t gdepth = 0;
-  std::vector<uint32_t> grid;
+  int build = 3;                       // build kernel: 5 waves per SIMD (batch_build_waves)
+  std::vector<uint32_t> build_grid;    // its persistent grid, per build kernel
+  std::vector<uint32_t> grid;          // the trace kernel's, per configuration
+  // the autotuned choice on 512 mixed meshes (1k-500k triangles, incoherent
+  // rays): trace configuration 0 (LDS depth 24), 1024-ray wave items, refill
+  // at 16 idle lanes
   int variant = 0;
-  uint32_t ray_tile = 2048, refill = 16;  // the autotuned choice on 512 mixed meshes
+  uint32_t ray_tile = 1024, refill = 16;
   uint32_t phase_begin[BATCH_PHASES + 1] = {};
   uint32_t trace_seg[BATCH_XCDS + 1] = {};
   uint32_t built_ray_tile = 0;
   hipStream_t own_stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  int graph_variant = -1;
+  int graph_variant = -1, graph_build = -1;
   uint32_t graph_ray_tile = 0, graph_refill = 0;
   std::vector<void*> allocs;
 };
@@ -128,6 +133,7 @@ int make_work(rq_ctx* c, rq_batch* b, uint32_t ray_tile) {
         case PH_KARRAS: k = n >= 2 ? tiles_of(n - 1, NODE_TILE) : 0; break;
         case PH_REFIT: k = tiles_of(n, REFIT_CHUNK); break;
         case PH_TOP: k = tiles_of(n, REFIT_CHUNK) > 1 ? 1 : 0; break;
+        case PH_WIDEN: k = n >= 2 ? tiles_of(n - 1, BUILD_BLOCK) : 0; break;
         default: break;
       }
       for (uint32_t t = 0; t < k; ++t) w.push_back(BatchWork{m, t});
@@ -153,7 +159,7 @@ BatchArgs batch_args(const rq_batch* b) {
   a.gstack = b->gstack;
   int depth, ts;
   batch_variant_info(b->variant, &depth, &ts);
-  a.gdepth = TRACE_MAX_STACK - depth;
+  a.gdepth = TRACE_MAX_STACK_WIDE - depth;  // the batch run walks the 4-wide nodes
   a.ray_tile = b->ray_tile;
   a.refill_idle = b->refill;
   return a;
@@ -166,7 +172,9 @@ int launch(rq_ctx* c, rq_batch* b, hipStream_t s, bool profile) {
   }
   LaunchCtx lc = launch_ctx(c, s);
   if (!profile) lc.prof = nullptr;
-  hipError_t e = launch_batch(lc, batch_args(b), b->variant, b->grid[b->variant]);
+  hipError_t e =
+      launch_batch(lc, batch_args(b), b->build, b->build_grid[b->build], b->variant,
+                   b->grid[b->variant]);
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "batch launch");
 }
 
@@ -203,7 +211,7 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     v->n = items[i].scene->ntri;
     v->owned = false;
     b->bvhs.push_back(v);
-    rc = bvh_alloc(c, v, v->n, false);
+    rc = bvh_alloc(c, v, v->n, true, false);
   }
   // device mesh table
   std::vector<BatchMeshDev> md(count);
@@ -220,7 +228,7 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     d.occ = it.occluded;
     const uint32_t n = v->n;
     work_cap += 1 + 2 * tiles_of(n, TRI_TILE) + SORT_PASSES * tiles_of(n, SORT_TILE_LEAN) +
-                tiles_of(n, NODE_TILE) + tiles_of(n, REFIT_CHUNK) + 1 +
+                tiles_of(n, NODE_TILE) + tiles_of(n, REFIT_CHUNK) + 1 + tiles_of(n, BUILD_BLOCK) +
                 2 * (size_t)tiles_of(it.nrays, kMinRayTile);
   }
   if (!rc) rc = balloc(c, b, std::max<size_t>(1, count) * sizeof(BatchMeshDev), (void**)&b->meshes);
@@ -240,11 +248,14 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
   }
   if (!rc) {
     size_t gmax = 0;
+    for (int k = 0; k < batch_num_build_variants(); ++k)
+      b->build_grid.push_back(batch_build_grid_blocks(c->device, k));
     for (int v = 0; v < batch_num_variants(); ++v) {
-      b->grid.push_back(batch_grid_blocks(c->device, v));
+      b->grid.push_back(batch_trace_grid_blocks(c->device, v));
       int depth, ts;
       batch_variant_info(v, &depth, &ts);
-      gmax = std::max(gmax, (size_t)b->grid.back() * BUILD_BLOCK * (TRACE_MAX_STACK - depth));
+      gmax = std::max(gmax,
+                      (size_t)b->grid.back() * BUILD_BLOCK * (TRACE_MAX_STACK_WIDE - depth));
     }
     rc = balloc(c, b, gmax * 2 * sizeof(int32_t), (void**)&b->gstack);
   }
@@ -275,13 +286,16 @@ int rq_batch_run(rq_ctx* c, rq_batch* b, void* stream) {
 
 int rq_batch_run_graph(rq_ctx* c, rq_batch* b, void* stream) {
   if (!c || !b) return RQ_E_INVALID;
-  hipStream_t s = stream ? (hipStream_t)stream : b->own_stream;
+  // the replay goes on the caller's stream (the null stream included), in
+  // order after the caller's earlier work there, rq_batch_run's among it:
+  // two runs of one batch share its control words and must not overlap
+  hipStream_t s = (hipStream_t)stream;
   if (b->built_ray_tile != b->ray_tile) {
     int rc = make_work(c, b, b->ray_tile);  // also drops the graph
     if (rc) return rc;
   }
-  if (!b->exec || b->graph_variant != b->variant || b->graph_ray_tile != b->ray_tile ||
-      b->graph_refill != b->refill) {
+  if (!b->exec || b->graph_variant != b->variant || b->graph_build != b->build ||
+      b->graph_ray_tile != b->ray_tile || b->graph_refill != b->refill) {
     drop_graph(b);
     hipError_t e = hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeRelaxed);
     if (e != hipSuccess) return hip_err(c, e, "graph capture begin");
@@ -295,6 +309,7 @@ int rq_batch_run_graph(rq_ctx* c, rq_batch* b, void* stream) {
       return hip_err(c, e, "graph instantiate");
     }
     b->graph_variant = b->variant;
+    b->graph_build = b->build;
     b->graph_ray_tile = b->ray_tile;
     b->graph_refill = b->refill;
   }
@@ -339,6 +354,9 @@ int rq_batch_set_config(rq_ctx* c, rq_batch* b, int index, uint32_t ray_tile,
 int rq_batch_autotune(rq_ctx* c, rq_batch* b, int repeats, int* best_index,
                       uint32_t* best_ray_tile, uint32_t* best_refill, double* best_ms) {
   if (!c || !b || repeats < 1) return RQ_E_INVALID;
+  // the timed runs go on the batch's own stream: after every earlier run
+  hipError_t se = hipDeviceSynchronize();
+  if (se != hipSuccess) return hip_err(c, se, "batch autotune");
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
@@ -360,9 +378,24 @@ int rq_batch_autotune(rq_ctx* c, rq_batch* b, int repeats, int* best_index,
         (void)hipEventElapsedTime(&ms, e0, e1);
         if (!rc && ms / repeats < best) best = ms / repeats, bv = v, bt = tile, br = rf;
       }
+  // then the build kernel, with the chosen trace configuration
+  b->variant = bv, b->ray_tile = bt, b->refill = br;
+  int bb = b->build;
+  for (int k = 0; k < batch_num_build_variants() && !rc; ++k) {
+    b->build = k;
+    for (int r = 0; r <= repeats && !rc; ++r) {
+      if (r == 1) (void)hipEventRecord(e0, b->own_stream);
+      rc = launch(c, b, b->own_stream, false);
+    }
+    (void)hipEventRecord(e1, b->own_stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (!rc && ms / repeats < best) best = ms / repeats, bb = k;
+  }
+  b->build = bb;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  b->variant = bv, b->ray_tile = bt, b->refill = br;
   if (rc) return rc;
   if (best_index) *best_index = bv;
   if (best_ray_tile) *best_ray_tile = bt;
@@ -395,6 +428,29 @@ int rq_batch_errors(rq_ctx* c, rq_batch* b, uint32_t* bits) {
   return RQ_OK;
 }
 
+int rq_batch_get_config(rq_ctx* c, const rq_batch* b, int* index, uint32_t* ray_tile,
+                        uint32_t* refill_idle) {
+  if (!c || !b || !index || !ray_tile || !refill_idle) return RQ_E_INVALID;
+  *index = b->variant, *ray_tile = b->ray_tile, *refill_idle = b->refill;
+  return RQ_OK;
+}
+
+int rq_batch_set_build_waves(rq_ctx* c, rq_batch* b, int waves) {
+  if (!c || !b) return RQ_E_INVALID;
+  for (int k = 0; k < batch_num_build_variants(); ++k)
+    if (batch_build_waves(k) == waves) {
+      b->build = k;
+      return RQ_OK;
+    }
+  return set_err(c, RQ_E_INVALID, "no build kernel with %d waves per SIMD", waves);
+}
+
+int rq_batch_get_build_waves(rq_ctx* c, const rq_batch* b, int* waves) {
+  if (!c || !b || !waves) return RQ_E_INVALID;
+  *waves = batch_build_waves(b->build);
+  return RQ_OK;
+}
+
 int rq_batch_occupancy(rq_ctx* c, rq_batch* b, uint32_t* members, uint32_t* grid) {
   if (!c || !b || !members || !grid) return RQ_E_INVALID;
   uint32_t m = 0;
@@ -403,7 +459,7 @@ int rq_batch_occupancy(rq_ctx* c, rq_batch* b, uint32_t* members, uint32_t* grid
     e = hipMemcpy(&m, b->ctl + BATCH_CTL_MEMBERS, sizeof m, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return hip_err(c, e, "batch occupancy");
   *members = m;
-  *grid = b->grid[b->variant];
+  *grid = b->build_grid[b->build];
   return RQ_OK;
 }
 

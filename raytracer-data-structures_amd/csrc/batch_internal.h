@@ -15,6 +15,7 @@ enum BatchPhase {
   PH_KARRAS,
   PH_REFIT,
   PH_TOP,
+  PH_WIDEN,  // 4-wide nodes from the refitted binary records
   PH_TRACE,
   BATCH_PHASES
 };
@@ -62,12 +63,23 @@ struct BatchArgs {
   uint32_t refill_idle;
 };
 
+// Batch configurations: the trace kernels of a batch run (4-wide walk).
 int batch_num_variants();
 void batch_variant_info(int v, int* depth, int* tstack);
-uint32_t batch_grid_blocks(int device, int v, bool trace_only = false);
-// Resets the control words, then one persistent launch (all phases, or the
-// trace phase alone).
-hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid,
-                        bool trace_only = false);
+// Build kernels by waves per SIMD (8, 7, 6, 5).
+int batch_num_build_variants();
+int batch_build_waves(int build);
+uint32_t batch_build_grid_blocks(int device, int build);
+uint32_t batch_trace_grid_blocks(int device, int v);
+// The serving loop's trace kernels (binary walk over its slots).
+int serve_num_variants();
+void serve_variant_info(int v, int* depth, int* tstack);
+uint32_t serve_grid_blocks(int device, int v);
+// A batch run: resets the control words, then the build kernel (every build
+// phase, grid barriers between them) and the trace kernel of `variant`.
+hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int build, uint32_t build_grid,
+                        int variant, uint32_t trace_grid);
+// The serving step: resets the control words, then the trace phase alone.
+hipError_t launch_serve_trace(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid);
 
 }  // namespace rq

@@ -103,7 +103,10 @@ __device__ inline T load_uniform(const T* p) {
   return t;
 }
 
-template <int DEPTH, bool TSTACK>
+// One trace item (a tile of one mesh's rays) by one wave: the wave's ray
+// counter in LDS hands out chunks of the tile as its lanes go idle; no other
+// wave takes part, so there is no block barrier at either end.
+template <int DEPTH, bool TSTACK, bool WIDE>
 __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* err,
                                   uint32_t ray_tile, uint32_t refill_idle, const BatchMeshDev& m,
                                   uint32_t tile, uint32_t* lds, uint32_t* s_ctr) {
@@ -112,6 +115,7 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   const uint32_t nr = min(ray_tile, m.nrays - r0);
   TraceArgs t{};
   t.nodes = m.v.B.nodes;
+  t.wide = m.v.B.wide;
   t.tris = m.v.B.tris;
   t.nprims = m.v.n;
   t.rays = rq_rays{m.rays.ox + r0, m.rays.oy + r0, m.rays.oz + r0, m.rays.dx + r0,
@@ -123,8 +127,8 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   t.refill_idle = refill_idle;
   t.chunk = 64;
   t.sphere = 0;  // batch meshes are triangle scenes
-  if (threadIdx.x == 0) *s_ctr = 0u;
-  __syncthreads();
+  // lane 0 resets the counter and is the lane that takes from it (trace_loop)
+  if ((threadIdx.x & 63) == 0) *s_ctr = 0u;
   int32_t* s_node = reinterpret_cast<int32_t*>(lds);
   float* s_tn = reinterpret_cast<float*>(lds + DEPTH * BUILD_BLOCK);
   const uint32_t gtid = blockIdx.x * BUILD_BLOCK + threadIdx.x;
@@ -133,17 +137,16 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   stream.ctr32 = s_ctr;
   if (any) {
     t.occ = m.occ + r0;
-    trace_loop<true, false, BUILD_BLOCK, DEPTH, false, false>(t, stream, s_node, s_tn, gtid,
-                                                              gstride);
+    trace_loop<true, false, BUILD_BLOCK, DEPTH, false, false, WIDE>(t, stream, s_node, s_tn,
+                                                                    gtid, gstride);
   } else {
     t.prim = m.prim + r0;
     t.t = m.t ? m.t + r0 : nullptr;
     t.u = m.u ? m.u + r0 : nullptr;
     t.v = m.v_ ? m.v_ + r0 : nullptr;
-    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false>(t, stream, s_node, s_tn, gtid,
-                                                                gstride);
+    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false, WIDE>(t, stream, s_node, s_tn,
+                                                                      gtid, gstride);
   }
-  __syncthreads();
 }
 
 // LDS words of the batch kernel: the largest of the lean sort tile, the
@@ -154,33 +157,55 @@ constexpr int batch_lds_words(int depth, bool tstack) {
               depth * BUILD_BLOCK * (tstack ? 2 : 1));
 }
 
-// Occupancy is set for the traversal phase (the longest): WAVES waves per
-// SIMD caps the registers of every phase, and the build phases are written to
-// fit that budget (lean 8-key sort tiles, descriptors loaded per phase).
-// TRACE_ONLY: the trace phase alone (ray tiles over resident BVHs, no grid
-// barrier), the continuous-batching step of the serving loop.
-template <int DEPTH, bool TSTACK, int WAVES, bool TRACE_ONLY>
+// Two kernels share this body. KM_BUILD runs the build phases of every mesh
+// (init ... widen) with grid barriers between them; KM_TRACE runs the trace
+// phase alone (ray tiles over built BVHs, no barrier): after KM_BUILD in a
+// batch run (WIDE: walking the 4-wide nodes of the widen phase), or as the
+// continuous-batching step of the serving loop (binary nodes). WAVES waves
+// per SIMD cap the registers: the build phases are written to fit 8 (lean
+// 8-key sort tiles, descriptors loaded per phase); the 4-wide walk needs
+// about 90 VGPRs (5 waves), the binary walk fits 7-8.
+enum KernelMode { KM_BUILD, KM_TRACE };
+template <int DEPTH, bool TSTACK, int WAVES, KernelMode MODE, bool WIDE>
 __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_batch(
     BatchArgs a) {
   __shared__ uint32_t lds[batch_lds_words(DEPTH, TSTACK)];
-  __shared__ uint32_t s_item, s_ctr;
+  __shared__ uint32_t s_item, s_wctr[BUILD_BLOCK / 64];
   uint32_t* const ctl = a.ctl;
-  // blocks the device places after the membership closed take no part
-  if (!TRACE_ONLY && !join_grid(ctl, &s_item)) return;
   const BatchWork* const items = a.items;
   const BatchMeshDev* const meshes = a.meshes;
-  // XCD this block runs on (hardware register; any value is safe: blocks
-  // drain every segment before the launch ends)
-  const uint32_t xcd = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) % BATCH_XCDS;
-  for (int ph = TRACE_ONLY ? (int)PH_TRACE : 0; ph < BATCH_PHASES; ++ph) {
-   const int nq = ph == PH_TRACE ? BATCH_XCDS : 1;
-   for (int k = 0; k < nq; ++k) {
-    const uint32_t q = (xcd + (uint32_t)k) % BATCH_XCDS;
+  if constexpr (MODE == KM_TRACE) {
+    // Waves take trace items on their own, from their XCD's segment first,
+    // and never wait for the rest of their block. The XCD comes from a
+    // hardware register; any value is safe: waves drain every segment.
+    const uint32_t xcd = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) % BATCH_XCDS;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (int k = 0; k < BATCH_XCDS; ++k) {
+      const uint32_t q = (xcd + (uint32_t)k) % BATCH_XCDS;
+      const uint32_t b0 = a.trace_seg[q], b1 = a.trace_seg[q + 1];
+      if (b1 == b0) continue;
+      for (;;) {
+        uint32_t it = 0;
+        if (lane == 0) it = b0 + atomicAdd(ctl + BATCH_CTL_XTICKET + q, 1u);
+        it = __builtin_amdgcn_readfirstlane(it);  // every lane is active here
+        if (it >= b1) break;
+        const BatchWork w = load_uniform(items + it);
+        const BatchMeshDev m = load_uniform(meshes + w.mesh);
+        trace_item<DEPTH, TSTACK, WIDE>(a.gstack, a.gdepth, ctl + BATCH_CTL_TRACE_ERR, a.ray_tile,
+                                        a.refill_idle, m, w.tile, lds, s_wctr + wave);
+      }
+    }
+    return;
+  }
+  // ---- the build phases, block by block, grid barriers between them
+  // blocks the device places after the membership closed take no part
+  if (!join_grid(ctl, &s_item)) return;
+  constexpr int first = 0, last = (int)PH_TRACE;
+  for (int ph = first; ph < last; ++ph) {
+   {
     // kernarg words, block-uniform
-    const uint32_t b0 = ph == PH_TRACE ? a.trace_seg[q] : a.phase_begin[ph];
-    const uint32_t b1 = ph == PH_TRACE ? a.trace_seg[q + 1] : a.phase_begin[ph + 1];
-    uint32_t* const ticket =
-        ph == PH_TRACE ? ctl + BATCH_CTL_XTICKET + q : ctl + BATCH_CTL_TICKET + ph;
+    const uint32_t b0 = a.phase_begin[ph], b1 = a.phase_begin[ph + 1];
+    uint32_t* const ticket = ctl + BATCH_CTL_TICKET + ph;
     for (;;) {
       if (b1 == b0) break;
       if (threadIdx.x == 0) s_item = b0 + atomicAdd(ticket, 1u);
@@ -245,18 +270,20 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
           refit_top_block(v);
           break;
         }
-        default: {
-          const BatchMeshDev m = load_uniform(mp);
-          trace_item<DEPTH, TSTACK>(a.gstack, a.gdepth, ctl + BATCH_CTL_TRACE_ERR, a.ray_tile,
-                                    a.refill_idle, m, w.tile, lds, &s_ctr);
+        case PH_WIDEN: {
+          VIEW;
+          const uint32_t i = w.tile * BUILD_BLOCK + threadIdx.x;
+          if (i + 1u < v.n) widen_global(v.B, i);
           break;
         }
+        default:
+          break;
       }
 #undef VIEW
       __syncthreads();
     }
    }
-    if (ph + 1 < BATCH_PHASES) grid_barrier(ctl, (uint32_t)(ph + 1));
+    if (ph + 1 < last) grid_barrier(ctl, (uint32_t)(ph + 1));
   }
 }
 
@@ -272,14 +299,34 @@ struct BatchVariant {
   bool tstack;
   BatchFn fn;
 };
-#define RQ_BATCH_VARIANTS(T)                                                               \
-  {16, false, k_batch<16, false, 7, T>}, {20, false, k_batch<20, false, 7, T>},              \
-      {8, true, k_batch<8, true, 7, T>}, {10, true, k_batch<10, true, 7, T>},                \
-      {16, false, k_batch<16, false, 8, T>}, {8, true, k_batch<8, true, 8, T>}
-const BatchVariant kBatchVariants[] = {RQ_BATCH_VARIANTS(false)};
-const BatchVariant kTraceVariants[] = {RQ_BATCH_VARIANTS(true)};
-#undef RQ_BATCH_VARIANTS
+// The build kernel of a batch run, by waves per SIMD (the register cap trades
+// occupancy against spills of the sort phase's look-back window).
+const BatchFn kBuild[] = {k_batch<0, false, 8, KM_BUILD, false>, k_batch<0, false, 7, KM_BUILD, false>,
+                          k_batch<0, false, 6, KM_BUILD, false>, k_batch<0, false, 5, KM_BUILD, false>};
+constexpr int kBuildWaves[] = {8, 7, 6, 5};
+constexpr int kNumBuildVariants = sizeof(kBuild) / sizeof(kBuild[0]);
+// Trace kernels of a batch run (4-wide walk): the batch configurations.
+const BatchVariant kBatchVariants[] = {
+    {24, false, k_batch<24, false, 5, KM_TRACE, true>},
+    {16, false, k_batch<16, false, 5, KM_TRACE, true>},
+    {32, false, k_batch<32, false, 4, KM_TRACE, true>},
+    {12, true, k_batch<12, true, 5, KM_TRACE, true>}};
+// Trace kernels of the serving loop (binary walk).
+const BatchVariant kServeVariants[] = {
+    {16, false, k_batch<16, false, 7, KM_TRACE, false>}, {20, false, k_batch<20, false, 7, KM_TRACE, false>},
+    {8, true, k_batch<8, true, 7, KM_TRACE, false>},     {10, true, k_batch<10, true, 7, KM_TRACE, false>},
+    {16, false, k_batch<16, false, 8, KM_TRACE, false>}, {8, true, k_batch<8, true, 8, KM_TRACE, false>}};
 constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+constexpr int kNumServeVariants = sizeof(kServeVariants) / sizeof(kServeVariants[0]);
+
+uint32_t grid_blocks(int device, BatchFn fn) {
+  int cus = 0, per = 0;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, BUILD_BLOCK, 0);
+  if (cus <= 0) cus = 256;
+  if (per <= 0) per = 1;
+  return (uint32_t)(cus * per);
+}
 
 }  // namespace
 
@@ -288,23 +335,38 @@ void batch_variant_info(int v, int* depth, int* tstack) {
   *depth = kBatchVariants[v].depth;
   *tstack = kBatchVariants[v].tstack ? 1 : 0;
 }
+int serve_num_variants() { return kNumServeVariants; }
+void serve_variant_info(int v, int* depth, int* tstack) {
+  *depth = kServeVariants[v].depth;
+  *tstack = kServeVariants[v].tstack ? 1 : 0;
+}
+int batch_num_build_variants() { return kNumBuildVariants; }
+int batch_build_waves(int b) { return kBuildWaves[b]; }
+uint32_t batch_build_grid_blocks(int device, int b) { return grid_blocks(device, kBuild[b]); }
+uint32_t batch_trace_grid_blocks(int device, int v) {
+  return grid_blocks(device, kBatchVariants[v].fn);
+}
+uint32_t serve_grid_blocks(int device, int v) { return grid_blocks(device, kServeVariants[v].fn); }
 
-uint32_t batch_grid_blocks(int device, int v, bool trace_only) {
-  int cus = 0, per = 0;
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per, (trace_only ? kTraceVariants : kBatchVariants)[v].fn, BUILD_BLOCK, 0);
-  if (cus <= 0) cus = 256;
-  if (per <= 0) per = 1;
-  return (uint32_t)(cus * per);
+hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int build, uint32_t build_grid,
+                        int variant, uint32_t trace_grid) {
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, lc.stream, a.ctl);
+  {
+    ProfScope ps(lc, "batch_build");
+    hipLaunchKernelGGL(kBuild[build], dim3(build_grid), dim3(BUILD_BLOCK), 0, lc.stream, a);
+  }
+  {
+    ProfScope ps(lc, "batch_trace");
+    hipLaunchKernelGGL(kBatchVariants[variant].fn, dim3(trace_grid), dim3(BUILD_BLOCK), 0,
+                       lc.stream, a);
+  }
+  return hipGetLastError();
 }
 
-hipError_t launch_batch(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid,
-                        bool trace_only) {
-  ProfScope ps(lc, trace_only ? "batch_trace" : "batch");
+hipError_t launch_serve_trace(const LaunchCtx& lc, const BatchArgs& a, int variant, uint32_t grid) {
+  ProfScope ps(lc, "batch_trace");
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, lc.stream, a.ctl);
-  hipLaunchKernelGGL((trace_only ? kTraceVariants : kBatchVariants)[variant].fn, dim3(grid),
-                     dim3(BUILD_BLOCK), 0, lc.stream, a);
+  hipLaunchKernelGGL(kServeVariants[variant].fn, dim3(grid), dim3(BUILD_BLOCK), 0, lc.stream, a);
   return hipGetLastError();
 }
 

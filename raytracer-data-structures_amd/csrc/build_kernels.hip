@@ -163,6 +163,10 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
       ProfScope ps(lc, "refit_top");
       hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, v);
     }
+    if (B.wide && n >= 2) {  // a batch mesh's BVH rebuilt on its own
+      ProfScope ps(lc, "widen");
+      hipLaunchKernelGGL(k_widen_all, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
+    }
   }
   return hipGetLastError();
 }

@@ -153,7 +153,7 @@ int ensure_height(rq_ctx* c, const rq_bvh* b) {
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "tree height");
 }
 
-int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide) {
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit) {
   BuildBuffers& B = b->B;
   B.ntiles = (n + SORT_TILE_LEAN - 1) / SORT_TILE_LEAN;
   size_t nn = std::max<uint32_t>(n, 1);
@@ -174,9 +174,9 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide) {
                  4 * nn,
                  4 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK),
                  wide ? 128 * ni : 0,
-                 wide ? 24 * nn : 0,
-                 wide ? 24 * nn : 0,
-                 wide ? 24 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK) : 0};
+                 chunk_refit ? 24 * nn : 0,
+                 chunk_refit ? 24 * nn : 0,
+                 chunk_refit ? 24 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK) : 0};
   size_t total = 0;
   for (size_t s : sz) total += align256(s);
   void* p;
@@ -408,7 +408,7 @@ int rq_build_lbvh(rq_ctx* c, const rq_scene* s, int mode, rq_bvh** out, void* st
   b->scene = s;
   b->mode = mode;
   b->n = s->ntri;
-  int rc = bvh_alloc(c, b, b->n, true);
+  int rc = bvh_alloc(c, b, b->n, true, true);
   if (rc) {
     delete b;
     return rc;

@@ -453,7 +453,7 @@ bool launch_step(rq_server* s) {
     e = hipMemcpyAsync(s->dev_items[buf], it, ni * sizeof(BatchWork), hipMemcpyHostToDevice,
                        s->s_trace);
   if (e == hipSuccess) e = hipEventRecord(st->start, s->s_trace);
-  if (e == hipSuccess) e = launch_batch(LaunchCtx{s->s_trace, nullptr}, a, s->variant, s->grid, true);
+  if (e == hipSuccess) e = launch_serve_trace(LaunchCtx{s->s_trace, nullptr}, a, s->variant, s->grid);
   if (e == hipSuccess)
     e = hipMemcpyAsync(s->err_host[buf], s->ctl + BATCH_CTL_BARRIER_ERR, 2 * sizeof(uint32_t),
                        hipMemcpyDeviceToHost, s->s_trace);
@@ -705,7 +705,7 @@ int rq_server_create(int device, const rq_server_config* cfg_in, rq_server** out
     sl.bvh.magic = BVH_MAGIC;
     sl.bvh.owned = false;
     sl.bvh.mode = RQ_MORTON_SCENE_AABB;
-    rc = bvh_alloc(c, &sl.bvh, sl.tcap, false);
+    rc = bvh_alloc(c, &sl.bvh, sl.tcap, false, false);
   }
   if (!rc) rc = ctx_alloc(c, std::max<size_t>(cfg.arena_bytes, 256), (void**)&s->arena_base);
   s->arena.init(cfg.arena_bytes);
@@ -732,8 +732,8 @@ int rq_server_create(int device, const rq_server_config* cfg_in, rq_server** out
   if (!rc && hipMemset(s->ctl, 0, BATCH_CTL_WORDS * sizeof(uint32_t)) != hipSuccess) rc = RQ_E_HIP;
   if (!rc) {
     int depth = 0, ts = 0;
-    batch_variant_info(s->variant, &depth, &ts);
-    s->grid = batch_grid_blocks(device, s->variant, true);
+    serve_variant_info(s->variant, &depth, &ts);
+    s->grid = serve_grid_blocks(device, s->variant);
     s->gdepth = TRACE_MAX_STACK - depth;
     rc = ctx_alloc(c, (size_t)s->grid * BUILD_BLOCK * s->gdepth * 2 * sizeof(int32_t),
                    (void**)&s->gstack);

@@ -156,7 +156,7 @@ Q_CLOSEST, Q_ANY = 1, 2
 REQ_QUEUED, REQ_ADMITTED, REQ_DONE = 0, 1, 2
 
 BATCH_PHASES = ("init", "bounds", "morton", "sort0", "sort1", "sort2", "sort3", "karras", "refit",
-                "refit_top", "trace")
+                "refit_top", "widen", "trace")
 
 
 def _load() -> C.CDLL:
@@ -265,6 +265,9 @@ def _load() -> C.CDLL:
         "rq_batch_phase_items": (i32, [vp, vp, vp, i32]),
         "rq_batch_errors": (i32, [vp, vp, C.POINTER(u32)]),
         "rq_batch_occupancy": (i32, [vp, vp, C.POINTER(u32), C.POINTER(u32)]),
+        "rq_batch_set_build_waves": (i32, [vp, vp, i32]),
+        "rq_batch_get_config": (i32, [vp, vp, C.POINTER(C.c_int), C.POINTER(u32), C.POINTER(u32)]),
+        "rq_batch_get_build_waves": (i32, [vp, vp, C.POINTER(C.c_int)]),
         "rq_server_defaults": (None, [C.POINTER(ServerConfig)]),
         "rq_server_create": (i32, [i32, C.POINTER(ServerConfig), C.POINTER(vp)]),
         "rq_server_destroy": (i32, [vp]),
@@ -912,6 +915,21 @@ class Batch:
         self.ctx._check(lib().rq_batch_autotune(self.ctx.h, self.h, repeats, C.byref(v),
                                                 C.byref(t), C.byref(r), C.byref(ms)))
         return v.value, t.value, r.value, ms.value
+
+    def config(self) -> tuple[int, int, int]:
+        """rq_batch_get_config: (trace configuration index, ray tile, refill threshold)."""
+        i, t, r = C.c_int(), C.c_uint32(), C.c_uint32()
+        self.ctx._check(lib().rq_batch_get_config(self.ctx.h, self.h, C.byref(i), C.byref(t),
+                                                  C.byref(r)))
+        return i.value, t.value, r.value
+
+    def build_waves(self, waves: int | None = None) -> int:
+        """rq_batch_set/get_build_waves: the build kernel's waves per SIMD."""
+        if waves is not None:
+            self.ctx._check(lib().rq_batch_set_build_waves(self.ctx.h, self.h, waves))
+        w = C.c_int()
+        self.ctx._check(lib().rq_batch_get_build_waves(self.ctx.h, self.h, C.byref(w)))
+        return w.value
 
     def occupancy(self) -> tuple[int, int]:
         """rq_batch_occupancy: (blocks that took part in the last run, launch grid)."""

@@ -157,6 +157,7 @@ def test_batch_phase_items(setup, rq):
     assert ph["karras"] == sum(cdiv(n - 1, 4096) for n in ns if n >= 2)
     assert ph["refit"] == sum(cdiv(n, 1024) for n in ns)
     assert ph["refit_top"] == sum(1 for n in ns if cdiv(n, 1024) > 1)
+    assert ph["widen"] == sum(cdiv(n - 1, 256) for n in ns if n >= 2)
 
 
 def test_batch_matches_per_mesh(setup):
@@ -212,6 +213,25 @@ def test_batch_every_config(setup, rq, tile):
         assert setup.batch.errors() == 0
         setup.check(got, f"config {c['index']} tile {tile}")
     setup.batch.set_config(0, 32768, 32)
+
+
+def test_batch_every_build_kernel(setup):
+    """The build kernel at each register cap (8, 7, 6, 5 waves per SIMD),
+    plain and replayed: the same trees and answers."""
+    saved = setup.batch.build_waves()
+    try:
+        for waves in (8, 7, 6, 5):
+            log(f"batch build kernel at {waves} waves")
+            assert setup.batch.build_waves(waves) == waves
+            for graph in (False, True):
+                setup.clear()
+                setup.batch.run_graph() if graph else setup.batch.run()
+                got = setup.results()
+                assert setup.batch.errors() == 0
+                setup.check(got, f"build waves {waves} graph {graph}")
+            setup.check_trees()
+    finally:
+        setup.batch.build_waves(saved)
 
 
 def test_batch_graph_after_queue_rebuild(setup):

@@ -188,7 +188,11 @@ def test_wide_after_blob_load(ctx, O, rq):
         ctx.release(s)
 
 
-def test_wide_variant_on_batch_bvh_walks_binary(ctx, O, rq):
+def test_batch_bvh_wide_nodes(ctx, O, rq):
+    """A batch mesh's BVH gets its 4-wide nodes from the batch kernel's widen
+    phase (the binary collapse, every internal node) and again from a
+    standalone rebuild of that BVH; a wide variant walks them to the
+    oracle's answers."""
     mesh = random_soup(3000, 21, spread=1.5, size=0.03)
     rays = concat_rays(rays_toward(2000, 48), random_rays(1000, 49))
     s = ctx.upload_scene(mesh)
@@ -201,14 +205,20 @@ def test_wide_variant_on_batch_bvh_walks_binary(ctx, O, rq):
         batch.run()
         ctx.sync()
         bb = batch.bvh(0)
-        with pytest.raises(rq.RQError):
-            ctx.export_wide(bb, mesh.ntri)
-        wv = _wide_variants(ctx)[0]["index"]
-        ctx.trace_config(wv, wv, 16)
         o = O.OracleBVH(mesh, 1).closest(rays)
-        g = ctx.closest_hit(bb, rays)
-        assert np.array_equal(g["prim"], o["prim"])
-        assert np.array_equal(_bits(g["t"]), _bits(o["t"]))
+        assert np.array_equal(hits["prim"].download(np.int32, n), o["prim"])
+        for rebuilt in (False, True):
+            if rebuilt:
+                ctx.rebuild(bb)
+            e, w = ctx.export(bb, mesh.ntri), ctx.export_wide(bb, mesh.ntri)
+            for i, (bx, ln) in _expected_collapse(e, mesh.ntri).items():
+                assert np.array_equal(w["links"][i], ln), (rebuilt, i)
+                assert np.array_equal(_bits(w["boxes"][i]), _bits(bx)), (rebuilt, i)
+            for wv in (v["index"] for v in _wide_variants(ctx)):
+                ctx.trace_config(wv, wv, 16)
+                g = ctx.closest_hit(bb, rays)
+                assert np.array_equal(g["prim"], o["prim"]), (rebuilt, wv)
+                assert np.array_equal(_bits(g["t"]), _bits(o["t"])), (rebuilt, wv)
         assert ctx.device_errors() == 0
     finally:
         ctx.trace_config(*saved)
