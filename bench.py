@@ -167,7 +167,7 @@ def cfg_arg(t) -> str | None:
 # ------------------------------------------------------- rocprofv3 counters
 COPY_PROBE_BYTES = 1 << 30
 KERNEL_SYMBOL = {"trace_closest": "k_trace<false, false", "trace_any": "k_trace<true, false",
-                 "batch": "k_batch<"}
+                 "batch_build": "KernelMode)0", "batch_trace": "KernelMode)1"}
 COPY_SYMBOL = "k_copy16"
 # Three passes: HBM bytes read, HBM bytes written, and the issue / cache counters
 # (at most 4 TCC, 8 SQ and 2 GRBM counters per pass).
@@ -884,10 +884,10 @@ def run_c4(ctx, args) -> int:
         a, b = counted_batch(ctx, bvhs[m], rays[m], any_hit=False)
         c, d = counted_batch(ctx, bvhs[m], rays[m], any_hit=True)
         vis_c, tst_c, vis_a, tst_a = vis_c + a, tst_c + b, vis_a + c, tst_a + d
-    alg = (BUILD_BYTES_PER_TRI * ntri + nray * (32 + 8) + 32 * vis_c + 36 * tst_c +
-           nray * (32 + 1) + 32 * vis_a + 36 * tst_a)
-    ms_tot, launches = prof["batch_graph"]
-    pmc = measure_pmc("c4", ["batch"], None) if args.traffic == "auto" else {}
+    alg_trace = (nray * (32 + 8) + 32 * vis_c + 36 * tst_c + nray * (32 + 1) + 32 * vis_a +
+                 36 * tst_a)
+    alg_build = BUILD_BYTES_PER_TRI * ntri
+    pmc = measure_pmc("c4", ["batch_trace", "batch_build"], None) if args.traffic == "auto" else {}
     line = base_line(2 * nray / batch_s / 1e6, batch_s * 1e3, args.steps, args.warmup, {
         "workload": "c4_batched_build_and_trace", "meshes": len(meshes), "triangles": ntri,
         "rays": nray, "queries": "build+closest+any",
@@ -905,8 +905,14 @@ def run_c4(ctx, args) -> int:
     line["traversal_counts"] = {"closest": {"node_visits": vis_c, "tri_tests": tst_c},
                                 "any": {"node_visits": vis_a, "tri_tests": tst_a}}
     line["autotune"] = tuned
-    line["roofline"] = roofline_entry("k_batch", alg, ms_tot / launches, pmc.get("batch"))
-    line["roofline"]["basis"] = "build bytes (364 per triangle) + " + ALG_BASIS
+    # the dominant kernel: the trace kernel (closest + any hit of every mesh)
+    line["roofline"] = roofline_entry("k_batch trace (4-wide walk)", alg_trace,
+                                      phase_ms["trace"], pmc.get("batch_trace"))
+    line["roofline"]["basis"] = ALG_BASIS
+    line["build_roofline"] = roofline_entry("k_batch build phases", alg_build, phase_ms["build"],
+                                            pmc.get("batch_build"))
+    line["build_roofline"]["basis"] = (f"build bytes ({BUILD_BYTES_PER_TRI} per triangle: "
+                                       "the build kernels' reads and writes)")
     line["cpu_baseline"] = None
     if not args.no_cpu:
         sample = list(range(0, len(meshes), 61))  # icospheres and soups

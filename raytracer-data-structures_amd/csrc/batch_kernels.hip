@@ -103,10 +103,11 @@ __device__ inline T load_uniform(const T* p) {
   return t;
 }
 
-// One trace item (a tile of one mesh's rays) by one wave: the wave's ray
-// counter in LDS hands out chunks of the tile as its lanes go idle; no other
-// wave takes part, so there is no block barrier at either end.
-template <int DEPTH, bool TSTACK, bool WIDE>
+// One trace item (a tile of one mesh's rays): a ray counter in LDS hands out
+// chunks of the tile as lanes go idle. WAVE_ITEM: one wave takes the item on
+// its own (its own counter, no block barrier at either end); otherwise the
+// block's waves share it and meet at a barrier when it is done.
+template <int DEPTH, bool TSTACK, bool WIDE, bool WAVE_ITEM>
 __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* err,
                                   uint32_t ray_tile, uint32_t refill_idle, const BatchMeshDev& m,
                                   uint32_t tile, uint32_t* lds, uint32_t* s_ctr) {
@@ -127,8 +128,13 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   t.refill_idle = refill_idle;
   t.chunk = 64;
   t.sphere = 0;  // batch meshes are triangle scenes
-  // lane 0 resets the counter and is the lane that takes from it (trace_loop)
-  if ((threadIdx.x & 63) == 0) *s_ctr = 0u;
+  if (WAVE_ITEM) {
+    // lane 0 resets the counter and is the lane that takes from it (trace_loop)
+    if ((threadIdx.x & 63) == 0) *s_ctr = 0u;
+  } else {
+    if (threadIdx.x == 0) *s_ctr = 0u;
+    __syncthreads();
+  }
   int32_t* s_node = reinterpret_cast<int32_t*>(lds);
   float* s_tn = reinterpret_cast<float*>(lds + DEPTH * BUILD_BLOCK);
   const uint32_t gtid = blockIdx.x * BUILD_BLOCK + threadIdx.x;
@@ -147,6 +153,7 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
     trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false, WIDE>(t, stream, s_node, s_tn,
                                                                       gtid, gstride);
   }
+  if (!WAVE_ITEM) __syncthreads();
 }
 
 // LDS words of the batch kernel: the largest of the lean sort tile, the
@@ -166,7 +173,7 @@ constexpr int batch_lds_words(int depth, bool tstack) {
 // 8-key sort tiles, descriptors loaded per phase); the 4-wide walk needs
 // about 90 VGPRs (5 waves), the binary walk fits 7-8.
 enum KernelMode { KM_BUILD, KM_TRACE };
-template <int DEPTH, bool TSTACK, int WAVES, KernelMode MODE, bool WIDE>
+template <int DEPTH, bool TSTACK, int WAVES, KernelMode MODE, bool WIDE, bool WAVE_ITEMS = WIDE>
 __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_batch(
     BatchArgs a) {
   __shared__ uint32_t lds[batch_lds_words(DEPTH, TSTACK)];
@@ -175,24 +182,36 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
   const BatchWork* const items = a.items;
   const BatchMeshDev* const meshes = a.meshes;
   if constexpr (MODE == KM_TRACE) {
-    // Waves take trace items on their own, from their XCD's segment first,
-    // and never wait for the rest of their block. The XCD comes from a
-    // hardware register; any value is safe: waves drain every segment.
+    // Trace items from the XCD's segment first (the XCD comes from a hardware
+    // register; any value is safe: every segment is drained). WAVE_ITEMS
+    // (the batch run): waves take items on their own and never wait for the
+    // rest of their block; otherwise (the serving step, whose tiles are sized
+    // for a block) a block takes an item and its waves share it.
     const uint32_t xcd = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) % BATCH_XCDS;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (int k = 0; k < BATCH_XCDS; ++k) {
       const uint32_t q = (xcd + (uint32_t)k) % BATCH_XCDS;
       const uint32_t b0 = a.trace_seg[q], b1 = a.trace_seg[q + 1];
       if (b1 == b0) continue;
+      uint32_t* const ticket = ctl + BATCH_CTL_XTICKET + q;
       for (;;) {
         uint32_t it = 0;
-        if (lane == 0) it = b0 + atomicAdd(ctl + BATCH_CTL_XTICKET + q, 1u);
-        it = __builtin_amdgcn_readfirstlane(it);  // every lane is active here
+        if (WAVE_ITEMS) {
+          if (lane == 0) it = b0 + atomicAdd(ticket, 1u);
+          it = __builtin_amdgcn_readfirstlane(it);  // every lane is active here
+        } else {
+          if (threadIdx.x == 0) s_item = b0 + atomicAdd(ticket, 1u);
+          __syncthreads();
+          it = __builtin_amdgcn_readfirstlane(s_item);
+          __syncthreads();
+        }
         if (it >= b1) break;
         const BatchWork w = load_uniform(items + it);
         const BatchMeshDev m = load_uniform(meshes + w.mesh);
-        trace_item<DEPTH, TSTACK, WIDE>(a.gstack, a.gdepth, ctl + BATCH_CTL_TRACE_ERR, a.ray_tile,
-                                        a.refill_idle, m, w.tile, lds, s_wctr + wave);
+        trace_item<DEPTH, TSTACK, WIDE, WAVE_ITEMS>(a.gstack, a.gdepth,
+                                                    ctl + BATCH_CTL_TRACE_ERR, a.ray_tile,
+                                                    a.refill_idle, m, w.tile, lds,
+                                                    s_wctr + (WAVE_ITEMS ? wave : 0u));
       }
     }
     return;
