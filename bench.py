@@ -709,6 +709,8 @@ def run_c3(ctx, args) -> int:
     occ = rq.DeviceArray(ctx, n)
     vc, tc = counted_batch(ctx, bvh, rays, any_hit=False)
     va, ta = counted_batch(ctx, bvh, rays, any_hit=True)
+    if args.sweep:
+        sweep(ctx, bvh, {"closest": rays, "any": rays})
     cfg = choose_trace(ctx, bvh, rays, rays, args, 2)
     how = ("--trace-config" if args.trace_config else
            "autotuned on these rays" if args.autotune else "fixed for this workload")
@@ -958,7 +960,7 @@ def sweep(ctx, bvh, batches) -> None:
         for refill in (16, 48, 56, 64):
             row = []
             for name, rays in batches.items():
-                anyh = name == "shadow"
+                anyh = name in ("shadow", "any")
                 if anyh and v["tstack"]:
                     row.append(f"{name}=   -   ")
                     continue

@@ -329,7 +329,9 @@ const BatchVariant kBatchVariants[] = {
     {24, false, k_batch<24, false, 5, KM_TRACE, true>},
     {16, false, k_batch<16, false, 5, KM_TRACE, true>},
     {32, false, k_batch<32, false, 4, KM_TRACE, true>},
-    {12, true, k_batch<12, true, 5, KM_TRACE, true>}};
+    {12, true, k_batch<12, true, 5, KM_TRACE, true>},
+    {28, false, k_batch<28, false, 5, KM_TRACE, true>},
+    {30, false, k_batch<30, false, 5, KM_TRACE, true>}};
 // Trace kernels of the serving loop (binary walk).
 const BatchVariant kServeVariants[] = {
     {16, false, k_batch<16, false, 7, KM_TRACE, false>}, {20, false, k_batch<20, false, 7, KM_TRACE, false>},

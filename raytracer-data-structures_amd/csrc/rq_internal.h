@@ -79,7 +79,7 @@ constexpr int TRACE_MAX_STACK = 64;
 // most 64 binary levels has at most 32 wide ones.
 constexpr int TRACE_MAX_STACK_WIDE = 96;
 constexpr int TRACE_MIN_LDS_DEPTH = 12;
-constexpr int TRACE_MAX_VARIANTS = 16;
+constexpr int TRACE_MAX_VARIANTS = 24;
 // Variants a new context uses: the 4-wide walk, 128 threads and 24 LDS entries
 // for closest hit, 256 threads and 16 entries for any hit (the fastest or
 // within a few per cent on camera, shadow, bounce and uniform random rays).

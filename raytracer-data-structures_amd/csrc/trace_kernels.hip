@@ -61,7 +61,8 @@ const Variant kVariants[] = {
     RQ_VARIANT(64, 12, true, false),   RQ_VARIANT(128, 24, false, true),
     RQ_VARIANT(128, 32, false, true), RQ_VARIANT(64, 32, false, true),
     RQ_VARIANT(128, 24, true, true),  RQ_VARIANT(64, 24, true, true),
-    RQ_VARIANT(256, 16, false, true),
+    RQ_VARIANT(256, 16, false, true),  RQ_VARIANT(256, 24, false, true),
+    RQ_VARIANT(256, 28, false, true),
 };
 #undef RQ_VARIANT
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
