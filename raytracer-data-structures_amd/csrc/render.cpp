@@ -23,6 +23,11 @@ struct rq_renderer {
   RaySoA prim, bnc, shad;  // shad holds 2n rays when bounce, else n
   int32_t *h0 = nullptr, *h1 = nullptr;
   float *t0 = nullptr, *t1 = nullptr;
+  // the second hits' shadow rays compacted (side-stream frames): rays, their
+  // samples and their count
+  RaySoA shadc{};
+  uint32_t* shidx = nullptr;
+  uint32_t* shcount = nullptr;
   uint8_t* occ = nullptr;
   float4* lit0 = nullptr;  // per sample: a diffuse primary hit's L.N, Phong term, bounce weight
   float2* lit1 = nullptr;  // per sample: a diffuse second hit's L.N and Phong term
@@ -157,6 +162,9 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
     hipError_t e = hipMemset(r->counters, 0, 16);
     if (e != hipSuccess) rc = hip_err(c, e, "counter init");
   }
+  if (!rc && r->second() && p->shadows) rc = alloc_rays(c, r, n, r->shadc);
+  if (!rc && r->second() && p->shadows) rc = ralloc(c, r, n * 4, (void**)&r->shidx);
+  if (!rc && r->second() && p->shadows) rc = ralloc(c, r, 4, (void**)&r->shcount);
   if (!rc && r->second() && p->shadows) {
     hipError_t e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_primary, hipEventDisableTiming);
@@ -215,12 +223,15 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
     rc = rq_closest_hit(c, b, &br, r->n, &h1, stream);
     if (rc) return rc;
-    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad, r->lit1);
+    const CompactOut co = r->side ? CompactOut{r->shadc, r->shidx, r->shcount, r->occ + r->n}
+                                  : CompactOut{};
+    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad, r->lit1, co);
     if (e != hipSuccess) return hip_err(c, e, "shade bounce");
   }
   if (r->side) {
-    rq_rays sr = as_rays(r->shad, r->n);
-    rc = rq_any_hit(c, b, &sr, r->n, r->occ + r->n, stream);
+    // the second hits' shadow rays, compacted: their count is on the device
+    const rq_rays sr = as_rays(r->shadc);
+    rc = any_hit_indexed(c, b, &sr, r->n, r->shcount, r->shidx, r->occ + r->n, stream);
     if (rc) return rc;
     e = hipStreamWaitEvent((hipStream_t)stream, r->ev_shadow, 0);
     if (e != hipSuccess) return hip_err(c, e, "frame streams");

@@ -217,14 +217,22 @@ __device__ inline void put_ray(const RaySoA& R, uint64_t i, V3 o, V3 d, float tm
 __device__ inline void put_inactive(const RaySoA& R, uint64_t i) {
   R.tmin[i] = 0.0f, R.tmax[i] = 0.0f;
 }
-__device__ inline void shadow_ray(const RaySoA& R, uint64_t i, V3 d, const Geo& g,
-                                  const ShadeParams& sp) {
+struct ShadowRay {
+  V3 o, d;
+  float tmax;
+};
+__device__ inline ShadowRay make_shadow(V3 d, const Geo& g, const ShadeParams& sp) {
   const V3 L{sp.light[0], sp.light[1], sp.light[2]};
   const V3 so = dot(d, g.n) < 0.0f ? add(g.p, mul(g.n, sp.bias)) : sub(g.p, mul(g.n, sp.bias));
   V3 ld = sub(L, g.p);
   const float dist2 = dot(ld, ld);
   ld = normalize(ld);
-  put_ray(R, i, so, ld, 0.0f, sqrtf(dist2));
+  return ShadowRay{so, ld, sqrtf(dist2)};
+}
+__device__ inline void shadow_ray(const RaySoA& R, uint64_t i, V3 d, const Geo& g,
+                                  const ShadeParams& sp) {
+  const ShadowRay r = make_shadow(d, g, sp);
+  put_ray(R, i, r.o, r.d, 0.0f, r.tmax);
 }
 // refract (main.cpp:223-233): eta * I + (eta * cosi - sqrt(k)) * n, or 0 on
 // total internal reflection; clamp as main.cpp:76-79.
@@ -314,23 +322,48 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
 
 // Stage 2: from bounce hits spawn their shadow ray (slot n + i) and keep
 // their shadow-independent colour terms.
+// With a compact output (co.rays.ox set), each shadow ray is also appended
+// to co.rays at a slot taken from co.count (one atomic per wave) with its
+// sample in co.idx, and samples without one get a 0 flag in co.occ: the
+// any-hit launch then traces only the live rays (a few thousand of millions
+// of samples on a mesh that mostly reflects to the sky).
 __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams sp, uint32_t n,
                                                        RaySoA bnc, const int32_t* __restrict__ h1,
                                                        const float* __restrict__ t1, RaySoA shad,
-                                                       float2* __restrict__ lit1) {
+                                                       float2* __restrict__ lit1, CompactOut co) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool in = i < n;
   if (!in) return;
   const int32_t p = h1[i];
   if (p < 0 || mat_of(s, p) != 0) {  // a second refracting / reflecting hit casts no shadow ray
     put_inactive(shad, (uint64_t)n + i);
+    if (co.rays.ox) co.occ[i] = 0;
     return;
   }
   const V3 o = get_o(bnc, i), d = get_d(bnc, i);
   const Geo g = hit_geo(s, p, o, d, t1[i]);
-  if (sp.shadows) shadow_ray(shad, (uint64_t)n + i, d, g, sp);
+  if (sp.shadows) {
+    const ShadowRay r = make_shadow(d, g, sp);
+    put_ray(shad, (uint64_t)n + i, r.o, r.d, 0.0f, r.tmax);
+    if (co.rays.ox) {
+      // every lane still here casts one: wave-aggregated slot
+      const uint64_t m = __ballot(1);
+      const int leader = __ffsll((unsigned long long)m) - 1;
+      const uint32_t lane = threadIdx.x & 63u;
+      uint32_t base = 0;
+      if ((int)lane == leader) base = atomicAdd(co.count, (uint32_t)__popcll(m));
+      base = (uint32_t)__shfl((int)base, leader, 64);
+      const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+      put_ray(co.rays, k, r.o, r.d, 0.0f, r.tmax);
+      co.idx[k] = i;
+    }
+  }
   const LitTerms lt = lit_terms(d, g, sp);
   lit1[i] = make_float2(lt.LdotN, lt.s25);
+}
+
+__global__ __launch_bounds__(64) void k_zero_word(uint32_t* w) {
+  if (threadIdx.x == 0) *w = 0u;
 }
 
 // Colour of sample i (main.cpp:418-490): sky on a miss; the material switch
@@ -521,10 +554,11 @@ hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const S
 
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
-                               const RaySoA& shad, float2* lit1) {
+                               const RaySoA& shad, float2* lit1, const CompactOut& co) {
+  if (co.rays.ox) hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, lc.stream, co.count);
   ProfScope ps(lc, "shade_bounce");
   hipLaunchKernelGGL(k_shade_bounce, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, bnc,
-                     h1, t1, shad, lit1);
+                     h1, t1, shad, lit1, co);
   return hipGetLastError();
 }
 

@@ -550,9 +550,14 @@ int rq_bvh_export_wide(rq_ctx* c, const rq_bvh* b, float* boxes, int32_t* links)
 // noalloc: the allocation-free entry (rq_closest_hit_noalloc): the device
 // scratch array in place of the stream's overflow area and counter slot, the
 // grid capped to fit it, no profiling events.
+// count_dev / out_index (device arrays, internal callers): the launch traces
+// min(n, *count_dev) rays, its slice schedule computed on the device, and
+// writes ray r's results at out_index[r]; it takes a counter slot zeroed by a
+// kernel, as a captured launch does (its slice count is not known here).
 static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,
                  uint8_t* occ, uint32_t* visits, uint32_t* tests, bool any, bool counted,
-                 void* stream, bool noalloc = false) {
+                 void* stream, bool noalloc = false, const uint32_t* count_dev = nullptr,
+                 const uint32_t* out_index = nullptr) {
   if (check_ctx(c) || !b || b->magic != BVH_MAGIC || !rays)
     return set_err(c, RQ_E_INVALID, "bad argument");
   if (n == 0) return RQ_OK;
@@ -571,6 +576,8 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.occ = occ;
   a.visits = visits;
   a.tests = tests;
+  a.nrays_dev = count_dev;
+  a.out_index = out_index;
   int variant = counted ? 0 : (any ? c->any_variant : c->closest_variant);
   int blk, depth, ts, wide;
   trace_variant_info(variant, &blk, &depth, &ts, &wide);
@@ -586,6 +593,8 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   hipError_t e = hipStreamIsCapturing((hipStream_t)stream, &cap);
   if (e != hipSuccess) return hip_err(c, e, "stream capture status");
   const bool captured = cap == hipStreamCaptureStatusActive;
+  // a counter slot whose start is not tracked here: zeroed in the stream first
+  const bool own_zero = captured || count_dev != nullptr;
   a.gdepth = (uint32_t)(trace_max_stack(variant) - depth);
   if (noalloc) {
     const size_t per_block = (size_t)blk * a.gdepth * 2 * sizeof(int32_t);
@@ -613,10 +622,10 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   // after the launches enqueued before them (the base). A base baked into a
   // captured graph would be stale at the next replay, so a captured query
   // takes a capture slot of its own, zeroed in the graph before the trace.
-  const uint32_t slot = captured ? COUNTER_SLOTS + (c->capture_seq++ % CAPTURE_SLOTS)
+  const uint32_t slot = own_zero ? COUNTER_SLOTS + (c->capture_seq++ % CAPTURE_SLOTS)
                                  : next_counter_slot(c);
   a.counter = reinterpret_cast<unsigned long long*>(c->counters) + slot;
-  a.ctr_base = captured ? 0ull : c->counter_base[slot];
+  a.ctr_base = own_zero ? 0ull : c->counter_base[slot];
   a.err = c->errword;
   // the launch grid (launch_trace takes the same), its waves and slice size
   a.grid = std::max(1u, std::min(c->trace_grid[variant], (n + (uint32_t)blk - 1) / (uint32_t)blk));
@@ -626,8 +635,9 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
       slice_schedule(n, trace_chunk(n, c->trace_grid[variant] * (uint32_t)blk / 64u,
                                     any ? c->slice_any : c->slice_closest));
   a.chunk = sched.chunk, a.chunk_small = sched.small, a.big_slices = sched.big;
+  if (count_dev) a.chunk = any ? c->slice_any : c->slice_closest;  // the largest slice
   a.sphere = b->scene && b->scene->kind == RQ_PRIM_SPHERE ? 1 : 0;
-  if (captured) {
+  if (own_zero) {
     e = launch_counter_reset(launch_ctx(c, stream), a.counter);
     if (e != hipSuccess) return hip_err(c, e, "counter reset");
   }
@@ -635,8 +645,19 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   if (e != hipSuccess) return hip_err(c, e, "trace launch");
   // the launch moves its slot's counter by its number of slices, or not at
   // all when the waves' first slices cover it (RayStream)
-  if (!captured && sched.total > waves) c->counter_base[slot] += sched.total;
+  if (!own_zero && sched.total > waves) c->counter_base[slot] += sched.total;
   return RQ_OK;
+}
+
+extern "C++" {
+namespace rq {
+int any_hit_indexed(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t capacity,
+                    const uint32_t* count_dev, const uint32_t* out_index, uint8_t* occ,
+                    void* stream) {
+  return query(c, b, rays, capacity, nullptr, occ, nullptr, nullptr, true, false, stream, false,
+               count_dev, out_index);
+}
+}  // namespace rq
 }
 
 int rq_closest_hit(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq_hits* hits,

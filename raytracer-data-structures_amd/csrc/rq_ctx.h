@@ -104,4 +104,10 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit);
 int ensure_height(rq_ctx* c, const rq_bvh* b);
 // Frees the pinned staging buffer of the host-pointer queries (rq_destroy).
 void stage_release(rq_ctx* c);
+// Any hit over min(capacity, *count_dev) rays whose count is known on the
+// device only, ray r's flag written at occ[out_index[r]] (the renderer's
+// compacted second-hit shadow rays).
+int any_hit_indexed(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t capacity,
+                    const uint32_t* count_dev, const uint32_t* out_index, uint8_t* occ,
+                    void* stream);
 }  // namespace rq

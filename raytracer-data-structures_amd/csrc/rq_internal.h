@@ -100,7 +100,8 @@ constexpr uint32_t TRACE_RAY_CHUNK_MAX = 4096;  // largest that may be configure
 // slices, halved down to one ray per lane (64) for smaller launches, so that
 // a small batch still spreads over as many waves as it can fill (each slice
 // after a wave's first costs one device atomic on the launch's counter).
-inline uint32_t trace_chunk(uint32_t n, uint32_t waves, uint32_t largest = TRACE_RAY_CHUNK) {
+__host__ __device__ inline uint32_t trace_chunk(uint32_t n, uint32_t waves,
+                                                uint32_t largest = TRACE_RAY_CHUNK) {
   uint32_t c = largest;
   while (c > 64 && (uint64_t)n < 2ull * c * waves) c >>= 1;
   return c;
@@ -111,7 +112,7 @@ inline uint32_t trace_chunk(uint32_t n, uint32_t waves, uint32_t largest = TRACE
 struct SliceSchedule {
   uint32_t chunk, small, big, total;  // sizes, number of large slices, number of slices
 };
-inline SliceSchedule slice_schedule(uint32_t n, uint32_t chunk) {
+__host__ __device__ inline SliceSchedule slice_schedule(uint32_t n, uint32_t chunk) {
   SliceSchedule q;
   q.chunk = chunk;
   q.small = chunk / 4 < 64 ? 64 : chunk / 4;
@@ -213,6 +214,13 @@ struct TraceArgs {
   uint32_t chunk_small; // rays per slice after the first big_slices (standalone kernels)
   uint32_t big_slices;  // slices of `chunk` rays before the small ones
   int32_t sphere;       // leaves are spheres (centre, r, r*r) instead of triangles
+  // Set when the ray count is known on the device only: the launch traces rays
+  // [0, min(nrays, *nrays_dev)) and computes its slice schedule from that
+  // count (`chunk` is then the largest slice; the counter starts at zero).
+  const uint32_t* nrays_dev;
+  // Set when results go elsewhere than to the ray's own index: ray r's
+  // outputs are written at out_index[r].
+  const uint32_t* out_index;
 };
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
                         int variant);

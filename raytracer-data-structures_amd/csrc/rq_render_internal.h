@@ -48,9 +48,18 @@ hipError_t launch_raygen_pinhole(const LaunchCtx& lc, const PinholeParams& p, co
 hipError_t launch_shade_primary(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                 uint32_t n, const RaySoA& prim, const int32_t* h0, const float* t0,
                                 const RaySoA& shad, const RaySoA& bnc, float4* lit0);
+// Compacted copy of the second hits' shadow rays (all null: not kept):
+// rays[k] with its sample idx[k], k < *count; occ: the second-hit flags,
+// zeroed for the samples without a shadow ray.
+struct CompactOut {
+  RaySoA rays;
+  uint32_t* idx;
+  uint32_t* count;
+  uint8_t* occ;
+};
 hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const ShadeParams& sp,
                                uint32_t n, const RaySoA& bnc, const int32_t* h1, const float* t1,
-                               const RaySoA& shad, float2* lit1);
+                               const RaySoA& shad, float2* lit1, const CompactOut& co);
 // Adds the number of active rays (tmin < tmax) of R[0, n) to *counter.
 hipError_t launch_count_active(const LaunchCtx& lc, const RaySoA& R, uint64_t n,
                                uint32_t* counter);

@@ -28,11 +28,22 @@ struct RayStream {
   uint64_t base = 0;                    // ctr64 at launch
   uint32_t wave = 0, waves = 0;         // this wave's index, waves in the grid
 };
-__device__ inline uint32_t slice_begin(const TraceArgs& a, uint64_t s) {
-  return s <= a.big_slices ? (uint32_t)min(s * a.chunk, (uint64_t)a.nrays)
-                           : (uint32_t)min((uint64_t)a.big_slices * a.chunk +
-                                               (s - a.big_slices) * a.chunk_small,
-                                           (uint64_t)a.nrays);
+// The launch's ray count and slice sizes, in scalar registers: from the
+// kernel arguments, or computed here from a count read from device memory
+// (TraceArgs::nrays_dev).
+struct Schedule {
+  uint32_t n, chunk, small, big;
+};
+__device__ inline Schedule schedule_of(const TraceArgs& a, uint32_t waves) {
+  if (!a.nrays_dev) return Schedule{a.nrays, a.chunk, a.chunk_small, a.big_slices};
+  const uint32_t n = min(a.nrays, __builtin_amdgcn_readfirstlane(*a.nrays_dev));
+  const SliceSchedule q = slice_schedule(n, trace_chunk(n, waves, a.chunk));
+  return Schedule{n, q.chunk, q.small, q.big};
+}
+__device__ inline uint32_t slice_begin(const Schedule& a, uint64_t s) {
+  return s <= a.big ? (uint32_t)min(s * a.chunk, (uint64_t)a.n)
+                    : (uint32_t)min((uint64_t)a.big * a.chunk + (s - a.big) * a.small,
+                                    (uint64_t)a.n);
 }
 
 struct RayState {
@@ -157,6 +168,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
   const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
   const int max_sp = DEPTH + (int)a.gdepth;
+  const Schedule sc = schedule_of(a, (uint32_t)rs.waves);
   const float slack = (SPHERES && a.sphere) ? SPHERE_SLACK : SLAB_SLACK;
 
   RayState r{};  // lanes that never take a ray still read their (unused) octant
@@ -166,8 +178,8 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
   uint32_t cnext = 0, cend = 0;
   bool wave_out = false;  // the global stream is exhausted for this wave
   if (rs.ctr64) {
-    cnext = slice_begin(a, rs.wave), cend = slice_begin(a, rs.wave + 1ull);
-    wave_out = cnext >= a.nrays;
+    cnext = slice_begin(sc, rs.wave), cend = slice_begin(sc, rs.wave + 1ull);
+    wave_out = cnext >= sc.n;
   }
   uint32_t ray = 0;
   int32_t node = 0, best = -1;
@@ -176,17 +188,18 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
   uint32_t vis = 0, tst = 0;
 
   auto finish = [&]() {
+    const uint32_t oi = a.out_index ? a.out_index[ray] : ray;
     if (ANY) {
-      a.occ[ray] = best >= 0 ? 1 : 0;
+      a.occ[oi] = best >= 0 ? 1 : 0;
     } else {
-      a.prim[ray] = best;
-      if (a.t) a.t[ray] = best >= 0 ? bt : INFINITY;
-      if (a.u) a.u[ray] = bu;
-      if (a.v) a.v[ray] = bv;
+      a.prim[oi] = best;
+      if (a.t) a.t[oi] = best >= 0 ? bt : INFINITY;
+      if (a.u) a.u[oi] = bu;
+      if (a.v) a.v[oi] = bv;
     }
     if (COUNT) {
-      a.visits[ray] = vis;
-      a.tests[ray] = tst;
+      a.visits[oi] = vis;
+      a.tests[oi] = tst;
     }
     active = false;
   };
@@ -229,22 +242,22 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     uint64_t m = __ballot(!active);
     for (int round = 0; round < 2 && m && !wave_out; ++round) {
       if (cnext >= cend) {
-        if (rs.ctr64 && slice_begin(a, rs.waves) >= a.nrays) {  // first slices cover all
+        if (rs.ctr64 && slice_begin(sc, rs.waves) >= sc.n) {  // first slices cover all
           wave_out = true;
           break;
         }
         uint64_t s = 0;
         if (lane == 0)
           s = rs.ctr64 ? atomicAdd(rs.ctr64, 1ull) - rs.base + rs.waves
-                       : atomicAdd(rs.ctr32, a.chunk);
+                       : atomicAdd(rs.ctr32, sc.chunk);
         s = __shfl(s, 0, 64);
-        const uint32_t b = rs.ctr64 ? slice_begin(a, s) : (uint32_t)min(s, (uint64_t)a.nrays);
-        if (b >= a.nrays) {
+        const uint32_t b = rs.ctr64 ? slice_begin(sc, s) : (uint32_t)min(s, (uint64_t)sc.n);
+        if (b >= sc.n) {
           wave_out = true;
           break;
         }
         cnext = b;
-        cend = rs.ctr64 ? slice_begin(a, s + 1) : min(b + a.chunk, a.nrays);
+        cend = rs.ctr64 ? slice_begin(sc, s + 1) : min(b + sc.chunk, sc.n);
       }
       const uint32_t take = min((uint32_t)__popcll(m), cend - cnext);
       const uint32_t rank = (uint32_t)__popcll(m & lt);
