@@ -223,10 +223,12 @@ int rq_create(rq_ctx** out, int device, size_t budget) {
   c->prof = new Profiler();
   size_t gmax = 0;
   for (int v = 0; v < trace_num_variants(); ++v) {
-    c->trace_grid[v] = trace_grid_blocks(device, v);
+    c->trace_grid[v] = trace_grid_blocks(device, v, false);
+    c->trace_grid_any[v] = trace_grid_blocks(device, v, true);
     int blk, depth, ts;
     trace_variant_info(v, &blk, &depth, &ts);
-    gmax = std::max(gmax, (size_t)c->trace_grid[v] * blk * (trace_max_stack(v) - depth));
+    gmax = std::max(gmax, (size_t)std::max(c->trace_grid[v], c->trace_grid_any[v]) * blk *
+                              (trace_max_stack(v) - depth));
   }
   // counter slots (eager, then capture) + one trailing word of traversal error bits
   const size_t cbytes = ((COUNTER_SLOTS + CAPTURE_SLOTS) * 2 + 1) * sizeof(uint32_t);
@@ -628,11 +630,12 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   a.ctr_base = own_zero ? 0ull : c->counter_base[slot];
   a.err = c->errword;
   // the launch grid (launch_trace takes the same), its waves and slice size
-  a.grid = std::max(1u, std::min(c->trace_grid[variant], (n + (uint32_t)blk - 1) / (uint32_t)blk));
+  const uint32_t full_grid = any ? c->trace_grid_any[variant] : c->trace_grid[variant];
+  a.grid = std::max(1u, std::min(full_grid, (n + (uint32_t)blk - 1) / (uint32_t)blk));
   a.refill_idle = any ? c->refill_any : c->refill_closest;
   const uint32_t waves = a.grid * (uint32_t)blk / 64u;
   const SliceSchedule sched =
-      slice_schedule(n, trace_chunk(n, c->trace_grid[variant] * (uint32_t)blk / 64u,
+      slice_schedule(n, trace_chunk(n, full_grid * (uint32_t)blk / 64u,
                                     any ? c->slice_any : c->slice_closest));
   a.chunk = sched.chunk, a.chunk_small = sched.small, a.big_slices = sched.big;
   if (count_dev) a.chunk = any ? c->slice_any : c->slice_closest;  // the largest slice

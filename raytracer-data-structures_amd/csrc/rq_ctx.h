@@ -15,7 +15,8 @@ struct rq_ctx {
   std::unordered_map<void*, size_t> sizes;  // live device allocations
   rq::Profiler* prof = nullptr;
   bool prof_on = false;
-  uint32_t trace_grid[rq::TRACE_MAX_VARIANTS] = {};  // persistent grid per kernel variant
+  // persistent grid per kernel variant and query kind (closest, any)
+  uint32_t trace_grid[rq::TRACE_MAX_VARIANTS] = {}, trace_grid_any[rq::TRACE_MAX_VARIANTS] = {};
   // chosen traversal variants (the 4-wide walks until tuned or set)
   int closest_variant = rq::TRACE_DEFAULT_CLOSEST, any_variant = rq::TRACE_DEFAULT_ANY;
   // idle lanes before a wave refills (64: only once every lane is idle), per query kind

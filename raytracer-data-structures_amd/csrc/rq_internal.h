@@ -227,7 +227,7 @@ hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool 
 int trace_num_variants();
 void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide = nullptr);
 int trace_max_stack(int variant);  // total stack entries per lane (LDS + overflow)
-uint32_t trace_grid_blocks(int device, int variant);
+uint32_t trace_grid_blocks(int device, int variant, bool any);
 hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64_t bytes);
 // Zeroes one ray-stream counter with a one-thread kernel (a memset node would
 // not be safe to replay in a captured graph).

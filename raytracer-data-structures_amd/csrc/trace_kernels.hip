@@ -15,8 +15,10 @@
 namespace rq {
 namespace {
 
-template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool WIDE = false>
-__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+// WAVES > 0: registers capped for that many waves per SIMD.
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool WIDE = false, int WAVES = 0>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1)))
+void k_trace(TraceArgs a) {
   __shared__ int32_t s_node[DEPTH * BLOCK];
   __shared__ float s_tn[TSTACK ? DEPTH * BLOCK : 1];
   RayStream rs;
@@ -50,9 +52,20 @@ struct Variant {
 };
 
 // Variant 0 is the canonical binary configuration (the counted kernels run
-// it); the wide variants need the BVH's 4-wide nodes.
-#define RQ_VARIANT(B, D, T, W) \
-  { B, D, T, W, k_trace<false, false, B, D, T, W>, k_trace<true, false, B, D, false, W> }
+// it); the wide variants need the BVH's 4-wide nodes. A wide any-hit kernel
+// whose LDS stack leaves room for 6 waves per SIMD (6 x 4 waves x 64 lanes x
+// DEPTH x 4 B <= 160 KB: DEPTH <= 26) is compiled for 6 (80 VGPRs, no spills;
+// left to itself the compiler takes 81 and gets 5): it waits on dependent
+// node fetches more than it issues, and the sixth wave hides more of that
+// wait (C2 shadow rays 11 %, C3 6 % faster). The closest-hit walk issues at
+// ~0.8 of the peak and spills at that cap, so it keeps its 5 waves. The
+// launch grid follows each kernel's own occupancy.
+constexpr int any_waves(bool wide, int depth) { return wide && depth <= 26 ? 6 : 0; }
+#define RQ_VARIANT(B, D, T, W)                                  \
+  {                                                             \
+    B, D, T, W, k_trace<false, false, B, D, T, W>,              \
+        k_trace<true, false, B, D, false, W, any_waves(W, D)>   \
+  }
 const Variant kVariants[] = {
     RQ_VARIANT(128, 24, false, false), RQ_VARIANT(128, 16, false, false),
     RQ_VARIANT(64, 24, false, false),  RQ_VARIANT(256, 16, false, false),
@@ -83,11 +96,11 @@ void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide) {
 
 int trace_max_stack(int v) { return kVariants[v].wide ? TRACE_MAX_STACK_WIDE : TRACE_MAX_STACK; }
 
-uint32_t trace_grid_blocks(int device, int v) {
+uint32_t trace_grid_blocks(int device, int v, bool any) {
   int cus = 0, per = 0;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kVariants[v].closest,
-                                                     kVariants[v].block, 0);
+  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per, any ? kVariants[v].any : kVariants[v].closest, kVariants[v].block, 0);
   if (cus <= 0) cus = 256;
   if (per <= 0) per = 2;
   return (uint32_t)(cus * per);
