@@ -42,7 +42,7 @@ typedef struct {
 typedef struct {
   int32_t lds_depth; /* stack entries per lane in LDS (rest in global memory) */
   int32_t tstack;    /* stack entries carry their entry distance             */
-  uint32_t ray_tile; /* rays per work item                                   */
+  uint32_t ray_tile; /* rays per work item (at least 256)                    */
   uint32_t refill_idle;
 } rq_batch_config;
 

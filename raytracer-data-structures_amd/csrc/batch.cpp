@@ -14,9 +14,9 @@
 using namespace rq;
 
 namespace {
-constexpr uint32_t kRayTiles[] = {1024, 2048, 4096, 8192, 32768};
+constexpr uint32_t kRayTiles[] = {256, 512, 1024, 2048, 8192};
 constexpr uint32_t kRefills[] = {16, 32, 64};
-constexpr uint32_t kMinRayTile = 1024;
+constexpr uint32_t kMinRayTile = 256;
 }  // namespace
 
 struct rq_batch {
@@ -33,10 +33,10 @@ struct rq_batch {
   std::vector<uint32_t> build_grid;    // its persistent grid, per build kernel
   std::vector<uint32_t> grid;          // the trace kernel's, per configuration
   // the autotuned choice on 512 mixed meshes (1k-500k triangles, incoherent
-  // rays): trace configuration 0 (LDS depth 24), 1024-ray wave items, refill
-  // at 16 idle lanes
-  int variant = 0;
-  uint32_t ray_tile = 1024, refill = 16;
+  // rays): trace configuration 1 (LDS depth 16), 512-ray items, refill at 16
+  // idle lanes
+  int variant = 1;
+  uint32_t ray_tile = 512, refill = 16;
   uint32_t phase_begin[BATCH_PHASES + 1] = {};
   uint32_t trace_seg[BATCH_XCDS + 1] = {};
   uint32_t built_ray_tile = 0;

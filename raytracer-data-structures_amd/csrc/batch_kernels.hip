@@ -156,6 +156,91 @@ __device__ inline void trace_item(int32_t* gstack, uint32_t gdepth, uint32_t* er
   if (!WAVE_ITEM) __syncthreads();
 }
 
+// Slices of one wave's walk over one mesh and query kind (the batch run): the
+// tile of the item the walk starts with, then the tiles of the items the wave
+// takes next from the XCD segments while they are of the same mesh and kind.
+// A mesh's items sit in a row in its segment (closest-hit tiles, then any-hit
+// tiles), so a wave lets its lanes run dry once per run of one mesh's items
+// rather than once per tile; the first item of another mesh or kind ends the
+// walk and starts the next one.
+struct ItemSlices {
+  static constexpr bool kActive = true;
+  const BatchArgs& args;
+  uint32_t xcd, k;  // segment cursor: segment (xcd + k) % BATCH_XCDS
+  uint32_t mesh, kind, nrays;
+  BatchWork held;   // the walk's first item, then the item that ended it
+  bool first, have_held;
+  // The wave's next item, its own XCD's segment first (every lane active).
+  __device__ bool take(BatchWork& w) {
+    for (; k < (uint32_t)BATCH_XCDS; ++k) {
+      const uint32_t q = (xcd + k) % BATCH_XCDS;
+      const uint32_t b0 = args.trace_seg[q], b1 = args.trace_seg[q + 1];
+      if (b1 == b0) continue;
+      uint32_t it = 0;
+      if ((threadIdx.x & 63u) == 0) it = b0 + atomicAdd(args.ctl + BATCH_CTL_XTICKET + q, 1u);
+      it = __builtin_amdgcn_readfirstlane(it);
+      if (it < b1) {
+        w = load_uniform(args.items + it);
+        return true;
+      }
+    }
+    return false;
+  }
+  __device__ bool next(uint32_t& b, uint32_t& e) {
+    BatchWork w;
+    if (first) {
+      w = held;
+      first = false;
+    } else {
+      if (!take(w)) return false;
+      if (w.mesh != mesh || (w.tile & BATCH_ANY_BIT) != kind) {
+        held = w, have_held = true;
+        return false;
+      }
+    }
+    b = (w.tile & ~BATCH_ANY_BIT) * args.ray_tile;
+    e = min(b + args.ray_tile, nrays);
+    return true;
+  }
+};
+
+// One wave's walk over the rays of one mesh and query kind, its slices from
+// src (the whole mesh's arrays: a ray's index is its index in the mesh).
+template <int DEPTH, bool TSTACK, bool WIDE>
+__device__ inline void trace_mesh(const BatchArgs& a, const BatchMeshDev& m, uint32_t* lds,
+                                  ItemSlices* src) {
+  TraceArgs t{};
+  t.nodes = m.v.B.nodes;
+  t.wide = m.v.B.wide;
+  t.tris = m.v.B.tris;
+  t.nprims = m.v.n;
+  t.rays = m.rays;
+  t.nrays = m.nrays;
+  t.gstack = a.gstack;
+  t.gdepth = a.gdepth;
+  t.err = a.ctl + BATCH_CTL_TRACE_ERR;
+  t.refill_idle = a.refill_idle;
+  t.chunk = 64;
+  t.sphere = 0;  // batch meshes are triangle scenes
+  int32_t* s_node = reinterpret_cast<int32_t*>(lds);
+  float* s_tn = reinterpret_cast<float*>(lds + DEPTH * BUILD_BLOCK);
+  const uint32_t gtid = blockIdx.x * BUILD_BLOCK + threadIdx.x;
+  const uint32_t gstride = gridDim.x * BUILD_BLOCK;
+  const RayStream stream;  // unused: the slices come from src
+  if (src->kind) {
+    t.occ = m.occ;
+    trace_loop<true, false, BUILD_BLOCK, DEPTH, false, false, WIDE>(t, stream, s_node, s_tn, gtid,
+                                                                    gstride, src);
+  } else {
+    t.prim = m.prim;
+    t.t = m.t;
+    t.u = m.u;
+    t.v = m.v_;
+    trace_loop<false, false, BUILD_BLOCK, DEPTH, TSTACK, false, WIDE>(t, stream, s_node, s_tn,
+                                                                      gtid, gstride, src);
+  }
+}
+
 // LDS words of the batch kernel: the largest of the lean sort tile, the
 // Morton histograms and the traversal stack.
 constexpr int cmax(int x, int y) { return x > y ? x : y; }
@@ -170,8 +255,8 @@ constexpr int batch_lds_words(int depth, bool tstack) {
 // batch run (WIDE: walking the 4-wide nodes of the widen phase), or as the
 // continuous-batching step of the serving loop (binary nodes). WAVES waves
 // per SIMD cap the registers: the build phases are written to fit 8 (lean
-// 8-key sort tiles, descriptors loaded per phase); the 4-wide walk needs
-// about 90 VGPRs (5 waves), the binary walk fits 7-8.
+// 8-key sort tiles, descriptors loaded per phase); the 4-wide walk fits 6
+// (79 VGPRs) where its LDS stack allows, the binary walk 7-8.
 enum KernelMode { KM_BUILD, KM_TRACE };
 template <int DEPTH, bool TSTACK, int WAVES, KernelMode MODE, bool WIDE, bool WAVE_ITEMS = WIDE>
 __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_batch(
@@ -188,6 +273,17 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
     // rest of their block; otherwise (the serving step, whose tiles are sized
     // for a block) a block takes an item and its waves share it.
     const uint32_t xcd = __builtin_amdgcn_s_getreg(XCC_ID_HWREG) % BATCH_XCDS;
+    if constexpr (WAVE_ITEMS) {
+      ItemSlices src{a, xcd, 0u, 0u, 0u, 0u, BatchWork{}, false, false};
+      if (!src.take(src.held)) return;
+      for (;;) {
+        const BatchMeshDev m = load_uniform(meshes + src.held.mesh);
+        src.mesh = src.held.mesh, src.kind = src.held.tile & BATCH_ANY_BIT, src.nrays = m.nrays;
+        src.first = true, src.have_held = false;
+        trace_mesh<DEPTH, TSTACK, WIDE>(a, m, lds, &src);
+        if (!src.have_held) return;
+      }
+    }
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (int k = 0; k < BATCH_XCDS; ++k) {
       const uint32_t q = (xcd + (uint32_t)k) % BATCH_XCDS;
@@ -326,10 +422,10 @@ constexpr int kBuildWaves[] = {8, 7, 6, 5};
 constexpr int kNumBuildVariants = sizeof(kBuild) / sizeof(kBuild[0]);
 // Trace kernels of a batch run (4-wide walk): the batch configurations.
 const BatchVariant kBatchVariants[] = {
-    {24, false, k_batch<24, false, 5, KM_TRACE, true>},
-    {16, false, k_batch<16, false, 5, KM_TRACE, true>},
+    {24, false, k_batch<24, false, 6, KM_TRACE, true>},
+    {16, false, k_batch<16, false, 6, KM_TRACE, true>},
     {32, false, k_batch<32, false, 4, KM_TRACE, true>},
-    {12, true, k_batch<12, true, 5, KM_TRACE, true>},
+    {12, true, k_batch<12, true, 6, KM_TRACE, true>},
     {28, false, k_batch<28, false, 5, KM_TRACE, true>},
     {30, false, k_batch<30, false, 5, KM_TRACE, true>}};
 // Trace kernels of the serving loop (binary walk).

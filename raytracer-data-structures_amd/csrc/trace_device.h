@@ -46,6 +46,15 @@ __device__ inline uint32_t slice_begin(const Schedule& a, uint64_t s) {
                                     (uint64_t)a.n);
 }
 
+// A wave takes its slices from the RayStream unless the caller gives a slice
+// source: Src::next(b, e) returns the wave's next slice [b, e) of a.rays, or
+// false when the wave is to finish its rays and return. It is called with
+// every lane of the wave active and its result is wave-uniform.
+struct NoSlices {
+  static constexpr bool kActive = false;
+  __device__ bool next(uint32_t&, uint32_t&) { return false; }
+};
+
 struct RayState {
   float ox, oy, oz, dx, dy, dz;
   float ix, iy, iz;  // 1/d
@@ -153,7 +162,8 @@ __device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) 
 // redone against the current best). Leaves reached from a node are tested in
 // the same loop iteration; the order of node visits and triangle tests is the
 // canonical one unless TSTACK skips an entry. Rays are taken a.chunk at a time
-// from *ctr (global or LDS); the loop returns when *ctr passes a.nrays.
+// from *ctr (global or LDS), or a slice at a time from *src; the loop returns
+// when the stream has no rays left for the wave and its lanes are idle.
 // SPHERES: the leaves may be spheres (a.sphere); false compiles triangles only.
 // WIDE: walk the 4-wide nodes (a.wide) instead of the binary ones: the slots
 // a ray's box test accepts are visited nearest first (closest hit; any hit in
@@ -161,9 +171,10 @@ __device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) 
 // with the same slack, so culling is as conservative as the binary walk and
 // the result (the lexicographic minimum) is the same; visit order differs.
 template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true,
-          bool WIDE = false>
+          bool WIDE = false, class Src = NoSlices>
 __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32_t* s_node,
-                                  float* s_tn, uint32_t gtid, uint32_t gstride) {
+                                  float* s_tn, uint32_t gtid, uint32_t gstride,
+                                  Src* src = nullptr) {
   static_assert(!(WIDE && COUNT), "counted traversal walks the binary nodes");
   const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
@@ -241,7 +252,14 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     // ---- refill idle lanes from the wave's slice of the ray stream -------
     uint64_t m = __ballot(!active);
     for (int round = 0; round < 2 && m && !wave_out; ++round) {
-      if (cnext >= cend) {
+      if (Src::kActive && cnext >= cend) {
+        uint32_t b = 0, e = 0;
+        if (!src->next(b, e)) {
+          wave_out = true;
+          break;
+        }
+        cnext = b, cend = e;
+      } else if (cnext >= cend) {
         if (rs.ctr64 && slice_begin(sc, rs.waves) >= sc.n) {  // first slices cover all
           wave_out = true;
           break;

@@ -202,7 +202,7 @@ def test_batch_graph_replays(setup):
     setup.check_trees()
 
 
-@pytest.mark.parametrize("tile", [1024, 8192])
+@pytest.mark.parametrize("tile", [256, 1024, 8192])
 def test_batch_every_config(setup, rq, tile):
     for c in rq.Batch.configs():
         log(f"batch config {c} tile={tile}")
@@ -257,7 +257,7 @@ def test_batch_graph_after_queue_rebuild(setup):
 def test_batch_autotune(setup):
     log("batch autotune")
     v, t, r, ms = setup.batch.autotune(1)
-    assert 0 <= v < len(_rq.Batch.configs()) and t >= 1024 and 1 <= r <= 64 and ms > 0
+    assert 0 <= v < len(_rq.Batch.configs()) and t >= 256 and 1 <= r <= 64 and ms > 0
     setup.clear()
     setup.batch.run_graph()
     setup.check(setup.results(), "autotuned")
