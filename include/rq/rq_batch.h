@@ -69,7 +69,7 @@ int rq_batch_set_config(rq_ctx* ctx, rq_batch* b, int index, uint32_t ray_tile,
                         uint32_t refill_idle);
 int rq_batch_get_config(rq_ctx* ctx, const rq_batch* b, int* index, uint32_t* ray_tile,
                         uint32_t* refill_idle);
-/* The build kernel's waves per SIMD (8, 7, 6 or 5: its register cap, traded
+/* The build kernel's waves per SIMD (8, 7, 6, 5 or 4: its register cap, traded
  * against occupancy); the autotuner chooses it after the trace configuration. */
 int rq_batch_set_build_waves(rq_ctx* ctx, rq_batch* b, int waves);
 int rq_batch_get_build_waves(rq_ctx* ctx, const rq_batch* b, int* waves);
@@ -80,6 +80,11 @@ int rq_batch_autotune(rq_ctx* ctx, rq_batch* b, int repeats, int* best_index,
                       uint32_t* best_ray_tile, uint32_t* best_refill, double* best_ms);
 /* Items of the work queue per phase (11 phases), for inspection. */
 int rq_batch_phase_items(rq_ctx* ctx, rq_batch* b, uint32_t* counts, int cap);
+/* Synchronises and reports the last run's build phases in ms (the first
+ * `cap` of the 10 build phases, init ... widen), from wall-clock stamps the
+ * build kernel takes at its grid barriers; returns the number of build
+ * phases. A phase the run did not stamp reads -1. */
+int rq_batch_phase_ms(rq_ctx* ctx, rq_batch* b, double* ms, int cap);
 /* Synchronises and reports the error bits of the last run: 1 = a grid
  * barrier gave up waiting (a member block did not arrive within seconds; the
  * results of that run are invalid), 2 = a ray needed a deeper stack than

@@ -131,9 +131,13 @@ int make_work(rq_ctx* c, rq_batch* b, uint32_t ray_tile) {
           k = tiles_of(n, SORT_TILE_LEAN);
           break;
         case PH_KARRAS: k = n >= 2 ? tiles_of(n - 1, NODE_TILE) : 0; break;
-        case PH_REFIT: k = tiles_of(n, REFIT_CHUNK); break;
-        case PH_TOP: k = tiles_of(n, REFIT_CHUNK) > 1 ? 1 : 0; break;
-        case PH_WIDEN: k = n >= 2 ? tiles_of(n - 1, BUILD_BLOCK) : 0; break;
+        case PH_REFIT: k = tiles_of(n, BUILD_BLOCK * BATCH_REFIT_CHUNKS); break;
+        // spanning nodes exist once there are two chunks; pass 1 writes the
+        // wide records of all the others
+        case PH_TOP:
+        case PH_WIDEN:
+          k = tiles_of(n, BUILD_BLOCK) > 1 ? tiles_of(n - 1, BUILD_BLOCK * BATCH_SPAN_TILES) : 0;
+          break;
         default: break;
       }
       for (uint32_t t = 0; t < k; ++t) w.push_back(BatchWork{m, t});
@@ -211,7 +215,7 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     v->n = items[i].scene->ntri;
     v->owned = false;
     b->bvhs.push_back(v);
-    rc = bvh_alloc(c, v, v->n, true, false);
+    rc = bvh_alloc(c, v, v->n, true, true);  // wide nodes, chunk-union refit
   }
   // device mesh table
   std::vector<BatchMeshDev> md(count);
@@ -227,9 +231,11 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     d.prim = it.hits.prim, d.t = it.hits.t, d.u = it.hits.u, d.v_ = it.hits.v;
     d.occ = it.occluded;
     const uint32_t n = v->n;
+    // init, bounds + Morton, sort, Karras, refit, spanning nodes + their wide
+    // records, trace
     work_cap += 1 + 2 * tiles_of(n, TRI_TILE) + SORT_PASSES * tiles_of(n, SORT_TILE_LEAN) +
-                tiles_of(n, NODE_TILE) + tiles_of(n, REFIT_CHUNK) + 1 + tiles_of(n, BUILD_BLOCK) +
-                2 * (size_t)tiles_of(it.nrays, kMinRayTile);
+                tiles_of(n, NODE_TILE) + tiles_of(n, BUILD_BLOCK) +
+                2 * (size_t)tiles_of(n, BUILD_BLOCK) + 2 * (size_t)tiles_of(it.nrays, kMinRayTile);
   }
   if (!rc) rc = balloc(c, b, std::max<size_t>(1, count) * sizeof(BatchMeshDev), (void**)&b->meshes);
   if (!rc && count) {
@@ -241,9 +247,9 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     b->work_cap = std::max<size_t>(work_cap, 1);
     rc = balloc(c, b, b->work_cap * sizeof(BatchWork), (void**)&b->work);
   }
-  if (!rc) rc = balloc(c, b, BATCH_CTL_WORDS * sizeof(uint32_t), (void**)&b->ctl);
+  if (!rc) rc = balloc(c, b, BATCH_CTL_ALLOC_WORDS * sizeof(uint32_t), (void**)&b->ctl);
   if (!rc) {
-    hipError_t e = hipMemset(b->ctl, 0, BATCH_CTL_WORDS * sizeof(uint32_t));
+    hipError_t e = hipMemset(b->ctl, 0, BATCH_CTL_ALLOC_WORDS * sizeof(uint32_t));
     if (e != hipSuccess) rc = hip_err(c, e, "batch control init");
   }
   if (!rc) {
@@ -409,6 +415,22 @@ int rq_batch_phase_items(rq_ctx* c, rq_batch* b, uint32_t* counts, int cap) {
   for (int ph = 0; ph < BATCH_PHASES && ph < cap; ++ph)
     counts[ph] = b->phase_begin[ph + 1] - b->phase_begin[ph];
   return BATCH_PHASES;
+}
+
+int rq_batch_phase_ms(rq_ctx* c, rq_batch* b, double* ms, int cap) {
+  if (!c || !b || !ms) return RQ_E_INVALID;
+  uint32_t w[BATCH_CTL_WORDS];
+  int khz = 0;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(w, b->ctl, sizeof w, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device);
+  if (e != hipSuccess) return hip_err(c, e, "batch phase times");
+  if (khz <= 0) return set_err(c, RQ_E_HIP, "no wall-clock rate");
+  uint64_t st[BATCH_PHASES];
+  std::memcpy(st, w + BATCH_CTL_CLOCK, sizeof st);
+  for (int ph = 0; ph < (int)PH_TRACE && ph < cap; ++ph)
+    ms[ph] = st[ph + 1] >= st[ph] && st[0] != ~0ull ? (double)(st[ph + 1] - st[ph]) / khz : -1.0;
+  return (int)PH_TRACE;
 }
 
 int rq_batch_errors(rq_ctx* c, rq_batch* b, uint32_t* bits) {

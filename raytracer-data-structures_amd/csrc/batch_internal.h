@@ -33,9 +33,27 @@ constexpr int BATCH_CTL_TRACE_ERR = BATCH_PHASES + 2;      // stack deeper than 
 constexpr int BATCH_CTL_XTICKET = BATCH_PHASES + 3;        // BATCH_XCDS trace segment tickets
 constexpr int BATCH_CTL_START = BATCH_CTL_XTICKET + BATCH_XCDS;  // blocks started (+ BATCH_CLOSED)
 constexpr int BATCH_CTL_MEMBERS = BATCH_CTL_START + 1;     // blocks taking part (0 until closed)
-constexpr int BATCH_CTL_WORDS = BATCH_CTL_MEMBERS + 1;
+// BATCH_PHASES 64-bit wall-clock stamps (8-byte aligned) of the build kernel:
+// stamp 0 = the first member's start, stamp p = phase p - 1 done (the last
+// member through barrier p, or out of the kernel after the last phase)
+constexpr int BATCH_CTL_CLOCK = (BATCH_CTL_MEMBERS + 2) & ~1;
+constexpr int BATCH_CTL_WORDS = BATCH_CTL_CLOCK + 2 * BATCH_PHASES;
+// The grid barrier's group words, after the control words: BATCH_BAR_GROUPS
+// arrival counters, then as many release words, each word on a 4 KB page of
+// its own, so that the arrivals and polls of different groups are served by
+// different memory channels rather than queueing on one word.
+constexpr int BATCH_BAR_GROUPS = 16;
+constexpr int BATCH_BAR_STRIDE = 1024;  // words
+constexpr int BATCH_BAR_BASE = BATCH_BAR_STRIDE;
+constexpr int BATCH_CTL_ALLOC_WORDS = BATCH_BAR_BASE + 2 * BATCH_BAR_GROUPS * BATCH_BAR_STRIDE;
+static_assert(BATCH_CTL_WORDS <= BATCH_BAR_BASE, "control words before the barrier groups");
 constexpr uint32_t BATCH_CLOSED = 0x80000000u;             // start word: membership closed
 constexpr uint32_t BATCH_ANY_BIT = 0x80000000u;            // trace item tile: any-hit query
+// Work items of the refit phases (see batch_kernels.hip): refit chunks of
+// BUILD_BLOCK leaves, BATCH_REFIT_CHUNKS per item; spanning nodes in tiles of
+// BUILD_BLOCK, BATCH_SPAN_TILES per item.
+constexpr uint32_t BATCH_REFIT_CHUNKS = 4;
+constexpr uint32_t BATCH_SPAN_TILES = 8;
 
 struct BatchMeshDev {
   BuildView v;

@@ -27,33 +27,57 @@ constexpr uint32_t JOIN_SPIN_LIMIT = 1u << 14;
 
 // Membership: the barriers count only the blocks that have started, so that
 // they never wait for a block the device has not placed (kernels on other
-// streams can hold CUs when the launch begins). A block joins by counting
-// itself in the start word at entry; a block that finds the membership
+// streams can hold CUs when the launch begins). A block joins by taking a
+// ticket from the start word at entry; a block that finds the membership
 // closed leaves at once. The first block to reach the first barrier waits
 // (bounded) until the whole grid has started, then closes the membership
-// with an atomic OR, whose returned count is the number of members. Work is
-// pulled from queues, so any number of members completes every phase: a
-// grid that could not be placed whole runs on fewer blocks, not wrongly.
-__device__ inline bool join_grid(uint32_t* ctl, uint32_t* s_flag) {
+// with an atomic OR, whose returned count is the number of members: the
+// members hold tickets 0 .. members - 1. Work is pulled from queues, so any
+// number of members completes every phase: a grid that could not be placed
+// whole runs on fewer blocks, not wrongly. Returns the block's ticket, or
+// ~0u for a block that takes no part.
+__device__ inline uint32_t join_grid(uint32_t* ctl, uint32_t* s_flag) {
   if (threadIdx.x == 0) *s_flag = atomicAdd(ctl + BATCH_CTL_START, 1u);
   __syncthreads();
-  const bool member = (*s_flag & BATCH_CLOSED) == 0u;
+  const uint32_t t = *s_flag;
   __syncthreads();
-  return member;
+  return (t & BATCH_CLOSED) ? ~0u : t;
 }
 
-// Every thread releases its writes at agent scope, the block arrives, thread 0
-// waits for all member blocks and acquires before the block goes on. The
-// members are running, so every wait ends; it is bounded all the same: a wait
-// that gives up sets the sticky error word, and once it is set no later
+__device__ inline uint32_t* bar_group(uint32_t* ctl, uint32_t g) {
+  return ctl + BATCH_BAR_BASE + (size_t)g * BATCH_BAR_STRIDE;
+}
+__device__ inline uint32_t* bar_release(uint32_t* ctl, uint32_t g) {
+  return ctl + BATCH_BAR_BASE + (size_t)(BATCH_BAR_GROUPS + g) * BATCH_BAR_STRIDE;
+}
+
+// Phase stamps (BATCH_CTL_CLOCK): the latest (or, for the start, earliest)
+// wall-clock reading over the member blocks, one atomic per block.
+__device__ inline unsigned long long* clock_stamp(uint32_t* ctl, uint32_t k) {
+  return reinterpret_cast<unsigned long long*>(ctl + BATCH_CTL_CLOCK) + k;
+}
+
+// Grid barrier after phase `phases_done` - 1. The block's waves meet at a
+// work-group barrier, which leaves their stores in the XCD's L2; thread 0's
+// agent-scope release then writes that L2 back (one write-back per block
+// rather than one per wave), and it counts the block on its group's counter
+// (group = ticket mod BATCH_BAR_GROUPS). The group's last arrival counts the
+// group on the top counter, and the top's last arrival publishes the phase
+// on every group's release word; each block waits on its own group's word
+// and acquires. The counters are cumulative over the launch, so the last
+// arrival of phase p is the one that brings a counter to p times its
+// members. One word shared by every block, for the arrivals and for the
+// polls, had made each barrier cost ~0.15-0.3 ms on a full grid. The
+// members are running, so every wait ends; it is bounded all the same: a
+// wait that gives up sets the sticky error word, and once it is set no later
 // barrier of the launch waits at all.
-__device__ inline void grid_barrier(uint32_t* ctl, uint32_t phases_done) {
-  __threadfence();
+__device__ inline void grid_barrier(uint32_t* ctl, uint32_t phases_done, uint32_t ticket) {
   __syncthreads();
   if (threadIdx.x == 0) {
-    uint32_t* const bar = ctl + BATCH_CTL_BARRIER;
+    __threadfence();
     uint32_t* const err = ctl + BATCH_CTL_BARRIER_ERR;
-    atomicAdd(bar, 1u);
+    const uint32_t g = ticket % (uint32_t)BATCH_BAR_GROUPS;
+    const uint32_t old = atomicAdd(bar_group(ctl, g), 1u);
     uint32_t members = ld_agent(ctl + BATCH_CTL_MEMBERS);
     if (members == 0u) {  // the first barrier of the launch: close the membership
       uint32_t spins = 0;
@@ -62,8 +86,8 @@ __device__ inline void grid_barrier(uint32_t* ctl, uint32_t phases_done) {
         if ((st & BATCH_CLOSED) || st >= gridDim.x || ++spins > JOIN_SPIN_LIMIT) break;
         __builtin_amdgcn_s_sleep(2);
       }
-      const uint32_t old = atomicOr(ctl + BATCH_CTL_START, BATCH_CLOSED);
-      if (!(old & BATCH_CLOSED)) atomicExch(ctl + BATCH_CTL_MEMBERS, old);
+      const uint32_t o = atomicOr(ctl + BATCH_CTL_START, BATCH_CLOSED);
+      if (!(o & BATCH_CLOSED)) atomicExch(ctl + BATCH_CTL_MEMBERS, o);
       spins = 0;
       while ((members = ld_agent(ctl + BATCH_CTL_MEMBERS)) == 0u) {
         if (++spins > BARRIER_SPIN_LIMIT) {
@@ -73,16 +97,27 @@ __device__ inline void grid_barrier(uint32_t* ctl, uint32_t phases_done) {
         __builtin_amdgcn_s_sleep(2);
       }
     }
-    const uint32_t target = members * phases_done;
-    uint32_t spins = 0;
-    while (members != 0u && ld_agent(bar) < target) {
-      if (++spins > BARRIER_SPIN_LIMIT || ((spins & 1023u) == 0u && ld_agent(err) != 0u)) {
-        atomicOr(err, 1u);
-        break;
+    if (members != 0u) {
+      constexpr uint32_t G = BATCH_BAR_GROUPS;
+      const uint32_t groups = min(members, G), in_group = (members - g + G - 1u) / G;
+      if (old + 1u == in_group * phases_done) {  // the group's last arrival
+        __threadfence();
+        if (atomicAdd(ctl + BATCH_CTL_BARRIER, 1u) + 1u == groups * phases_done) {
+          __threadfence();  // the grid's last arrival: release every group
+          for (uint32_t k = 0; k < G; ++k) atomicMax(bar_release(ctl, k), phases_done);
+        }
       }
-      __builtin_amdgcn_s_sleep(2);
+      uint32_t spins = 0;
+      while (ld_agent(bar_release(ctl, g)) < phases_done) {
+        if (++spins > BARRIER_SPIN_LIMIT || ((spins & 1023u) == 0u && ld_agent(err) != 0u)) {
+          atomicOr(err, 1u);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
     }
     __threadfence();
+    atomicMax(clock_stamp(ctl, phases_done), (unsigned long long)wall_clock64());
   }
   __syncthreads();
 }
@@ -241,12 +276,20 @@ __device__ inline void trace_mesh(const BatchArgs& a, const BatchMeshDev& m, uin
   }
 }
 
-// LDS words of the batch kernel: the largest of the lean sort tile, the
-// Morton histograms and the traversal stack.
+// The batch build's refit chunks: one leaf per thread of a block. A work
+// item of the refit phase covers BATCH_REFIT_CHUNKS chunks, one of the
+// spanning-node phases BATCH_SPAN_TILES tiles of BUILD_BLOCK internal nodes:
+// every item costs a ticket, an atomic on one word that all blocks share,
+// and items of a single chunk or tile made the phases wait on those tickets.
+constexpr int BATCH_REFIT_CHUNK = BUILD_BLOCK;
+
+// LDS words of the batch kernel: the build kernel's largest phase (lean sort
+// tile, Morton histograms, refit chunk), the trace kernel's stack.
 constexpr int cmax(int x, int y) { return x > y ? x : y; }
-constexpr int batch_lds_words(int depth, bool tstack) {
-  return cmax(cmax(sort_lds_words(SORT_ITEMS_LEAN), SORT_PASSES * 256),
-              depth * BUILD_BLOCK * (tstack ? 2 : 1));
+constexpr int batch_lds_words(bool build, int depth, bool tstack) {
+  return build ? cmax(cmax(sort_lds_words(SORT_ITEMS_LEAN), SORT_PASSES * 256),
+                      refit_chunk_lds_words<BATCH_REFIT_CHUNK>())
+               : depth * BUILD_BLOCK * (tstack ? 2 : 1);
 }
 
 // Two kernels share this body. KM_BUILD runs the build phases of every mesh
@@ -261,7 +304,7 @@ enum KernelMode { KM_BUILD, KM_TRACE };
 template <int DEPTH, bool TSTACK, int WAVES, KernelMode MODE, bool WIDE, bool WAVE_ITEMS = WIDE>
 __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAVES))) void k_batch(
     BatchArgs a) {
-  __shared__ uint32_t lds[batch_lds_words(DEPTH, TSTACK)];
+  __shared__ uint32_t lds[batch_lds_words(MODE == KM_BUILD, DEPTH, TSTACK)];
   __shared__ uint32_t s_item, s_wctr[BUILD_BLOCK / 64];
   uint32_t* const ctl = a.ctl;
   const BatchWork* const items = a.items;
@@ -314,7 +357,9 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
   }
   // ---- the build phases, block by block, grid barriers between them
   // blocks the device places after the membership closed take no part
-  if (!join_grid(ctl, &s_item)) return;
+  const uint32_t ticket = join_grid(ctl, &s_item);
+  if (ticket == ~0u) return;
+  if (threadIdx.x == 0) atomicMin(clock_stamp(ctl, 0), (unsigned long long)wall_clock64());
   constexpr int first = 0, last = (int)PH_TRACE;
   for (int ph = first; ph < last; ++ph) {
    {
@@ -375,20 +420,29 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
           karras_tile(v, w.tile);
           break;
         }
-        case PH_REFIT: {
+        case PH_REFIT: {  // chunk-union refit, pass 1 (with the chunks' wide records)
           VIEW;
-          refit_local_tile(v, w.tile, lds);
+          for (uint32_t c = 0; c < (uint32_t)BATCH_REFIT_CHUNKS; ++c) {
+            const uint32_t chunk = w.tile * BATCH_REFIT_CHUNKS + c;
+            if (chunk * BATCH_REFIT_CHUNK >= v.n) break;
+            refit_chunk_tile<BATCH_REFIT_CHUNK>(v, chunk, lds);
+            __syncthreads();  // the chunk's LDS is read to the end before the next one
+          }
           break;
         }
-        case PH_TOP: {
+        case PH_TOP: {  // pass 2: the nodes spanning chunks, a wave per 64 nodes
           VIEW;
-          refit_top_block(v);
+          for (uint32_t t = 0; t < (uint32_t)BATCH_SPAN_TILES; ++t)
+            refit_span_wave<BATCH_REFIT_CHUNK>(
+                v.B, v.n, (w.tile * BATCH_SPAN_TILES + t) * BUILD_BLOCK + threadIdx.x);
           break;
         }
-        case PH_WIDEN: {
+        case PH_WIDEN: {  // wide records of the spanning nodes
           VIEW;
-          const uint32_t i = w.tile * BUILD_BLOCK + threadIdx.x;
-          if (i + 1u < v.n) widen_global(v.B, i);
+          for (uint32_t t = 0; t < (uint32_t)BATCH_SPAN_TILES; ++t) {
+            const uint32_t i = (w.tile * BATCH_SPAN_TILES + t) * BUILD_BLOCK + threadIdx.x;
+            if (i + 1u < v.n && spans_chunks<BATCH_REFIT_CHUNK>(v.B, i)) widen_global(v.B, i);
+          }
           break;
         }
         default:
@@ -398,14 +452,20 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
       __syncthreads();
     }
    }
-    if (ph + 1 < last) grid_barrier(ctl, (uint32_t)(ph + 1));
+    if (ph + 1 < last) grid_barrier(ctl, (uint32_t)(ph + 1), ticket);
   }
+  if (threadIdx.x == 0) atomicMax(clock_stamp(ctl, last), (unsigned long long)wall_clock64());
 }
 
 // Clears the queue tickets, the barrier and the error words before a run (a
 // kernel, so that the reset lives in the same captured graph as the run).
 __global__ void k_batch_reset(uint32_t* ctl) {
-  if (threadIdx.x < BATCH_CTL_WORDS) ctl[threadIdx.x] = 0u;
+  static_assert(BATCH_CTL_WORDS <= 64 && 2 * BATCH_BAR_GROUPS <= 64,
+                "one 64-thread block clears the control and barrier words");
+  // the start stamp is a minimum: it starts at the largest value
+  const bool start = threadIdx.x == BATCH_CTL_CLOCK || threadIdx.x == BATCH_CTL_CLOCK + 1;
+  if (threadIdx.x < BATCH_CTL_WORDS) ctl[threadIdx.x] = start ? 0xFFFFFFFFu : 0u;
+  if (threadIdx.x < 2 * BATCH_BAR_GROUPS) *bar_group(ctl, threadIdx.x) = 0u;
 }
 
 typedef void (*BatchFn)(BatchArgs);
@@ -417,8 +477,9 @@ struct BatchVariant {
 // The build kernel of a batch run, by waves per SIMD (the register cap trades
 // occupancy against spills of the sort phase's look-back window).
 const BatchFn kBuild[] = {k_batch<0, false, 8, KM_BUILD, false>, k_batch<0, false, 7, KM_BUILD, false>,
-                          k_batch<0, false, 6, KM_BUILD, false>, k_batch<0, false, 5, KM_BUILD, false>};
-constexpr int kBuildWaves[] = {8, 7, 6, 5};
+                          k_batch<0, false, 6, KM_BUILD, false>, k_batch<0, false, 5, KM_BUILD, false>,
+                          k_batch<0, false, 4, KM_BUILD, false>};
+constexpr int kBuildWaves[] = {8, 7, 6, 5, 4};
 constexpr int kNumBuildVariants = sizeof(kBuild) / sizeof(kBuild[0]);
 // Trace kernels of a batch run (4-wide walk): the batch configurations.
 const BatchVariant kBatchVariants[] = {

@@ -670,8 +670,9 @@ __device__ inline uint32_t leaf_depth(const BuildBuffers& B, uint32_t n, uint32_
   return d + 1u;
 }
 
-// ---- chunk-union refit (standalone build) ----------------------------------------
-// Pass 1, one block of REFIT_CHUNK threads per chunk of REFIT_CHUNK leaves:
+// ---- chunk-union refit ------------------------------------------------------------
+// Pass 1, one block of CHUNK threads per chunk of CHUNK leaves (the standalone
+// build: REFIT_CHUNK = 1024; the batch kernel: its 256-thread blocks):
 // each thread writes its leaf record, the block scans the leaf boxes into
 // per-leaf prefix and suffix unions of the chunk (and the chunk's union), and
 // each leaf climbs as in the frontier refit while the node's leaf range lies
@@ -695,28 +696,40 @@ __device__ inline void box_load(const float* src, float* b) {
   b[0] = a.x, b[1] = a.y, b[2] = c.x, b[3] = c.y, b[4] = e.x, b[5] = e.y;
 }
 
-// LDS: REFIT_CHUNK arrival counters, 2 * 16 wave unions (6 floats each), the
+// LDS: CHUNK arrival counters, 2 * CHUNK / 64 wave unions (6 floats each), the
 // child boxes of the chunk's nodes ([node - chunk start][side][6]), and the
 // parent links of the chunk's leaves and internal nodes with the internal
 // nodes' leaf ranges (loaded once, so the climb reads no global memory).
-constexpr int REFIT_CHUNK_LDS_WORDS =
-    REFIT_CHUNK + 2 * (REFIT_CHUNK / 64) * 6 + REFIT_CHUNK * 12 + REFIT_CHUNK * 6;
+template <int CHUNK>
+constexpr int refit_chunk_lds_words() {
+  return CHUNK + 2 * (CHUNK / 64) * 6 + CHUNK * 12 + CHUNK * 6;
+}
+constexpr int REFIT_CHUNK_LDS_WORDS = refit_chunk_lds_words<REFIT_CHUNK>();
 
+// Whether internal node i's leaf range crosses a boundary of CHUNK-leaf chunks.
+template <int CHUNK>
+__device__ inline bool spans_chunks(const BuildBuffers& B, uint32_t i) {
+  const uint2 rg = B.range[i];
+  return rg.x / CHUNK != rg.y / CHUNK;
+}
+
+template <int CHUNK = REFIT_CHUNK>
 __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
-  static_assert(REFIT_CHUNK == 1024, "one leaf per thread of a 1024-thread block");
+  static_assert(CHUNK % 64 == 0 && CHUNK <= 1024, "one leaf per thread, whole waves");
+  static_assert(CHUNK >= REFIT_CHUNK_MIN, "the chunk unions are allocated for REFIT_CHUNK_MIN");
   uint32_t* s_flag = lds;
-  float* s_wpre = reinterpret_cast<float*>(lds + REFIT_CHUNK);  // [wave][6] exclusive prefix
-  float* s_wsuf = s_wpre + (REFIT_CHUNK / 64) * 6;               // [wave][6] exclusive suffix
-  float* s_box = s_wsuf + (REFIT_CHUNK / 64) * 6;                // [node][side][6] child boxes
-  int32_t* s_pleaf = reinterpret_cast<int32_t*>(s_box + REFIT_CHUNK * 12);  // parent of leaf
-  int32_t* s_pint = s_pleaf + REFIT_CHUNK;                                   // parent of node
-  uint2* s_rng = reinterpret_cast<uint2*>(s_pint + REFIT_CHUNK);             // range of node
-  int2* s_kid = reinterpret_cast<int2*>(s_rng + REFIT_CHUNK);                // children of node
+  float* s_wpre = reinterpret_cast<float*>(lds + CHUNK);  // [wave][6] exclusive prefix
+  float* s_wsuf = s_wpre + (CHUNK / 64) * 6;               // [wave][6] exclusive suffix
+  float* s_box = s_wsuf + (CHUNK / 64) * 6;                // [node][side][6] child boxes
+  int32_t* s_pleaf = reinterpret_cast<int32_t*>(s_box + CHUNK * 12);  // parent of leaf
+  int32_t* s_pint = s_pleaf + CHUNK;                                   // parent of node
+  uint2* s_rng = reinterpret_cast<uint2*>(s_pint + CHUNK);             // range of node
+  int2* s_kid = reinterpret_cast<int2*>(s_rng + CHUNK);                // children of node
   const BuildBuffers& B = v.B;
   const uint32_t n = v.n;
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint32_t chunk_lo = chunk * REFIT_CHUNK;
-  const uint32_t chunk_hi = min(chunk_lo + REFIT_CHUNK, n) - 1;
+  const uint32_t chunk_lo = chunk * CHUNK;
+  const uint32_t chunk_hi = min(chunk_lo + CHUNK, n) - 1;
   const uint32_t k = chunk_lo + tid;
   const bool valid = k <= chunk_hi;
   s_flag[tid] = 0u;
@@ -753,14 +766,14 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
   __syncthreads();
   float ep[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   float es[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  if (tid < REFIT_CHUNK / 64) {
-    for (uint32_t w = 0; w < REFIT_CHUNK / 64; ++w) {
+  if (tid < CHUNK / 64) {
+    for (uint32_t w = 0; w < CHUNK / 64; ++w) {
       if (w < tid) box_union(ep, s_wpre + w * 6);
       if (w > tid) box_union(es, s_wpre + w * 6);
     }
   }
   __syncthreads();  // every wave union read before it is overwritten
-  if (tid < REFIT_CHUNK / 64) {
+  if (tid < CHUNK / 64) {
 #pragma unroll
     for (int q = 0; q < 6; ++q) s_wpre[tid * 6 + q] = ep[q], s_wsuf[tid * 6 + q] = es[q];
   }
@@ -849,20 +862,21 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
 // through its spanning nodes one at a time, all 64 lanes striding over the
 // chunks strictly inside the span (a node near the root spans them all).
 // Every lane must reach this call (the wave shuffles).
+template <int CHUNK = REFIT_CHUNK>
 __device__ inline void refit_span_wave(const BuildBuffers& B, uint32_t n, uint32_t i) {
   const uint32_t lane = threadIdx.x & 63u;
   uint2 rg = make_uint2(0u, 0u);
   bool span = false;
   if (i < n - 1) {
     rg = B.range[i];
-    span = rg.x / REFIT_CHUNK != rg.y / REFIT_CHUNK;
+    span = rg.x / CHUNK != rg.y / CHUNK;
   }
   uint64_t m = __ballot(span);
   while (m) {
     const int src = __ffsll((unsigned long long)m) - 1;
     m &= m - 1ull;
-    const uint32_t ca = (uint32_t)__shfl((int)(rg.x / REFIT_CHUNK), src, 64);
-    const uint32_t cb = (uint32_t)__shfl((int)(rg.y / REFIT_CHUNK), src, 64);
+    const uint32_t ca = (uint32_t)__shfl((int)(rg.x / CHUNK), src, 64);
+    const uint32_t cb = (uint32_t)__shfl((int)(rg.y / CHUNK), src, 64);
     float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY}, o[6];
     for (uint32_t c = ca + 1 + lane; c < cb; c += 64) {
       box_load(B.cbox + 6ull * c, o);

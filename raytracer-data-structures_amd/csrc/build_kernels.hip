@@ -77,8 +77,7 @@ __global__ __launch_bounds__(256) void k_widen_all(BuildBuffers B, uint32_t n) {
 __global__ __launch_bounds__(256) void k_widen_span(BuildBuffers B, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  const uint2 rg = B.range[i];
-  if (rg.x / REFIT_CHUNK != rg.y / REFIT_CHUNK) widen_global(B, i);
+  if (spans_chunks<REFIT_CHUNK>(B, i)) widen_global(B, i);
 }
 
 // Tree height: each block's deepest leaf to B.flags[block] (free once the

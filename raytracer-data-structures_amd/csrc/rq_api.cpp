@@ -176,7 +176,7 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit) {
                  wide ? 128 * ni : 0,
                  chunk_refit ? 24 * nn : 0,
                  chunk_refit ? 24 * nn : 0,
-                 chunk_refit ? 24 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK) : 0};
+                 chunk_refit ? 24 * (size_t)((nn + REFIT_CHUNK_MIN - 1) / REFIT_CHUNK_MIN) : 0};
   size_t total = 0;
   for (size_t s : sz) total += align256(s);
   void* p;

@@ -62,6 +62,9 @@ constexpr int SORT_PASSES = 4;  // 30-bit codes, 8-bit digits
 // one leaf per thread, the batch kernel four); the top pass runs in one
 // work-group.
 constexpr int REFIT_CHUNK = 1024;
+// The smallest refit chunk (the batch kernel's 256-thread blocks): the
+// chunk-union buffers hold one union per chunk of this size.
+constexpr int REFIT_CHUNK_MIN = 256;
 constexpr int REFIT_TOP_BLOCK = 1024;
 
 // Lookback status word: [31:30] flag, [29:0] count.
@@ -144,7 +147,7 @@ struct BuildBuffers {
   // are 6 floats (lo xyz, hi xyz).
   float* pre = nullptr;           // 6n: union of the leaves from the chunk start to k
   float* suf = nullptr;           // 6n: union of the leaves from k to the chunk end
-  float* cbox = nullptr;          // 6 per REFIT_CHUNK chunk: its leaves' union
+  float* cbox = nullptr;          // 6 per refit chunk (up to n / REFIT_CHUNK_MIN): its leaves' union
   uint32_t ntiles = 0;            // tiles of SORT_TILE_LEAN keys
 };
 

@@ -728,8 +728,8 @@ int rq_server_create(int device, const rq_server_config* cfg_in, rq_server** out
         rc = RQ_E_HIP;
     }
   }
-  if (!rc) rc = ctx_alloc(c, BATCH_CTL_WORDS * sizeof(uint32_t), (void**)&s->ctl);
-  if (!rc && hipMemset(s->ctl, 0, BATCH_CTL_WORDS * sizeof(uint32_t)) != hipSuccess) rc = RQ_E_HIP;
+  if (!rc) rc = ctx_alloc(c, BATCH_CTL_ALLOC_WORDS * sizeof(uint32_t), (void**)&s->ctl);
+  if (!rc && hipMemset(s->ctl, 0, BATCH_CTL_ALLOC_WORDS * sizeof(uint32_t)) != hipSuccess) rc = RQ_E_HIP;
   if (!rc) {
     int depth = 0, ts = 0;
     serve_variant_info(s->variant, &depth, &ts);

@@ -263,6 +263,7 @@ def _load() -> C.CDLL:
         "rq_batch_autotune": (i32, [vp, vp, i32, C.POINTER(i32), C.POINTER(u32), C.POINTER(u32),
                                     C.POINTER(C.c_double)]),
         "rq_batch_phase_items": (i32, [vp, vp, vp, i32]),
+        "rq_batch_phase_ms": (i32, [vp, vp, vp, i32]),
         "rq_batch_errors": (i32, [vp, vp, C.POINTER(u32)]),
         "rq_batch_occupancy": (i32, [vp, vp, C.POINTER(u32), C.POINTER(u32)]),
         "rq_batch_set_build_waves": (i32, [vp, vp, i32]),
@@ -941,6 +942,15 @@ class Batch:
         counts = np.zeros(len(BATCH_PHASES), np.uint32)
         lib().rq_batch_phase_items(self.ctx.h, self.h, _ptr(counts), counts.size)
         return dict(zip(BATCH_PHASES, counts.tolist()))
+
+    def phase_ms(self) -> dict:
+        """The last run's build phases in ms (device wall-clock stamps at
+        the build kernel's grid barriers; waits for the device)."""
+        ms = np.zeros(len(BATCH_PHASES), np.float64)
+        k = lib().rq_batch_phase_ms(self.ctx.h, self.h, _ptr(ms), ms.size)
+        if k < 0:
+            self.ctx._check(k)
+        return dict(zip(BATCH_PHASES[:k], ms[:k].tolist()))
 
     def errors(self) -> int:
         bits = C.c_uint32()

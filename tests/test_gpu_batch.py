@@ -155,9 +155,30 @@ def test_batch_phase_items(setup, rq):
     assert ph["sort0"] == sum(cdiv(n, 2048) for n in ns)
     assert ph["sort3"] == ph["sort0"]
     assert ph["karras"] == sum(cdiv(n - 1, 4096) for n in ns if n >= 2)
+    # chunk-union refit in 256-leaf chunks, 4 per item; pass 2 and the
+    # spanning nodes' wide records take 2048 internal nodes per item once a
+    # mesh has two chunks
     assert ph["refit"] == sum(cdiv(n, 1024) for n in ns)
-    assert ph["refit_top"] == sum(1 for n in ns if cdiv(n, 1024) > 1)
-    assert ph["widen"] == sum(cdiv(n - 1, 256) for n in ns if n >= 2)
+    assert ph["refit_top"] == sum(cdiv(n - 1, 2048) for n in ns if cdiv(n, 256) > 1)
+    assert ph["widen"] == ph["refit_top"]
+
+
+def test_batch_phase_times(setup, rq):
+    """Device stamps of the build kernel's phases (init ... widen): one
+    value per build phase, none negative, together no longer than the build
+    kernel's event time."""
+    ctx = setup.ctx
+    ctx.profile(True)
+    ctx.profile_reset()
+    setup.batch.run()
+    ctx.sync()
+    ctx.profile(False)
+    build_ms = ctx.profile_read()["batch_build"][0]
+    ph = setup.batch.phase_ms()
+    log(f"batch phase ms {ph}, build kernel {build_ms:.3f} ms")
+    assert list(ph) == list(rq.BATCH_PHASES[:-1])
+    assert all(v >= 0.0 for v in ph.values())
+    assert 0.0 < sum(ph.values()) <= build_ms * 1.05 + 0.01
 
 
 def test_batch_matches_per_mesh(setup):
@@ -216,11 +237,11 @@ def test_batch_every_config(setup, rq, tile):
 
 
 def test_batch_every_build_kernel(setup):
-    """The build kernel at each register cap (8, 7, 6, 5 waves per SIMD),
+    """The build kernel at each register cap (8, 7, 6, 5, 4 waves per SIMD),
     plain and replayed: the same trees and answers."""
     saved = setup.batch.build_waves()
     try:
-        for waves in (8, 7, 6, 5):
+        for waves in (8, 7, 6, 5, 4):
             log(f"batch build kernel at {waves} waves")
             assert setup.batch.build_waves(waves) == waves
             for graph in (False, True):
