@@ -58,13 +58,15 @@ __global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(BuildView v) {
 
 __global__ __launch_bounds__(REFIT_TOP_BLOCK) void k_refit_top(BuildView v) { refit_top_block(v); }
 
-__global__ __launch_bounds__(REFIT_CHUNK) void k_refit_chunk(BuildView v) {
-  __shared__ uint32_t lds[REFIT_CHUNK_LDS_WORDS];
-  refit_chunk_tile(v, blockIdx.x, lds);
+template <int CHUNK>
+__global__ __launch_bounds__(CHUNK) void k_refit_chunk(BuildView v) {
+  __shared__ uint32_t lds[refit_chunk_lds_words<CHUNK>()];
+  refit_chunk_tile<CHUNK>(v, blockIdx.x, lds);
 }
 
+template <int CHUNK>
 __global__ __launch_bounds__(256) void k_refit_span(BuildBuffers B, uint32_t n) {
-  refit_span_wave(B, n, blockIdx.x * blockDim.x + threadIdx.x);
+  refit_span_wave<CHUNK>(B, n, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Wide nodes from the finished binary records: every internal node (after a
@@ -74,11 +76,19 @@ __global__ __launch_bounds__(256) void k_widen_all(BuildBuffers B, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n - 1) widen_global(B, i);
 }
+template <int CHUNK>
 __global__ __launch_bounds__(256) void k_widen_span(BuildBuffers B, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
-  if (spans_chunks<REFIT_CHUNK>(B, i)) widen_global(B, i);
+  if (spans_chunks<CHUNK>(B, i)) widen_global(B, i);
 }
+
+// Leaves per chunk (and threads per block) of the standalone chunk-union
+// refit. 256-leaf chunks (eight blocks per CU instead of two) measured the
+// same for pass 1 on a 1.31 M-triangle mesh (0.139 vs 0.138 ms) and made the
+// spanning-node pass slower (0.035 vs 0.020 ms): the pass moves ~370 bytes
+// per leaf and is bound by them, not by its occupancy.
+constexpr int BUILD_REFIT_CHUNK = REFIT_CHUNK;
 
 // Tree height: each block's deepest leaf to B.flags[block] (free once the
 // refit is done), k_height reduces them (a device atomic per wave on one word
@@ -141,17 +151,19 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
   }
   const uint32_t nchunks = tiles_of(n, REFIT_CHUNK);
   if (B.pre) {  // chunk-union refit
+    constexpr int C = BUILD_REFIT_CHUNK;
+    const uint32_t cchunks = tiles_of(n, C);
     {
       ProfScope ps(lc, "refit");
-      hipLaunchKernelGGL(k_refit_chunk, dim3(nchunks), dim3(REFIT_CHUNK), 0, s, v);
+      hipLaunchKernelGGL(k_refit_chunk<C>, dim3(cchunks), dim3(C), 0, s, v);
     }
-    if (nchunks > 1) {
+    if (cchunks > 1) {
       ProfScope ps(lc, "refit_top");
-      hipLaunchKernelGGL(k_refit_span, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
+      hipLaunchKernelGGL(k_refit_span<C>, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
     }
-    if (nchunks > 1 && B.wide) {
+    if (cchunks > 1 && B.wide) {
       ProfScope ps(lc, "widen");
-      hipLaunchKernelGGL(k_widen_span, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
+      hipLaunchKernelGGL(k_widen_span<C>, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
     }
   } else {  // frontier refit
     {
