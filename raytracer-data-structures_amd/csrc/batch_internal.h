@@ -91,7 +91,7 @@ uint32_t batch_build_grid_blocks(int device, int build);
 uint32_t batch_trace_grid_blocks(int device, int v);
 // The serving loop's trace kernels (binary walk over its slots).
 int serve_num_variants();
-void serve_variant_info(int v, int* depth, int* tstack);
+void serve_variant_info(int v, int* depth, int* tstack, int* wide);
 uint32_t serve_grid_blocks(int device, int v);
 // A batch run: resets the control words, then the build kernel (every build
 // phase, grid barriers between them) and the trace kernel of `variant`.

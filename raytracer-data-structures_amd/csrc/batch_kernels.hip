@@ -489,11 +489,15 @@ const BatchVariant kBatchVariants[] = {
     {12, true, k_batch<12, true, 6, KM_TRACE, true>},
     {28, false, k_batch<28, false, 5, KM_TRACE, true>},
     {30, false, k_batch<30, false, 5, KM_TRACE, true>}};
-// Trace kernels of the serving loop (binary walk).
+// Trace kernels of the serving loop: binary walks (0-5), then 4-wide walks
+// with block-shared items (6-7; the slots keep wide nodes).
 const BatchVariant kServeVariants[] = {
     {16, false, k_batch<16, false, 7, KM_TRACE, false>}, {20, false, k_batch<20, false, 7, KM_TRACE, false>},
     {8, true, k_batch<8, true, 7, KM_TRACE, false>},     {10, true, k_batch<10, true, 7, KM_TRACE, false>},
-    {16, false, k_batch<16, false, 8, KM_TRACE, false>}, {8, true, k_batch<8, true, 8, KM_TRACE, false>}};
+    {16, false, k_batch<16, false, 8, KM_TRACE, false>}, {8, true, k_batch<8, true, 8, KM_TRACE, false>},
+    {16, false, k_batch<16, false, 5, KM_TRACE, true, false>},
+    {16, false, k_batch<16, false, 6, KM_TRACE, true, false>}};
+constexpr int kServeWideFirst = 6;
 constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 constexpr int kNumServeVariants = sizeof(kServeVariants) / sizeof(kServeVariants[0]);
 
@@ -514,9 +518,10 @@ void batch_variant_info(int v, int* depth, int* tstack) {
   *tstack = kBatchVariants[v].tstack ? 1 : 0;
 }
 int serve_num_variants() { return kNumServeVariants; }
-void serve_variant_info(int v, int* depth, int* tstack) {
+void serve_variant_info(int v, int* depth, int* tstack, int* wide) {
   *depth = kServeVariants[v].depth;
   *tstack = kServeVariants[v].tstack ? 1 : 0;
+  *wide = v >= kServeWideFirst ? 1 : 0;
 }
 int batch_num_build_variants() { return kNumBuildVariants; }
 int batch_build_waves(int b) { return kBuildWaves[b]; }

@@ -66,6 +66,25 @@ def edge_rays():
     return {k: a[:, i].copy() for i, k in enumerate(RAY_FIELDS)}
 
 
+def nonfinite_rays():
+    """Rays with a NaN or infinite origin or direction component (one axis,
+    or all of them), each after a copy of itself that is finite and aimed at
+    the origin region; none of the non-finite rays can have a finite hit."""
+    rows = []
+    bad = (np.nan, np.inf, -np.inf)
+    base_o, base_d = (0.1, -0.2, 0.0), (0.0, 0.0, -1.0)
+    for v in bad:
+        for which in ("o", "d"):
+            for axes in ((0,), (1,), (2,), (0, 1, 2)):
+                o, d = list(base_o), list(base_d)
+                for a in axes:
+                    (o if which == "o" else d)[a] = v
+                rows.append((*base_o, *base_d, 0.0, np.inf))
+                rows.append((*o, *d, 0.0, np.inf))
+    a = np.array(rows, np.float32)
+    return {k: a[:, i].copy() for i, k in enumerate(RAY_FIELDS)}
+
+
 def mesh_configs():
     """Name -> mesh builder for the small parity sweep."""
     return {

@@ -5,8 +5,8 @@ import numpy as np
 import pytest
 
 from conftest import log
-from helpers import (concat_rays, duplicate_mesh, edge_rays, mesh_configs, random_rays,
-                     random_soup, rays_toward)
+from helpers import (concat_rays, duplicate_mesh, edge_rays, mesh_configs, nonfinite_rays,
+                     random_rays, random_soup, rays_toward)
 
 pytestmark = pytest.mark.gpu
 
@@ -157,6 +157,41 @@ def test_all_trace_variants_identical(ctx, O, rq):
                 assert np.array_equal(_bits(g["u"]), _bits(o["u"])), v
                 ga = ctx.any_hit(b, rays)
                 assert np.array_equal(ga["occ"], oa["occ"]), v
+        assert ctx.device_errors() == 0
+    finally:
+        ctx.trace_config(*saved)
+        ctx.release(b)
+        ctx.release(s)
+
+
+def test_nonfinite_rays_every_variant(ctx, O, rq):
+    """Rays with NaN or infinite components miss in the oracle and in brute
+    force (no finite t exists), and every kernel variant, binary and 4-wide,
+    returns the oracle's answers for them and for the finite rays beside them
+    (a ray whose every axis is NaN passes every box, the empty slots of the
+    wide nodes included, which their zero links keep out of the walk)."""
+    mesh = rq.icosphere(3, 5, center=(0, 0, -5))
+    rays = concat_rays(nonfinite_rays(), rays_toward(500, 41))
+    finite = np.isfinite(np.stack([rays[k] for k in ("ox", "oy", "oz", "dx", "dy", "dz")])).all(0)
+    s = ctx.upload_scene(mesh)
+    b = ctx.build_lbvh(s, 1)
+    ob = O.OracleBVH(mesh, 1)
+    o = ob.closest(rays)
+    oa = ob.any(rays)
+    bf = O.closest_brute(mesh, rays)
+    assert (o["prim"][~finite] == -1).all() and not oa["occ"][~finite].any()
+    assert np.array_equal(o["prim"], bf["prim"])
+    assert (o["prim"][finite] >= 0).any()
+    saved = ctx.trace_config()
+    try:
+        for v in ctx.trace_variants():
+            log(f"non-finite rays, variant {v}")
+            ctx.trace_config(v["index"], v["index"] if not v["tstack"] else 0, 16)
+            g = ctx.closest_hit(b, rays)
+            assert np.array_equal(g["prim"], o["prim"]), v
+            assert np.array_equal(_bits(g["t"][finite]), _bits(o["t"][finite])), v
+            ga = ctx.any_hit(b, rays)
+            assert np.array_equal(ga["occ"], oa["occ"]), v
         assert ctx.device_errors() == 0
     finally:
         ctx.trace_config(*saved)
