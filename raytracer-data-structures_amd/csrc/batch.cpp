@@ -215,7 +215,7 @@ int rq_batch_create(rq_ctx* c, const rq_batch_item* items, uint32_t count, int m
     v->n = items[i].scene->ntri;
     v->owned = false;
     b->bvhs.push_back(v);
-    rc = bvh_alloc(c, v, v->n, true, true);  // wide nodes, chunk-union refit
+    rc = bvh_alloc(c, v, v->n);
   }
   // device mesh table
   std::vector<BatchMeshDev> md(count);

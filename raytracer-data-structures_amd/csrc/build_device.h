@@ -442,32 +442,14 @@ __device__ inline void karras_tile(const BuildView& v, uint32_t tile) {
 }
 
 // ---- phases 8-9: bottom-up refit ---------------------------------------------------
-// Pass 1 (one unit per REFIT_CHUNK leaves): each leaf writes its triangle
-// record and its box into the parent's child slot; the second arriver at a
-// node (atomic counter) unions the two child boxes and climbs. A node whose
-// leaf range lies inside the chunk is completed by two threads of this block,
-// so its counter needs only work-group scope (on a part with one L2 per XCD an
-// agent-scope acquire/release costs an L2 write-back and invalidate). A node
-// spanning chunks is left to pass 2: the arriving thread deposits its box,
-// counts itself with a relaxed device atomic, and the second of two arrivals
-// lists the node in the chunk's frontier. min/max is order-independent and
-// zeros are canonicalised, so the boxes are bit-exact whatever the order.
+// Helpers of the chunk-union refit below: a node's box goes into its parent's
+// child slot (put_slot); the root box and height into misc (put_root).
 __device__ inline void put_slot(NodeRec* nodes, uint32_t node, uint32_t side, const float* lo,
                                 const float* hi) {
   float* slo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
   float* shi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
   slo[0] = lo[0], slo[1] = lo[1], slo[2] = lo[2];
   shi[0] = hi[0], shi[1] = hi[1], shi[2] = hi[2];
-}
-__device__ inline void union_slot(const NodeRec* nodes, uint32_t node, uint32_t side, float* lo,
-                                  float* hi) {
-  const float* olo = side ? &nodes[node].lo1.x : &nodes[node].lo0.x;
-  const float* ohi = side ? &nodes[node].hi1.x : &nodes[node].hi0.x;
-#pragma unroll
-  for (int q = 0; q < 3; ++q) {
-    lo[q] = fminf(lo[q], olo[q]);
-    hi[q] = fmaxf(hi[q], ohi[q]);
-  }
 }
 __device__ inline void put_root(uint32_t* misc, const float* lo, const float* hi, uint32_t h) {
   float* root = reinterpret_cast<float*>(misc + MISC_ROOT);
@@ -501,102 +483,6 @@ __device__ inline void leaf_record(const BuildView& v, uint32_t k, TriRec& tr, f
     hi[0] = fmaxf(fmaxf(v0.x, v1.x), v2.x) + 0.0f, hi[1] = fmaxf(fmaxf(v0.y, v1.y), v2.y) + 0.0f;
     hi[2] = fmaxf(fmaxf(v0.z, v1.z), v2.z) + 0.0f;
   }
-}
-
-__device__ inline void refit_local_tile(const BuildView& v, uint32_t chunk, uint32_t* lds) {
-  uint32_t* s_nf = lds;
-  if (threadIdx.x == 0) *s_nf = 0u;
-  __syncthreads();
-  const BuildBuffers& B = v.B;
-  const uint32_t n = v.n;
-  const uint32_t chunk_lo = chunk * REFIT_CHUNK;
-  const uint32_t chunk_hi = min(chunk_lo + REFIT_CHUNK, n) - 1;
-  // blockDim.x threads share the chunk's leaves (the standalone kernel runs
-  // REFIT_CHUNK threads, one leaf each; the batch kernel BUILD_BLOCK)
-  for (uint32_t k = chunk_lo + threadIdx.x; k <= chunk_hi; k += blockDim.x) {
-    TriRec tr;
-    float lo[3], hi[3];
-    leaf_record(v, k, tr, lo, hi);
-    B.tris[k] = tr;
-    if (n == 1) {
-      put_root(B.misc, lo, hi, 0u);
-      break;
-    }
-    uint32_t h = 0, cur = (n - 1) + k;
-    for (int level = 0;; ++level) {
-      const int32_t pe = B.parent[cur];
-      const uint32_t node = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
-      if (pe < 0 || node >= n - 1 || level > 64) {  // height <= 63 for 30-bit codes
-        atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
-        break;
-      }
-      put_slot(B.nodes, node, side, lo, hi);
-      B.slot_h[2 * node + side] = h;
-      const uint2 rg = B.range[node];
-      if (rg.x < chunk_lo || rg.y > chunk_hi) {
-        // spans chunks: finished by pass 2 once both children are in
-        const uint32_t old =
-            __hip_atomic_fetch_add(&B.flags[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == 1u) B.frontier[chunk_lo + atomicAdd(s_nf, 1u)] = node;
-        break;
-      }
-      const uint32_t old =
-          __hip_atomic_fetch_add(&B.flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0u) break;
-      union_slot(B.nodes, node, side ^ 1u, lo, hi);
-      h = 1u + max(h, B.slot_h[2 * node + (side ^ 1u)]);
-      if (node == 0u) {
-        put_root(B.misc, lo, hi, h);
-        break;
-      }
-      cur = node;
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) B.fcount[chunk] = *s_nf;
-  __syncthreads();
-}
-
-// Pass 2 (one block): the nodes spanning chunks, bottom-up from the frontier
-// lists of pass 1 (their child slots were written before this block started
-// reading: a kernel boundary or an acquire on the chunk counters); counters
-// shared inside the block need only work-group scope.
-__device__ inline void refit_top_block(const BuildView& v) {
-  const BuildBuffers& B = v.B;
-  const uint32_t n = v.n;
-  const uint32_t nchunks = tiles_of(n, REFIT_CHUNK);
-  if (nchunks <= 1) return;
-  for (uint32_t bch = threadIdx.x; bch < nchunks; bch += blockDim.x) {
-    const uint32_t cnt = B.fcount[bch];
-    for (uint32_t e = 0; e < cnt; ++e) {
-      uint32_t node = B.frontier[bch * REFIT_CHUNK + e];
-      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-      union_slot(B.nodes, node, 0u, lo, hi);
-      union_slot(B.nodes, node, 1u, lo, hi);
-      uint32_t h = 1u + max(B.slot_h[2 * node], B.slot_h[2 * node + 1]);
-      for (int level = 0;; ++level) {
-        if (node == 0u) {
-          put_root(B.misc, lo, hi, h);
-          break;
-        }
-        const int32_t pe = B.parent[node];
-        const uint32_t q = (uint32_t)pe >> 1, side = (uint32_t)pe & 1u;
-        if (pe < 0 || q >= n - 1 || level > 64) {
-          atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
-          break;
-        }
-        put_slot(B.nodes, q, side, lo, hi);
-        B.slot_h[2 * q + side] = h;
-        const uint32_t old =
-            __hip_atomic_fetch_add(&B.flags[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old == 0u) break;
-        union_slot(B.nodes, q, side ^ 1u, lo, hi);
-        h = 1u + max(h, B.slot_h[2 * q + (side ^ 1u)]);
-        node = q;
-      }
-    }
-  }
-  __syncthreads();
 }
 
 // ---- 4-wide collapse ------------------------------------------------------------
@@ -675,13 +561,14 @@ __device__ inline uint32_t leaf_depth(const BuildBuffers& B, uint32_t n, uint32_
 // build: REFIT_CHUNK = 1024; the batch kernel: its 256-thread blocks):
 // each thread writes its leaf record, the block scans the leaf boxes into
 // per-leaf prefix and suffix unions of the chunk (and the chunk's union), and
-// each leaf climbs as in the frontier refit while the node's leaf range lies
-// inside the chunk: such a node's index lies in the chunk too (a Karras node
-// i is an end of its own range), so its arrival counter lives in LDS. A node
-// spanning chunks only receives its child's box. Pass 2 gives every spanning
-// node [a, b] its box directly: suffix(a) | chunks strictly between | prefix(b)
-// (min / max are order-independent, so this is the box the climb would
-// build). The height is left to the depth walk of the widen kernel.
+// each leaf climbs while the node's leaf range lies inside the chunk: the
+// second of a node's two arrivals (an arrival counter) unions the sibling's
+// box and climbs on. Such a node's index lies in the chunk too (a Karras
+// node i is an end of its own range), so its counter and child boxes live in
+// LDS. A node spanning chunks only receives its child's box. Pass 2 gives
+// every spanning node [a, b] its box directly: suffix(a) | chunks strictly
+// between | prefix(b) (min / max are order-independent, so this is the box a
+// climb would build). The height is left to the on-demand depth walk.
 __device__ inline void box_union(float* b, const float* o) {
   b[0] = fminf(b[0], o[0]), b[1] = fminf(b[1], o[1]), b[2] = fminf(b[2], o[2]);
   b[3] = fmaxf(b[3], o[3]), b[4] = fmaxf(b[4], o[4]), b[5] = fmaxf(b[5], o[5]);

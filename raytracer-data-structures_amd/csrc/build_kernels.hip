@@ -51,13 +51,6 @@ constexpr uint32_t SORT_LEAN_MAX_KEYS = 1u << 20;
 
 __global__ __launch_bounds__(BUILD_BLOCK) void k_karras(BuildView v) { karras_tile<BUILD_BLOCK>(v, blockIdx.x); }
 
-__global__ __launch_bounds__(REFIT_CHUNK) void k_refit_local(BuildView v) {
-  __shared__ uint32_t lds[1];
-  refit_local_tile(v, blockIdx.x, lds);
-}
-
-__global__ __launch_bounds__(REFIT_TOP_BLOCK) void k_refit_top(BuildView v) { refit_top_block(v); }
-
 template <int CHUNK>
 __global__ __launch_bounds__(CHUNK) void k_refit_chunk(BuildView v) {
   __shared__ uint32_t lds[refit_chunk_lds_words<CHUNK>()];
@@ -149,35 +142,21 @@ hipError_t launch_build(const LaunchCtx& lc, const float* vx, const float* vy, c
     ProfScope ps(lc, "karras");
     hipLaunchKernelGGL(k_karras, dim3(tiles_of(n - 1, BUILD_BLOCK)), dim3(BUILD_BLOCK), 0, s, v);
   }
-  const uint32_t nchunks = tiles_of(n, REFIT_CHUNK);
-  if (B.pre) {  // chunk-union refit
-    constexpr int C = BUILD_REFIT_CHUNK;
-    const uint32_t cchunks = tiles_of(n, C);
-    {
-      ProfScope ps(lc, "refit");
-      hipLaunchKernelGGL(k_refit_chunk<C>, dim3(cchunks), dim3(C), 0, s, v);
-    }
-    if (cchunks > 1) {
-      ProfScope ps(lc, "refit_top");
-      hipLaunchKernelGGL(k_refit_span<C>, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
-    }
-    if (cchunks > 1 && B.wide) {
-      ProfScope ps(lc, "widen");
-      hipLaunchKernelGGL(k_widen_span<C>, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
-    }
-  } else {  // frontier refit
-    {
-      ProfScope ps(lc, "refit");
-      hipLaunchKernelGGL(k_refit_local, dim3(nchunks), dim3(REFIT_CHUNK), 0, s, v);
-    }
-    if (nchunks > 1) {
-      ProfScope ps(lc, "refit_top");
-      hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(REFIT_TOP_BLOCK), 0, s, v);
-    }
-    if (B.wide && n >= 2) {  // a batch mesh's BVH rebuilt on its own
-      ProfScope ps(lc, "widen");
-      hipLaunchKernelGGL(k_widen_all, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
-    }
+  // chunk-union refit (pass 1 with the chunk-internal wide records, pass 2
+  // for the nodes spanning chunks, then their wide records)
+  constexpr int C = BUILD_REFIT_CHUNK;
+  const uint32_t cchunks = tiles_of(n, C);
+  {
+    ProfScope ps(lc, "refit");
+    hipLaunchKernelGGL(k_refit_chunk<C>, dim3(cchunks), dim3(C), 0, s, v);
+  }
+  if (cchunks > 1) {
+    ProfScope ps(lc, "refit_top");
+    hipLaunchKernelGGL(k_refit_span<C>, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
+  }
+  if (cchunks > 1 && B.wide) {
+    ProfScope ps(lc, "widen");
+    hipLaunchKernelGGL(k_widen_span<C>, dim3(tiles_of(n - 1, 256)), dim3(256), 0, s, B, n);
   }
   return hipGetLastError();
 }

@@ -376,7 +376,7 @@ int rq_bvh_load(rq_ctx* c, const rq_scene* s, const void* blob, size_t bytes, rq
   b->scene = s;
   b->mode = h.mode;
   b->n = h.n;
-  rc = bvh_alloc(c, b, b->n, true, true);
+  rc = bvh_alloc(c, b, b->n);
   if (rc) {
     delete b;
     return rc;

@@ -153,7 +153,7 @@ int ensure_height(rq_ctx* c, const rq_bvh* b) {
   return e == hipSuccess ? RQ_OK : hip_err(c, e, "tree height");
 }
 
-int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit) {
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n) {
   BuildBuffers& B = b->B;
   B.ntiles = (n + SORT_TILE_LEAN - 1) / SORT_TILE_LEAN;
   size_t nn = std::max<uint32_t>(n, 1);
@@ -172,11 +172,10 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit) {
                  64 * ni,
                  48 * nn,
                  4 * nn,
-                 4 * (size_t)((nn + REFIT_CHUNK - 1) / REFIT_CHUNK),
-                 wide ? 128 * ni : 0,
-                 chunk_refit ? 24 * nn : 0,
-                 chunk_refit ? 24 * nn : 0,
-                 chunk_refit ? 24 * (size_t)((nn + REFIT_CHUNK_MIN - 1) / REFIT_CHUNK_MIN) : 0};
+                 128 * ni,
+                 24 * nn,
+                 24 * nn,
+                 24 * (size_t)((nn + REFIT_CHUNK_MIN - 1) / REFIT_CHUNK_MIN)};
   size_t total = 0;
   for (size_t s : sz) total += align256(s);
   void* p;
@@ -186,7 +185,7 @@ int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit) {
   void** dst[] = {(void**)&B.codes, (void**)&B.keys_a, (void**)&B.keys_b, (void**)&B.vals_a,
                   (void**)&B.vals_b, (void**)&B.status, (void**)&B.misc, (void**)&B.parent,
                   (void**)&B.flags, (void**)&B.slot_h, (void**)&B.range, (void**)&B.nodes,
-                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.fcount, (void**)&B.wide,
+                  (void**)&B.tris, (void**)&B.frontier, (void**)&B.wide,
                   (void**)&B.pre, (void**)&B.suf, (void**)&B.cbox};
   for (size_t i = 0; i < sizeof(sz) / sizeof(sz[0]); ++i) {
     *dst[i] = sz[i] ? q : nullptr;
@@ -410,7 +409,7 @@ int rq_build_lbvh(rq_ctx* c, const rq_scene* s, int mode, rq_bvh** out, void* st
   b->scene = s;
   b->mode = mode;
   b->n = s->ntri;
-  int rc = bvh_alloc(c, b, b->n, true, true);
+  int rc = bvh_alloc(c, b, b->n);
   if (rc) {
     delete b;
     return rc;
@@ -583,7 +582,7 @@ static int query(rq_ctx* c, const rq_bvh* b, const rq_rays* rays, uint32_t n, rq
   int variant = counted ? 0 : (any ? c->any_variant : c->closest_variant);
   int blk, depth, ts, wide;
   trace_variant_info(variant, &blk, &depth, &ts, &wide);
-  if (wide && !b->B.wide) {  // no wide nodes (a batch's BVH): the canonical binary walk
+  if (wide && !b->B.wide) {  // no wide nodes: the canonical binary walk
     variant = 0;
     trace_variant_info(variant, &blk, &depth, &ts, &wide);
   }

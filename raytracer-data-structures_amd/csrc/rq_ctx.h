@@ -95,11 +95,10 @@ uint32_t next_counter_slot(rq_ctx* c);
 // MAX_TRACE_STREAMS streams is exceeded, or when allocation is not allowed.
 // At the limit, the area of a stream with no work left is taken over.
 int32_t* stream_gstack(rq_ctx* c, void* stream, bool may_alloc = true);
-// Build buffers for n primitives; `wide`: also the 4-wide node array;
-// `chunk_refit`: the chunk-union refit buffers (the standalone build and blob
-// load; batch meshes refit by frontier lists and widen in a phase of their
-// own, server slots walk binary nodes).
-int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n, bool wide, bool chunk_refit);
+// Build buffers for n primitives: the binary and 4-wide node arrays, the
+// triangle records and the chunk-union refit's unions (every BVH: standalone,
+// blob-loaded, batch meshes and server slots).
+int bvh_alloc(rq_ctx* c, rq_bvh* b, uint32_t n);
 // Makes the tree height in b's misc words known (blocking; the chunk-union
 // refit leaves it to this call).
 int ensure_height(rq_ctx* c, const rq_bvh* b);

@@ -65,7 +65,6 @@ constexpr int REFIT_CHUNK = 1024;
 // The smallest refit chunk (the batch kernel's 256-thread blocks): the
 // chunk-union buffers hold one union per chunk of this size.
 constexpr int REFIT_CHUNK_MIN = 256;
-constexpr int REFIT_TOP_BLOCK = 1024;
 
 // Lookback status word: [31:30] flag, [29:0] count.
 constexpr uint32_t LB_AGG = 1u << 30;
@@ -135,16 +134,14 @@ struct BuildBuffers {
   uint32_t* status = nullptr;     // SORT_PASSES * ntiles * 256 (tiles of SORT_TILE_LEAN)
   uint32_t* misc = nullptr;       // see MiscLayout
   int32_t* parent = nullptr;      // 2n-1: (node << 1) | side
-  uint32_t* flags = nullptr;      // n-1 refit arrival counters
-  uint32_t* slot_h = nullptr;     // 2(n-1) child heights per node side
+  uint32_t* flags = nullptr;      // n-1 words, zeroed by Karras: tree-height and treelet scratch
+  uint32_t* slot_h = nullptr;     // 2(n-1) words of treelet scratch (heights, counts)
   uint2* range = nullptr;         // n-1 leaf range [first, last] of each internal node
-  uint32_t* frontier = nullptr;   // n: per refit chunk, spanning nodes ready for the top pass
-  uint32_t* fcount = nullptr;     // cdiv(n, REFIT_CHUNK) entries per chunk
+  uint32_t* frontier = nullptr;   // n words of treelet scratch (treelet roots)
   NodeRec* nodes = nullptr;       // n-1
   TriRec* tris = nullptr;         // n
-  WideNode* wide = nullptr;       // n-1 (see WideNode), or null: binary only
-  // Chunk-union refit (allocated with `wide`; null: frontier refit). Boxes
-  // are 6 floats (lo xyz, hi xyz).
+  WideNode* wide = nullptr;       // n-1 (see WideNode)
+  // Chunk-union refit. Boxes are 6 floats (lo xyz, hi xyz).
   float* pre = nullptr;           // 6n: union of the leaves from the chunk start to k
   float* suf = nullptr;           // 6n: union of the leaves from k to the chunk end
   float* cbox = nullptr;          // 6 per refit chunk (up to n / REFIT_CHUNK_MIN): its leaves' union

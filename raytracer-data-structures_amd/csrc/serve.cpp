@@ -708,7 +708,7 @@ int rq_server_create(int device, const rq_server_config* cfg_in, rq_server** out
     sl.bvh.magic = BVH_MAGIC;
     sl.bvh.owned = false;
     sl.bvh.mode = RQ_MORTON_SCENE_AABB;
-    rc = bvh_alloc(c, &sl.bvh, sl.tcap, true, true);  // wide nodes, chunk-union refit
+    rc = bvh_alloc(c, &sl.bvh, sl.tcap);
   }
   if (!rc) rc = ctx_alloc(c, std::max<size_t>(cfg.arena_bytes, 256), (void**)&s->arena_base);
   s->arena.init(cfg.arena_bytes);
