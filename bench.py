@@ -637,6 +637,8 @@ def run_frames(ctx, args) -> int:
     roofline = roofline_entry(dom, alg[dom], avg[dom], pmc.get(dom))
     roof_closest = (roofline_entry("trace_closest", alg["trace_closest"], avg["trace_closest"],
                                    pmc.get("trace_closest")) if "trace_closest" in avg else None)
+    roof_any = (roofline_entry("trace_any", alg["trace_any"], avg["trace_any"], pmc.get("trace_any"))
+                if "trace_any" in avg else None)
     build_kernels = ("build_init", "centroid_bounds", "morton_hist", "radix_onesweep", "karras",
                      "refit", "refit_top", "widen")
     build_ms = sum(breakdown[k][0] for k in build_kernels if k in breakdown) / breakdown_frames
@@ -666,6 +668,7 @@ def run_frames(ctx, args) -> int:
     line["trace_config"] = trace_dict(ctx, cfg, how)
     line["roofline"] = roofline
     line["roofline_closest"] = roof_closest
+    line["roofline_any"] = roof_any
     line["serving"] = None
     line["cpu_baseline"] = None
     if args.config == "c2" and not args.no_serve:
