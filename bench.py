@@ -56,7 +56,7 @@ BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
 # refill, closest slice, any slice): the fastest of a sweep of every variant and
 # refill threshold on MI355X (--sweep); fixed so that a profile of the same
 # command times the same kernels. --autotune times the variants again instead.
-TRACE_CONFIG = {"c1": None, "c2": "14,64,14,64,2048,1024", "c3": "14,16,9,16,2048,2048"}
+TRACE_CONFIG = {"c1": None, "c2": "17,64,14,64,2048,1024", "c3": "14,16,9,16,2048,2048"}
 # Frames whose LBVH rebuild runs on a second stream while the previous frame
 # renders (measured on MI355X: C1 0.324 -> 0.210 ms per frame; C2 3.55 -> 3.60,
 # its traversal already keeps VALU issue at ~0.77 and the build only competes).

@@ -230,12 +230,14 @@ int rq_any_hit_counted(rq_ctx* ctx, const rq_bvh* b, const rq_rays* rays, uint32
  * beyond the best hit when popped), and the node format walked: binary
  * (64-byte nodes, two children) or wide (the binary tree collapsed into
  * 128-byte 4-child nodes, written by every rq_build_lbvh / rq_rebuild_lbvh /
- * rq_bvh_load). Results are identical for all. A new context uses wide
- * variants (9 for closest hit, 14 for any hit, refill 64, slices of 1024); a
- * BVH without wide nodes (one owned by a batch) is walked by variant 0 when
- * a wide variant is configured. */
+ * rq_bvh_load), and the waves per SIMD the closest-hit and any-hit kernels
+ * are compiled for (0: as many as their registers allow; 6: registers
+ * capped for six, which hides more of the dependent node fetches). Results
+ * are identical for all. A new context uses wide variants (9 for closest
+ * hit, 14 for any hit, refill 64, slices of 1024). */
 typedef struct {
   int32_t block, lds_depth, tstack, wide;
+  int32_t closest_waves, any_waves;
 } rq_trace_variant;
 int rq_trace_variant_count(void);
 int rq_trace_variant_get(int index, rq_trace_variant* out);

@@ -687,7 +687,8 @@ int rq_trace_variant_count(void) { return trace_num_variants(); }
 
 int rq_trace_variant_get(int v, rq_trace_variant* out) {
   if (v < 0 || v >= trace_num_variants() || !out) return RQ_E_INVALID;
-  trace_variant_info(v, &out->block, &out->lds_depth, &out->tstack, &out->wide);
+  trace_variant_info(v, &out->block, &out->lds_depth, &out->tstack, &out->wide,
+                     &out->closest_waves, &out->any_waves);
   return RQ_OK;
 }
 

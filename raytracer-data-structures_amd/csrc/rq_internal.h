@@ -225,7 +225,8 @@ struct TraceArgs {
 hipError_t launch_trace(const LaunchCtx& lc, const TraceArgs& a, bool any, bool counted,
                         int variant);
 int trace_num_variants();
-void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide = nullptr);
+void trace_variant_info(int v, int* block, int* depth, int* tstack, int* wide = nullptr,
+                        int* closest_waves = nullptr, int* any_waves = nullptr);
 int trace_max_stack(int variant);  // total stack entries per lane (LDS + overflow)
 uint32_t trace_grid_blocks(int device, int variant, bool any);
 hipError_t launch_copy16(const LaunchCtx& lc, const void* src, void* dst, uint64_t bytes);

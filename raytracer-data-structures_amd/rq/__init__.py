@@ -83,7 +83,7 @@ class BvhInfo(C.Structure):
 
 class TraceVariant(C.Structure):
     _fields_ = [("block", C.c_int32), ("lds_depth", C.c_int32), ("tstack", C.c_int32),
-                ("wide", C.c_int32)]
+                ("wide", C.c_int32), ("closest_waves", C.c_int32), ("any_waves", C.c_int32)]
 
 
 class Mesh(C.Structure):
@@ -782,7 +782,7 @@ class Context:
             v = TraceVariant()
             lib().rq_trace_variant_get(i, C.byref(v))
             out.append(dict(index=i, block=v.block, lds_depth=v.lds_depth, tstack=v.tstack,
-                            wide=v.wide))
+                            wide=v.wide, closest_waves=v.closest_waves, any_waves=v.any_waves))
         return out
 
     def trace_config(self, closest: int | None = None, any_hit: int | None = None,
