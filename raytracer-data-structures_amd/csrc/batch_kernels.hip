@@ -287,8 +287,9 @@ constexpr int BATCH_REFIT_CHUNK = BUILD_BLOCK;
 // tile, Morton histograms, refit chunk), the trace kernel's stack.
 constexpr int cmax(int x, int y) { return x > y ? x : y; }
 constexpr int batch_lds_words(bool build, int depth, bool tstack) {
-  return build ? cmax(cmax(sort_lds_words(SORT_ITEMS_LEAN), SORT_PASSES * 256),
-                      refit_chunk_lds_words<BATCH_REFIT_CHUNK>())
+  return build ? cmax(cmax(cmax(sort_lds_words(SORT_ITEMS_LEAN), SORT_PASSES * 256),
+                           refit_chunk_lds_words<BATCH_REFIT_CHUNK>()),
+                      karras_lds_words<NODE_TILE>())
                : depth * BUILD_BLOCK * (tstack ? 2 : 1);
 }
 
@@ -417,7 +418,7 @@ __global__ __launch_bounds__(BUILD_BLOCK) __attribute__((amdgpu_waves_per_eu(WAV
         }
         case PH_KARRAS: {
           VIEW;
-          karras_tile(v, w.tile);
+          karras_tile(v, w.tile, lds);
           break;
         }
         case PH_REFIT: {  // chunk-union refit, pass 1 (with the chunks' wide records)

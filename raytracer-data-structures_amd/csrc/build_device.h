@@ -385,20 +385,56 @@ __device__ inline void onesweep_tile(const BuildView& v, uint32_t tile, uint32_t
   __syncthreads();
 }
 
+// Child links of the internal nodes as Karras emits them, (left, right) per
+// node: kept in the slot_h words (2(n-1), free until a treelet pass uses them
+// as scratch) until the refit writes each node record whole.
+__device__ inline int2* links_of(const BuildBuffers& B) { return reinterpret_cast<int2*>(B.slot_h); }
+
 // ---- phase 7: Karras 2012 emission for a tile of internal nodes ----------------
 // Internal node i in [0, n-2] from the sorted codes; equal codes fall back to
 // the index (augmented key code<<32 | i).
-__device__ inline int kdelta(const uint32_t* __restrict__ k, int64_t n, int64_t i, uint32_t ki,
-                             int64_t j) {
+//
+// A node's searches read the keys around it, a chain of dependent loads (the
+// direction, the doubling bound, the binary searches for the far end and the
+// split); most nodes span a few leaves. A unit first stages the keys of its
+// nodes and KARRAS_HALO on each side in LDS, and a search reads a key from
+// there when it lies in that window, from global memory otherwise.
+constexpr int KARRAS_HALO = 256;
+typedef __attribute__((address_space(3))) const uint32_t* LdsWords;  // an LDS array
+struct KeyWindow {
+  const uint32_t* __restrict__ keys;  // all n sorted keys (global)
+  LdsWords win;                       // keys[lo, hi), read with LDS loads
+  int64_t lo, hi;
+  // The LDS word is read at a clamped index whatever j, and replaced by the
+  // global one only outside the window: with one read per branch the
+  // compiler merges them into a generic (flat) load, which waits on both the
+  // LDS and the vector memory counters.
+  __device__ uint32_t operator()(int64_t j) const {
+    uint32_t k = win[min(max(j - lo, (int64_t)0), hi - lo - 1)];
+    if (j < lo || j >= hi) k = keys[j];
+    return k;
+  }
+};
+// Stages keys [lo, hi) of the n keys into win (the block's threads); the
+// caller synchronises before the searches read it.
+__device__ inline KeyWindow stage_keys(const uint32_t* __restrict__ keys, int64_t n, int64_t lo,
+                                       int64_t hi, uint32_t* win) {
+  lo = max(lo, (int64_t)0), hi = min(hi, n);
+  for (int64_t j = lo + threadIdx.x; j < hi; j += blockDim.x) win[j - lo] = keys[j];
+  return KeyWindow{keys, (LdsWords)win, lo, hi};
+}
+
+template <class K>
+__device__ inline int kdelta(const K& k, int64_t n, int64_t i, uint32_t ki, int64_t j) {
   if (j < 0 || j >= n) return -1;
-  const uint32_t kj = k[j];
+  const uint32_t kj = k(j);
   return ki != kj ? __clz(ki ^ kj) : 32 + __clz((uint32_t)i ^ (uint32_t)j);
 }
 
-__device__ inline void karras_node(const BuildView& v, int64_t i) {
-  const uint32_t* keys = v.B.keys_a;
+template <class K>
+__device__ inline void karras_node(const BuildView& v, const K& keys, int64_t i) {
   const int64_t N = v.n;
-  const uint32_t ki = keys[i];
+  const uint32_t ki = keys(i);
   const int d = kdelta(keys, N, i, ki, i + 1) > kdelta(keys, N, i, ki, i - 1) ? 1 : -1;
   const int dmin = kdelta(keys, N, i, ki, i - d);
   int64_t lmax = 2;
@@ -416,8 +452,9 @@ __device__ inline void karras_node(const BuildView& v, int64_t i) {
   const int64_t g = i + s * d + min(d, 0);
   const int32_t left = (min(i, j) == g) ? ~(int32_t)g : (int32_t)g;
   const int32_t right = (max(i, j) == g + 1) ? ~(int32_t)(g + 1) : (int32_t)(g + 1);
-  reinterpret_cast<int32_t*>(&v.B.nodes[i].lo0)[3] = left;
-  reinterpret_cast<int32_t*>(&v.B.nodes[i].hi0)[3] = right;
+  // the child links go to a compact array (consecutive nodes, whole lines);
+  // the refit writes them into the node records with the boxes
+  links_of(v.B)[i] = make_int2(left, right);
   const int64_t ul = left >= 0 ? left : (N - 1) + ~left;
   const int64_t ur = right >= 0 ? right : (N - 1) + ~right;
   v.B.flags[i] = 0u;
@@ -430,15 +467,24 @@ __device__ inline void karras_node(const BuildView& v, int64_t i) {
   v.B.parent[ur] = (int32_t)((i << 1) | 1);
 }
 
+// LDS words of a Karras unit of TILE nodes: its keys and the halo.
+template <int TILE>
+constexpr int karras_lds_words() {
+  return TILE + 2 * KARRAS_HALO;
+}
+
 // TILE internal nodes per unit: the batch kernel queues NODE_TILE, the
 // standalone kernel one node per thread (each node is an independent chain of
-// dependent key loads, so more units hide more latency).
+// dependent key loads, so more units hide more latency). lds holds
+// karras_lds_words<TILE>() words.
 template <int TILE = NODE_TILE>
-__device__ inline void karras_tile(const BuildView& v, uint32_t tile) {
+__device__ inline void karras_tile(const BuildView& v, uint32_t tile, uint32_t* lds) {
   if (v.n < 2) return;
-  const int64_t end = min((int64_t)(tile + 1) * TILE, (int64_t)v.n - 1);
-  for (int64_t i = (int64_t)tile * TILE + threadIdx.x; i < end; i += blockDim.x)
-    karras_node(v, i);
+  const int64_t begin = (int64_t)tile * TILE;
+  const int64_t end = min(begin + TILE, (int64_t)v.n - 1);
+  const KeyWindow w = stage_keys(v.B.keys_a, v.n, begin - KARRAS_HALO, end + 1 + KARRAS_HALO, lds);
+  __syncthreads();
+  for (int64_t i = begin + threadIdx.x; i < end; i += blockDim.x) karras_node(v, w, i);
 }
 
 // ---- phases 8-9: bottom-up refit ---------------------------------------------------
@@ -625,7 +671,7 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
   if (k < n - 1) {
     s_pint[tid] = k == 0u ? -1 : B.parent[k];  // the root has no parent
     s_rng[tid] = B.range[k];
-    s_kid[tid] = make_int2(__float_as_int(B.nodes[k].lo0.w), __float_as_int(B.nodes[k].hi0.w));
+    s_kid[tid] = links_of(B)[k];
   }
   float box[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   if (valid) {
@@ -685,11 +731,14 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
       atomicOr(&B.misc[MISC_ERR], ERR_REFIT_DEPTH);
       break;
     }
-    put_slot(B.nodes, node, side, lo, hi);
-    // a node outside the chunk's index span cannot have its range inside it
-    if (node < chunk_lo || node > chunk_hi) break;  // spanning: pass 2
-    const uint2 rg = s_rng[node - chunk_lo];
-    if (rg.x < chunk_lo || rg.y > chunk_hi) break;  // spanning: pass 2
+    // a node outside the chunk's index span cannot have its range inside it;
+    // a spanning parent takes the box in its record's slot (pass 2 finishes
+    // it), a parent inside the chunk in LDS (its record is written whole below)
+    if (node < chunk_lo || node > chunk_hi || s_rng[node - chunk_lo].x < chunk_lo ||
+        s_rng[node - chunk_lo].y > chunk_hi) {
+      put_slot(B.nodes, node, side, lo, hi);
+      break;
+    }
     float* mine = s_box + (node - chunk_lo) * 12;
 #pragma unroll
     for (int q = 0; q < 6; ++q) mine[6 * side + q] = box[q];
@@ -711,16 +760,22 @@ __device__ inline void refit_chunk_tile(const BuildView& v, uint32_t chunk, uint
     }
     cur = node;
   }
-  if (!B.wide) return;
-  // Wide records of the chunk's completed nodes (both children arrived, so
-  // every box below them is final in LDS): thread t writes node chunk start +
-  // t, so the 128-byte records go out in consecutive lines. An internal child
-  // of such a node lies in the chunk too.
+  // Node records and wide records of the chunk's completed nodes (both
+  // children arrived, so every box below them is final in LDS): thread t
+  // writes node chunk start + t, so the 64- and 128-byte records go out in
+  // consecutive lines. An internal child of such a node lies in the chunk too.
   __syncthreads();
   if (k >= n - 1 || s_flag[tid] != 2u) return;
   const float* mine = s_box + tid * 12;
-  WideSlots ws;
   const int2 kid = s_kid[tid];
+  NodeRec nr;
+  nr.lo0 = make_float4(mine[0], mine[1], mine[2], __int_as_float(kid.x));
+  nr.hi0 = make_float4(mine[3], mine[4], mine[5], __int_as_float(kid.y));
+  nr.lo1 = make_float4(mine[6], mine[7], mine[8], 0.0f);
+  nr.hi1 = make_float4(mine[9], mine[10], mine[11], 0.0f);
+  B.nodes[k] = nr;
+  if (!B.wide) return;
+  WideSlots ws;
 #pragma unroll
   for (int sd = 0; sd < 2; ++sd) {
     const int32_t c = sd ? kid.y : kid.x;
@@ -776,6 +831,10 @@ __device__ inline void refit_span_wave(const BuildBuffers& B, uint32_t n, uint32
       box_union(box, o);
     }
     if ((int)lane != src) continue;
+    // the spanning node's own links (its slots were filled by its children)
+    const int2 kid = links_of(B)[i];
+    reinterpret_cast<int32_t*>(&B.nodes[i].lo0)[3] = kid.x;
+    reinterpret_cast<int32_t*>(&B.nodes[i].hi0)[3] = kid.y;
     box_load(B.suf + 6ull * rg.x, o);
     box_union(box, o);
     box_load(B.pre + 6ull * rg.y, o);

@@ -49,7 +49,10 @@ const OnesweepFn kOnesweepLean[SORT_PASSES] = {k_onesweep_lean<0>, k_onesweep_le
                                                k_onesweep_lean<2>, k_onesweep_lean<3>};
 constexpr uint32_t SORT_LEAN_MAX_KEYS = 1u << 20;
 
-__global__ __launch_bounds__(BUILD_BLOCK) void k_karras(BuildView v) { karras_tile<BUILD_BLOCK>(v, blockIdx.x); }
+__global__ __launch_bounds__(BUILD_BLOCK) void k_karras(BuildView v) {
+  __shared__ uint32_t lds[karras_lds_words<BUILD_BLOCK>()];
+  karras_tile<BUILD_BLOCK>(v, blockIdx.x, lds);
+}
 
 template <int CHUNK>
 __global__ __launch_bounds__(CHUNK) void k_refit_chunk(BuildView v) {
