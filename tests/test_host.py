@@ -45,6 +45,22 @@ def test_headers_compile_as_c_and_cpp(tmp_path):
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{inc}", str(cpp)], check=True)
 
 
+def test_trace_variant_table(rq):
+    """The kernel variant table (a static table, no device needed): variant 0
+    is the canonical binary walk; every wide any-hit kernel whose LDS stack
+    leaves room for six waves per SIMD (6 x 4 waves x 64 lanes x depth x 4 B
+    <= 160 KB) is capped for six; 17 and 18 cap their closest-hit kernels too."""
+    vs = rq.Context.trace_variants()
+    assert vs[0] == dict(index=0, block=128, lds_depth=24, tstack=0, wide=0, closest_waves=0,
+                         any_waves=0)
+    for v in vs:
+        fits6 = v["wide"] and 6 * 4 * 64 * v["lds_depth"] * 4 <= 160 * 1024
+        assert v["any_waves"] == (6 if fits6 else 0), v
+        assert v["closest_waves"] in (0, 6)
+    six = [v["index"] for v in vs if v["closest_waves"] == 6]
+    assert six == [17, 18] and all(vs[i]["wide"] and not vs[i]["tstack"] for i in six)
+
+
 def test_acc_type_values(rq):
     assert [int(x) for x in rq.AccType] == [0, 1, 2, 3, 4, 5]
     assert rq.AccType.GPU_LBVH.name == "GPU_LBVH"
