@@ -125,7 +125,8 @@ int rq_server_submit(rq_server* s, const rq_request* r, uint64_t* ticket);
 /* Any thread: asks a queued or running request to stop. It finishes with
  * state RQ_E_CANCELED (its host outputs are left unspecified): at once if it
  * was not admitted yet, else once the trace launches that hold its rays have
- * retired, without its result copy; the launches of other requests are not
+ * retired and the result copies of the chunks they traced are over (no
+ * further chunk is traced); the launches of other requests are not
  * affected. RQ_E_INVALID for an unknown ticket or a finished request. */
 int rq_server_cancel(rq_server* s, uint64_t ticket);
 int rq_server_poll(rq_server* s, uint64_t ticket, rq_request_status* st);

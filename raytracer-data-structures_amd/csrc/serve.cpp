@@ -116,6 +116,11 @@ struct Req {
   uint32_t assigned = 0;      // rays handed to steps
   uint32_t pending = 0;       // chunks in steps not yet retired
   bool out_enqueued = false;
+  // every output array is page-locked host memory: each chunk's results are
+  // copied when its step is launched (behind the step's end event); a
+  // pageable array would make such a copy block the loop until the step ends,
+  // so those requests get one copy once their last chunk is traced
+  bool stream_out = false;
 };
 
 struct Chunk {
@@ -132,6 +137,18 @@ struct Step {
 
 constexpr int kStepBufs = 2;           // steps in flight (double-buffered tables)
 constexpr uint32_t kMinStepTile = 256;  // smallest ray tile of a step (one ray per thread)
+
+// Whether p points to page-locked host memory (hipHostMalloc / rq_host_alloc).
+bool page_locked(const void* p) {
+  if (!p) return true;
+  hipPointerAttribute_t a{};
+  const hipError_t e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // a pageable pointer is not an error of a later call
+    return false;
+  }
+  return a.type == hipMemoryTypeHost;
+}
 
 size_t request_bytes(const rq_request& r) {
   const size_t n = r.nrays;
@@ -461,6 +478,27 @@ bool launch_step(rq_server* s) {
     e = hipMemcpyAsync(s->err_host[buf], s->ctl + BATCH_CTL_BARRIER_ERR, 2 * sizeof(uint32_t),
                        hipMemcpyDeviceToHost, s->s_trace);
   if (e == hipSuccess) e = hipEventRecord(st->end, s->s_trace);
+  // Each chunk's results go to the caller's (page-locked) host arrays as soon
+  // as the step ends: copies on the output stream behind the step's end
+  // event, so they overlap the later steps, and a request's last copy is its
+  // last chunk's, queued when that chunk is launched (not when the loop sees
+  // the step end).
+  if (e == hipSuccess) e = hipStreamWaitEvent(s->s_out, st->end, 0);
+  for (const Chunk& c : st->chunks) {
+    Req* q = c.q;
+    if (!q->stream_out) continue;
+    if (e == hipSuccess && q->prim)
+      e = hipMemcpyAsync(q->r.prim + c.first, q->prim + c.first, 4ull * c.count,
+                         hipMemcpyDeviceToHost, s->s_out);
+    if (e == hipSuccess && q->t)
+      e = hipMemcpyAsync(q->r.t + c.first, q->t + c.first, 4ull * c.count, hipMemcpyDeviceToHost,
+                         s->s_out);
+    if (e == hipSuccess && q->occ)
+      e = hipMemcpyAsync(q->r.occluded + c.first, q->occ + c.first, c.count,
+                         hipMemcpyDeviceToHost, s->s_out);
+    if (e == hipSuccess) e = hipEventRecord(q->out, s->s_out);
+    q->out_enqueued = true;
+  }
   for (const Chunk& c : st->chunks) {
     c.q->assigned += c.count;
     ++c.q->pending;
@@ -479,23 +517,23 @@ bool launch_step(rq_server* s) {
   return true;
 }
 
+// Retires finished steps in launch order. The result copies of page-locked
+// requests were queued at launch; a pageable request gets one copy of all its
+// results once its last chunk has retired.
 void enqueue_out(rq_server* s, Req* q, hipEvent_t after) {
   const size_t n = q->r.nrays;
-  hipError_t e = hipSuccess;
-  if (after) e = hipStreamWaitEvent(s->s_out, after, 0);
-  if (e == hipSuccess && n && q->prim)
+  hipError_t e = hipStreamWaitEvent(s->s_out, after, 0);
+  if (e == hipSuccess && q->prim)
     e = hipMemcpyAsync(q->r.prim, q->prim, 4 * n, hipMemcpyDeviceToHost, s->s_out);
-  if (e == hipSuccess && n && q->t)
+  if (e == hipSuccess && q->t)
     e = hipMemcpyAsync(q->r.t, q->t, 4 * n, hipMemcpyDeviceToHost, s->s_out);
-  if (e == hipSuccess && n && q->occ)
+  if (e == hipSuccess && q->occ)
     e = hipMemcpyAsync(q->r.occluded, q->occ, n, hipMemcpyDeviceToHost, s->s_out);
   if (e == hipSuccess) e = hipEventRecord(q->out, s->s_out);
   if (e != hipSuccess) q->err = RQ_E_HIP;
   q->out_enqueued = true;
 }
 
-// Retires finished steps in launch order and queues the copies of requests
-// whose last chunk was in them.
 void retire_steps(rq_server* s) {
   for (;;) {
     Step* st = nullptr;
@@ -514,8 +552,8 @@ void retire_steps(rq_server* s) {
       --q->pending;
       if (bad) q->err = e != hipSuccess ? RQ_E_HIP : RQ_E_DEVICE;
       rays += (uint64_t)c.count * ((q->prim ? 1 : 0) + (q->occ ? 1 : 0));
-      if (q->pending == 0 && q->assigned >= q->r.nrays && !q->out_enqueued && !q->err &&
-          !q->cancel.load())
+      if (!q->stream_out && q->pending == 0 && q->assigned >= q->r.nrays && !q->out_enqueued &&
+          !q->err && !q->cancel.load())
         enqueue_out(s, q, st->end);
     }
     {
@@ -535,16 +573,22 @@ void complete(rq_server* s) {
     Req* q = s->admitted[i];
     bool done = false;
     if (q->ready && q->pending == 0) {
-      if (q->err || q->r.nrays == 0) {
-        done = true;  // failed, or a mesh load without rays
-      } else if (q->cancel.load() && !q->out_enqueued) {
-        q->err = RQ_E_CANCELED, done = true;  // no step holds it and no copy was queued
-      } else if (q->out_enqueued) {
+      // the result copies queued so far are over (they read the arena range)
+      bool copied = true;
+      if (q->out_enqueued) {
         const hipError_t e = hipEventQuery(q->out);
-        if (e == hipSuccess)
-          done = true;
-        else if (e != hipErrorNotReady)
-          q->err = RQ_E_HIP, done = true;
+        if (e == hipErrorNotReady)
+          copied = false;
+        else if (e != hipSuccess)
+          q->err = RQ_E_HIP;
+      }
+      if (!copied) {
+      } else if (q->err || q->r.nrays == 0) {
+        done = true;  // failed, or a mesh load without rays
+      } else if (q->cancel.load()) {
+        q->err = RQ_E_CANCELED, done = true;  // no step holds it, no copy is running
+      } else if (q->assigned >= q->r.nrays && q->out_enqueued) {
+        done = true;  // every chunk traced and copied
       }
     }
     if (!done) {
@@ -807,6 +851,7 @@ int submit(rq_server* s, const rq_request* r, uint64_t* ticket, bool resident) {
     if (hipEventCreateWithFlags(&q->prep, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&q->out, hipEventDisableTiming) != hipSuccess)
       return srv_err(s, RQ_E_HIP, "request events");
+    q->stream_out = page_locked(r->prim) && page_locked(r->t) && page_locked(r->occluded);
   }
   std::lock_guard<std::mutex> g(s->mu);
   auto refuse = [&](int code, const char* why) {

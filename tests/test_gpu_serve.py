@@ -236,17 +236,22 @@ def test_serve_resident_slots_are_never_recycled(rq):
             srv.load_mesh(rq.icosphere(2, 4, center=(0, 0, -5)))
 
 
-def test_serve_cancel(rq):
+@pytest.mark.parametrize("pinned", [False, True])
+def test_serve_cancel(rq, pinned):
     """Cancelled requests finish as RQ_E_CANCELED, queued ones at once and a
     running one once its launches retire; the others complete with their
-    alone results, and the server keeps serving."""
-    log("serve: cancel")
+    alone results, and the server keeps serving. Page-locked outputs take each
+    chunk's result copy at its launch, pageable ones one copy at the end."""
+    log(f"serve: cancel, page-locked outputs {pinned}")
     cfg = dict(SMALL, max_inflight=2, chunk_rays=1 << 12, step_rays=1 << 13)
     box = ((-2.0, -2.0, -7.0), (2.0, 2.0, -3.0))
     with rq.Server(0, **cfg) as srv:
         slot = srv.load_mesh(rq.icosphere(5, 9, center=(0, 0, -5)))
         n = 60000
-        outs = [np.full(n, -7, np.int32) for _ in range(8)]
+        keep = [rq.HostArray(n, np.int32) for _ in range(8)] if pinned else []
+        outs = [h.array for h in keep] if pinned else [np.empty(n, np.int32) for _ in range(8)]
+        for o in outs:
+            o.fill(-7)
         tks = [srv.submit(n, rq.Q_CLOSEST, prim=outs[k], mesh=slot, box=box, seed=200 + k)
                for k in range(8)]
         cancelled = {1, 5, 6}
