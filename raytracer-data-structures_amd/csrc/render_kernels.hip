@@ -327,13 +327,11 @@ __global__ __launch_bounds__(256) void k_shade_primary(SceneView s, ShadeParams 
 // sample in co.idx, and samples without one get a 0 flag in co.occ: the
 // any-hit launch then traces only the live rays (a few thousand of millions
 // of samples on a mesh that mostly reflects to the sky).
-__global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams sp, uint32_t n,
-                                                       RaySoA bnc, const int32_t* __restrict__ h1,
-                                                       const float* __restrict__ t1, RaySoA shad,
-                                                       float2* __restrict__ lit1, CompactOut co) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool in = i < n;
-  if (!in) return;
+__device__ inline void shade_bounce_sample(const SceneView& s, const ShadeParams& sp, uint32_t n,
+                                           const RaySoA& bnc, const int32_t* __restrict__ h1,
+                                           const float* __restrict__ t1, const RaySoA& shad,
+                                           float2* __restrict__ lit1, const CompactOut& co,
+                                           uint32_t i) {
   const int32_t p = h1[i];
   if (p < 0 || mat_of(s, p) != 0) {  // a second refracting / reflecting hit casts no shadow ray
     put_inactive(shad, (uint64_t)n + i);
@@ -360,6 +358,18 @@ __global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams s
   }
   const LitTerms lt = lit_terms(d, g, sp);
   lit1[i] = make_float2(lt.LdotN, lt.s25);
+}
+
+// Grid-stride: the launch keeps a bounded grid (it runs beside the primary
+// shadow rays' persistent any-hit launch, which holds most of the device's
+// wave slots, and few resident waves each taking many samples get through
+// faster there than a block per 256 samples).
+__global__ __launch_bounds__(256) void k_shade_bounce(SceneView s, ShadeParams sp, uint32_t n,
+                                                       RaySoA bnc, const int32_t* __restrict__ h1,
+                                                       const float* __restrict__ t1, RaySoA shad,
+                                                       float2* __restrict__ lit1, CompactOut co) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    shade_bounce_sample(s, sp, n, bnc, h1, t1, shad, lit1, co, i);
 }
 
 __global__ __launch_bounds__(64) void k_zero_word(uint32_t* w) {
@@ -557,8 +567,8 @@ hipError_t launch_shade_bounce(const LaunchCtx& lc, const SceneView& s, const Sh
                                const RaySoA& shad, float2* lit1, const CompactOut& co) {
   if (co.rays.ox) hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(64), 0, lc.stream, co.count);
   ProfScope ps(lc, "shade_bounce");
-  hipLaunchKernelGGL(k_shade_bounce, dim3(cdiv(n, 256)), dim3(256), 0, lc.stream, s, sp, n, bnc,
-                     h1, t1, shad, lit1, co);
+  hipLaunchKernelGGL(k_shade_bounce, dim3(std::min<uint32_t>(cdiv(n, 256), 2048)), dim3(256), 0,
+                     lc.stream, s, sp, n, bnc, h1, t1, shad, lit1, co);
   return hipGetLastError();
 }
 
