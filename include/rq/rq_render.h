@@ -79,7 +79,9 @@ int rq_render_read_image(rq_ctx* ctx, rq_renderer* r, float* rgb, uint8_t* ppm);
  * primary/bounce hit ids (-1 = miss or no ray), shadow flags, sample colour (3 floats). */
 int rq_render_read_samples(rq_ctx* ctx, rq_renderer* r, int32_t* hit0, int32_t* hit1,
                            uint8_t* shadow0, uint8_t* shadow1, float* sample_rgb);
-/* Device SoA of the primary rays of the last frame (valid until destroy). */
+/* Device SoA of the primary rays of the last frame (valid until destroy).
+ * Read-only: the origin, tmin and tmax arrays of the renderer's batches are
+ * shared constant arrays (zeros, +inf). */
 int rq_render_primary_rays(rq_ctx* ctx, rq_renderer* r, rq_rays* out, uint32_t* n);
 /* Device SoA of one ray batch of the last frame: which = 0 primary, 1 bounce,
  * 2 shadow (n or 2n slots; unused slots are inactive rays). */

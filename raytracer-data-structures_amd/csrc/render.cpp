@@ -26,6 +26,11 @@ struct rq_renderer {
   // the second hits' shadow rays compacted (side-stream frames): rays, their
   // samples and their count
   RaySoA shadc{};
+  // shared constant arrays, written once at creation: zeros (2n floats when a
+  // second ray per sample is traced) serve as every batch's tmin and as the
+  // camera rays' origins; infs (n floats) as the camera rays' tmax
+  float* zeros = nullptr;
+  float* infs = nullptr;
   uint32_t* shidx = nullptr;
   uint32_t* shcount = nullptr;
   uint8_t* occ = nullptr;
@@ -52,13 +57,21 @@ int ralloc(rq_ctx* c, rq_renderer* r, size_t bytes, void** p) {
   return rc;
 }
 
+// A ray batch whose tmin is the renderer's shared array of zeros (every ray
+// the renderer spawns starts its interval at 0).
 int alloc_rays(rq_ctx* c, rq_renderer* r, size_t n, RaySoA& R) {
-  float** f[8] = {&R.ox, &R.oy, &R.oz, &R.dx, &R.dy, &R.dz, &R.tmin, &R.tmax};
+  float** f[7] = {&R.ox, &R.oy, &R.oz, &R.dx, &R.dy, &R.dz, &R.tmax};
   for (auto* q : f) {
     int rc = ralloc(c, r, n * 4, (void**)q);
     if (rc) return rc;
   }
+  R.tmin = r->zeros;
   return RQ_OK;
+}
+// The batch as the shading kernels write it: tmin left to the shared zeros.
+RaySoA write_view(RaySoA R) {
+  R.tmin = nullptr;
+  return R;
 }
 
 void free_renderer(rq_ctx* c, rq_renderer* r) {
@@ -132,7 +145,21 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
   r->sh.materials = p->materials;
   const bool second = r->second();
   const size_t ns = second ? 2 * n : n;
-  int rc = alloc_rays(c, r, n, r->prim);
+  // shared constant arrays: zeros (every tmin, the camera rays' origins) and
+  // +inf (the camera rays' tmax), written once here
+  int rc = ralloc(c, r, ns * 4, (void**)&r->zeros);
+  if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->infs);
+  if (!rc) {
+    hipError_t e = hipMemset(r->zeros, 0, ns * 4);
+    if (e == hipSuccess) e = hipMemsetD32((hipDeviceptr_t)r->infs, 0x7F800000, n);
+    if (e != hipSuccess) rc = hip_err(c, e, "renderer constants");
+  }
+  // camera rays: only their directions are written per frame
+  if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->prim.dx);
+  if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->prim.dy);
+  if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->prim.dz);
+  r->prim.ox = r->prim.oy = r->prim.oz = r->prim.tmin = r->zeros;
+  r->prim.tmax = r->infs;
   if (!rc && second) rc = alloc_rays(c, r, n, r->bnc);
   if (!rc) rc = alloc_rays(c, r, ns, r->shad);
   if (!rc) rc = ralloc(c, r, n * 4, (void**)&r->h0);
@@ -191,7 +218,10 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   LaunchCtx lc = launch_ctx(c, stream);
   const rq_scene* s = b->scene;
   SceneView sv{s->vx, s->vy, s->vz, s->idx, s->r, s->rgb, r->p.materials ? s->mat : nullptr};
-  hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, r->prim);
+  // the camera rays' directions only: their origins and intervals are the
+  // shared constant arrays
+  const RaySoA gen{nullptr, nullptr, nullptr, r->prim.dx, r->prim.dy, r->prim.dz, nullptr, nullptr};
+  hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, gen);
   if (e != hipSuccess) return hip_err(c, e, "raygen");
   rq_rays pr = as_rays(r->prim);
   if (r->p.mode == RQ_RENDER_OCCLUSION) {
@@ -203,7 +233,12 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   rq_hits h0{r->h0, r->t0, nullptr, nullptr};
   int rc = rq_closest_hit(c, b, &pr, r->n, &h0, stream);
   if (rc) return rc;
-  e = launch_shade_primary(lc, sv, r->sh, r->n, r->prim, r->h0, r->t0, r->shad, r->bnc, r->lit0);
+  // the shading kernels read the camera rays with a null origin (all at the
+  // origin) and write the spawned rays without their tmin (all zero)
+  RaySoA cam = r->prim;
+  cam.ox = cam.oy = cam.oz = nullptr;
+  e = launch_shade_primary(lc, sv, r->sh, r->n, cam, r->h0, r->t0, write_view(r->shad),
+                           write_view(r->bnc), r->lit0);
   if (e != hipSuccess) return hip_err(c, e, "shade primary");
   if (r->side) {
     // the primary shadow rays on the side stream, concurrently with the
@@ -223,9 +258,11 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
     rc = rq_closest_hit(c, b, &br, r->n, &h1, stream);
     if (rc) return rc;
-    const CompactOut co = r->side ? CompactOut{r->shadc, r->shidx, r->shcount, r->occ + r->n}
+    const CompactOut co = r->side ? CompactOut{write_view(r->shadc), r->shidx, r->shcount,
+                                               r->occ + r->n}
                                   : CompactOut{};
-    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, r->shad, r->lit1, co);
+    e = launch_shade_bounce(lc, sv, r->sh, r->n, r->bnc, r->h1, r->t1, write_view(r->shad),
+                            r->lit1, co);
     if (e != hipSuccess) return hip_err(c, e, "shade bounce");
   }
   if (r->side) {
@@ -240,7 +277,7 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     rc = rq_any_hit(c, b, &sr, r->second() ? 2 * r->n : r->n, r->occ, stream);
     if (rc) return rc;
   }
-  e = launch_shade_final(lc, sv, r->sh, r->npix, r->p.spp, r->prim, r->bnc, r->h0, r->t0, r->h1,
+  e = launch_shade_final(lc, sv, r->sh, r->npix, r->p.spp, cam, r->bnc, r->h0, r->t0, r->h1,
                          r->t1, r->occ, r->lit0, r->lit1, r->srgb, r->img);
   if (e != hipSuccess) return hip_err(c, e, "shade final");
   return RQ_OK;

@@ -80,9 +80,11 @@ __global__ __launch_bounds__(256) void k_raygen_camera(CamParams cp, const doubl
       (float)((2.0 * (((double)x + r1) * (double)cp.invW) - 1.0) * (double)cp.angle * (double)cp.aspect);
   const float yy = (float)((1.0 - 2.0 * (((double)y + r2) * (double)cp.invH)) * (double)cp.angle);
   const V3 d = normalize({xx, yy, -1.0f});
-  ox[i] = 0.0f, oy[i] = 0.0f, oz[i] = 0.0f;
+  // null origin / interval arrays: the caller keeps shared constant ones
+  // (every camera ray starts at the origin with [0, inf))
+  if (ox) ox[i] = 0.0f, oy[i] = 0.0f, oz[i] = 0.0f;
   dx[i] = d.x, dy[i] = d.y, dz[i] = d.z;
-  tmin[i] = 0.0f, tmax[i] = INFINITY;
+  if (tmin) tmin[i] = 0.0f, tmax[i] = INFINITY;
 }
 
 __global__ __launch_bounds__(256) void k_raygen_uniform(uint32_t n, uint64_t seed, float lox,
@@ -209,13 +211,17 @@ __device__ inline V3 lit_color(LitTerms t, const ShadeParams& sp, V3 surf, bool 
 __device__ inline V3 direct(V3 d, const Geo& g, const ShadeParams& sp, V3 surf, bool shadow) {
   return lit_color(lit_terms(d, g, sp), sp, surf, shadow);
 }
+// Every ray the renderer spawns starts its interval at 0; a batch written
+// with R.tmin null keeps a shared array of zeros for it (not written here).
 __device__ inline void put_ray(const RaySoA& R, uint64_t i, V3 o, V3 d, float tmin, float tmax) {
   R.ox[i] = o.x, R.oy[i] = o.y, R.oz[i] = o.z;
   R.dx[i] = d.x, R.dy[i] = d.y, R.dz[i] = d.z;
-  R.tmin[i] = tmin, R.tmax[i] = tmax;
+  if (R.tmin) R.tmin[i] = tmin;
+  R.tmax[i] = tmax;
 }
 __device__ inline void put_inactive(const RaySoA& R, uint64_t i) {
-  R.tmin[i] = 0.0f, R.tmax[i] = 0.0f;
+  if (R.tmin) R.tmin[i] = 0.0f;
+  R.tmax[i] = 0.0f;
 }
 struct ShadowRay {
   V3 o, d;
@@ -254,7 +260,11 @@ __device__ inline V3 refract(V3 I, V3 N, float ior) {
   if (k < 0.0f) return {0.0f, 0.0f, 0.0f};
   return add(mul(I, eta), mul(n, eta * c - sqrtf(k)));
 }
-__device__ inline V3 get_o(const RaySoA& R, uint64_t i) { return {R.ox[i], R.oy[i], R.oz[i]}; }
+// A batch read with R.ox null starts every ray at the origin (camera rays).
+__device__ inline V3 get_o(const RaySoA& R, uint64_t i) {
+  if (!R.ox) return {0.0f, 0.0f, 0.0f};
+  return {R.ox[i], R.oy[i], R.oz[i]};
+}
 __device__ inline V3 get_d(const RaySoA& R, uint64_t i) { return {R.dx[i], R.dy[i], R.dz[i]}; }
 
 // Frame ray counts on demand: the active rays (tmin < tmax) of one batch.
