@@ -111,6 +111,14 @@ __device__ inline void cas_pair(uint32_t& ka, int32_t& ca, uint32_t& kb, int32_t
   const int32_t c0 = sw ? cb : ca, c1 = sw ? ca : cb;
   ka = k0, kb = k1, ca = c0, cb = c1;
 }
+// The same exchange for hit / miss keys (any hit): a miss before a hit swaps.
+// The flags stay lane masks, so only the two link selects are vector work.
+__device__ inline void cas_hit(bool& ha, int32_t& ca, bool& hb, int32_t& cb) {
+  const bool sw = !ha && hb;
+  const int32_t c0 = sw ? cb : ca, c1 = sw ? ca : cb;
+  const bool h0 = ha || hb, h1 = ha && hb;
+  ha = h0, hb = h1, ca = c0, cb = c1;
+}
 
 // Möller–Trumbore, main.cpp:137-162 (v_0 read as v0), t restricted to
 // (tmin, tmax). Dot products (x*x' + y*y') + z*z', no contraction.
@@ -379,11 +387,22 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
         auto key = [](bool h, float t) -> uint32_t {
           return h ? (ANY ? 0u : __float_as_uint(t)) : 0xFFFFFFFFu;
         };
-        uint32_t k0 = key(h0, t0), k1 = key(h1, t1), k2 = key(h2, t2), k3 = key(h3, t3);
+        uint32_t k0, k1, k2, k3;
         int32_t c0 = ln.x, c1 = ln.y, c2 = ln.z, c3 = ln.w;
-        cas_pair(k0, c0, k1, c1), cas_pair(k2, c2, k3, c3);
-        cas_pair(k0, c0, k2, c2), cas_pair(k1, c1, k3, c3);
-        cas_pair(k1, c1, k2, c2);
+        if (ANY) {
+          // the same network on the hit flags (every pushed entry is a hit,
+          // key 0)
+          bool g0 = h0, g1 = h1, g2 = h2, g3 = h3;
+          cas_hit(g0, c0, g1, c1), cas_hit(g2, c2, g3, c3);
+          cas_hit(g0, c0, g2, c2), cas_hit(g1, c1, g3, c3);
+          cas_hit(g1, c1, g2, c2);
+          k0 = k1 = k2 = k3 = 0u;
+        } else {
+          k0 = key(h0, t0), k1 = key(h1, t1), k2 = key(h2, t2), k3 = key(h3, t3);
+          cas_pair(k0, c0, k1, c1), cas_pair(k2, c2, k3, c3);
+          cas_pair(k0, c0, k2, c2), cas_pair(k1, c1, k3, c3);
+          cas_pair(k1, c1, k2, c2);
+        }
         if (k == 0) {
           if (!pop()) finish();
         } else {
