@@ -1,6 +1,7 @@
 // Block-level device functions of the LBVH build (Morton -> onesweep radix
 // sort -> Karras emission -> two-pass refit). Each function is one unit of
-// work for one block of BUILD_BLOCK (256) threads: the standalone kernels in
+// work for one block of BUILD_BLOCK (256) threads (the triangle passes take
+// the block size as a parameter): the standalone kernels in
 // build_kernels.hip launch one block per unit, the persistent batch kernel in
 // batch_kernels.hip pulls the same units from its work queue. LDS scratch is
 // passed in so that a kernel calling several phases holds one buffer.
@@ -16,7 +17,7 @@
 namespace rq {
 
 constexpr int BUILD_BLOCK = 256;
-constexpr int TRI_TILE = 4096;   // triangles per bounds / Morton unit (16 per thread)
+constexpr int TRI_TILE = 4096;   // triangles per bounds / Morton unit
 constexpr int NODE_TILE = 4096;  // internal nodes per Karras unit
 // LDS words needed by the largest phase (the onesweep tile)
 __host__ __device__ constexpr int sort_lds_words(int items) {
@@ -129,10 +130,12 @@ __device__ inline void build_init_part(const BuildView& v, uint32_t first, uint3
 }
 
 // ---- phase 1: centroid bounds of one tile (SCENE_AABB map) ------------------------
+template <int BLOCK = BUILD_BLOCK>
 __device__ inline void bounds_tile(const BuildView& v, uint32_t tile, uint32_t* lds) {
+  constexpr int W = BLOCK / 64;  // waves; lds holds 6 * W words
   uint32_t lo[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, hi[3] = {0u, 0u, 0u};
   const uint32_t end = min((tile + 1) * (uint32_t)TRI_TILE, v.n);
-  for (uint32_t t = tile * TRI_TILE + threadIdx.x; t < end; t += BUILD_BLOCK) {
+  for (uint32_t t = tile * TRI_TILE + threadIdx.x; t < end; t += BLOCK) {
     const float3 c = centroid(v, t);
     const uint32_t o[3] = {f2ord(c.x), f2ord(c.y), f2ord(c.z)};
 #pragma unroll
@@ -149,13 +152,13 @@ __device__ inline void bounds_tile(const BuildView& v, uint32_t tile, uint32_t* 
   const int wave = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int k = 0; k < 3; ++k) lds[k * 4 + wave] = lo[k], lds[(3 + k) * 4 + wave] = hi[k];
+    for (int k = 0; k < 3; ++k) lds[k * W + wave] = lo[k], lds[(3 + k) * W + wave] = hi[k];
   }
   __syncthreads();
   if (threadIdx.x < 6) {  // one device atomic per bound per tile
     const int k = threadIdx.x;
-    uint32_t m = lds[k * 4];
-    for (int w = 1; w < BUILD_BLOCK / 64; ++w) m = k < 3 ? min(m, lds[k * 4 + w]) : max(m, lds[k * 4 + w]);
+    uint32_t m = lds[k * W];
+    for (int w = 1; w < W; ++w) m = k < 3 ? min(m, lds[k * W + w]) : max(m, lds[k * W + w]);
     if (k < 3)
       atomicMin(&v.B.misc[MISC_BOUNDS + k], m);
     else
@@ -165,9 +168,10 @@ __device__ inline void bounds_tile(const BuildView& v, uint32_t tile, uint32_t* 
 }
 
 // ---- phase 2: Morton codes of one tile + the four 8-bit digit histograms ------
+template <int BLOCK = BUILD_BLOCK>
 __device__ inline void morton_tile(const BuildView& v, uint32_t tile, uint32_t* lds) {
   uint32_t* s_hist = lds;  // SORT_PASSES * 256
-  for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BUILD_BLOCK) s_hist[i] = 0u;
+  for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BLOCK) s_hist[i] = 0u;
   float lo[3] = {0, 0, 0}, ext[3] = {0, 0, 0};
   if (v.mode == RQ_MORTON_SCENE_AABB) {
 #pragma unroll
@@ -178,7 +182,7 @@ __device__ inline void morton_tile(const BuildView& v, uint32_t tile, uint32_t* 
   }
   __syncthreads();
   const uint32_t end = min((tile + 1) * (uint32_t)TRI_TILE, v.n);
-  for (uint32_t t = tile * TRI_TILE + threadIdx.x; t < end; t += BUILD_BLOCK) {
+  for (uint32_t t = tile * TRI_TILE + threadIdx.x; t < end; t += BLOCK) {
     const float3 c = centroid(v, t);
     float q[3] = {c.x, c.y, c.z};
 #pragma unroll
@@ -195,7 +199,7 @@ __device__ inline void morton_tile(const BuildView& v, uint32_t tile, uint32_t* 
       atomicAdd(&s_hist[p * 256 + ((code >> (8 * p)) & 255u)], 1u);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BUILD_BLOCK) {
+  for (int i = threadIdx.x; i < SORT_PASSES * 256; i += BLOCK) {
     const uint32_t c = s_hist[i];
     if (c) atomicAdd(&v.B.misc[MISC_HIST + i], c);
   }
