@@ -57,10 +57,12 @@ BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
 # refill threshold on MI355X (--sweep); fixed so that a profile of the same
 # command times the same kernels. --autotune times the variants again instead.
 TRACE_CONFIG = {"c1": None, "c2": "17,64,14,64,2048,1024", "c3": "14,16,9,16,2048,2048"}
-# Frames whose LBVH rebuild runs on a second stream while the previous frame
-# renders (measured on MI355X: C1 0.324 -> 0.210 ms per frame; C2 3.55 -> 3.60,
-# its traversal already keeps VALU issue at ~0.77 and the build only competes).
-PIPELINE = {"c1": True, "c2": False}
+# The LBVH rebuild of frame k+1 on a second stream (rq.FramePipeline): "frame"
+# beside the whole of frame k, "tail" after frame k's last closest-hit launch,
+# beside its shading only; None: sequential. Measured on MI355X (ms per frame,
+# two runs each): C1 sequential 0.314, frame 0.219, tail 0.290; C2 sequential
+# 3.097, frame 3.058, tail 3.051.
+PIPELINE = {"c1": "frame", "c2": "tail"}
 ALG_BASIS = ("algorithmic bytes of the canonical traversal (rays in, hits out, 32 B per node "
              "box tested, 36 B per triangle tested); node and triangle reads are mostly served "
              "from L2 and the Infinity Cache, so the fraction can exceed 1")
@@ -559,7 +561,8 @@ def run_frames(ctx, args) -> int:
     # Every frame rebuilds its LBVH; pipelined, the rebuild for frame k+1 runs
     # on its own stream while frame k renders (rq.FramePipeline)
     pipe = rq.FramePipeline(ctx, ren, scene, bvh, rq.MORTON_SCENE_AABB, args.treelets,
-                            pipelined=args.pipeline)
+                            pipelined=args.pipeline is not None,
+                            overlap=args.pipeline or "frame")
     frame = pipe.frame
 
     for _ in range(args.warmup):
@@ -651,7 +654,7 @@ def run_frames(ctx, args) -> int:
         "bounce": int(params.bounce), "rays_per_frame": rays_per_frame,
         "rays_primary": int(cnt.primary), "rays_bounce": int(cnt.bounce),
         "rays_shadow": int(cnt.shadow), "lbvh_rebuilt_per_frame": True,
-        "rebuild_overlaps_previous_frame": bool(args.pipeline),
+        "rebuild_overlaps_previous_frame": args.pipeline or False,
         "bvh_height": int(info.height)})
     line["lbvh_build_ms"] = round(build_ms, 4)
     line["lbvh_build_roofline"] = {
@@ -1002,10 +1005,10 @@ def main() -> int:
     ap.add_argument("--serve-seconds-short", type=float, default=2.0,
                     help="c2: measured seconds of the serving run reported beside the frame")
     ap.add_argument("--no-serve", action="store_true", help="c2: no serving run")
-    ap.add_argument("--pipeline", default="auto", choices=["auto", "on", "off"],
-                    help="c1/c2: rebuild the next frame's LBVH while a frame renders (auto: "
-                         "on for c1, whose launches leave the device idle; off for c2, whose "
-                         "traversal keeps every SIMD's issue busy)")
+    ap.add_argument("--pipeline", default="auto", choices=["auto", "frame", "tail", "off"],
+                    help="c1/c2: rebuild the next frame's LBVH while a frame renders: beside "
+                         "the whole frame (frame), beside its shading after its last "
+                         "closest-hit launch (tail), or not (off); auto: the workload's choice")
     ap.add_argument("--traffic", default="auto", choices=["auto", "off"],
                     help="HBM bytes and issue counters per launch from rocprofv3 --pmc child runs")
     ap.add_argument("--pmc-probe", action="store_true",
@@ -1025,8 +1028,8 @@ def main() -> int:
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the library copy kernel over BYTES (counter calibration)")
     args = ap.parse_args()
-    args.pipeline = (args.pipeline == "on" or
-                     (args.pipeline == "auto" and PIPELINE.get(args.config, False)))
+    args.pipeline = (PIPELINE.get(args.config) if args.pipeline == "auto" else
+                     None if args.pipeline == "off" else args.pipeline)
     if args.treelets:
         CHILD_ARGS.extend(["--treelets", str(args.treelets)])
     if args.gpus != 1:

@@ -71,6 +71,15 @@ int rq_renderer_create(rq_ctx* ctx, const rq_render_params* p, rq_renderer** out
 int rq_renderer_destroy(rq_ctx* ctx, rq_renderer* r);
 /* Enqueue one frame on `stream` against bvh (its scene supplies normals). */
 int rq_render_frame(rq_ctx* ctx, rq_renderer* r, const rq_bvh* bvh, void* stream);
+/* Makes `stream` wait, on the device (no host wait), until the last frame
+ * enqueued by rq_render_frame has finished its last closest-hit launch (the
+ * any-hit launch of an occlusion frame): the rest of that frame is shading and
+ * the second hits' shadow rays. Frames on one stream run in order, so every
+ * earlier frame is then complete. A rebuild of the BVH the previous frame read
+ * can start there and overlap the frame's tail (rq_stream_order after the
+ * whole frame is the other choice). No frame yet: returns at once.
+ * No counterpart in the reference (its frames are sequential host loops). */
+int rq_render_wait_traced(rq_ctx* ctx, rq_renderer* r, void* stream);
 int rq_render_counts(rq_ctx* ctx, rq_renderer* r, rq_frame_counts* out);
 /* Image after the last frame: rgb = per-pixel sum over samples (w*h*3
  * floats, may be NULL); ppm = (uint8)(min(1, rgb/spp) * 255) (main.cpp:521-523). */

@@ -44,6 +44,11 @@ struct rq_renderer {
   // are traced on the frame's stream (default priority, ordered by events)
   hipStream_t side = nullptr;
   hipEvent_t ev_primary = nullptr, ev_shadow = nullptr;
+  // recorded on the frame's stream after its last closest-hit launch (or the
+  // occlusion frame's any-hit launch): what is left of the frame then is
+  // shading and the few second-hit shadow rays (rq_render_wait_traced)
+  hipEvent_t ev_traced = nullptr;
+  bool traced_recorded = false;
   std::vector<void*> allocs;
   // a second ray per sample: the mirror bounce, or a refraction ray
   bool second() const { return p.mode == RQ_RENDER_SHADE && (p.bounce || p.materials); }
@@ -78,6 +83,7 @@ void free_renderer(rq_ctx* c, rq_renderer* r) {
   for (void* p : r->allocs) ctx_free(c, p);
   if (r->ev_primary) (void)hipEventDestroy(r->ev_primary);
   if (r->ev_shadow) (void)hipEventDestroy(r->ev_shadow);
+  if (r->ev_traced) (void)hipEventDestroy(r->ev_traced);
   if (r->side) (void)rq_stream_destroy(c, r->side);  // and the stream's stack overflow area
   delete r;
 }
@@ -192,6 +198,10 @@ int rq_renderer_create(rq_ctx* c, const rq_render_params* p, rq_renderer** out) 
   if (!rc && r->second() && p->shadows) rc = alloc_rays(c, r, n, r->shadc);
   if (!rc && r->second() && p->shadows) rc = ralloc(c, r, n * 4, (void**)&r->shidx);
   if (!rc && r->second() && p->shadows) rc = ralloc(c, r, 4, (void**)&r->shcount);
+  if (!rc) {
+    const hipError_t e = hipEventCreateWithFlags(&r->ev_traced, hipEventDisableTiming);
+    if (e != hipSuccess) rc = hip_err(c, e, "renderer event");
+  }
   if (!rc && r->second() && p->shadows) {
     hipError_t e = hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->ev_primary, hipEventDisableTiming);
@@ -224,14 +234,22 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
   hipError_t e = launch_raygen_camera(lc, r->cam, r->jit, gen);
   if (e != hipSuccess) return hip_err(c, e, "raygen");
   rq_rays pr = as_rays(r->prim);
+  auto traced = [&]() -> int {
+    const hipError_t er = hipEventRecord(r->ev_traced, (hipStream_t)stream);
+    if (er != hipSuccess) return hip_err(c, er, "traced event");
+    r->traced_recorded = true;
+    return RQ_OK;
+  };
   if (r->p.mode == RQ_RENDER_OCCLUSION) {
     int rc = rq_any_hit(c, b, &pr, r->n, r->occ, stream);
+    if (!rc) rc = traced();
     if (rc) return rc;
     e = launch_shade_occlusion(lc, r->sh, r->npix, r->p.spp, r->occ, r->h0, r->srgb, r->img);
     return e == hipSuccess ? RQ_OK : hip_err(c, e, "shade occlusion");
   }
   rq_hits h0{r->h0, r->t0, nullptr, nullptr};
   int rc = rq_closest_hit(c, b, &pr, r->n, &h0, stream);
+  if (!rc && !r->second()) rc = traced();
   if (rc) return rc;
   // the shading kernels read the camera rays with a null origin (all at the
   // origin) and write the spawned rays without their tmin (all zero)
@@ -257,6 +275,7 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
     rq_rays br = as_rays(r->bnc);
     rq_hits h1{r->h1, r->t1, nullptr, nullptr};
     rc = rq_closest_hit(c, b, &br, r->n, &h1, stream);
+    if (!rc) rc = traced();
     if (rc) return rc;
     const CompactOut co = r->side ? CompactOut{write_view(r->shadc), r->shidx, r->shcount,
                                                r->occ + r->n}
@@ -281,6 +300,13 @@ int rq_render_frame(rq_ctx* c, rq_renderer* r, const rq_bvh* b, void* stream) {
                          r->t1, r->occ, r->lit0, r->lit1, r->srgb, r->img);
   if (e != hipSuccess) return hip_err(c, e, "shade final");
   return RQ_OK;
+}
+
+int rq_render_wait_traced(rq_ctx* c, rq_renderer* r, void* stream) {
+  if (!c || !r) return RQ_E_INVALID;
+  if (!r->traced_recorded) return RQ_OK;  // no frame yet: nothing to wait for
+  const hipError_t e = hipStreamWaitEvent((hipStream_t)stream, r->ev_traced, 0);
+  return e == hipSuccess ? RQ_OK : hip_err(c, e, "wait traced");
 }
 
 int rq_render_counts(rq_ctx* c, rq_renderer* r, rq_frame_counts* out) {

@@ -231,6 +231,7 @@ def _load() -> C.CDLL:
         "rq_renderer_create": (i32, [vp, C.POINTER(RenderParams), C.POINTER(vp)]),
         "rq_renderer_destroy": (i32, [vp, vp]),
         "rq_render_frame": (i32, [vp, vp, vp, vp]),
+        "rq_render_wait_traced": (i32, [vp, vp, vp]),
         "rq_render_counts": (i32, [vp, vp, C.POINTER(FrameCounts)]),
         "rq_render_read_image": (i32, [vp, vp, vp, vp]),
         "rq_render_read_samples": (i32, [vp, vp, vp, vp, vp, vp, vp]),
@@ -1120,6 +1121,11 @@ class Renderer:
     def frame(self, bvh: int, stream=None) -> None:
         self.ctx._check(lib().rq_render_frame(self.ctx.h, self.h, bvh, stream))
 
+    def wait_traced(self, stream) -> None:
+        """rq_render_wait_traced: `stream` waits (on the device) until the last
+        frame has finished its last closest-hit launch."""
+        self.ctx._check(lib().rq_render_wait_traced(self.ctx.h, self.h, stream))
+
     def counts(self) -> FrameCounts:
         c = FrameCounts()
         self.ctx._check(lib().rq_render_counts(self.ctx.h, self.h, C.byref(c)))
@@ -1173,15 +1179,24 @@ class FramePipeline:
     (rq_rebuild_lbvh, then rq_bvh_treelets when `treelets` > 0); the renderer
     runs on a third. rq_stream_order makes a render wait for its own BVH's
     rebuild and a rebuild wait for the render that last read its BVH, with no
-    host wait. Not pipelined (`pipelined` False): rebuild and render one after
-    the other on the default stream, with one BVH. The scene must not change
-    while frames are in flight. frame() returns the BVH handle it rendered.
+    host wait. `overlap` "frame": the rebuild for frame k+1 is ordered after
+    the render of frame k-1 and runs beside all of frame k (for short frames
+    whose launches leave the device idle); "tail": it is ordered after frame
+    k's last closest-hit launch (rq_render_wait_traced) and runs beside the
+    frame's shading and second-hit shadow rays only, so it does not compete
+    with a traversal that keeps every SIMD issuing. Not pipelined
+    (`pipelined` False): rebuild and render one after the other on the
+    default stream, with one BVH. The scene must not change while frames are
+    in flight. frame() returns the BVH handle it rendered.
     """
 
     def __init__(self, ctx: Context, renderer: Renderer, scene: int, bvh: int,
                  morton_mode: int = MORTON_SCENE_AABB, treelets: int = 0,
-                 pipelined: bool = True):
+                 pipelined: bool = True, overlap: str = "frame"):
+        if overlap not in ("frame", "tail"):
+            raise ValueError(f"overlap must be 'frame' or 'tail', not {overlap!r}")
         self.ctx, self.ren, self.treelets, self.pipelined = ctx, renderer, treelets, pipelined
+        self.overlap = overlap
         self.bvhs = [bvh]
         self.streams: list = []
         self.k = 0
@@ -1203,10 +1218,18 @@ class FramePipeline:
         cur, nxt = k % 2, (k + 1) % 2
         if k == 0:
             self._rebuild(self.bvhs[cur], s_build[cur])
-        self.ctx.stream_order(s_build[nxt], s_render)  # after the render that read bvhs[nxt]
-        self._rebuild(self.bvhs[nxt], s_build[nxt])    # frame k+1's BVH
-        self.ctx.stream_order(s_render, s_build[cur])  # after this frame's rebuild
-        self.ren.frame(self.bvhs[cur], stream=s_render)
+        if self.overlap == "frame":
+            self.ctx.stream_order(s_build[nxt], s_render)  # after the render that read bvhs[nxt]
+            self._rebuild(self.bvhs[nxt], s_build[nxt])    # frame k+1's BVH
+            self.ctx.stream_order(s_render, s_build[cur])  # after this frame's rebuild
+            self.ren.frame(self.bvhs[cur], stream=s_render)
+        else:
+            self.ctx.stream_order(s_render, s_build[cur])  # after this frame's rebuild
+            self.ren.frame(self.bvhs[cur], stream=s_render)
+            # frame k+1's BVH once frame k has traced: frame k-1, the last
+            # reader of bvhs[nxt], ran before it on the same stream
+            self.ren.wait_traced(s_build[nxt])
+            self._rebuild(self.bvhs[nxt], s_build[nxt])
         self.k = k + 1
         return self.bvhs[cur]
 

@@ -132,13 +132,16 @@ def test_render_frames_back_to_back_on_a_stream(ctx, rq):
         ctx.release(s)
 
 
-def test_pipelined_frames_equal_single_frames(ctx, rq):
+@pytest.mark.parametrize("overlap,bounce", [("frame", 1), ("tail", 1), ("tail", 0)])
+def test_pipelined_frames_equal_single_frames(ctx, rq, overlap, bounce):
     """rq.FramePipeline: the LBVH rebuild for frame k+1 runs on its own stream
-    while frame k renders (two BVHs alternating, ordered by rq_stream_order);
-    every frame equals a frame rendered after a plain rebuild, bit for bit."""
+    beside frame k (two BVHs alternating, ordered by rq_stream_order, or, with
+    overlap "tail", after frame k's last closest-hit launch by
+    rq_render_wait_traced); every frame equals a frame rendered after a plain
+    rebuild, bit for bit."""
     from rq import workloads as W
     mesh, p, _ = W.c1()
-    p.bounce = 1
+    p.bounce = bounce
     s = ctx.upload_scene(mesh)
     b = ctx.build_lbvh(s, rq.MORTON_SCENE_AABB)
     r = rq.Renderer(ctx, p)
@@ -149,10 +152,10 @@ def test_pipelined_frames_equal_single_frames(ctx, rq):
         ctx.sync()
         ref = r.samples()
         _, ref_ppm = r.image()
-        pipe = rq.FramePipeline(ctx, r, s, b, rq.MORTON_SCENE_AABB)
+        pipe = rq.FramePipeline(ctx, r, s, b, rq.MORTON_SCENE_AABB, overlap=overlap)
         used = []
         for k in range(8):
-            log(f"pipelined frame {k}")
+            log(f"pipelined frame {k} overlap {overlap} bounce {bounce}")
             used.append(pipe.frame())
             if k % 2 == 1:  # frames 0, 2, ... are left in flight behind the next one
                 ctx.sync(pipe.render_stream())

@@ -155,3 +155,10 @@ def test_render_defaults_are_reference_constants(rq):
     assert list(p.light) == [0.0, 3.0, 30.0]          # light2, main.cpp:775
     assert np.allclose(list(p.sky), [0.6, 0.8, 1.0])   # main.cpp:312
     assert p.bias == np.float32(1e-4)                  # main.cpp:404
+
+
+def test_frame_pipeline_rejects_unknown_overlap(rq):
+    """rq.FramePipeline takes overlap "frame" or "tail" and rejects anything
+    else before it touches the device."""
+    with pytest.raises(ValueError):
+        rq.FramePipeline(None, None, 0, 0, overlap="tails")
