@@ -1011,8 +1011,10 @@ def sweep(ctx, bvh, batches) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 100 steps: a pipelined frame's last rebuild (not overlapped by a next
+    # frame) adds build / steps to the step time
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--serve-seconds", type=float, default=10.0,
                     help="c5: measured seconds of arrivals (after 1 s of warm-up)")
