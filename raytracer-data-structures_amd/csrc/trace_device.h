@@ -178,12 +178,18 @@ __device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) 
 // slot order), the others pushed farthest first. The same boxes are tested
 // with the same slack, so culling is as conservative as the binary walk and
 // the result (the lexicographic minimum) is the same; visit order differs.
-template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true,
-          bool WIDE = false, class Src = NoSlices>
-__device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32_t* s_node,
-                                  float* s_tn, uint32_t gtid, uint32_t gstride,
-                                  Src* src = nullptr) {
+// ROWS, how the wide walk addresses a node's rows: a tree of
+// WIDE_UNIFORM_MAX_NODES nodes or more has rows beyond a 32-bit byte offset,
+// addressed in 64 bits with the near / far rows selected per value (ROWS_BIG);
+// smaller trees address them by 32-bit offsets from the array's base
+// (ROWS_SMALL); ROWS_ANY decides per launch inside one code path.
+constexpr int ROWS_SMALL = 0, ROWS_BIG = 1, ROWS_ANY = 2;
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES, bool WIDE,
+          int ROWS, class Src>
+__device__ inline void trace_loop_impl(const TraceArgs& a, const RayStream& rs, int32_t* s_node,
+                                       float* s_tn, uint32_t gtid, uint32_t gstride, Src* src) {
   static_assert(!(WIDE && COUNT), "counted traversal walks the binary nodes");
+  const bool big = ROWS == ROWS_ANY ? a.nprims > WIDE_UNIFORM_MAX_NODES : ROWS == ROWS_BIG;
   const int tid = threadIdx.x, lane = tid & 63;
   const uint64_t lt = (1ull << lane) - 1ull;
   const int max_sp = DEPTH + (int)a.gdepth;
@@ -316,8 +322,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
       const uint64_t am = __ballot(active);
       const int first = am ? __ffsll((long long)am) - 1 : 0;
       const uint32_t oref = (uint32_t)__shfl((int)oct, first, 64);
-      const bool uni = __ballot(active && oct != oref) == 0ull &&
-                       a.nprims <= WIDE_UNIFORM_MAX_NODES;
+      const bool uni = __ballot(active && oct != oref) == 0ull && !big;
       uni_rows = __builtin_amdgcn_readfirstlane((uint32_t)uni);
       s_oct = __builtin_amdgcn_readfirstlane(oref);
       mx = __ballot(r.sx), my = __ballot(r.sy), mz = __ballot(r.sz);
@@ -331,7 +336,7 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
         // the node's row offset, shared by both offset paths (computed here, a
         // path that only issues loads needs no wait behind the other's loads)
         const uint32_t base = (uint32_t)node << 7;
-        if (!uni_rows && a.nprims <= WIDE_UNIFORM_MAX_NODES) {
+        if (!big && !uni_rows) {
           // mixed octants: the same rows at per-lane byte offsets (an OR into
           // the node's 128-byte row, cheaper than a select per value). The
           // offsets are formed from the wave's sign masks at each visit
@@ -349,7 +354,10 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
           fy = *reinterpret_cast<const float4*>(wb + 32 + (base | (qy ^ 16u)));
           nz = *reinterpret_cast<const float4*>(wb + 64 + (base | qz));
           fz = *reinterpret_cast<const float4*>(wb + 64 + (base | (qz ^ 16u)));
-        } else if (uni_rows) {
+          // keeps the two paths' loads in their own branches: merged after
+          // them, the two sets of addresses became 64-bit address phis
+          if (ROWS == ROWS_SMALL) asm volatile("" ::: "memory");
+        } else if (ROWS == ROWS_ANY ? uni_rows != 0u : !big) {
           // rows at byte offsets lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80;
           // the near row of an axis is hi when its sign bit is set
           const char* wb = reinterpret_cast<const char*>(a.wide);
@@ -372,8 +380,17 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
         }
         // the link row after the rows: the uniform path, which statically
         // follows the per-lane path's loads, then starts with nothing in
-        // flight to wait on
-        const int4 ln = w->link;
+        // flight to wait on. It is addressed like the rows (scalar base + the
+        // 32-bit row offset) unless the tree is BIG.
+        // (The link row's offset is opaque to the optimiser: shared with the
+        // rows' offset as one 64-bit value across the branches, it turned
+        // every row load into a 64-bit address.)
+        uint32_t lofs = base | 96u;
+        if (ROWS == ROWS_SMALL) asm volatile("" : "+v"(lofs));
+        const int4 ln =
+            ROWS != ROWS_SMALL
+                ? w->link
+                : *reinterpret_cast<const int4*>(reinterpret_cast<const char*>(a.wide) + lofs);
         float t0, t1, t2, t3;
         const bool h0 = slab_w(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, r, bt, slack, t0) & (ln.x != 0);
         const bool h1 = slab_w(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, r, bt, slack, t1) & (ln.y != 0);
@@ -470,6 +487,32 @@ __device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32
     }
   }
 
+}
+
+// One code path for trees of any size (the batch and serving kernels, whose
+// launches walk many trees and hold their build phases in the same kernel).
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true,
+          bool WIDE = false, class Src = NoSlices>
+__device__ inline void trace_loop(const TraceArgs& a, const RayStream& rs, int32_t* s_node,
+                                  float* s_tn, uint32_t gtid, uint32_t gstride,
+                                  Src* src = nullptr) {
+  trace_loop_impl<ANY, COUNT, BLOCK, DEPTH, TSTACK, SPHERES, WIDE, ROWS_ANY, Src>(
+      a, rs, s_node, s_tn, gtid, gstride, src);
+}
+// A path per row form, chosen once per launch (the query kernels: one tree).
+// Below 2^25 nodes every row, the link row included, is a load at a 32-bit
+// offset from the scalar base, with no 64-bit address arithmetic; the closest-
+// hit kernels also drop their register spills.
+template <bool ANY, bool COUNT, int BLOCK, int DEPTH, bool TSTACK, bool SPHERES = true,
+          bool WIDE = false>
+__device__ inline void trace_loop_split(const TraceArgs& a, const RayStream& rs, int32_t* s_node,
+                                        float* s_tn, uint32_t gtid, uint32_t gstride) {
+  if (WIDE && a.nprims > WIDE_UNIFORM_MAX_NODES)
+    trace_loop_impl<ANY, COUNT, BLOCK, DEPTH, TSTACK, SPHERES, WIDE, ROWS_BIG, NoSlices>(
+        a, rs, s_node, s_tn, gtid, gstride, nullptr);
+  else
+    trace_loop_impl<ANY, COUNT, BLOCK, DEPTH, TSTACK, SPHERES, WIDE, ROWS_SMALL, NoSlices>(
+        a, rs, s_node, s_tn, gtid, gstride, nullptr);
 }
 
 }  // namespace rq

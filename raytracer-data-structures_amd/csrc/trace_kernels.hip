@@ -26,9 +26,8 @@ void k_trace(TraceArgs a) {
   rs.base = a.ctr_base;
   rs.wave = (blockIdx.x * BLOCK + threadIdx.x) >> 6;
   rs.waves = gridDim.x * (BLOCK / 64);
-  trace_loop<ANY, COUNT, BLOCK, DEPTH, TSTACK, true, WIDE>(a, rs, s_node, s_tn,
-                                                           blockIdx.x * BLOCK + threadIdx.x,
-                                                           gridDim.x * BLOCK);
+  trace_loop_split<ANY, COUNT, BLOCK, DEPTH, TSTACK, true, WIDE>(
+      a, rs, s_node, s_tn, blockIdx.x * BLOCK + threadIdx.x, gridDim.x * BLOCK);
 }
 
 // Device scratch of the allocation-free closest-hit entry
