@@ -56,7 +56,7 @@ BUILD_BYTES_PER_TRI = 48 + 52 + 60 + 28 + 176
 # refill, closest slice, any slice): the fastest of a sweep of every variant and
 # refill threshold on MI355X (--sweep); fixed so that a profile of the same
 # command times the same kernels. --autotune times the variants again instead.
-TRACE_CONFIG = {"c1": None, "c2": "17,64,14,64,2048,1024", "c3": "14,16,9,16,2048,2048"}
+TRACE_CONFIG = {"c1": None, "c2": "17,64,14,64,2048,1024", "c3": "17,16,17,16,2048,2048"}
 # The LBVH rebuild of frame k+1 on a second stream (rq.FramePipeline): "frame"
 # beside the whole of frame k, "tail" after frame k's last closest-hit launch,
 # beside its shading only; None: sequential. Measured on MI355X (ms per frame,
