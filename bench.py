@@ -60,8 +60,8 @@ TRACE_CONFIG = {"c1": None, "c2": "17,64,14,64,2048,1024", "c3": "17,16,17,16,20
 # The LBVH rebuild of frame k+1 on a second stream (rq.FramePipeline): "frame"
 # beside the whole of frame k, "tail" after frame k's last closest-hit launch,
 # beside its shading only; None: sequential. Measured on MI355X (ms per frame,
-# two runs each): C1 sequential 0.314, frame 0.219, tail 0.290; C2 sequential
-# 3.097, frame 3.058, tail 3.051.
+# two runs each): --config c1 sequential 0.314, frame 0.219, tail 0.290;
+# --config c2 sequential 3.097, frame 3.058, tail 3.051.
 PIPELINE = {"c1": "frame", "c2": "tail"}
 ALG_BASIS = ("algorithmic bytes of the canonical traversal (rays in, hits out, 32 B per node "
              "box tested, 36 B per triangle tested); node and triangle reads are mostly served "
@@ -546,7 +546,7 @@ def base_line(value, ms_per_step, steps, warmup, config) -> dict:
 
 
 def run_frames(ctx, args) -> int:
-    """C1 / C2: rebuild + one frame of the wavefront renderer per step."""
+    """--config c1 / c2: rebuild + one frame of the wavefront renderer per step."""
     mesh, params, workload = (W.c2 if args.config == "c2" else W.c1)()
     log(f"{workload}: {mesh.ntri} triangles")
     scene = ctx.upload_scene(mesh)
@@ -592,7 +592,7 @@ def run_frames(ctx, args) -> int:
         return 0
 
     # ---- timed region: K frames, HIP events on the launch streams around the
-    # traversal launches only (events around every kernel cost ~4 % of a C2 frame)
+    # traversal launches only (events around every kernel cost ~4 % of a 1.31 M-triangle frame)
     ctx.profile_only(["trace_closest", "trace_any"])
     ctx.profile(True)
     ctx.profile_reset()

@@ -1,18 +1,20 @@
 """Parity at the benchmark workloads' own sizes, against the CPU oracle.
 
-C2 (1,310,720 triangles, 1920x1080x4 spp, mirror bounce + shadow rays):
+The large-mesh frame (1,310,720 triangles, 1920x1080x4 spp, mirror bounce +
+shadow rays):
     every primary ray bit-exact; primary / bounce hit ids and both shadow-flag
     sets identical over the whole frame; sample colours within 2e-5 and the
     image within 1/255; closest hit (prim, t, u, v bit-exact) of every primary
     and bounce ray and any hit of every shadow ray, for every traversal variant.
-C3 (256 L5 icospheres, 5,242,880 triangles, 2^24 uniform rays): the LBVH
+The raw ray batch (256 L5 icospheres, 5,242,880 triangles, 2^24 uniform
+rays): the LBVH
     (codes, order, topology, boxes) bit-exact; every device ray bit-exact;
     closest and any hit on every 16th ray (1,048,576) for every variant, and
     after two treelet rounds.
-C4 (512 meshes, 36.5 M triangles, 176 M rays, one persistent launch): every
+The 512-mesh batch (36.5 M triangles, 176 M rays, one persistent launch): every
     mesh's LBVH bit-exact; a stride of every mesh's rays (about 2,000 per
     mesh) closest and any hit.
-C5 (64 resident meshes of the serving distribution, 20.1 M triangles):
+The server (64 resident meshes of the serving distribution, 20.1 M triangles):
     240 Poisson requests (half pinhole, half uniform, 5 % uploads of a new
     mesh into 8 upload slots beside the 64 resident ones) served by
     continuous batching; every request's results on a stride equal the
@@ -89,7 +91,7 @@ def _variants(any_hit):
     return [v for v in _rq.Context.trace_variants() if not (any_hit and v["tstack"])]
 
 
-# ------------------------------------------------------------------- C2
+# ------------------------------------- large-mesh frame (1.31 M triangles)
 @pytest.fixture(scope="module")
 def c2(ctx, O):
     mesh, p, _ = W.c2()
@@ -182,7 +184,7 @@ def test_c2_any_every_variant(ctx, O, c2):
         assert ctx.device_errors() == 0
 
 
-# ------------------------------------------------------------------- C3
+# ------------------------------------ raw ray batch (5.24 M triangles, 2^24 rays)
 C3_STRIDE = 16
 
 
@@ -262,7 +264,7 @@ def test_c3_after_treelets(ctx, c3):
     _c3_queries(ctx, c3, [wide[0], wide[-1]], [wide[0], wide[-1]], " treelets")
 
 
-# ------------------------------------------------------------------- C4
+# ----------------------------------------------------- 512-mesh batch
 C4_RAYS_PER_MESH = 2048
 
 
@@ -331,7 +333,7 @@ def test_c4_full_workload(ctx, O):
             ctx.release(s)
 
 
-# ------------------------------------------------------------------- C5
+# -------------------------------------------------- server, 64 meshes
 C5_REQUESTS = 240
 C5_RAYS_PER_REQUEST = 4096
 C5_UPLOAD_SLOTS = 8

@@ -107,7 +107,7 @@ def test_query_empty_scene_misses(ctx, rq):
 
 
 def test_query_c1_camera(ctx, O, rq):
-    """C1 mesh (81,920 triangles) with the reference camera rays at 640x480."""
+    """The small frame's mesh (81,920 triangles) with the reference camera rays at 640x480."""
     mesh = rq.icosphere(6, 1, center=(0, 0, -5))
     jit = O.mt19937_jitter(2 * 640 * 480)
     rays = O.camera_rays_ref(640, 480, 1, jit)
