@@ -93,7 +93,8 @@ int rq_render_read_samples(rq_ctx* ctx, rq_renderer* r, int32_t* hit0, int32_t* 
  * shared constant arrays (zeros, +inf). */
 int rq_render_primary_rays(rq_ctx* ctx, rq_renderer* r, rq_rays* out, uint32_t* n);
 /* Device SoA of one ray batch of the last frame: which = 0 primary, 1 bounce,
- * 2 shadow (n or 2n slots; unused slots are inactive rays). */
+ * 2 shadow (n or 2n slots; unused slots are inactive rays). Read-only, like
+ * the primary rays: every batch's tmin is the renderer's shared zero array. */
 int rq_render_rays(rq_ctx* ctx, rq_renderer* r, int which, rq_rays* out, uint32_t* n);
 
 /* Batch ray generators into caller-owned device SoA buffers. */
