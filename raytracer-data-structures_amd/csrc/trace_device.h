@@ -178,6 +178,26 @@ __device__ inline bool sphere_hit(const TriRec& S, const RayState& r, float& t) 
 // slot order), the others pushed farthest first. The same boxes are tested
 // with the same slack, so culling is as conservative as the binary walk and
 // the result (the lexicographic minimum) is the same; visit order differs.
+// The wide records as a raw buffer resource (scalar registers): row loads at
+// a 32-bit offset from it, with the row's fixed part as a scalar or immediate
+// offset, need no 64-bit address arithmetic. Dword 3 selects the 32-bit data
+// format that gfx9 buffer instructions require.
+__device__ inline __amdgpu_buffer_rsrc_t wide_rsrc(const TraceArgs& a) {
+  const uint64_t bytes = (uint64_t)a.nprims * sizeof(WideNode);
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<WideNode*>(a.wide), (short)0,
+                                           (int)(uint32_t)min(bytes, (uint64_t)0xFFFFFFFFu),
+                                           0x00020000);
+}
+__device__ inline float4 row_load(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                     __uint_as_float(v[3]));
+}
+__device__ inline int4 link_load(__amdgpu_buffer_rsrc_t r, uint32_t voff) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, 96, 0);
+  return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+}
+
 // ROWS, how the wide walk addresses a node's rows: a tree of
 // WIDE_UNIFORM_MAX_NODES nodes or more has rows beyond a 32-bit byte offset,
 // addressed in 64 bits with the near / far rows selected per value (ROWS_BIG);
@@ -317,6 +337,7 @@ __device__ inline void trace_loop_impl(const TraceArgs& a, const RayStream& rs, 
     // recomputes it.
     uint32_t uni_rows = 0u, s_oct = 0u;
     uint64_t mx = 0, my = 0, mz = 0;  // lanes whose 1/d has that sign bit set
+    const __amdgpu_buffer_rsrc_t wrs = wide_rsrc(a);
     if (WIDE) {
       const uint32_t oct = (uint32_t)r.sx | ((uint32_t)r.sy << 1) | ((uint32_t)r.sz << 2);
       const uint64_t am = __ballot(active);
@@ -336,6 +357,7 @@ __device__ inline void trace_loop_impl(const TraceArgs& a, const RayStream& rs, 
         // the node's row offset, shared by both offset paths (computed here, a
         // path that only issues loads needs no wait behind the other's loads)
         const uint32_t base = (uint32_t)node << 7;
+        int4 ln;
         if (!big && !uni_rows) {
           // mixed octants: the same rows at per-lane byte offsets (an OR into
           // the node's 128-byte row, cheaper than a select per value). The
@@ -343,33 +365,23 @@ __device__ inline void trace_loop_impl(const TraceArgs& a, const RayStream& rs, 
           // (volatile: not hoisted out of the loop, where they would hold
           // three registers across it; the closest-hit kernel would reach 91
           // and leave less room beside a concurrent any-hit launch).
-          const char* wb = reinterpret_cast<const char*>(a.wide);
           uint32_t qx, qy, qz;
           asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qx) : "s"(mx));
           asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qy) : "s"(my));
           asm volatile("v_cndmask_b32_e64 %0, 0, 16, %1" : "=v"(qz) : "s"(mz));
-          nx = *reinterpret_cast<const float4*>(wb + (base | qx));
-          fx = *reinterpret_cast<const float4*>(wb + (base | (qx ^ 16u)));
-          ny = *reinterpret_cast<const float4*>(wb + 32 + (base | qy));
-          fy = *reinterpret_cast<const float4*>(wb + 32 + (base | (qy ^ 16u)));
-          nz = *reinterpret_cast<const float4*>(wb + 64 + (base | qz));
-          fz = *reinterpret_cast<const float4*>(wb + 64 + (base | (qz ^ 16u)));
-          // keeps the two paths' loads in their own branches: merged after
-          // them, the two sets of addresses became 64-bit address phis
-          if (ROWS == ROWS_SMALL) asm volatile("" ::: "memory");
+          nx = row_load(wrs, base | qx, 0u), fx = row_load(wrs, base | (qx ^ 16u), 0u);
+          ny = row_load(wrs, base | qy, 32u), fy = row_load(wrs, base | (qy ^ 16u), 32u);
+          nz = row_load(wrs, base | qz, 64u), fz = row_load(wrs, base | (qz ^ 16u), 64u);
+          if (ROWS == ROWS_ANY) ln = link_load(wrs, base);
         } else if (ROWS == ROWS_ANY ? uni_rows != 0u : !big) {
           // rows at byte offsets lox 0, hix 16, loy 32, hiy 48, loz 64, hiz 80;
           // the near row of an axis is hi when its sign bit is set
-          const char* wb = reinterpret_cast<const char*>(a.wide);
           const uint32_t ox = (s_oct & 1u) << 4, oy = 32u + ((s_oct & 2u) << 3),
                          oz = 64u + ((s_oct & 4u) << 2);
-          const uint64_t off = base;
-          nx = *reinterpret_cast<const float4*>(wb + ox + off);
-          fx = *reinterpret_cast<const float4*>(wb + (ox ^ 16u) + off);
-          ny = *reinterpret_cast<const float4*>(wb + oy + off);
-          fy = *reinterpret_cast<const float4*>(wb + (oy ^ 16u) + off);
-          nz = *reinterpret_cast<const float4*>(wb + oz + off);
-          fz = *reinterpret_cast<const float4*>(wb + (oz ^ 16u) + off);
+          nx = row_load(wrs, base, ox), fx = row_load(wrs, base, ox ^ 16u);
+          ny = row_load(wrs, base, oy), fy = row_load(wrs, base, oy ^ 16u);
+          nz = row_load(wrs, base, oz), fz = row_load(wrs, base, oz ^ 16u);
+          if (ROWS == ROWS_ANY) ln = link_load(wrs, base);
         } else {
           const float4 lx = w->lox, hx = w->hix, ly = w->loy, hy = w->hiy, lz = w->loz,
                        hz = w->hiz;
@@ -377,20 +389,14 @@ __device__ inline void trace_loop_impl(const TraceArgs& a, const RayStream& rs, 
           nx = r.sx ? hx : lx, fx = r.sx ? lx : hx;
           ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
           nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
+          if (ROWS != ROWS_SMALL) ln = w->link;
         }
         // the link row after the rows: the uniform path, which statically
         // follows the per-lane path's loads, then starts with nothing in
-        // flight to wait on. It is addressed like the rows (scalar base + the
-        // 32-bit row offset) unless the tree is BIG.
-        // (The link row's offset is opaque to the optimiser: shared with the
-        // rows' offset as one 64-bit value across the branches, it turned
-        // every row load into a 64-bit address.)
-        uint32_t lofs = base | 96u;
-        if (ROWS == ROWS_SMALL) asm volatile("" : "+v"(lofs));
-        const int4 ln =
-            ROWS != ROWS_SMALL
-                ? w->link
-                : *reinterpret_cast<const int4*>(reinterpret_cast<const char*>(a.wide) + lofs);
+        // flight to wait on. (A walk that decides the row form per launch
+        // loads it in each branch instead: one load after them would be a
+        // select between a buffer and a 64-bit load.)
+        if (ROWS == ROWS_SMALL) ln = link_load(wrs, base);
         float t0, t1, t2, t3;
         const bool h0 = slab_w(nx.x, ny.x, nz.x, fx.x, fy.x, fz.x, r, bt, slack, t0) & (ln.x != 0);
         const bool h1 = slab_w(nx.y, ny.y, nz.y, fx.y, fy.y, fz.y, r, bt, slack, t1) & (ln.y != 0);
